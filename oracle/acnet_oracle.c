@@ -1,0 +1,471 @@
+/*
+ * ORACLE -- test infrastructure only (see acnet_oracle.h for the pins).
+ *
+ * CPU restatement of the reference ACNetLegacy 2x pass:
+ *   CPUProcessor<ACNetLegacy>::process   core/src/processor/cpu/CPUProcessor.cpp:1371-1390
+ *   conv3x3_1to8_relu                    core/internal/.../CPU/Backend.hpp:17-34, Common.hpp:166-197
+ *   conv3x3_8to8_relu                    Backend.hpp:35-38, Common.hpp:116-164
+ *   conv3x3_8to8_relu_deconv2x2_8to1     Backend.hpp:39-56, Common.hpp:344-393
+ *   summation orders                     Generic.hpp:12-82, X86/AVX.hpp:14-150 (FMA),
+ *                                        X86/AVX512.hpp:56-110 (falls back to the 256-bit
+ *                                        FMA path for 8 channels and 9 taps)
+ *   toFloat / fromFloat                  core/internal/AC/Core/Internal/Util.hpp:49-78
+ *   weight offsets                       core/include/AC/Core/Model/ACNet.hpp:34-60
+ *
+ * Build with -ffp-contract=off: every multiply-add that is not an explicit
+ * fmaf() below is rounded twice, as in the reference's intrinsics.
+ */
+#include "acnet_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NB 8          /* blocks (core/src/Model.cpp:10) */
+#define CH 8          /* feature channels */
+
+static inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* ---- element conversion (Util.hpp:49-78) ------------------------------------------ */
+
+float orc_half_to_float(uint16_t h)
+{
+    uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    uint32_t exp = (h >> 10) & 0x1f;
+    uint32_t man = h & 0x3ffu;
+    uint32_t bits;
+    if (exp == 0) {
+        if (man == 0) bits = sign;
+        else {                       /* subnormal: value = man * 2^-24 */
+            float f = (float)man * 5.9604644775390625e-08f;
+            memcpy(&bits, &f, 4);
+            bits |= sign;
+        }
+    } else if (exp == 31) bits = sign | 0x7f800000u | (man << 13);
+    else bits = sign | ((exp + 112u) << 23) | (man << 13);
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
+/* IEEE round-to-nearest-even float -> binary16 */
+uint16_t orc_float_to_half(float f)
+{
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0));
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);          /* rounds to inf */
+    if (ax < 0x38800000u) {                                             /* subnormal / zero */
+        float a;
+        memcpy(&a, &ax, 4);
+        /* a * 2^24 is exact in float range; round half to even */
+        float s = a * 16777216.0f;
+        float r = nearbyintf(s);
+        return (uint16_t)(sign | (uint32_t)r);
+    }
+    uint32_t mant = ax & 0x7fffffu;
+    uint32_t e = (ax >> 23) - 112u;
+    uint32_t hv = (e << 10) | (mant >> 13);
+    uint32_t rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (hv & 1u))) hv++;
+    return (uint16_t)(sign | hv);
+}
+
+float orc_to_float(const void* p, int type)
+{
+    switch (type) {
+    case ORC_U8: return (float)(*(const uint8_t*)p) / 255.0f;
+    case ORC_U16: return (float)(*(const uint16_t*)p) / 65535.0f;
+    case ORC_F16: return orc_half_to_float(*(const uint16_t*)p);
+    default: return *(const float*)p;
+    }
+}
+
+void orc_from_float(float v, void* p, int type)
+{
+    float s = v < 0.0f ? 0.0f : (v < 1.0f ? v : 1.0f);
+    switch (type) {
+    case ORC_U8: *(uint8_t*)p = (uint8_t)(s * 255.0f + 0.5f); break;
+    case ORC_U16: *(uint16_t*)p = (uint16_t)(s * 65535.0f + 0.5f); break;
+    case ORC_F16: *(uint16_t*)p = orc_float_to_half(s); break;
+    default: *(float*)p = s; break;
+    }
+}
+
+static inline int elem_size(int type) { return type & 0xff; }
+
+/* ---- the two reference summation orders ------------------------------------------- */
+
+/* AVX.hpp:24-30: ((s0+s4)+(s2+s6)) + ((s1+s5)+(s3+s7)) */
+static inline float hsum8(const float* s)
+{
+    float a0 = s[0] + s[4], a1 = s[1] + s[5], a2 = s[2] + s[6], a3 = s[3] + s[7];
+    float b0 = a0 + a2, b1 = a1 + a3;
+    return b0 + b1;
+}
+
+/* L0: OpImplFMA::conv_cin1<8,9> (AVX.hpp:93-124): fma over taps 0..7, hsum, then
+ * the 9th tap as a separate multiply and add, then + bias. */
+static inline void conv_cin1_fma(const float* r, const float* k, const float* b, float* out)
+{
+    for (int n = 0; n < CH; n++) {
+        const float* kp = k + n * 9;
+        float s[8];
+        for (int p = 0; p < 8; p++) s[p] = fmaf(r[p], kp[p], 0.0f);
+        float sum = hsum8(s);
+        float t = r[8] * kp[8];
+        sum = sum + t;
+        out[n] = sum + b[n];
+    }
+}
+
+/* L0: OpImplGeneric::conv_cin1 (Generic.hpp:60-69): left fold of 9 products, + bias */
+static inline void conv_cin1_generic(const float* r, const float* k, const float* b, float* out)
+{
+    for (int n = 0; n < CH; n++) {
+        const float* kp = k + n * 9;
+        float sum = r[0] * kp[0];
+        for (int p = 1; p < 9; p++) { float t = r[p] * kp[p]; sum = sum + t; }
+        out[n] = sum + b[n];
+    }
+}
+
+/* 8->8: OpImplFMA::conv<8,8,9> (AVX.hpp:32-59,126-146): per output, one 8-lane
+ * accumulator over channels, fma over taps 0..8, hsum, added to the bias. */
+static inline void conv8_fma(const float* const* rp, const float* k, const float* b, float* out)
+{
+    for (int n = 0; n < CH; n++) {
+        const float* kn = k + n * 72;
+        float s[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        for (int p = 0; p < 9; p++)
+            for (int c = 0; c < 8; c++) s[c] = fmaf(rp[p][c], kn[p * 8 + c], s[c]);
+        out[n] = b[n] + hsum8(s);
+    }
+}
+
+/* 8->8: OpImplGeneric::conv (Generic.hpp:71-81): sum over channels (outer) of
+ * the left-folded 9-tap block sums, + bias. */
+static inline void conv8_generic(const float* const* rp, const float* k, const float* b, float* out)
+{
+    for (int n = 0; n < CH; n++) {
+        const float* kn = k + n * 72;
+        float sum = 0.0f;
+        for (int c = 0; c < 8; c++) {
+            float bs = rp[0][c] * kn[c];
+            for (int p = 1; p < 9; p++) { float t = rp[p][c] * kn[p * 8 + c]; bs = bs + t; }
+            sum = sum + bs;
+        }
+        out[n] = sum + b[n];
+    }
+}
+
+/* deconv dot<8>: FMA path (AVX.hpp:62-91) = 0 + hsum(products); Generic = left fold */
+static inline float dot8_fma(const float* v, const float* k)
+{
+    float s[8];
+    for (int c = 0; c < 8; c++) s[c] = fmaf(v[c], k[c], 0.0f);
+    return 0.0f + hsum8(s);
+}
+static inline float dot8_generic(const float* v, const float* k)
+{
+    float sum = v[0] * k[0];
+    for (int c = 1; c < 8; c++) { float t = v[c] * k[c]; sum = sum + t; }
+    return sum;
+}
+
+/* ---- the pass (CPUProcessor.cpp:1371-1390) ---------------------------------------- */
+
+int orc_acnet_pass(const void* src, long src_stride, int w, int h, int src_type,
+                   void* dst, long dst_stride, int dst_type,
+                   const float* kernels, const float* biases, int order, int threads)
+{
+    if (!src || !dst || w <= 0 || h <= 0 || !kernels || !biases) return -22;
+    size_t plane = (size_t)w * h * CH;
+    float* t1 = (float*)malloc(plane * sizeof(float));
+    float* t2 = (float*)malloc(plane * sizeof(float));
+    if (!t1 || !t2) { free(t1); free(t2); return -12; }
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+    const int fma_order = (order == ORC_ORDER_FMA);
+    const unsigned char* sb = (const unsigned char*)src;
+    const int ses = elem_size(src_type);
+
+    /* L0: conv3x3 1->8 + ReLU, replicate-clamped taps (Common.hpp:166-197) */
+    {
+        const float* k = kernels;          /* kernelOffset(0) = 0 */
+        const float* b = biases;           /* biasOffset(0) = 0 */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++) {
+            int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+            for (int j = 0; j < w; j++) {
+                int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+                int ys[3] = { i - tp, i, i + bp }, xs[3] = { j - lp, j, j + rp };
+                float r[9];
+                for (int a = 0; a < 3; a++)
+                    for (int c = 0; c < 3; c++)
+                        r[a * 3 + c] = orc_to_float(sb + ys[a] * src_stride + (long)xs[c] * ses, src_type);
+                float* out = t1 + ((size_t)i * w + j) * CH;
+                if (fma_order) conv_cin1_fma(r, k, b, out); else conv_cin1_generic(r, k, b, out);
+                for (int n = 0; n < CH; n++) out[n] = out[n] > 0.0f ? out[n] : 0.0f;   /* max(v, 0) */
+            }
+        }
+    }
+
+    float* tin = t1;
+    float* tout = t2;
+    /* L1..L7: conv3x3 8->8 + ReLU (Common.hpp:116-164) */
+    for (int l = 1; l < NB; l++) {
+        const float* k = kernels + 72 + 576 * (l - 1);
+        const float* b = biases + 8 * l;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++) {
+            int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+            for (int j = 0; j < w; j++) {
+                int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+                int ys[3] = { i - tp, i, i + bp }, xs[3] = { j - lp, j, j + rp };
+                const float* rpp[9];
+                for (int a = 0; a < 3; a++)
+                    for (int c = 0; c < 3; c++)
+                        rpp[a * 3 + c] = tin + ((size_t)ys[a] * w + xs[c]) * CH;
+                float* out = tout + ((size_t)i * w + j) * CH;
+                if (fma_order) conv8_fma(rpp, k, b, out); else conv8_generic(rpp, k, b, out);
+                for (int n = 0; n < CH; n++) out[n] = out[n] > 0.0f ? out[n] : 0.0f;
+            }
+        }
+        float* tmp = tin; tin = tout; tout = tmp;
+    }
+
+    /* L8 + L9: conv3x3 8->8 + ReLU, then deconv 2x2 stride 2 8->1 (Common.hpp:344-393) */
+    {
+        const float* k = kernels + 72 + 576 * (NB - 1);     /* layer 8 */
+        const float* b = biases + 8 * NB;
+        const float* k2 = kernels + 72 + 576 * NB;          /* layer 9: offset 4680 */
+        unsigned char* db = (unsigned char*)dst;
+        const int des = elem_size(dst_type);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++) {
+            int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+            for (int j = 0; j < w; j++) {
+                int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+                int ys[3] = { i - tp, i, i + bp }, xs[3] = { j - lp, j, j + rp };
+                const float* rpp[9];
+                for (int a = 0; a < 3; a++)
+                    for (int c = 0; c < 3; c++)
+                        rpp[a * 3 + c] = tin + ((size_t)ys[a] * w + xs[c]) * CH;
+                float s[CH];
+                if (fma_order) conv8_fma(rpp, k, b, s); else conv8_generic(rpp, k, b, s);
+                for (int n = 0; n < CH; n++) s[n] = s[n] > 0.0f ? s[n] : 0.0f;
+                for (int dy = 0; dy < 2; dy++)
+                    for (int dx = 0; dx < 2; dx++) {
+                        const float* kk = k2 + 8 * (dy * 2 + dx);
+                        float v = fma_order ? dot8_fma(s, kk) : dot8_generic(s, kk);
+                        orc_from_float(v, db + (long)(2 * i + dy) * dst_stride + (long)(2 * j + dx) * des, dst_type);
+                    }
+            }
+        }
+    }
+    free(t1);
+    free(t2);
+    return 0;
+}
+
+/* ---- FP64 restatement (error reference only) --------------------------------------- */
+
+int orc_acnet_pass_f64(const float* src, int w, int h, double* dst,
+                       const float* kernels, const float* biases, int threads)
+{
+    if (!src || !dst || w <= 0 || h <= 0) return -22;
+    size_t plane = (size_t)w * h * CH;
+    double* t1 = (double*)malloc(plane * sizeof(double));
+    double* t2 = (double*)malloc(plane * sizeof(double));
+    if (!t1 || !t2) { free(t1); free(t2); return -12; }
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++)
+            for (int n = 0; n < CH; n++) {
+                double acc = biases[n];
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        int y = clampi(i + dy, 0, h - 1), x = clampi(j + dx, 0, w - 1);
+                        acc += (double)src[(size_t)y * w + x] * (double)kernels[n * 9 + (dy + 1) * 3 + (dx + 1)];
+                    }
+                t1[((size_t)i * w + j) * CH + n] = acc > 0.0 ? acc : 0.0;
+            }
+    double* tin = t1;
+    double* tout = t2;
+    for (int l = 1; l <= NB; l++) {
+        const float* k = kernels + 72 + 576 * (l - 1);
+        const float* b = biases + 8 * l;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++)
+                for (int n = 0; n < CH; n++) {
+                    double acc = b[n];
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            int y = clampi(i + dy, 0, h - 1), x = clampi(j + dx, 0, w - 1);
+                            const double* r = tin + ((size_t)y * w + x) * CH;
+                            const float* kk = k + n * 72 + ((dy + 1) * 3 + (dx + 1)) * 8;
+                            for (int c = 0; c < CH; c++) acc += r[c] * (double)kk[c];
+                        }
+                    tout[((size_t)i * w + j) * CH + n] = acc > 0.0 ? acc : 0.0;
+                }
+        double* tmp = tin; tin = tout; tout = tmp;
+    }
+    const float* k2 = kernels + 72 + 576 * NB;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+            const double* s = tin + ((size_t)i * w + j) * CH;
+            for (int dy = 0; dy < 2; dy++)
+                for (int dx = 0; dx < 2; dx++) {
+                    double v = 0.0;
+                    for (int c = 0; c < CH; c++) v += s[c] * (double)k2[8 * (dy * 2 + dx) + c];
+                    v = v < 0.0 ? 0.0 : (v < 1.0 ? v : 1.0);
+                    dst[(size_t)(2 * i + dy) * (2 * w) + (2 * j + dx)] = v;
+                }
+        }
+    free(t1);
+    free(t2);
+    return 0;
+}
+
+/* ---- colour split / merge (ImageProcess.cpp:38-61, :191-215) ----------------------- */
+
+int orc_rgb2yuv(const void* rgb, long rgb_stride, int w, int h, int channels, int type,
+                void* y, long y_stride, void* uv, long uv_stride)
+{
+    if (channels != 3 && channels != 4) return -22;
+    const int es = elem_size(type);
+    const int uvc = channels - 1;
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+            const unsigned char* in = (const unsigned char*)rgb + i * rgb_stride + (long)j * channels * es;
+            unsigned char* yo = (unsigned char*)y + i * y_stride + (long)j * es;
+            unsigned char* uo = (unsigned char*)uv + i * uv_stride + (long)j * uvc * es;
+            float r = orc_to_float(in, type), g = orc_to_float(in + es, type), b = orc_to_float(in + 2 * es, type);
+            float a0 = 0.299f * r, a1 = 0.587f * g, a2 = 0.114f * b;
+            float yy = (a0 + a1) + a2;
+            float u = 0.564f * (b - yy) + 0.5f;
+            float v = 0.713f * (r - yy) + 0.5f;
+            orc_from_float(yy, yo, type);
+            orc_from_float(u, uo, type);
+            orc_from_float(v, uo + es, type);
+            if (channels == 4) orc_from_float(orc_to_float(in + 3 * es, type), uo + 2 * es, type);
+        }
+    return 0;
+}
+
+int orc_yuv2rgb(const void* y, long y_stride, const void* uv, long uv_stride,
+                int w, int h, int channels, int type, void* rgb, long rgb_stride)
+{
+    if (channels != 3 && channels != 4) return -22;
+    const int es = elem_size(type);
+    const int uvc = channels - 1;
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+            const unsigned char* yi = (const unsigned char*)y + i * y_stride + (long)j * es;
+            const unsigned char* ui = (const unsigned char*)uv + i * uv_stride + (long)j * uvc * es;
+            unsigned char* out = (unsigned char*)rgb + i * rgb_stride + (long)j * channels * es;
+            float yy = orc_to_float(yi, type);
+            float u = orc_to_float(ui, type) - 0.5f;
+            float v = orc_to_float(ui + es, type) - 0.5f;
+            float r = yy + 1.403f * v;
+            float g = (yy - 0.344f * u) - 0.714f * v;
+            float b = yy + 1.773f * u;
+            orc_from_float(r, out, type);
+            orc_from_float(g, out + es, type);
+            orc_from_float(b, out + 2 * es, type);
+            if (channels == 4) orc_from_float(orc_to_float(ui + 2 * es, type), out + 3 * es, type);
+        }
+    return 0;
+}
+
+/* ---- 2x resize, closed-form taps (parity with stb_image_resize2 unpinned) ----------- */
+
+/* Output pixel centre 2i+0.5 maps to source i + (-0.25 | +0.25).  Bilinear: taps
+ * (0.25, 0.75) on (i-1, i) for even outputs and (0.75, 0.25) on (i, i+1) for odd.
+ * Catmull-Rom (a = -0.5) at fractional offsets 0.75 / 0.25 from the left tap. */
+static void taps2x(int mode, int odd, int* off, float* wt, int* nt)
+{
+    if (mode == 0) {
+        *nt = 2;
+        if (!odd) { off[0] = -1; off[1] = 0; wt[0] = 0.25f; wt[1] = 0.75f; }
+        else { off[0] = 0; off[1] = 1; wt[0] = 0.75f; wt[1] = 0.25f; }
+    } else {
+        *nt = 4;
+        /* t = 0.75 for even (between i-1 and i), t = 0.25 for odd (between i and i+1) */
+        static const float even_w[4] = { -0.0234375f, 0.2265625f, 0.8671875f, -0.0703125f };
+        static const float odd_w[4] = { -0.0703125f, 0.8671875f, 0.2265625f, -0.0234375f };
+        const float* src = odd ? odd_w : even_w;
+        int base = odd ? -1 : -2;
+        for (int t = 0; t < 4; t++) { off[t] = base + t; wt[t] = src[t]; }
+    }
+}
+
+int orc_resize2x(const void* src, long src_stride, int w, int h, int channels, int type,
+                 void* dst, long dst_stride, int mode)
+{
+    if (w <= 0 || h <= 0 || channels <= 0) return -22;
+    const int es = elem_size(type);
+    int W2 = 2 * w;
+    float* tmp = (float*)malloc(sizeof(float) * (size_t)W2 * h * channels);
+    if (!tmp) return -12;
+    int off[4], nt;
+    float wt[4];
+    for (int i = 0; i < h; i++)
+        for (int x = 0; x < W2; x++) {
+            taps2x(mode, x & 1, off, wt, &nt);
+            for (int c = 0; c < channels; c++) {
+                float acc = 0.0f;
+                for (int t = 0; t < nt; t++) {
+                    int sx = clampi((x >> 1) + off[t], 0, w - 1);
+                    const unsigned char* p = (const unsigned char*)src + i * src_stride + ((long)sx * channels + c) * es;
+                    float m = wt[t] * orc_to_float(p, type);
+                    acc = acc + m;
+                }
+                tmp[((size_t)i * W2 + x) * channels + c] = acc;
+            }
+        }
+    for (int y = 0; y < 2 * h; y++) {
+        taps2x(mode, y & 1, off, wt, &nt);
+        for (int x = 0; x < W2; x++)
+            for (int c = 0; c < channels; c++) {
+                float acc = 0.0f;
+                for (int t = 0; t < nt; t++) {
+                    int sy = clampi((y >> 1) + off[t], 0, h - 1);
+                    float m = wt[t] * tmp[((size_t)sy * W2 + x) * channels + c];
+                    acc = acc + m;
+                }
+                orc_from_float(acc, (unsigned char*)dst + y * dst_stride + ((long)x * channels + c) * es, type);
+            }
+    }
+    free(tmp);
+    return 0;
+}

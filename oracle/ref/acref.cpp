@@ -1,0 +1,31 @@
+// Dispatcher for oracle/_ref -- test infrastructure only.  arch numbering follows the
+// reference's x86 CPU arch list without Eigen (core/src/processor/cpu/CPUProcessor.cpp:17-105):
+// 1 = Generic, 4 = FMA, 5 = AVX512.
+extern "C" {
+int acref_pass_generic(const void*, long, int, int, int, void*, long, int, const float*, const float*);
+int acref_pass_fma(const void*, long, int, int, int, void*, long, int, const float*, const float*);
+int acref_pass_avx512(const void*, long, int, int, int, void*, long, int, const float*, const float*);
+
+int acref_has_arch(int arch)
+{
+    __builtin_cpu_init();
+    switch (arch) {
+    case 1: return 1;
+    case 4: return __builtin_cpu_supports("avx") && __builtin_cpu_supports("fma");
+    case 5: return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("fma");
+    default: return 0;
+    }
+}
+
+int acref_pass(int arch, const void* src, long sstride, int w, int h, int stype,
+               void* dst, long dstride, int dtype, const float* kernels, const float* biases)
+{
+    if (!acref_has_arch(arch)) return -95;
+    switch (arch) {
+    case 1: return acref_pass_generic(src, sstride, w, h, stype, dst, dstride, dtype, kernels, biases);
+    case 4: return acref_pass_fma(src, sstride, w, h, stype, dst, dstride, dtype, kernels, biases);
+    case 5: return acref_pass_avx512(src, sstride, w, h, stype, dst, dstride, dtype, kernels, biases);
+    default: return -22;
+    }
+}
+}

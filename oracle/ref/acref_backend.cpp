@@ -1,0 +1,118 @@
+// Reference-kernel driver for oracle/_ref -- test infrastructure only.
+//
+// Compiled once per ISA (BACKEND_HEADER / BACKEND_OP / BACKEND_SUFFIX set by
+// oracle/Makefile) directly against the reference's own headers where they
+// lie: core/internal/AC/Core/Internal/Processor/CPU/{Generic.hpp,X86/*.hpp}
+// (the per-ISA dot/conv micro-kernels) and core/internal/AC/Core/Internal/Util.hpp
+// (toFloat/fromFloat).  Those headers need nothing that is absent here.
+//
+// The per-pixel replicate-clamped loops of Common.hpp:116-197,344-393 cannot be
+// included (Common.hpp pulls in Image.hpp, which needs a CMake-generated export
+// header), so the three loops below restate them and call the reference kernels.
+#include <cstdint>
+#include <cstdlib>
+#include <vector>
+
+#include "AC/Core/Internal/Util.hpp"
+#include BACKEND_HEADER
+
+namespace {
+using Op = ac::core::cpu::BACKEND_OP;
+
+template <typename IN>
+void layer0(const void* src, long sstride, int w, int h, float* out, const float* k, const float* b)
+{
+#pragma omp parallel for schedule(guided)
+    for (int i = 0; i < h; i++) {
+        int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+        auto row = [&](int y) { return reinterpret_cast<const IN*>(static_cast<const std::uint8_t*>(src) + y * sstride); };
+        for (int j = 0; j < w; j++) {
+            int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+            alignas(64) const float r[] = {
+                ac::core::toFloat(row(i - tp)[j - lp]), ac::core::toFloat(row(i - tp)[j]), ac::core::toFloat(row(i - tp)[j + rp]),
+                ac::core::toFloat(row(i)[j - lp]),      ac::core::toFloat(row(i)[j]),      ac::core::toFloat(row(i)[j + rp]),
+                ac::core::toFloat(row(i + bp)[j - lp]), ac::core::toFloat(row(i + bp)[j]), ac::core::toFloat(row(i + bp)[j + rp]),
+            };
+            float* o = out + (static_cast<long>(i) * w + j) * 8;
+            Op::conv_cin1<8, 9>(r, o, k, b);
+            for (int n = 0; n < 8; n++) o[n] = o[n] > 0.0f ? o[n] : 0.0f;   // ReLU: std::max(v, 0.0f)
+        }
+    }
+}
+
+void layer8(const float* in, int w, int h, float* out, const float* k, const float* b)
+{
+#pragma omp parallel for schedule(guided)
+    for (int i = 0; i < h; i++) {
+        int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+        for (int j = 0; j < w; j++) {
+            int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+            auto px = [&](int y, int x) { return in + (static_cast<long>(y) * w + x) * 8; };
+            const float* rptr[] = { px(i - tp, j - lp), px(i - tp, j), px(i - tp, j + rp),
+                                    px(i, j - lp),      px(i, j),      px(i, j + rp),
+                                    px(i + bp, j - lp), px(i + bp, j), px(i + bp, j + rp) };
+            alignas(64) float sum[8]{};
+            Op::conv<8, 8, 9>(rptr, sum, k, b);
+            float* o = out + (static_cast<long>(i) * w + j) * 8;
+            for (int n = 0; n < 8; n++) o[n] = sum[n] > 0.0f ? sum[n] : 0.0f;
+        }
+    }
+}
+
+template <typename OUT>
+void last(const float* in, int w, int h, void* dst, long dstride, const float* k, const float* b, const float* k2)
+{
+#pragma omp parallel for schedule(guided)
+    for (int i = 0; i < h; i++) {
+        int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+        for (int j = 0; j < w; j++) {
+            int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+            auto px = [&](int y, int x) { return in + (static_cast<long>(y) * w + x) * 8; };
+            const float* rptr[] = { px(i - tp, j - lp), px(i - tp, j), px(i - tp, j + rp),
+                                    px(i, j - lp),      px(i, j),      px(i, j + rp),
+                                    px(i + bp, j - lp), px(i + bp, j), px(i + bp, j + rp) };
+            alignas(64) float sum[8]{};
+            Op::conv<8, 8, 9>(rptr, sum, k, b);
+            for (int n = 0; n < 8; n++) sum[n] = sum[n] > 0.0f ? sum[n] : 0.0f;
+            for (int dy = 0; dy < 2; dy++)
+                for (int dx = 0; dx < 2; dx++) {
+                    auto o = reinterpret_cast<OUT*>(static_cast<std::uint8_t*>(dst) + (2 * i + dy) * dstride) + 2 * j + dx;
+                    *o = ac::core::fromFloat<OUT>(Op::dot<8>(sum, k2 + 8 * (dy * 2 + dx)));
+                }
+        }
+    }
+}
+} // namespace
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+
+// src_type/dst_type: 0x001 u8, 0x002 u16, 0x204 f32 (core/include/AC/Core/Image.hpp:367-371)
+extern "C" int CAT(acref_pass_, BACKEND_SUFFIX)(const void* src, long sstride, int w, int h, int stype,
+                                                void* dst, long dstride, int dtype,
+                                                const float* kernels, const float* biases)
+{
+    std::vector<float> t1(static_cast<size_t>(w) * h * 8), t2(static_cast<size_t>(w) * h * 8);
+    switch (stype) {   // CPUProcessor.cpp:1384: conv3x3_1to8_relu(src, tmpO, kernel(0), bias(0))
+    case 0x001: layer0<std::uint8_t>(src, sstride, w, h, t1.data(), kernels, biases); break;
+    case 0x002: layer0<std::uint16_t>(src, sstride, w, h, t1.data(), kernels, biases); break;
+    case 0x204: layer0<float>(src, sstride, w, h, t1.data(), kernels, biases); break;
+    default: return -22;
+    }
+    float* ti = t1.data();
+    float* to = t2.data();
+    for (int l = 1; l < 8; l++) {   // :1386-1390 blocks()-1 middle layers
+        layer8(ti, w, h, to, kernels + 72 + 576 * (l - 1), biases + 8 * l);
+        std::swap(ti, to);
+    }
+    const float* k8 = kernels + 72 + 576 * 7;
+    const float* b8 = biases + 64;
+    const float* k9 = kernels + 72 + 576 * 8;
+    switch (dtype) {
+    case 0x001: last<std::uint8_t>(ti, w, h, dst, dstride, k8, b8, k9); break;
+    case 0x002: last<std::uint16_t>(ti, w, h, dst, dstride, k8, b8, k9); break;
+    case 0x204: last<float>(ti, w, h, dst, dstride, k8, b8, k9); break;
+    default: return -22;
+    }
+    return 0;
+}
