@@ -1,0 +1,108 @@
+"""ctypes binding of the native library ``lib/libac_hip.so``.
+
+Every entry point used here is declared in ``include/achip.h`` (the thin device
+C ABI) or ``include/AC/Core/*.h`` (the reference-compatible C binding).  There is
+no fallback: if the library is missing or fails to load, ``lib()`` raises.
+"""
+import ctypes
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "lib", "libac_hip.so")
+
+_lib = None
+
+c_void_p = ctypes.c_void_p
+c_int = ctypes.c_int
+c_long = ctypes.c_long
+c_double = ctypes.c_double
+c_char_p = ctypes.c_char_p
+
+
+class ACImage(ctypes.Structure):
+    """binding/c/include/AC/Core/Image.h:52-61"""
+    _fields_ = [("width", c_int), ("height", c_int), ("channels", c_int), ("stride", c_int),
+                ("element_type", c_int), ("ptr", c_void_p), ("hptr", c_void_p)]
+
+
+class ACProcessor(ctypes.Structure):
+    """binding/c/include/AC/Core/Processor.h:15-21"""
+    _fields_ = [("device", c_int), ("type", c_char_p), ("model", c_char_p), ("hptr", c_void_p)]
+
+
+# name -> (restype, argtypes); the C-ABI surface this package relies on
+SIGNATURES = {
+    # include/achip.h
+    "achip_device_count": (c_int, []),
+    "achip_net_create": (c_int, [c_int, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
+    "achip_net_destroy": (None, [c_void_p]),
+    "achip_net_device": (c_int, [c_void_p]),
+    "achip_pass": (c_int, [c_void_p, c_int, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p]),
+    "achip_pass_config": (c_int, [c_void_p, c_int, c_int, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p]),
+    "achip_config_count": (c_int, [c_int]),
+    "achip_config_info": (c_int, [c_int, c_int] + [ctypes.POINTER(c_int)] * 5),
+    "achip_last_error": (c_char_p, []),
+    # include/AC/Core/Image.h
+    "ac_image_alloc": (ctypes.POINTER(ACImage), []),
+    "ac_image_free": (None, [ctypes.POINTER(ctypes.POINTER(ACImage))]),
+    "ac_image_ref": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage)]),
+    "ac_image_unref": (None, [ctypes.POINTER(ACImage)]),
+    "ac_image_create": (c_int, [ctypes.POINTER(ACImage)]),
+    "ac_image_map": (c_int, [ctypes.POINTER(ACImage)]),
+    "ac_image_from": (c_int, [ctypes.POINTER(ACImage), c_void_p]),
+    "ac_image_view": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage), c_int, c_int, c_int, c_int]),
+    "ac_image_clone": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage)]),
+    "ac_image_to": (c_int, [ctypes.POINTER(ACImage), c_void_p, c_int]),
+    # include/AC/Core/Processor.h
+    "ac_processor_alloc": (ctypes.POINTER(ACProcessor), []),
+    "ac_processor_free": (None, [ctypes.POINTER(ctypes.POINTER(ACProcessor))]),
+    "ac_processor_ref": (c_int, [ctypes.POINTER(ACProcessor), ctypes.POINTER(ACProcessor)]),
+    "ac_processor_unref": (None, [ctypes.POINTER(ACProcessor)]),
+    "ac_processor_create": (c_int, [ctypes.POINTER(ACProcessor)]),
+    "ac_processor_process": (c_int, [ctypes.POINTER(ACProcessor), ctypes.POINTER(ACImage), ctypes.POINTER(ACImage), c_double]),
+    "ac_processor_ok": (c_int, [ctypes.POINTER(ACProcessor)]),
+    "ac_processor_error": (c_char_p, [ctypes.POINTER(ACProcessor)]),
+    "ac_processor_name": (c_char_p, [ctypes.POINTER(ACProcessor)]),
+    "ac_processor_type": (c_int, [ctypes.POINTER(ACProcessor)]),
+    "ac_processor_type_name": (c_char_p, [ctypes.POINTER(ACProcessor)]),
+    "ac_processor_info": (c_char_p, [c_int]),
+    "ac_processor_list_info": (c_char_p, []),
+}
+
+
+def lib():
+    """Load the native library once; raise if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"native library {LIB_PATH} is not built; run anime4kcpp_amd.build.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().achip_last_error().decode(errors="replace")
+
+
+class AchipError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__(f"{where} failed ({code}): {last_error()}")
+        self.code = code
+
+
+def check(code, where):
+    if code != 0:
+        raise AchipError(code, where)
+    return code
+
+
+def check_ac(code, where):
+    """C API error codes (include/AC/Error.h): 0 success, negative errno otherwise."""
+    if code != 0:
+        raise RuntimeError(f"{where} failed ({code})")
+    return code

@@ -1,0 +1,247 @@
+// Thin device-level C ABI (include/achip.h): weight upload, argument validation and
+// kernel dispatch for the ACNetLegacy HIP path.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "achip.h"
+#include "acnet_device.hpp"
+#include "acnet_kernels.hpp"
+
+struct ACHipNet {
+    int device;
+    float* wts;   // device weight blob, acnet_device.hpp layout
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(ACHIP_EIO, "%s: %s", what, hipGetErrorString(e));
+}
+
+// Restores the calling thread's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    hipError_t err = hipSuccess;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) err = hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+int elem_size(int type)
+{
+    switch (type) {
+    case achip::U8: return 1;
+    case achip::U16: return 2;
+    case achip::F16: return 2;
+    case achip::F32: return 4;
+    default: return 0;
+    }
+}
+
+// The buffer [ptr, ptr + bytes) must be device (or managed) memory of `device`, inside
+// one allocation.  A failed query leaves a sticky runtime error, which is cleared here
+// so that it is not reported by the next launch check.
+int check_device_buffer(const void* ptr, size_t bytes, int device, const char* name)
+{
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ACHIP_EINVAL, "%s: %p is not HIP device memory", name, ptr);
+    }
+    if (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged)
+        return fail(ACHIP_EINVAL, "%s: %p is not device memory (memory type %d)", name, ptr, (int)attr.type);
+    if (attr.type == hipMemoryTypeDevice && attr.device != device)
+        return fail(ACHIP_EINVAL, "%s: %p belongs to device %d, the net is on device %d", name, ptr, attr.device, device);
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    e = hipMemGetAddressRange(&base, &size, const_cast<void*>(ptr));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ACHIP_EINVAL, "%s: cannot query the allocation of %p", name, ptr);
+    }
+    const char* b = static_cast<const char*>(base);
+    const char* p = static_cast<const char*>(ptr);
+    if (p < b || static_cast<size_t>(p - b) + bytes > size)
+        return fail(ACHIP_EINVAL, "%s: image needs %zu bytes at %p, allocation holds %zu from %p",
+                    name, bytes, ptr, size - static_cast<size_t>(p - b), base);
+    return ACHIP_OK;
+}
+
+int validate(const ACHipNet* net, const void* src, long sstride, int stype, int w, int h,
+             const void* dst, long dstride, int dtype)
+{
+    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    const int ies = elem_size(stype), oes = elem_size(dtype);
+    if (!ies) return fail(ACHIP_EINVAL, "unsupported source element type 0x%x", stype);
+    if (!oes) return fail(ACHIP_EINVAL, "unsupported destination element type 0x%x", dtype);
+    if (w <= 0 || h <= 0) return fail(ACHIP_EINVAL, "empty image %dx%d", w, h);
+    if (w > (1 << 29) || h > (1 << 29)) return fail(ACHIP_EINVAL, "image %dx%d too large", w, h);
+    if (!src || !dst) return fail(ACHIP_EINVAL, "NULL image pointer");
+    if (sstride < static_cast<long>(w) * ies || sstride % ies)
+        return fail(ACHIP_EINVAL, "source stride %ld invalid for width %d x %d bytes", sstride, w, ies);
+    if (dstride < 2L * w * oes || dstride % oes)
+        return fail(ACHIP_EINVAL, "destination stride %ld invalid for width %d x %d bytes", dstride, 2 * w, oes);
+    if (reinterpret_cast<uintptr_t>(src) % ies || reinterpret_cast<uintptr_t>(dst) % oes)
+        return fail(ACHIP_EINVAL, "image pointer not aligned to its element size");
+    const size_t sbytes = static_cast<size_t>(h - 1) * sstride + static_cast<size_t>(w) * ies;
+    const size_t dbytes = static_cast<size_t>(2 * h - 1) * dstride + static_cast<size_t>(2 * w) * oes;
+    int r = check_device_buffer(src, sbytes, net->device, "src");
+    if (r) return r;
+    r = check_device_buffer(dst, dbytes, net->device, "dst");
+    if (r) return r;
+    const char* s0 = static_cast<const char*>(src);
+    const char* d0 = static_cast<const char*>(dst);
+    if (s0 < d0 + dbytes && d0 < s0 + sbytes) return fail(ACHIP_EINVAL, "src and dst overlap");
+    return ACHIP_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int achip_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int achip_net_create(int device, const float* kernels, const float* biases, ACHipNet** out)
+{
+    if (!out) return fail(ACHIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!kernels || !biases) return fail(ACHIP_EINVAL, "NULL weights");
+    const int n = achip_device_count();
+    if (n <= 0) return fail(ACHIP_ENODEV, "no HIP device");
+    if (device < 0 || device >= n) {
+        // best device by compute units x clock, as the reference picks its GPU when the
+        // index is out of range (CUDAProcessor.cpp:233-251)
+        long best = -1;
+        for (int i = 0; i < n; i++) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, i) != hipSuccess) continue;
+            long score = static_cast<long>(prop.multiProcessorCount) * prop.clockRate;
+            if (score > best) { best = score; device = i; }
+        }
+        if (device < 0 || device >= n) device = 0;
+    }
+    // repack the reference layout into the kernels' layout
+    std::vector<float> blob(achip::kBlobFloats);
+    for (int p = 0; p < 9; p++)
+        for (int o = 0; o < 8; o++) blob[achip::kOffW0 + p * 8 + o] = kernels[o * 9 + p];
+    for (int o = 0; o < 8; o++) blob[achip::kOffB0 + o] = biases[o];
+    for (int l = 1; l <= 8; l++) {
+        const float* k = kernels + 72 + 576 * (l - 1);
+        for (int p = 0; p < 9; p++)
+            for (int c = 0; c < 8; c++)
+                for (int o = 0; o < 8; o++) blob[achip::offW(l) + (p * 8 + c) * 8 + o] = k[o * 72 + p * 8 + c];
+        for (int o = 0; o < 8; o++) blob[achip::offB(l) + o] = biases[8 * l + o];
+    }
+    const float* k2 = kernels + 72 + 576 * 8;
+    for (int c = 0; c < 8; c++)
+        for (int q = 0; q < 4; q++) blob[achip::kOffK2 + c * 4 + q] = k2[q * 8 + c];
+
+    DeviceGuard g(device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    float* d = nullptr;
+    hipError_t e = hipMalloc(&d, blob.size() * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(weights)");
+    e = hipMemcpy(d, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "hipMemcpy(weights)");
+    }
+    *out = new ACHipNet{ device, d };
+    g_error.clear();
+    return ACHIP_OK;
+}
+
+void achip_net_destroy(ACHipNet* net)
+{
+    if (!net) return;
+    {
+        DeviceGuard g(net->device);
+        if (net->wts) (void)hipFree(net->wts);
+    }
+    delete net;
+}
+
+int achip_net_device(const ACHipNet* net) { return net ? net->device : ACHIP_EINVAL; }
+
+int achip_config_count(int precision)
+{
+    switch (precision) {
+    case ACHIP_FP32: return achip::kFp32ConfigCount;
+    default: return 0;
+    }
+}
+
+int achip_config_info(int precision, int i, int* tw, int* th, int* nt, int* p, int* lds_bytes)
+{
+    if (i < 0 || i >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d for precision %d", i, precision);
+    const achip::Fp32Config& c = achip::kFp32Configs[i];
+    if (tw) *tw = c.tw;
+    if (th) *th = c.th;
+    if (nt) *nt = c.nt;
+    if (p) *p = c.p;
+    if (lds_bytes) *lds_bytes = c.lds_bytes;
+    return ACHIP_OK;
+}
+
+int achip_pass_config(ACHipNet* net, int precision, int config,
+                      const void* src, long src_stride, int src_type, int w, int h,
+                      void* dst, long dst_stride, int dst_type, void* stream)
+{
+    if (precision != ACHIP_FP32) return fail(ACHIP_EINVAL, "precision %d not available", precision);
+    if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
+    int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
+    if (r) return r;
+    DeviceGuard g(net->device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    hipError_t e = achip::launch_fp32(config, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
+                                      net->wts, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "launch acnet_fp32_fused");
+    return ACHIP_OK;
+}
+
+int achip_pass(ACHipNet* net, int precision, const void* src, long src_stride, int src_type, int w, int h,
+               void* dst, long dst_stride, int dst_type, void* stream)
+{
+    return achip_pass_config(net, precision, 0, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, stream);
+}
+
+const char* achip_last_error(void) { return g_error.c_str(); }
+
+} // extern "C"

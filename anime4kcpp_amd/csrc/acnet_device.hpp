@@ -1,0 +1,63 @@
+// Device-side helpers shared by the ACNetLegacy HIP kernels (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace achip {
+
+// Element types, encoded like ac::core::Image::ElementType
+// (core/include/AC/Core/Image.hpp:367-371): kind << 8 | bytes.
+enum ElemType : int { U8 = 0x001, U16 = 0x002, F16 = 0x202, F32 = 0x204 };
+
+// Device weight blob: the reference arrays (ACNet.hpp:34-60) repacked on upload so
+// the kernels' uniform weight reads are contiguous scalar loads.
+//   L0     w[p][n]    72 floats, then bias[8]
+//   L1..L8 w[p][c][n] 576 floats, then bias[8]          (p = 3*ky + kx)
+//   L9     k2[c][q]   32 floats                          (q = 2*dy + dx)
+constexpr int kOffW0 = 0;
+constexpr int kOffB0 = 72;
+constexpr int kLayerStride = 584;
+__host__ __device__ constexpr int offW(int l) { return 80 + (l - 1) * kLayerStride; }   // l in 1..8
+__host__ __device__ constexpr int offB(int l) { return offW(l) + 576; }
+constexpr int kOffK2 = 80 + 8 * kLayerStride;     // 4752
+constexpr int kBlobFloats = kOffK2 + 32;         // 4784
+
+// toFloat (core/internal/AC/Core/Internal/Util.hpp:49-61): unsigned -> v / max, correctly
+// rounded division as the host's IEEE divide.
+template <typename T> __device__ inline float to_float(T v);
+template <> __device__ inline float to_float<uint8_t>(uint8_t v) { return __fdiv_rn(static_cast<float>(v), 255.0f); }
+template <> __device__ inline float to_float<uint16_t>(uint16_t v) { return __fdiv_rn(static_cast<float>(v), 65535.0f); }
+template <> __device__ inline float to_float<__half>(__half v) { return __half2float(v); }
+template <> __device__ inline float to_float<float>(float v) { return v; }
+
+// fromFloat (Util.hpp:63-78): saturate to [0,1]; unsigned types then (T)(s*max + 0.5f)
+// with the multiply and the add rounded separately and the conversion truncating.
+template <typename T> __device__ inline T from_float(float v);
+__device__ inline float saturate01(float v) { return v < 0.0f ? 0.0f : (v < 1.0f ? v : 1.0f); }
+template <> __device__ inline uint8_t from_float<uint8_t>(float v)
+{
+    return static_cast<uint8_t>(static_cast<unsigned>(__fadd_rn(__fmul_rn(saturate01(v), 255.0f), 0.5f)));
+}
+template <> __device__ inline uint16_t from_float<uint16_t>(float v)
+{
+    return static_cast<uint16_t>(static_cast<unsigned>(__fadd_rn(__fmul_rn(saturate01(v), 65535.0f), 0.5f)));
+}
+template <> __device__ inline __half from_float<__half>(float v) { return __float2half_rn(saturate01(v)); }
+template <> __device__ inline float from_float<float>(float v) { return saturate01(v); }
+
+template <typename T>
+__device__ inline const T* row_ptr(const void* base, long stride, int y)
+{
+    return reinterpret_cast<const T*>(static_cast<const unsigned char*>(base) + static_cast<long>(y) * stride);
+}
+template <typename T>
+__device__ inline T* row_ptr(void* base, long stride, int y)
+{
+    return reinterpret_cast<T*>(static_cast<unsigned char*>(base) + static_cast<long>(y) * stride);
+}
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+} // namespace achip

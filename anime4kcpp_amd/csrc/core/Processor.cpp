@@ -1,0 +1,261 @@
+// ac::core::Processor: type/model string parsing, factory, and the process(src, dst,
+// factor) orchestration around the 2x passes (core/src/processor/Processor.cpp:12-333).
+//
+// This library builds one processor type, HIP (MI355X, ACNetLegacy).  The reference's
+// CPU, OpenCL and CUDA types and its other model families are not part of it: asking for
+// them yields a processor whose ok() is false and whose error() says so -- never a
+// silent substitute.
+#include <cctype>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "AC/Core/Model.hpp"
+#include "AC/Core/Processor.hpp"
+#include "core_internal.hpp"
+
+namespace ac::core::detail
+{
+    // Processor.cpp:12-25, plus "hip": unknown strings mean CPU, "auto" is -1.
+    int findProcessorType(const char* type) noexcept
+    {
+        if (type)
+        {
+            std::string s = type;
+            for (char& ch : s) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+            if (s == "auto") return -1;
+            if (s == "opencl") return Processor::OpenCL;
+            if (s == "cuda") return Processor::CUDA;
+            if (s.rfind("hip", 0) == 0) return Processor::HIP;
+        }
+        return Processor::CPU;
+    }
+
+    // Arithmetic of the HIP path's 8->8 layers from the type string: "hip" (FP32),
+    // "hip-fp16" / "hip:fp16", "hip-bf16" / "hip:bf16".
+    int findPrecision(const char* type) noexcept
+    {
+        if (!type) return 0;
+        std::string s = type;
+        for (char& ch : s) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+        if (s.find("fp16") != std::string::npos || s.find("f16") != std::string::npos) return 1;
+        if (s.find("bf16") != std::string::npos) return 2;
+        return 0;
+    }
+
+    // Processor.cpp:26-187 restricted to what this library implements: ACNetLegacy
+    // ("acnet-legacy-{gan,hdn0..3}"; "hdn" without a digit is HDN0, the first digit
+    // 0-3 after it selects the variant; no model string means GAN).  Other families
+    // report their name in `family`.
+    bool findModel(const char* model, model::ACNetLegacy::Variant& variant, std::string& family) noexcept
+    {
+        variant = model::ACNetLegacy::Variant::GAN;
+        if (!model) return true;
+        std::string s = model;
+        for (char& ch : s) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+        if (s.find("fsrcnnx") != std::string::npos) { family = "FSRCNNX"; return false; }
+        if (s.find("artcnn") != std::string::npos) { family = "ArtCNN"; return false; }
+        if (s.find("arnet") != std::string::npos) { family = "ARNet"; return false; }
+        if (s.find("acnet") != std::string::npos)
+        {
+            if (s.find("legacy") == std::string::npos) { family = "ACNet"; return false; }
+            if (s.find("hdn") != std::string::npos)
+            {
+                variant = model::ACNetLegacy::Variant::HDN0;
+                for (char ch : s)
+                {
+                    if (ch == '0') variant = model::ACNetLegacy::Variant::HDN0;
+                    else if (ch == '1') variant = model::ACNetLegacy::Variant::HDN1;
+                    else if (ch == '2') variant = model::ACNetLegacy::Variant::HDN2;
+                    else if (ch == '3') variant = model::ACNetLegacy::Variant::HDN3;
+                    else continue;
+                    break;
+                }
+            }
+        }
+        return true;
+    }
+
+    // ceil(log2(v)) from the IEEE exponent and mantissa (core/internal/.../Util.hpp:80-89)
+    int ceilLog2(const double v) noexcept
+    {
+        std::uint64_t bits = 0;
+        std::memcpy(&bits, &v, sizeof(bits));
+        return static_cast<int>((((bits >> 52) & 0x7ff) - 1023) + ((bits << 12) != 0));
+    }
+
+    const char* typeNameOf(int type) noexcept
+    {
+        switch (type)
+        {
+        case Processor::CPU: return "CPU";
+        case Processor::OpenCL: return "OpenCL";
+        case Processor::CUDA: return "CUDA";
+        case Processor::HIP: return "HIP";
+        default: return "unknown";
+        }
+    }
+
+    // A processor this library cannot provide: ok() is false everywhere, process() does
+    // nothing.
+    class UnavailableProcessor final : public Processor
+    {
+    public:
+        UnavailableProcessor(const int type, std::string message) noexcept : t(type), msg(std::move(message)) {}
+        bool ok() noexcept override { return false; }
+        const char* error() noexcept override { return msg.c_str(); }
+        const char* name() const noexcept override { return "unavailable"; }
+        int type() const noexcept override { return t; }
+        const char* typeName() const noexcept override { return typeNameOf(t); }
+
+    private:
+        void process(const Image&, Image&) override {}
+        bool processChain(const Image&, Image&, int) override { return false; }
+
+    private:
+        int t;
+        std::string msg;
+    };
+}
+
+ac::core::Processor::Processor() noexcept : idx(0) {}
+ac::core::Processor::~Processor() = default;
+
+ac::core::Image ac::core::Processor::process(const Image& src, const double factor)
+{
+    Image dst{};
+    process(src, dst, factor);
+    return dst;
+}
+
+// Processor.cpp:199-276.  power = factor > 2 ? ceil(log2(factor)) : 1 passes, then a resize
+// by factor / 2^power; multi-channel input is split into Y (through the passes) and UV
+// (resized by the full factor) and merged back.
+void ac::core::Processor::process(const Image& src, Image& dst, const double factor)
+{
+    if (src.empty()) return;
+    const int power = factor > 2.0 ? detail::ceilLog2(factor) : 1;
+    const double fxy = factor / static_cast<double>(1 << power);
+
+    Image y = src, uv{};
+    if (src.channels() > 1 && !colorSplit(src, y, uv)) return;
+
+    if (!dst.empty())
+    {
+        if (src.channels() == 1)
+        {
+            if (fxy == 1.0) processChain(y, dst, power);
+            else
+            {
+                Image out{};
+                if (processChain(y, out, power)) resizeTo(out, dst, 0.0, 0.0);
+            }
+        }
+        else
+        {
+            Image out{};
+            if (!processChain(y, out, power)) return;
+            if (fxy != 1.0 && !resizeTo(out, out, fxy, fxy)) return;
+            Image uvs{};
+            if (!resizeTo(uv, uvs, factor, factor)) return;
+            colorMerge(out, uvs, dst);
+        }
+    }
+    else
+    {
+        Image out{};
+        if (!processChain(y, out, power)) return;
+        if (!resizeTo(out, dst, fxy, fxy)) return;
+        if (src.channels() > 1)
+        {
+            Image uvs{}, rgb{};
+            if (!resizeTo(uv, uvs, factor, factor)) return;
+            if (!colorMerge(dst, uvs, rgb)) return;
+            dst = rgb;
+        }
+    }
+}
+
+// The reference's pass loop: intermediate images keep the input's element type; the last
+// pass writes into dst (allocated with the input's type when empty).
+bool ac::core::Processor::processChain(const Image& src, Image& dst, const int passes)
+{
+    Image in{}, out{ src };
+    for (int i = 0; i < passes - 1; i++)
+    {
+        in = out;
+        out = Image{};
+        out.create(in.width() * 2, in.height() * 2, 1, in.type());
+        process(in, out);
+        if (!ok()) return false;
+    }
+    if (dst.empty()) dst.create(out.width() * 2, out.height() * 2, 1, out.type());
+    process(out, dst);
+    return ok();
+}
+
+bool ac::core::Processor::colorSplit(const Image&, Image&, Image&)
+{
+    setError("multi-channel input is not supported by this processor");
+    return false;
+}
+
+bool ac::core::Processor::colorMerge(const Image&, const Image&, Image&)
+{
+    setError("multi-channel output is not supported by this processor");
+    return false;
+}
+
+// ImageResize.cpp:136-163 semantics for the identity cases (dst = src, no copy); other
+// sizes need a processor that implements resizing.
+bool ac::core::Processor::resizeTo(const Image& src, Image& dst, const double fx, const double fy)
+{
+    if (src.empty()) return false;
+    if ((fx > 0.0 && fy > 0.0 && fx == 1.0 && fy == 1.0) ||
+        (!(fx > 0.0 && fy > 0.0) && !dst.empty() && dst.width() == src.width() && dst.height() == src.height()))
+    {
+        dst = src;
+        return true;
+    }
+    setError("non power-of-two factors need a resize, which this processor does not implement");
+    return false;
+}
+
+void ac::core::Processor::setError(const char*) noexcept {}
+
+bool ac::core::Processor::ok() noexcept
+{
+    return true;
+}
+const char* ac::core::Processor::error() noexcept
+{
+    return "NO ERROR";
+}
+
+// Processor.cpp:286-317: "auto" tries the GPU first.  Here it is HIP and only HIP.
+std::shared_ptr<ac::core::Processor> ac::core::Processor::create(const char* const type, const int device, const char* const model)
+{
+    model::ACNetLegacy::Variant variant{};
+    std::string family;
+    int t = detail::findProcessorType(type);
+    if (t == -1) t = Processor::HIP;
+    if (t != Processor::HIP)
+        return std::make_shared<detail::UnavailableProcessor>(t, std::string("processor type '") + detail::typeNameOf(t) +
+            "' is not part of this library, which provides the HIP processor only");
+    if (!detail::findModel(model, variant, family))
+        return std::make_shared<detail::UnavailableProcessor>(t, "model family " + family +
+            " is not part of this library, which provides ACNetLegacy only");
+    return detail::createHIP(device, model::ACNetLegacy{ variant }, detail::findPrecision(type));
+}
+
+template<>
+AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::ACNetLegacy>(const int idx, const model::ACNetLegacy& model)
+{
+    return detail::createHIP(idx, model, 0);
+}
+
+const char* ac::core::Processor::listInfo()
+{
+    return info<Processor::HIP>();
+}

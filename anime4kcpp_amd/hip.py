@@ -1,0 +1,108 @@
+"""Device-level Python mirror of the HIP ACNetLegacy path (torch tensors as device
+memory; all compute goes through the C ABI of include/achip.h)."""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _native
+from ._native import check
+
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+VARIANTS = ("gan", "hdn0", "hdn1", "hdn2", "hdn3")
+PRECISIONS = {"fp32": 0, "fp16": 1, "bf16": 2}
+
+U8, U16, F16, F32 = 0x001, 0x002, 0x202, 0x204
+
+
+def load_weights(variant):
+    """(kernels[4712], biases[72]) float32 of an ACNetLegacy variant (ACNet.hpp:34-60)."""
+    v = variant.lower()
+    if v not in VARIANTS:
+        raise ValueError(f"unknown ACNetLegacy variant {variant!r}")
+    k = np.fromfile(os.path.join(WEIGHTS_DIR, f"acnet_legacy_{v}.kernels.f32"), dtype="<f4")
+    b = np.fromfile(os.path.join(WEIGHTS_DIR, f"acnet_legacy_{v}.biases.f32"), dtype="<f4")
+    return np.ascontiguousarray(k), np.ascontiguousarray(b)
+
+
+def elem_type(dtype):
+    import torch
+    table = {torch.uint8: U8, torch.uint16: U16, torch.float16: F16, torch.float32: F32}
+    if dtype not in table:
+        raise TypeError(f"unsupported element type {dtype}")
+    return table[dtype]
+
+
+def device_count():
+    return _native.lib().achip_device_count()
+
+
+class Net:
+    """One ACNetLegacy weight set resident on one HIP device."""
+
+    def __init__(self, variant="hdn0", device=0, kernels=None, biases=None):
+        if kernels is None:
+            kernels, biases = load_weights(variant)
+        k = np.ascontiguousarray(kernels, dtype=np.float32)
+        b = np.ascontiguousarray(biases, dtype=np.float32)
+        if k.size != 4712 or b.size != 72:
+            raise ValueError("ACNetLegacy needs 4712 kernel and 72 bias floats")
+        h = ctypes.c_void_p()
+        check(_native.lib().achip_net_create(device, k.ctypes.data, b.ctypes.data, ctypes.byref(h)), "achip_net_create")
+        self._h = h
+        self.variant = variant
+        self.device = _native.lib().achip_net_device(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _native.lib().achip_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def upscale(self, src, dst=None, precision="fp32", config=None, stream=None, out_dtype=None):
+        """One 2x pass: src (H x W, 1 channel) -> dst (2H x 2W); launches on `stream`
+        (a torch.cuda.Stream or raw hipStream_t int; default: torch's current stream)."""
+        import torch
+        if src.dim() != 2 or not src.is_cuda:
+            raise ValueError("src must be a 2-D device tensor")
+        h, w = src.shape
+        if dst is None:
+            dst = torch.empty((2 * h, 2 * w), dtype=out_dtype or src.dtype, device=src.device)
+        if dst.dim() != 2 or tuple(dst.shape) != (2 * h, 2 * w):
+            raise ValueError(f"dst must be {2 * h} x {2 * w}")
+        if src.stride(1) != 1 or dst.stride(1) != 1:
+            raise ValueError("rows must be contiguous")
+        if stream is None:
+            stream = torch.cuda.current_stream(src.device).cuda_stream
+        elif hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        args = (ctypes.c_void_p(src.data_ptr()), src.stride(0) * src.element_size(), elem_type(src.dtype), w, h,
+                ctypes.c_void_p(dst.data_ptr()), dst.stride(0) * dst.element_size(), elem_type(dst.dtype),
+                ctypes.c_void_p(stream))
+        L = _native.lib()
+        if config is None:
+            check(L.achip_pass(self._h, prec, *args), "achip_pass")
+        else:
+            check(L.achip_pass_config(self._h, prec, config, *args), "achip_pass_config")
+        return dst
+
+
+def configs(precision="fp32"):
+    L = _native.lib()
+    prec = PRECISIONS[precision]
+    out = []
+    for i in range(L.achip_config_count(prec)):
+        vals = [ctypes.c_int() for _ in range(5)]
+        check(L.achip_config_info(prec, i, *[ctypes.byref(v) for v in vals]), "achip_config_info")
+        out.append(dict(zip(("tw", "th", "nt", "p", "lds_bytes"), [v.value for v in vals])))
+    return out

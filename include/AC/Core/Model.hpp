@@ -1,0 +1,6 @@
+#ifndef AC_CORE_MODEL_HPP
+#define AC_CORE_MODEL_HPP
+
+#include "AC/Core/Model/ACNet.hpp"
+
+#endif

@@ -1,0 +1,83 @@
+/*
+ * achip.h -- the thin device-level C ABI under the HIP processor.
+ *
+ * Plain pointers and sizes only.  Images are device pointers with byte strides;
+ * element types use the ac::core::Image encoding (0x001 u8, 0x002 u16, 0x202 f16,
+ * 0x204 f32).  Every call validates its arguments on the host (sizes, strides,
+ * alignment, and that each buffer is device memory of the net's device large enough
+ * for the image) before anything is launched; an invalid call returns an error code
+ * and launches nothing.  Error codes are negative errno values:
+ * -22 invalid argument, -19 no device, -12 out of memory, -5 HIP runtime error.
+ * achip_last_error() returns the calling thread's last message.
+ *
+ * It replaces, for the ACNetLegacy path, the reference's per-backend 2x pass:
+ *   CPUProcessor<ACNetLegacy>::process   core/src/processor/cpu/CPUProcessor.cpp:1371-1390
+ *   CUDAProcessor<ACNetLegacy>::process  core/src/processor/cuda/CUDAProcessor.cpp:383-422
+ * behind the private virtual Processor::process(src, dst) (core/include/AC/Core/Processor.hpp:37).
+ */
+#ifndef ACHIP_H
+#define ACHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define ACHIP_API __attribute__((visibility("default")))
+#else
+#define ACHIP_API
+#endif
+
+/* arithmetic of the 8->8 layers */
+#define ACHIP_FP32 0   /* fused FP32 VALU kernel */
+#define ACHIP_FP16 1   /* FP16 MFMA implicit GEMM, FP32 accumulate */
+#define ACHIP_BF16 2   /* BF16 MFMA implicit GEMM, FP32 accumulate */
+
+#define ACHIP_OK 0
+#define ACHIP_EIO (-5)
+#define ACHIP_ENOMEM (-12)
+#define ACHIP_ENODEV (-19)
+#define ACHIP_EINVAL (-22)
+
+typedef struct ACHipNet ACHipNet;
+
+/* Number of HIP devices; 0 when none (never an error). */
+ACHIP_API int achip_device_count(void);
+
+/* Upload one ACNetLegacy weight set (kernels: 4712 floats, biases: 72 floats, the
+ * reference layout of core/include/AC/Core/Model/ACNet.hpp:34-60) to a device.
+ * Replaces the weight upload in the reference's GPU processor constructor
+ * (core/src/processor/cuda/CUDAProcessor.cpp:291-326).  device < 0 or out of range
+ * picks the device with the most compute units.  Returns 0 and *out on success. */
+ACHIP_API int achip_net_create(int device, const float* kernels, const float* biases, ACHipNet** out);
+ACHIP_API void achip_net_destroy(ACHipNet* net);
+ACHIP_API int achip_net_device(const ACHipNet* net);
+
+/* One 2x pass of the whole network on device memory: src w x h (1 channel) ->
+ * dst 2w x 2h (1 channel), on `stream` (a hipStream_t; NULL = the null stream).
+ * Asynchronous: returns after the launch.  Output element type follows dst_type
+ * (Common.hpp:344-393 writes fromFloat<OUT>). */
+ACHIP_API int achip_pass(ACHipNet* net, int precision,
+                         const void* src, long src_stride, int src_type, int w, int h,
+                         void* dst, long dst_stride, int dst_type, void* stream);
+
+/* The same with an explicit kernel configuration (autotuning and tests). */
+ACHIP_API int achip_pass_config(ACHipNet* net, int precision, int config,
+                                const void* src, long src_stride, int src_type, int w, int h,
+                                void* dst, long dst_stride, int dst_type, void* stream);
+
+/* Kernel configurations of a precision: tile width/height (input pixels), threads per
+ * workgroup, pixels per thread run, LDS bytes.  Returns the count, or fills config i. */
+ACHIP_API int achip_config_count(int precision);
+ACHIP_API int achip_config_info(int precision, int i, int* tw, int* th, int* nt, int* p, int* lds_bytes);
+
+/* Calling thread's last error message ("" if none). */
+ACHIP_API const char* achip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

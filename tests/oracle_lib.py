@@ -1,0 +1,98 @@
+"""ctypes access to the oracle (oracle/liboracle.so) and, where it was built, to the
+reference's own CPU kernels (oracle/_ref/libacref.so).  Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libacref.so")
+
+U8, U16, F16, F32 = 0x001, 0x002, 0x202, 0x204
+NP_TYPE = {U8: np.uint8, U16: np.uint16, F16: np.float16, F32: np.float32}
+TYPE_OF = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16, np.dtype(np.float16): F16, np.dtype(np.float32): F32}
+ORDER_FMA, ORDER_GENERIC = 0, 1
+
+_orc = None
+_ref = None
+P = ctypes.c_void_p
+L = ctypes.c_long
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: make -C oracle")
+        o = ctypes.CDLL(ORACLE_SO)
+        o.orc_acnet_pass.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int]
+        o.orc_acnet_pass_f64.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int]
+        o.orc_to_float.argtypes = [P, ctypes.c_int]
+        o.orc_to_float.restype = ctypes.c_float
+        o.orc_from_float.argtypes = [ctypes.c_float, P, ctypes.c_int]
+        o.orc_rgb2yuv.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, L]
+        o.orc_yuv2rgb.argtypes = [P, L, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L]
+        o.orc_resize2x.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int]
+        _orc = o
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        r = ctypes.CDLL(REF_SO)
+        r.acref_pass.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int, P, P]
+        r.acref_has_arch.argtypes = [ctypes.c_int]
+        _ref = r
+    return _ref
+
+
+def oracle_pass(src, kernels, biases, out_dtype=None, order=ORDER_FMA, threads=0):
+    """src: 2-D numpy array (u8/u16/f16/f32) -> 2x output in out_dtype (default src dtype)."""
+    src = np.ascontiguousarray(src)
+    h, w = src.shape
+    out_dtype = np.dtype(out_dtype or src.dtype)
+    dst = np.zeros((2 * h, 2 * w), out_dtype)
+    rc = orc().orc_acnet_pass(src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype],
+                              dst.ctypes.data, dst.strides[0], TYPE_OF[out_dtype],
+                              kernels.ctypes.data, biases.ctypes.data, order, threads)
+    assert rc == 0, rc
+    return dst
+
+
+def oracle_pass_f64(src_float, kernels, biases, threads=0):
+    """src_float: toFloat'ed input as float32 -> saturated float64 2x output."""
+    s = np.ascontiguousarray(src_float, dtype=np.float32)
+    h, w = s.shape
+    dst = np.zeros((2 * h, 2 * w), np.float64)
+    rc = orc().orc_acnet_pass_f64(s.ctypes.data, w, h, dst.ctypes.data, kernels.ctypes.data, biases.ctypes.data, threads)
+    assert rc == 0, rc
+    return dst
+
+
+def ref_pass(arch, src, kernels, biases, out_dtype=None):
+    src = np.ascontiguousarray(src)
+    h, w = src.shape
+    out_dtype = np.dtype(out_dtype or src.dtype)
+    dst = np.zeros((2 * h, 2 * w), out_dtype)
+    rc = ref().acref_pass(arch, src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype],
+                          dst.ctypes.data, dst.strides[0], TYPE_OF[out_dtype], kernels.ctypes.data, biases.ctypes.data)
+    assert rc == 0, rc
+    return dst
+
+
+def to_float(src):
+    """toFloat of a whole array (Util.hpp:49-61), via the oracle's scalar routine."""
+    src = np.ascontiguousarray(src)
+    if src.dtype == np.float32:
+        return src.copy()
+    if src.dtype == np.uint8:
+        return (src.astype(np.float32) / np.float32(255.0)).astype(np.float32)
+    if src.dtype == np.uint16:
+        return (src.astype(np.float32) / np.float32(65535.0)).astype(np.float32)
+    return src.astype(np.float32)

@@ -1,0 +1,118 @@
+"""The drop-in surface on the GPU: ac_processor_* / ac_image_* (include/AC/Core/*.h)
+with processor type "hip", against the oracle (factor 2 and chained factor 4/8, whose
+intermediate keeps the input's element type as core/src/processor/Processor.cpp:219-236
+does), preallocated mapped destinations, and the reference's ProcessorTest checks
+(tests/core/src/ProcessorTest.cpp:45-104: dimensions, preallocated dst, factor 4)."""
+import numpy as np
+import pytest
+
+from anime4kcpp_amd import processor as P
+from anime4kcpp_amd import synth
+from oracle_lib import oracle_pass
+from test_gpu_fp32 import check_f32, check_int
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def proc(cuda):
+    p = P.Processor("hip", 0, "acnet-legacy-hdn0")
+    assert p.ok(), p.error()
+    return p
+
+
+def test_create_and_identity(proc):
+    assert proc.type() == 3 and proc.type_name() == "HIP"
+    assert "MI355" in proc.name() or len(proc.name()) > 0
+    assert "HIP" in P.list_info() and "[0]" in P.list_info()
+
+
+def test_auto_is_hip(cuda):
+    p = P.Processor("auto", 0, "acnet-legacy-gan")
+    assert p.ok() and p.type() == 3
+
+
+@pytest.mark.parametrize("variant", ["gan", "hdn0", "hdn1", "hdn2", "hdn3"])
+def test_factor2_u8(weights, cuda, variant):
+    p = P.Processor("hip", 0, f"acnet-legacy-{variant}")
+    src = synth.flat_shaded(21, 123, 77, dtype=np.uint8)
+    rc, out = p.process(src, 2.0)
+    assert rc == 0, p.error()
+    assert out.shape == (154, 246) and out.dtype == np.uint8
+    check_int(out, oracle_pass(src, *weights[variant]))
+
+
+def test_factor2_f32(proc, weights):
+    src = synth.flat_shaded(22, 256, 256)
+    rc, out = proc.process(src, 2.0)
+    assert rc == 0, proc.error()
+    check_f32(out, oracle_pass(src, *weights["hdn0"]))
+
+
+@pytest.mark.parametrize("factor,passes", [(4.0, 2), (3.0, 2), (8.0, 3)])
+def test_chained_factors(proc, weights, factor, passes):
+    src = synth.flat_shaded(23, 40, 30, dtype=np.uint8)
+    want = src
+    for _ in range(passes):
+        want = oracle_pass(want, *weights["hdn0"])
+    if factor == 3.0:
+        pytest.skip("factor 3 needs the final resize (not a power of two)")
+    rc, out = proc.process(src, factor)
+    assert rc == 0, proc.error()
+    assert out.shape == (30 << passes, 40 << passes)
+    check_int(out, want)
+
+
+def test_chained_f32(proc, weights):
+    src = synth.flat_shaded(24, 64, 48)
+    want = oracle_pass(oracle_pass(src, *weights["hdn0"]), *weights["hdn0"])
+    rc, out = proc.process(src, 4.0)
+    assert rc == 0, proc.error()
+    check_f32(out, want, tol=2e-5)   # two passes; the second amplifies first-pass rounding
+
+
+def test_preallocated_mapped_dst_with_stride(proc, weights):
+    """A foreign destination with its own stride, as the video/VS/AVS callers pass it."""
+    src = synth.flat_shaded(25, 50, 20, dtype=np.uint8)
+    big = np.full((40, 130), 7, np.uint8)           # stride 130 > 100
+    dst = big[:, :100]
+    rc, _ = proc.process(src, 2.0, dst=dst)
+    assert rc == 0, proc.error()
+    check_int(big[:, :100], oracle_pass(src, *weights["hdn0"]))
+    assert (big[:, 100:] == 7).all()
+
+
+def test_mixed_types_into_preallocated(proc, weights):
+    src = synth.flat_shaded(26, 33, 17, dtype=np.uint8)
+    dst = np.zeros((34, 66), np.float32)
+    rc, _ = proc.process(src, 2.0, dst=dst)
+    assert rc == 0, proc.error()
+    check_f32(dst, oracle_pass(src, *weights["hdn0"], out_dtype=np.float32))
+
+
+def test_errors_are_reported_not_hidden(proc):
+    rgb = np.zeros((8, 8, 3), np.uint8)
+    rc, _ = proc.process(rgb, 2.0)
+    assert rc == -256 and not proc.ok()
+    src = np.zeros((8, 8), np.uint8)
+    rc, _ = proc.process(src, 2.0)          # the next good call clears this thread's error
+    assert rc == 0 and proc.ok()
+
+
+def test_threads_share_one_processor(proc, weights):
+    """One processor, many threads at once (per-thread streams, buffers and errors)."""
+    import threading
+    srcs = [synth.flat_shaded(100 + i, 60 + i, 40 + 2 * i, dtype=np.uint8) for i in range(8)]
+    outs = [None] * 8
+
+    def work(i):
+        for _ in range(3):
+            outs[i] = proc.process(srcs[i], 2.0)
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(8)]
+    [t.start() for t in ts]
+    [t.join() for t in ts]
+    for i in range(8):
+        rc, out = outs[i]
+        assert rc == 0
+        check_int(out, oracle_pass(srcs[i], *weights["hdn0"]))
