@@ -25,6 +25,12 @@ class ACImage(ctypes.Structure):
                 ("element_type", c_int), ("ptr", c_void_p), ("hptr", c_void_p)]
 
 
+class ACHipImage(ctypes.Structure):
+    """include/achip.h ACHipImage"""
+    _fields_ = [("src", c_void_p), ("src_stride", c_long), ("dst", c_void_p), ("dst_stride", c_long),
+                ("w", c_int), ("h", c_int)]
+
+
 class ACProcessor(ctypes.Structure):
     """binding/c/include/AC/Core/Processor.h:15-21"""
     _fields_ = [("device", c_int), ("type", c_char_p), ("model", c_char_p), ("hptr", c_void_p)]
@@ -42,6 +48,8 @@ SIGNATURES = {
     "achip_config_count": (c_int, [c_int]),
     "achip_config_info": (c_int, [c_int, c_int] + [ctypes.POINTER(c_int)] * 5),
     "achip_last_error": (c_char_p, []),
+    "achip_batch_workspace_bytes": (ctypes.c_size_t, [c_int]),
+    "achip_pass_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p]),
     # include/AC/Core/Image.h
     "ac_image_alloc": (ctypes.POINTER(ACImage), []),
     "ac_image_free": (None, [ctypes.POINTER(ctypes.POINTER(ACImage))]),

@@ -2,6 +2,10 @@
 // kernel dispatch for the ACNetLegacy HIP path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -14,7 +18,9 @@
 
 struct ACHipNet {
     int device;
-    float* wts;   // device weight blob, acnet_device.hpp layout
+    float* wts;      // device FP32 weight blob, acnet_device.hpp layout
+    void* wa16;      // MFMA A operands, FP16
+    void* wabf16;    // MFMA A operands, BF16
 };
 
 namespace {
@@ -53,6 +59,50 @@ struct DeviceGuard {
         if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
     }
 };
+
+// IEEE binary32 -> binary16 / bfloat16 bits, round to nearest even (finite inputs)
+uint16_t f32_to_f16_bits(float f)
+{
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+    if (ax >= 0x477ff000u) return static_cast<uint16_t>(sign | 0x7c00u);
+    if (ax < 0x38800000u) {
+        float a;
+        std::memcpy(&a, &ax, 4);
+        return static_cast<uint16_t>(sign | static_cast<uint32_t>(std::nearbyint(a * 16777216.0f)));
+    }
+    uint32_t hv = (((ax >> 23) - 112u) << 10) | ((ax & 0x7fffffu) >> 13);
+    const uint32_t rem = ax & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (hv & 1u))) hv++;
+    return static_cast<uint16_t>(sign | hv);
+}
+uint16_t f32_to_bf16_bits(float f)
+{
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    x += 0x7fffu + ((x >> 16) & 1u);
+    return static_cast<uint16_t>(x >> 16);
+}
+
+// Per-lane MFMA A operands (acnet_device.hpp) for one 16-bit type.
+std::vector<uint16_t> mfma_operands(const float* kernels, bool bf16)
+{
+    std::vector<uint16_t> a(static_cast<size_t>(8) * achip::kMfmaLayerHalves);
+    for (int l = 1; l <= 8; l++) {
+        const float* k = kernels + 72 + 576 * (l - 1);
+        for (int t = 0; t < 3; t++)
+            for (int lane = 0; lane < 64; lane++) {
+                const int i = lane % 16, g = lane / 16;
+                const int n = i % 8, ro = i / 8, ky = g - ro, kx = t;
+                for (int c = 0; c < 8; c++) {
+                    const float w = (ky >= 0 && ky <= 2) ? k[n * 72 + (ky * 3 + kx) * 8 + c] : 0.0f;
+                    a[(static_cast<size_t>(l - 1) * 3 * 64 + t * 64 + lane) * 8 + c] = bf16 ? f32_to_bf16_bits(w) : f32_to_f16_bits(w);
+                }
+            }
+    }
+    return a;
+}
 
 int elem_size(int type)
 {
@@ -124,7 +174,36 @@ int validate(const ACHipNet* net, const void* src, long sstride, int stype, int 
     return ACHIP_OK;
 }
 
+constexpr size_t kBatchHeader = 256;   // counter, padded
+
 } // namespace
+
+namespace achip {
+
+int persistent_grid(const void* kernel, int threads, int tiles)
+{
+    static std::mutex mtx;
+    static std::map<std::pair<const void*, int>, int> cache;   // (kernel, device) -> workgroups
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -1;
+    int per_device = 0;
+    {
+        std::lock_guard<std::mutex> lk(mtx);
+        auto it = cache.find({ kernel, dev });
+        if (it != cache.end()) per_device = it->second;
+    }
+    if (!per_device) {
+        int blocks = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, threads, 0) != hipSuccess) return -1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+        per_device = std::max(1, blocks) * cus;
+        std::lock_guard<std::mutex> lk(mtx);
+        cache[{ kernel, dev }] = per_device;
+    }
+    return std::min(per_device, std::max(tiles, 1));
+}
+
+} // namespace achip
 
 extern "C" {
 
@@ -173,17 +252,26 @@ int achip_net_create(int device, const float* kernels, const float* biases, ACHi
     for (int c = 0; c < 8; c++)
         for (int q = 0; q < 4; q++) blob[achip::kOffK2 + c * 4 + q] = k2[q * 8 + c];
 
+    const std::vector<uint16_t> a16 = mfma_operands(kernels, false);
+    const std::vector<uint16_t> abf = mfma_operands(kernels, true);
+
     DeviceGuard g(device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
-    float* d = nullptr;
-    hipError_t e = hipMalloc(&d, blob.size() * sizeof(float));
+    // one allocation: FP32 blob, then the two A-operand blobs, each 256-byte aligned
+    const size_t off16 = (blob.size() * sizeof(float) + 255) / 256 * 256;
+    const size_t offbf = off16 + achip::kMfmaBlobBytes;
+    const size_t total = offbf + achip::kMfmaBlobBytes;
+    char* d = nullptr;
+    hipError_t e = hipMalloc(&d, total);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(weights)");
     e = hipMemcpy(d, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + off16, a16.data(), achip::kMfmaBlobBytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + offbf, abf.data(), achip::kMfmaBlobBytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(d);
         return hip_fail(e, "hipMemcpy(weights)");
     }
-    *out = new ACHipNet{ device, d };
+    *out = new ACHipNet{ device, reinterpret_cast<float*>(d), d + off16, d + offbf };
     g_error.clear();
     return ACHIP_OK;
 }
@@ -204,6 +292,8 @@ int achip_config_count(int precision)
 {
     switch (precision) {
     case ACHIP_FP32: return achip::kFp32ConfigCount;
+    case ACHIP_FP16:
+    case ACHIP_BF16: return achip::kMfmaConfigCount;
     default: return 0;
     }
 }
@@ -211,7 +301,7 @@ int achip_config_count(int precision)
 int achip_config_info(int precision, int i, int* tw, int* th, int* nt, int* p, int* lds_bytes)
 {
     if (i < 0 || i >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d for precision %d", i, precision);
-    const achip::Fp32Config& c = achip::kFp32Configs[i];
+    const achip::Fp32Config& c = precision == ACHIP_FP32 ? achip::kFp32Configs[i] : achip::kMfmaConfigs[i];
     if (tw) *tw = c.tw;
     if (th) *th = c.th;
     if (nt) *nt = c.nt;
@@ -224,15 +314,22 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
                       const void* src, long src_stride, int src_type, int w, int h,
                       void* dst, long dst_stride, int dst_type, void* stream)
 {
-    if (precision != ACHIP_FP32) return fail(ACHIP_EINVAL, "precision %d not available", precision);
+    if (precision != ACHIP_FP32 && precision != ACHIP_FP16 && precision != ACHIP_BF16)
+        return fail(ACHIP_EINVAL, "precision %d not available", precision);
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
     int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
     if (r) return r;
     DeviceGuard g(net->device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
-    hipError_t e = achip::launch_fp32(config, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
-                                      net->wts, static_cast<hipStream_t>(stream));
-    if (e != hipSuccess) return hip_fail(e, "launch acnet_fp32_fused");
+    hipError_t e;
+    if (precision == ACHIP_FP32)
+        e = achip::launch_fp32(config, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
+                               net->wts, static_cast<hipStream_t>(stream));
+    else
+        e = achip::launch_mfma(precision == ACHIP_BF16, config, src_type, dst_type, src, src_stride, w, h, dst,
+                               dst_stride, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16,
+                               static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return ACHIP_OK;
 }
 
@@ -240,6 +337,64 @@ int achip_pass(ACHipNet* net, int precision, const void* src, long src_stride, i
                void* dst, long dst_stride, int dst_type, void* stream)
 {
     return achip_pass_config(net, precision, 0, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, stream);
+}
+
+size_t achip_batch_workspace_bytes(int n)
+{
+    return n <= 0 ? kBatchHeader : kBatchHeader + static_cast<size_t>(n) * sizeof(achip::BatchDesc);
+}
+
+int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int dst_type,
+                     const ACHipImage* images, int n, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (precision != ACHIP_FP32 && precision != ACHIP_FP16 && precision != ACHIP_BF16)
+        return fail(ACHIP_EINVAL, "precision %d not available", precision);
+    if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
+    if (!images || n <= 0) return fail(ACHIP_EINVAL, "empty batch");
+    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    const size_t need = achip_batch_workspace_bytes(n);
+    if (!workspace || workspace_bytes < need)
+        return fail(ACHIP_EINVAL, "workspace of %zu bytes, the batch needs %zu", workspace_bytes, need);
+    if (reinterpret_cast<uintptr_t>(workspace) % 256) return fail(ACHIP_EINVAL, "workspace not 256-byte aligned");
+    int r = check_device_buffer(workspace, need, net->device, "workspace");
+    if (r) return r;
+    int tw = 0, th = 0;
+    achip_config_info(precision, config, &tw, &th, nullptr, nullptr, nullptr);
+    std::vector<char> host(need, 0);
+    auto* desc = reinterpret_cast<achip::BatchDesc*>(host.data() + kBatchHeader);
+    long total = 0;
+    for (int i = 0; i < n; i++) {
+        const ACHipImage& im = images[i];
+        r = validate(net, im.src, im.src_stride, src_type, im.w, im.h, im.dst, im.dst_stride, dst_type);
+        if (r) {
+            std::string msg = g_error;
+            return fail(r, "image %d: %s", i, msg.c_str());
+        }
+        const char* ws = static_cast<const char*>(workspace);
+        const char* d0 = static_cast<const char*>(im.dst);
+        const size_t dbytes = static_cast<size_t>(2 * im.h - 1) * im.dst_stride + static_cast<size_t>(2 * im.w) * (dst_type & 0xff);
+        if (ws < d0 + dbytes && d0 < ws + need) return fail(ACHIP_EINVAL, "image %d: dst overlaps the workspace", i);
+        const int tx = (im.w + tw - 1) / tw, ty = (im.h + th - 1) / th;
+        desc[i] = achip::BatchDesc{ im.src, im.dst, im.src_stride, im.dst_stride, im.w, im.h, tx, static_cast<int>(total) };
+        total += static_cast<long>(tx) * ty;
+        if (total > (1L << 30)) return fail(ACHIP_EINVAL, "batch too large (%ld tiles)", total);
+    }
+    DeviceGuard g(net->device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    // counter (= 0) and descriptors in one copy; from pageable memory the copy is staged
+    // before the call returns, so `host` may go out of scope
+    hipError_t e = hipMemcpyAsync(workspace, host.data(), need, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(batch descriptors)");
+    auto* ddesc = reinterpret_cast<const achip::BatchDesc*>(static_cast<char*>(workspace) + kBatchHeader);
+    int* counter = static_cast<int*>(workspace);
+    if (precision == ACHIP_FP32)
+        e = achip::launch_fp32_batch(config, src_type, dst_type, ddesc, n, static_cast<int>(total), counter, net->wts, st);
+    else
+        e = achip::launch_mfma_batch(precision == ACHIP_BF16, config, src_type, dst_type, ddesc, n, static_cast<int>(total),
+                                     counter, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16, st);
+    if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+    return ACHIP_OK;
 }
 
 const char* achip_last_error(void) { return g_error.c_str(); }

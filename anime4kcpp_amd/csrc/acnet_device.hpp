@@ -24,6 +24,13 @@ __host__ __device__ constexpr int offB(int l) { return offW(l) + 576; }
 constexpr int kOffK2 = 80 + 8 * kLayerStride;     // 4752
 constexpr int kBlobFloats = kOffK2 + 32;         // 4784
 
+// MFMA A operands for layers 1..8, one blob per 16-bit type: [layer 1..8][t 0..2][lane 0..63]
+// x 8 values.  Lane l of MFMA t holds row i = l % 16 (output channel i % 8 of output row
+// y + i / 8) and K block g = l / 16 (patch row g, patch column t, channels 0..7):
+// the weight of tap (ky = g - i / 8, kx = t) when 0 <= ky <= 2, else 0.
+constexpr int kMfmaLayerHalves = 3 * 64 * 8;
+constexpr int kMfmaBlobBytes = 8 * kMfmaLayerHalves * 2;   // 24576
+
 // toFloat (core/internal/AC/Core/Internal/Util.hpp:49-61): unsigned -> v / max, correctly
 // rounded division as the host's IEEE divide.
 template <typename T> __device__ inline float to_float(T v);

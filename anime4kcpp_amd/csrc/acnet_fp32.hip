@@ -102,19 +102,17 @@ __device__ __forceinline__ void mid_item(const float4* __restrict__ fa, const fl
         for (int n = 0; n < 8; n++) out[p][n] = fmaxf(out[p][n], 0.0f);
 }
 
+// The whole network on one tile whose top-left input pixel is (x0, y0).  fa/fb/img are the
+// workgroup's LDS arrays.  Ends without a barrier: a caller reusing the LDS for another
+// tile synchronises first.
 template <typename TIN, typename TOUT, int TW, int TH, int NT, int P>
-__global__ __launch_bounds__(NT) void acnet_fp32_fused(const void* __restrict__ src, long sstride, int W, int H,
-                                                       void* __restrict__ dst, long dstride,
-                                                       const float* __restrict__ wts, int zero)
+__device__ __forceinline__ void fp32_tile(const void* __restrict__ src, long sstride, int W, int H,
+                                          void* __restrict__ dst, long dstride,
+                                          const float* __restrict__ wts, int zero, int x0, int y0,
+                                          float4* __restrict__ fa, float4* __restrict__ fb, float* __restrict__ img)
 {
     using C = Fp32Tile<TW, TH, NT, P>;
-    __shared__ float4 fa[C::NPIX];
-    __shared__ float4 fb[C::NPIX];
-    __shared__ float img[C::IW * C::IH];
-
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TW;
-    const int y0 = blockIdx.y * TH;
     const int bx = x0 - 8, by = y0 - 8;      // buffer origin; input tile origin is (bx-1, by-1)
 
     // ---- stage the input tile: toFloat, replicate clamp ----
@@ -256,6 +254,48 @@ __global__ __launch_bounds__(NT) void acnet_fp32_fused(const void* __restrict__ 
     }
 }
 
+template <typename TIN, typename TOUT, int TW, int TH, int NT, int P>
+__global__ __launch_bounds__(NT) void acnet_fp32_fused(const void* __restrict__ src, long sstride, int W, int H,
+                                                       void* __restrict__ dst, long dstride,
+                                                       const float* __restrict__ wts, int zero)
+{
+    using C = Fp32Tile<TW, TH, NT, P>;
+    __shared__ float4 fa[C::NPIX];
+    __shared__ float4 fb[C::NPIX];
+    __shared__ float img[C::IW * C::IH];
+    fp32_tile<TIN, TOUT, TW, TH, NT, P>(src, sstride, W, H, dst, dstride, wts, zero,
+                                        blockIdx.x * TW, blockIdx.y * TH, fa, fb, img);
+}
+
+// Persistent batched form: every workgroup takes the next tile of the batch from a
+// counter in global memory until the batch is done; tiles are numbered image by image
+// (BatchDesc::first_tile is the prefix sum).  Every workgroup reaches the exit when
+// the counter passes `total`.
+template <typename TIN, typename TOUT, int TW, int TH, int NT, int P>
+__global__ __launch_bounds__(NT) void acnet_fp32_batch(const BatchDesc* __restrict__ desc, int n, int total,
+                                                       int* __restrict__ counter,
+                                                       const float* __restrict__ wts, int zero)
+{
+    using C = Fp32Tile<TW, TH, NT, P>;
+    __shared__ float4 fa[C::NPIX];
+    __shared__ float4 fb[C::NPIX];
+    __shared__ float img[C::IW * C::IH];
+    __shared__ int s_tile;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = s_tile;
+        if (t >= total) break;
+        const BatchDesc d = find_tile(desc, n, t);
+        const int local = t - d.first_tile;
+        const int ty = local / d.tiles_x;
+        const int tx = local - ty * d.tiles_x;
+        fp32_tile<TIN, TOUT, TW, TH, NT, P>(d.src, d.sstride, d.w, d.h, d.dst, d.dstride, wts, zero,
+                                            tx * TW, ty * TH, fa, fb, img);
+        __syncthreads();
+    }
+}
+
 // ---- launch table ---------------------------------------------------------------------
 
 template <typename TIN, typename TOUT, int TW, int TH, int NT, int P>
@@ -266,6 +306,39 @@ static hipError_t launch_one(const void* src, long ss, int W, int H, void* dst, 
     hipLaunchKernelGGL((acnet_fp32_fused<TIN, TOUT, TW, TH, NT, P>), grid, dim3(NT), 0, stream,
                        src, ss, W, H, dst, ds, wts, 0);
     return hipGetLastError();
+}
+
+template <typename TIN, typename TOUT, int TW, int TH, int NT, int P>
+static hipError_t launch_batch_one(const BatchDesc* desc, int n, int total, int* counter, const float* wts,
+                                   hipStream_t stream)
+{
+    auto kern = acnet_fp32_batch<TIN, TOUT, TW, TH, NT, P>;
+    const int grid = persistent_grid(reinterpret_cast<const void*>(kern), NT, total);
+    if (grid <= 0) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, stream, desc, n, total, counter, wts, 0);
+    return hipGetLastError();
+}
+
+template <int TW, int TH, int NT, int P>
+static hipError_t launch_batch_types(int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
+                                     const float* wts, hipStream_t stream)
+{
+#define ACHIP_OUT(TIN)                                                                                          \
+    switch (dtype) {                                                                                            \
+    case U8: return launch_batch_one<TIN, uint8_t, TW, TH, NT, P>(desc, n, total, counter, wts, stream);        \
+    case U16: return launch_batch_one<TIN, uint16_t, TW, TH, NT, P>(desc, n, total, counter, wts, stream);      \
+    case F16: return launch_batch_one<TIN, __half, TW, TH, NT, P>(desc, n, total, counter, wts, stream);        \
+    case F32: return launch_batch_one<TIN, float, TW, TH, NT, P>(desc, n, total, counter, wts, stream);         \
+    default: return hipErrorInvalidValue;                                                                       \
+    }
+    switch (stype) {
+    case U8: ACHIP_OUT(uint8_t)
+    case U16: ACHIP_OUT(uint16_t)
+    case F16: ACHIP_OUT(__half)
+    case F32: ACHIP_OUT(float)
+    default: return hipErrorInvalidValue;
+    }
+#undef ACHIP_OUT
 }
 
 template <int TW, int TH, int NT, int P>
@@ -295,6 +368,16 @@ const Fp32Config kFp32Configs[] = {
     { 32, 32, 256, 4, Fp32Tile<32, 32, 256, 4>::lds_bytes },
 };
 const int kFp32ConfigCount = sizeof(kFp32Configs) / sizeof(kFp32Configs[0]);
+
+hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
+                             const float* wts, hipStream_t stream)
+{
+    switch (cfg) {
+    case 0: return launch_batch_types<64, 32, 512, 4>(stype, dtype, desc, n, total, counter, wts, stream);
+    case 1: return launch_batch_types<32, 32, 256, 4>(stype, dtype, desc, n, total, counter, wts, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t launch_fp32(int cfg, int stype, int dtype, const void* src, long ss, int W, int H,
                        void* dst, long ds, const float* wts, hipStream_t stream)

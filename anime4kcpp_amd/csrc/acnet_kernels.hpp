@@ -5,6 +5,31 @@
 
 namespace achip {
 
+// One image of a batched launch (device memory).  Tiles of the batch are numbered image by
+// image: image i owns tiles [first_tile, first_tile + tiles_x * tiles_y).
+struct BatchDesc {
+    const void* src;
+    void* dst;
+    long sstride, dstride;
+    int w, h;
+    int tiles_x, first_tile;
+};
+
+// The image that owns tile t (binary search over first_tile; desc sorted by it).
+__device__ inline BatchDesc find_tile(const BatchDesc* __restrict__ desc, int n, int t)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (desc[mid].first_tile <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return desc[lo];
+}
+
+// Workgroups of a persistent launch: resident workgroups per CU x CUs, at most `tiles`.
+int persistent_grid(const void* kernel, int threads, int tiles);
+
 // One tile shape of the fused FP32 kernel: TW x TH input-resolution pixels per
 // workgroup, NT threads, P-pixel vertical runs per thread.
 struct Fp32Config {
@@ -19,5 +44,18 @@ extern const int kFp32ConfigCount;
 // blob (acnet_device.hpp layout).  The caller has validated every argument.
 hipError_t launch_fp32(int cfg, int stype, int dtype, const void* src, long sstride, int W, int H,
                        void* dst, long dstride, const float* wts, hipStream_t stream);
+
+// Fused pass with layers 1..8 on MFMA.  bf16 = 0: FP16 operands, 1: BF16 operands; wa is
+// the per-lane A-operand blob of that type (acnet_device.hpp layout).
+extern const Fp32Config kMfmaConfigs[];
+extern const int kMfmaConfigCount;
+hipError_t launch_mfma(int bf16, int cfg, int stype, int dtype, const void* src, long sstride, int W, int H,
+                       void* dst, long dstride, const float* wts, const void* wa, hipStream_t stream);
+
+// Persistent batched launches: `counter` (device int, zero at launch) is the tile queue.
+hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
+                             const float* wts, hipStream_t stream);
+hipError_t launch_mfma_batch(int bf16, int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total,
+                             int* counter, const float* wts, const void* wa, hipStream_t stream);
 
 } // namespace achip

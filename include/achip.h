@@ -68,6 +68,30 @@ ACHIP_API int achip_pass_config(ACHipNet* net, int precision, int config,
                                 const void* src, long src_stride, int src_type, int w, int h,
                                 void* dst, long dst_stride, int dst_type, void* stream);
 
+/* One image of a batched pass (device pointers; 1 channel; dst is 2w x 2h). */
+typedef struct ACHipImage {
+    const void* src;
+    long src_stride;
+    void* dst;
+    long dst_stride;
+    int w;
+    int h;
+} ACHipImage;
+
+/* Device workspace a batch of n images needs (the tile-queue counter and the image
+ * descriptors). */
+ACHIP_API size_t achip_batch_workspace_bytes(int n);
+
+/* 2x passes of n images of uneven sizes (all of src_type -> dst_type) in ONE persistent
+ * launch: resident workgroups pull tiles from a queue until the batch is done.
+ * `workspace` is device memory of at least achip_batch_workspace_bytes(n) bytes, owned by
+ * the caller and not touched by any other call until this one has completed on `stream`.
+ * No counterpart in the reference, whose callers loop over images one pass at a time
+ * (tools/benchmark/src/Benchmark.cpp:58-63, cli/src/Main.cpp:112-116). */
+ACHIP_API int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int dst_type,
+                               const ACHipImage* images, int n, void* workspace, size_t workspace_bytes,
+                               void* stream);
+
 /* Kernel configurations of a precision: tile width/height (input pixels), threads per
  * workgroup, pixels per thread run, LDS bytes.  Returns the count, or fills config i. */
 ACHIP_API int achip_config_count(int precision);
