@@ -49,6 +49,7 @@ SIGNATURES = {
     "achip_config_info": (c_int, [c_int, c_int] + [ctypes.POINTER(c_int)] * 5),
     "achip_last_error": (c_char_p, []),
     "achip_batch_workspace_bytes": (ctypes.c_size_t, [c_int]),
+    "achip_autotune": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_float)]),
     "achip_pass_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p]),
     # include/AC/Core/Image.h
     "ac_image_alloc": (ctypes.POINTER(ACImage), []),

@@ -8,6 +8,8 @@
 #include <mutex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
+#include <tuple>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -21,6 +23,7 @@ struct ACHipNet {
     float* wts;      // device FP32 weight blob, acnet_device.hpp layout
     void* wa16;      // MFMA A operands, FP16
     void* wabf16;    // MFMA A operands, BF16
+    float* wa32;     // FP32 MFMA A operands
 };
 
 namespace {
@@ -104,6 +107,28 @@ std::vector<uint16_t> mfma_operands(const float* kernels, bool bf16)
     return a;
 }
 
+// Per-lane FP32 MFMA A operands (acnet_device.hpp).
+std::vector<float> mfma32_operands(const float* kernels)
+{
+    std::vector<float> a(achip::kMfma32BlobFloats, 0.0f);
+    for (int t = 0; t < 3; t++)
+        for (int lane = 0; lane < 64; lane++) {
+            const int i = lane % 16, g = lane / 16, n = i % 8, ky = g - i / 8;
+            if (ky >= 0 && ky <= 2) a[t * 64 + lane] = kernels[n * 9 + ky * 3 + t];
+        }
+    for (int l = 1; l <= 8; l++) {
+        const float* k = kernels + 72 + 576 * (l - 1);
+        float* al = a.data() + achip::kMfma32L0Floats + (l - 1) * achip::kMfma32LayerFloats;
+        for (int t = 0; t < 3; t++)
+            for (int c = 0; c < 8; c++)
+                for (int lane = 0; lane < 64; lane++) {
+                    const int i = lane % 16, g = lane / 16, n = i % 8, ky = g - i / 8;
+                    if (ky >= 0 && ky <= 2) al[(t * 8 + c) * 64 + lane] = k[n * 72 + (ky * 3 + t) * 8 + c];
+                }
+    }
+    return a;
+}
+
 int elem_size(int type)
 {
     switch (type) {
@@ -175,6 +200,22 @@ int validate(const ACHipNet* net, const void* src, long sstride, int stype, int 
 }
 
 constexpr size_t kBatchHeader = 256;   // counter, padded
+
+// autotuner cache: (device, precision, src type, dst type, log2 w, log2 h) -> config
+std::mutex g_tune_mtx;
+std::map<std::tuple<int, int, int, int, int, int>, int> g_tune;
+
+int ceil_log2i(int v)
+{
+    int l = 0;
+    while ((1 << l) < v && l < 30) l++;
+    return l;
+}
+
+std::tuple<int, int, int, int, int, int> tune_key(int device, int precision, int st, int dt, int w, int h)
+{
+    return { device, precision, st, dt, ceil_log2i(w), ceil_log2i(h) };
+}
 
 } // namespace
 
@@ -254,24 +295,27 @@ int achip_net_create(int device, const float* kernels, const float* biases, ACHi
 
     const std::vector<uint16_t> a16 = mfma_operands(kernels, false);
     const std::vector<uint16_t> abf = mfma_operands(kernels, true);
+    const std::vector<float> a32 = mfma32_operands(kernels);
 
     DeviceGuard g(device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
     // one allocation: FP32 blob, then the two A-operand blobs, each 256-byte aligned
     const size_t off16 = (blob.size() * sizeof(float) + 255) / 256 * 256;
     const size_t offbf = off16 + achip::kMfmaBlobBytes;
-    const size_t total = offbf + achip::kMfmaBlobBytes;
+    const size_t off32 = offbf + achip::kMfmaBlobBytes;
+    const size_t total = off32 + a32.size() * sizeof(float);
     char* d = nullptr;
     hipError_t e = hipMalloc(&d, total);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(weights)");
     e = hipMemcpy(d, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d + off16, a16.data(), achip::kMfmaBlobBytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d + offbf, abf.data(), achip::kMfmaBlobBytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + off32, a32.data(), a32.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(d);
         return hip_fail(e, "hipMemcpy(weights)");
     }
-    *out = new ACHipNet{ device, reinterpret_cast<float*>(d), d + off16, d + offbf };
+    *out = new ACHipNet{ device, reinterpret_cast<float*>(d), d + off16, d + offbf, reinterpret_cast<float*>(d + off32) };
     g_error.clear();
     return ACHIP_OK;
 }
@@ -291,7 +335,7 @@ int achip_net_device(const ACHipNet* net) { return net ? net->device : ACHIP_EIN
 int achip_config_count(int precision)
 {
     switch (precision) {
-    case ACHIP_FP32: return achip::kFp32ConfigCount;
+    case ACHIP_FP32: return achip::kFp32ConfigCount + 1;   // the FP32 MFMA kernel, then the VALU ones
     case ACHIP_FP16:
     case ACHIP_BF16: return achip::kMfmaConfigCount;
     default: return 0;
@@ -301,7 +345,9 @@ int achip_config_count(int precision)
 int achip_config_info(int precision, int i, int* tw, int* th, int* nt, int* p, int* lds_bytes)
 {
     if (i < 0 || i >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d for precision %d", i, precision);
-    const achip::Fp32Config& c = precision == ACHIP_FP32 ? achip::kFp32Configs[i] : achip::kMfmaConfigs[i];
+    const achip::Fp32Config mfma32{ 64, 32, 512, 2, achip::kMfma32LdsBytes };
+    const achip::Fp32Config& c = precision != ACHIP_FP32 ? achip::kMfmaConfigs[i]
+                               : (i == 0 ? mfma32 : achip::kFp32Configs[i - 1]);
     if (tw) *tw = c.tw;
     if (th) *th = c.th;
     if (nt) *nt = c.nt;
@@ -316,14 +362,31 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
 {
     if (precision != ACHIP_FP32 && precision != ACHIP_FP16 && precision != ACHIP_BF16)
         return fail(ACHIP_EINVAL, "precision %d not available", precision);
+    if (config == ACHIP_AUTO && net) {
+        config = 0;
+        bool found = false;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mtx);
+            auto it = g_tune.find(tune_key(net->device, precision, src_type, dst_type, w, h));
+            if (it != g_tune.end()) { config = it->second; found = true; }
+        }
+        const char* env = std::getenv("ACHIP_AUTOTUNE");
+        if (!found && env && env[0] == '1') {
+            int best = 0;
+            if (achip_autotune(net, precision, src_type, dst_type, w, h, &best, nullptr) == ACHIP_OK) config = best;
+        }
+    }
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
     int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
     if (r) return r;
     DeviceGuard g(net->device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
     hipError_t e;
-    if (precision == ACHIP_FP32)
-        e = achip::launch_fp32(config, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
+    if (precision == ACHIP_FP32 && config == 0)
+        e = achip::launch_mfma32(src_type, dst_type, src, src_stride, w, h, dst, dst_stride, net->wts, net->wa32,
+                                 static_cast<hipStream_t>(stream));
+    else if (precision == ACHIP_FP32)
+        e = achip::launch_fp32(config - 1, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
                                net->wts, static_cast<hipStream_t>(stream));
     else
         e = achip::launch_mfma(precision == ACHIP_BF16, config, src_type, dst_type, src, src_stride, w, h, dst,
@@ -336,7 +399,71 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
 int achip_pass(ACHipNet* net, int precision, const void* src, long src_stride, int src_type, int w, int h,
                void* dst, long dst_stride, int dst_type, void* stream)
 {
-    return achip_pass_config(net, precision, 0, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, stream);
+    return achip_pass_config(net, precision, ACHIP_AUTO, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, stream);
+}
+
+int achip_autotune(ACHipNet* net, int precision, int src_type, int dst_type, int w, int h, int* best, float* ms)
+{
+    if (!net || !best) return fail(ACHIP_EINVAL, "NULL argument");
+    const int n = achip_config_count(precision);
+    if (n <= 0) return fail(ACHIP_EINVAL, "precision %d not available", precision);
+    const int ies = elem_size(src_type), oes = elem_size(dst_type);
+    if (!ies || !oes || w <= 0 || h <= 0 || w > (1 << 15) || h > (1 << 15)) return fail(ACHIP_EINVAL, "bad tuning shape");
+    DeviceGuard g(net->device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    const long sp = static_cast<long>(w) * ies, dp = 2L * w * oes;
+    void* s = nullptr;
+    void* d = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ACHIP_OK;
+    auto cleanup = [&]() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+        if (s) (void)hipFree(s);
+        if (d) (void)hipFree(d);
+    };
+    hipError_t e = hipMalloc(&s, static_cast<size_t>(sp) * h);
+    if (e == hipSuccess) e = hipMalloc(&d, static_cast<size_t>(dp) * 2 * h);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(s, 0x40, static_cast<size_t>(sp) * h, st);
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "autotune setup");
+    }
+    float best_ms = 1e30f;
+    int best_cfg = 0;
+    for (int c = 0; c < n && rc == ACHIP_OK; c++) {
+        rc = achip_pass_config(net, precision, c, s, sp, src_type, w, h, d, dp, dst_type, st);   // warm-up
+        if (rc) break;
+        const int reps = 3;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && rc == ACHIP_OK; r++)
+            rc = achip_pass_config(net, precision, c, s, sp, src_type, w, h, d, dp, dst_type, st);
+        (void)hipEventRecord(e1, st);
+        e = hipEventSynchronize(e1);
+        if (e != hipSuccess) {
+            cleanup();
+            return hip_fail(e, "autotune run");
+        }
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        t /= reps;
+        if (ms) ms[c] = t;
+        if (t < best_ms) { best_ms = t; best_cfg = c; }
+    }
+    cleanup();
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mtx);
+        g_tune[tune_key(net->device, precision, src_type, dst_type, w, h)] = best_cfg;
+    }
+    *best = best_cfg;
+    return ACHIP_OK;
 }
 
 size_t achip_batch_workspace_bytes(int n)
@@ -388,8 +515,11 @@ int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(batch descriptors)");
     auto* ddesc = reinterpret_cast<const achip::BatchDesc*>(static_cast<char*>(workspace) + kBatchHeader);
     int* counter = static_cast<int*>(workspace);
-    if (precision == ACHIP_FP32)
-        e = achip::launch_fp32_batch(config, src_type, dst_type, ddesc, n, static_cast<int>(total), counter, net->wts, st);
+    if (precision == ACHIP_FP32 && config == 0)
+        e = achip::launch_mfma32_batch(src_type, dst_type, ddesc, n, static_cast<int>(total), counter, net->wts,
+                                       net->wa32, st);
+    else if (precision == ACHIP_FP32)
+        e = achip::launch_fp32_batch(config - 1, src_type, dst_type, ddesc, n, static_cast<int>(total), counter, net->wts, st);
     else
         e = achip::launch_mfma_batch(precision == ACHIP_BF16, config, src_type, dst_type, ddesc, n, static_cast<int>(total),
                                      counter, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16, st);

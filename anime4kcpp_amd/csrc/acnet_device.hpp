@@ -31,6 +31,14 @@ constexpr int kBlobFloats = kOffK2 + 32;         // 4784
 constexpr int kMfmaLayerHalves = 3 * 64 * 8;
 constexpr int kMfmaBlobBytes = 8 * kMfmaLayerHalves * 2;   // 24576
 
+// FP32 MFMA (16x16x4) A operands: layer 0 as [t 0..2][lane] (K = patch row g = lane/16,
+// one channel), then layers 1..8 as [layer][s = 8*t + c][lane]: lane l holds row
+// i = l % 16 (channel i % 8 of output row y + i / 8) at K = patch row g, patch column t,
+// channel c -- the weight of tap (ky = g - i / 8, kx = t) when 0 <= ky <= 2, else 0.
+constexpr int kMfma32L0Floats = 3 * 64;
+constexpr int kMfma32LayerFloats = 24 * 64;
+constexpr int kMfma32BlobFloats = kMfma32L0Floats + 8 * kMfma32LayerFloats;   // 12480
+
 // toFloat (core/internal/AC/Core/Internal/Util.hpp:49-61): unsigned -> v / max, correctly
 // rounded division as the host's IEEE divide.
 template <typename T> __device__ inline float to_float(T v);
@@ -63,6 +71,29 @@ template <typename T>
 __device__ inline T* row_ptr(void* base, long stride, int y)
 {
     return reinterpret_cast<T*>(static_cast<unsigned char*>(base) + static_cast<long>(y) * stride);
+}
+
+// Two horizontally adjacent output pixels: one store of twice the element size when the
+// pair is aligned for it (whole cache lines per wave instead of byte-interleaved partial
+// writes), else two stores.
+template <typename T> struct Bits;
+template <> struct Bits<uint8_t> { using one = uint8_t; using two = uint16_t; };
+template <> struct Bits<uint16_t> { using one = uint16_t; using two = uint32_t; };
+template <> struct Bits<__half> { using one = uint16_t; using two = uint32_t; };
+template <> struct Bits<float> { using one = uint32_t; using two = uint64_t; };
+template <typename T>
+__device__ inline void store_pair(T* p, float a, float b)
+{
+    using One = typename Bits<T>::one;
+    using Two = typename Bits<T>::two;
+    const T va = from_float<T>(a), vb = from_float<T>(b);
+    if ((reinterpret_cast<uintptr_t>(p) & (2 * sizeof(T) - 1)) == 0) {
+        *reinterpret_cast<Two*>(p) = static_cast<Two>(__builtin_bit_cast(One, va)) |
+                                     (static_cast<Two>(__builtin_bit_cast(One, vb)) << (8 * sizeof(T)));
+    } else {
+        p[0] = va;
+        p[1] = vb;
+    }
 }
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }

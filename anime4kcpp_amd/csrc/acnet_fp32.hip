@@ -243,8 +243,7 @@ __device__ __forceinline__ void fp32_tile(const void* __restrict__ src, long sst
                                     o[dx] = acc;
                                 }
                                 TOUT* orow = row_ptr<TOUT>(dst, dstride, 2 * (gy + p) + dy);
-                                orow[2 * gx] = from_float<TOUT>(o[0]);
-                                orow[2 * gx + 1] = from_float<TOUT>(o[1]);
+                                store_pair<TOUT>(orow + 2 * gx, o[0], o[1]);
                             }
                         }
                     }

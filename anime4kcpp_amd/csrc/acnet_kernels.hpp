@@ -52,6 +52,23 @@ extern const int kMfmaConfigCount;
 hipError_t launch_mfma(int bf16, int cfg, int stype, int dtype, const void* src, long sstride, int W, int H,
                        void* dst, long dstride, const float* wts, const void* wa, hipStream_t stream);
 
+// Fused pass with every conv layer on FP32 MFMA (tile 64 x 32, 512 threads); wa32 is the
+// per-lane FP32 A-operand blob (acnet_device.hpp layout).
+extern const int kMfma32LdsBytes;
+hipError_t launch_mfma32(int stype, int dtype, const void* src, long sstride, int W, int H,
+                         void* dst, long dstride, const float* wts, const float* wa32, hipStream_t stream);
+hipError_t launch_mfma32_batch(int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
+                               const float* wts, const float* wa32, hipStream_t stream);
+
+// Colour split RGB(A) -> Y + UV(A), merge back, and resize (mode 0 bilinear, 1 Catmull-Rom);
+// interleaved channels, one element type, byte strides (acnet_color.hip).
+hipError_t launch_color_split(int type, int channels, const void* src, long ss, int W, int H,
+                              void* y, long ys, void* uv, long uvs, hipStream_t stream);
+hipError_t launch_color_merge(int type, int channels, const void* y, long ys, const void* uv, long uvs, int W, int H,
+                              void* dst, long ds, hipStream_t stream);
+hipError_t launch_resize(int type, int channels, int mode, const void* src, long ss, int sw, int sh,
+                         void* dst, long ds, int dw, int dh, hipStream_t stream);
+
 // Persistent batched launches: `counter` (device int, zero at launch) is the tile queue.
 hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
                              const float* wts, hipStream_t stream);

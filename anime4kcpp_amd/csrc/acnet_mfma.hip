@@ -50,13 +50,51 @@ template <> struct Mma<__bf16> {
 template <int TW, int TH, int NT>
 struct MfmaTile {
     static constexpr int NW = NT / 64;
-    static constexpr int BW = TW + 16, BH = TH + 16;
+    static constexpr int BW = TW + 16, BH = TH + 16;                // tile + margin 8
+    // The plane is padded so that the 16-lane segments and 2-row pairs of a layer may
+    // overhang its region (they read in bounds and are discarded): columns up to
+    // 8 + 16 * ceil((TW + 14) / 16), rows up to BH.
+    static constexpr int PW = 8 + 16 * ((TW + 14 + 15) / 16) + 1;
+    static constexpr int PH = BH + 1;
     static constexpr int IW = TW + 18, IH = TH + 18;
     static constexpr int items(int m) { return ((TW + 2 * m + 15) / 16) * ((TH + 2 * m + 1) / 2); }
     static constexpr int kMaxItems = (items(7) + NW - 1) / NW;     // per wave, layers 1..7
     static constexpr int kL0Runs = 4;                               // layer-0 pixels per thread run
-    static constexpr int lds_bytes = BW * BH * 16 + IW * IH * 4;
+    static constexpr int lds_bytes = PW * PH * 16 + IW * IH * 4;
 };
+
+// Replicate ring.  After a layer of margin m has written its in-image region, copy the
+// image's edge pixels one pixel outward wherever the margin-m band reaches past the image
+// border.  The next layer then reads every tap without clamping and still sees the
+// reference's per-layer replicate border (Common.hpp:122-143).  Interior tiles need none.
+// (A row pair whose second row lies outside a cut region still reads input row y+2 for
+// it, with zero weights for row y -- but 0 x NaN is NaN.  In a tile whose band touches the
+// image border such a row may never be written, so those tiles zero the plane first;
+// interior tiles only read rows their earlier layers wrote.)
+template <int TW, int TH, int NT>
+__device__ inline void fill_ring(uint4* __restrict__ plane, int x0, int y0, int W, int H, int m)
+{
+    using C = MfmaTile<TW, TH, NT>;
+    const bool L = x0 - m <= -1, R = x0 + TW + m >= W + 1;
+    const bool T = y0 - m <= -1, B = y0 + TH + m >= H + 1;
+    if (!(L || R || T || B)) return;
+    const int bx = x0 - 8, by = y0 - 8;
+    const int gx0 = max(x0 - m, 0), gx1 = min(x0 + TW + m, W);
+    const int gy0 = max(y0 - m, 0), gy1 = min(y0 + TH + m, H);
+    const int ry0 = gy0 - (T ? 1 : 0), nrow = gy1 + (B ? 1 : 0) - ry0;
+    const int rx0 = gx0 - (L ? 1 : 0), ncol = gx1 + (R ? 1 : 0) - rx0;
+    const int nL = L ? nrow : 0, nR = R ? nrow : 0, nT = T ? ncol : 0, nB = B ? ncol : 0;
+    const int total = nL + nR + nT + nB;
+    for (int i = threadIdx.x; i < total; i += NT) {
+        int gx, gy;
+        if (i < nL) { gx = -1; gy = ry0 + i; }
+        else if (i < nL + nR) { gx = W; gy = ry0 + i - nL; }
+        else if (i < nL + nR + nT) { gx = rx0 + i - nL - nR; gy = -1; }
+        else { gx = rx0 + i - nL - nR - nT; gy = H; }
+        const int sx = clampi(gx, 0, W - 1), sy = clampi(gy, 0, H - 1);
+        plane[(gy - by) * C::PW + gx - bx] = plane[(sy - by) * C::PW + sx - bx];
+    }
+}
 
 // The whole network on one tile whose top-left input pixel is (x0, y0); plane/img are the
 // workgroup's LDS arrays.  Ends without a barrier.
@@ -71,8 +109,14 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
     using v8 = typename M::v8;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);    // wave-uniform
     const int bx = x0 - 8, by = y0 - 8;
+
+    // ---- border tiles (the margin-8 band reaches the image edge) start from a zeroed
+    // plane and keep a replicate ring (see fill_ring) ----
+    const bool border = x0 - 8 <= 0 || y0 - 8 <= 0 || x0 + TW + 8 >= W || y0 + TH + 8 >= H;
+    if (border)
+        for (int i = tid; i < C::PW * C::PH; i += NT) plane[i] = make_uint4(0u, 0u, 0u, 0u);
 
     // ---- stage the input tile: toFloat, replicate clamp ----
     for (int i = tid; i < C::IW * C::IH; i += NT) {
@@ -126,27 +170,36 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
                     q.y = M::pack2(fmaxf(acc[p][2], 0.f), fmaxf(acc[p][3], 0.f));
                     q.z = M::pack2(fmaxf(acc[p][4], 0.f), fmaxf(acc[p][5], 0.f));
                     q.w = M::pack2(fmaxf(acc[p][6], 0.f), fmaxf(acc[p][7], 0.f));
-                    plane[(ly + p) * C::BW + lx] = q;
+                    plane[(ly + p) * C::PW + lx] = q;
                 }
             }
         }
     }
     __syncthreads();
 
+    if (border) {
+        fill_ring<TW, TH, NT>(plane, x0, y0, W, H, 8);
+        __syncthreads();
+    }
+
     // ---- layers 1..8 on MFMA ----
+    // Work item = 16 pixels x 2 rows of a layer's region, items numbered row pair by row
+    // pair and dealt to the waves round-robin.  Per lane, B for patch column t is the pixel
+    // (row gy - 1 + g, column x + j + t - 1): one scalar item base plus a per-lane constant,
+    // and the three reads differ by 16 bytes.
     const int j = lane & 15;          // pixel column inside the item
     const int g = lane >> 4;          // lane group: patch row (B), D row block
     const int half = g & 1;           // channels 0-3 or 4-7 of D
     const int ro = g >> 1;            // output row y or y+1 of D
-    const float* k2 = wts + kOffK2;
+    const int lane_rd = g * C::PW + j;                         // B read, pixels
+    const int lane_wr = (ro * C::PW + j) * 2 + half;           // write-back, 8-byte units
 #pragma nounroll
-    for (int l = 1; l <= 8; l++) {
+    for (int l = 1; l <= 7; l++) {
         const int m = 8 - l;
         const int gx0 = max(x0 - m, 0), gy0 = max(y0 - m, 0);
         const int gx1 = min(x0 + TW + m, W), gy1 = min(y0 + TH + m, H);
         const int nseg = (gx1 - gx0 + 15) >> 4;
         const int nitems = nseg * ((gy1 - gy0 + 1) >> 1);
-        // A operands of this layer (per lane, arranged on the host) and this lane's 4 biases
         const uint4* al = wa + (l - 1) * 3 * 64 + lane;
         const v8 a0 = __builtin_bit_cast(v8, al[0]);
         const v8 a1 = __builtin_bit_cast(v8, al[64]);
@@ -154,74 +207,126 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
         const float* bl = wts + offB(l) + 4 * half;
         const float b0 = bl[0], b1 = bl[1], b2 = bl[2], b3 = bl[3];
 
-        uint2 res[C::kMaxItems];
+        // items in pairs: one uniform branch per pair, so the two items' LDS reads and MFMA
+        // chains share a scheduling region and overlap (a missing second item repeats the
+        // first, and is not written back)
+        uint2 res[C::kMaxItems + 1];
+        int pair = wave / nseg, seg = wave - (wave / nseg) * nseg;   // item = wave
 #pragma unroll
-        for (int ii = 0; ii < C::kMaxItems; ii++) {
-            const int item = wave + ii * C::NW;           // wave-uniform
-            if (item < nitems) {
-                const int pair = item / nseg;
-                const int seg = item - pair * nseg;
-                const int gx = gx0 + seg * 16 + j;
-                const int gy = gy0 + 2 * pair;
-                // B: pixel gx + t - 1 of input row gy - 1 + g, clamped to the image, then
-                // to the buffer (lanes past the region read in bounds and are discarded)
-                int row = clampi(gy - 1 + g, 0, H - 1) - by;
-                row = min(row, C::BH - 1);
-                const int cc = min(gx, W - 1);
-                const int c0 = min(max(cc - 1, 0) - bx, C::BW - 1);
-                const int c1 = min(cc - bx, C::BW - 1);
-                const int c2 = min(min(cc + 1, W - 1) - bx, C::BW - 1);
-                const uint4* prow = plane + row * C::BW;
-                const v8 q0 = __builtin_bit_cast(v8, prow[c0]);
-                const v8 q1 = __builtin_bit_cast(v8, prow[c1]);
-                const v8 q2 = __builtin_bit_cast(v8, prow[c2]);
-                f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
-                acc = M::mma(a0, q0, acc);
-                acc = M::mma(a1, q1, acc);
-                acc = M::mma(a2, q2, acc);
-                const float v0 = fmaxf(acc[0] + b0, 0.f), v1 = fmaxf(acc[1] + b1, 0.f);
-                const float v2 = fmaxf(acc[2] + b2, 0.f), v3 = fmaxf(acc[3] + b3, 0.f);
-                if (l < 8) {
-                    res[ii].x = M::pack2(v0, v1);
-                    res[ii].y = M::pack2(v2, v3);
-                } else {
-                    // deconv 2x2 stride 2 (8 -> 1) in FP32: this lane's 4 channels, then the
-                    // partner lane (lane ^ 16) holds the other 4
-                    float s[4];
+        for (int ii = 0; ii < C::kMaxItems; ii += 2) {
+            if (wave + ii * C::NW < nitems) {
+                int pb = pair, sb = seg + C::NW;
+                while (sb >= nseg) { sb -= nseg; pb++; }
+                const int baseA = (gy0 + 2 * pair - 1 - by) * C::PW + (gx0 + 16 * seg - 1 - bx);
+                const int baseB = (wave + (ii + 1) * C::NW < nitems)
+                                      ? (gy0 + 2 * pb - 1 - by) * C::PW + (gx0 + 16 * sb - 1 - bx) : baseA;
+                const uint4* pA = plane + baseA + lane_rd;
+                const uint4* pB = plane + baseB + lane_rd;
+                const v8 qa0 = __builtin_bit_cast(v8, pA[0]), qb0 = __builtin_bit_cast(v8, pB[0]);
+                const v8 qa1 = __builtin_bit_cast(v8, pA[1]), qb1 = __builtin_bit_cast(v8, pB[1]);
+                const v8 qa2 = __builtin_bit_cast(v8, pA[2]), qb2 = __builtin_bit_cast(v8, pB[2]);
+                f32x4 ca = { 0.f, 0.f, 0.f, 0.f }, cb = { 0.f, 0.f, 0.f, 0.f };
+                ca = M::mma(a0, qa0, ca);
+                cb = M::mma(a0, qb0, cb);
+                ca = M::mma(a1, qa1, ca);
+                cb = M::mma(a1, qb1, cb);
+                ca = M::mma(a2, qa2, ca);
+                cb = M::mma(a2, qb2, cb);
+                res[ii].x = M::pack2(fmaxf(ca[0] + b0, 0.f), fmaxf(ca[1] + b1, 0.f));
+                res[ii].y = M::pack2(fmaxf(ca[2] + b2, 0.f), fmaxf(ca[3] + b3, 0.f));
+                res[ii + 1].x = M::pack2(fmaxf(cb[0] + b0, 0.f), fmaxf(cb[1] + b1, 0.f));
+                res[ii + 1].y = M::pack2(fmaxf(cb[2] + b2, 0.f), fmaxf(cb[3] + b3, 0.f));
+            }
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const float w0 = half ? k2[16 + q] : k2[q];
-                        const float w1 = half ? k2[20 + q] : k2[4 + q];
-                        const float w2 = half ? k2[24 + q] : k2[8 + q];
-                        const float w3 = half ? k2[28 + q] : k2[12 + q];
-                        s[q] = fmaf(v3, w3, fmaf(v2, w2, fmaf(v1, w1, v0 * w0)));
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; q++) s[q] += __shfl_xor(s[q], 16);
-                    const int oy = gy + ro;
-                    if (gx < gx1 && oy < gy1) {
-                        TOUT* orow = row_ptr<TOUT>(dst, dstride, 2 * oy + half);
-                        orow[2 * gx] = from_float<TOUT>(half ? s[2] : s[0]);
-                        orow[2 * gx + 1] = from_float<TOUT>(half ? s[3] : s[1]);
-                    }
-                }
+            for (int k = 0; k < 2; k++) {
+                seg += C::NW;
+                while (seg >= nseg) { seg -= nseg; pair++; }
             }
         }
-        if (l < 8) {
-            __syncthreads();
+        __syncthreads();
+        pair = wave / nseg;
+        seg = wave - pair * nseg;
 #pragma unroll
-            for (int ii = 0; ii < C::kMaxItems; ii++) {
-                const int item = wave + ii * C::NW;
-                if (item < nitems) {
-                    const int pair = item / nseg;
-                    const int seg = item - pair * nseg;
-                    const int gx = gx0 + seg * 16 + j;
-                    const int oy = gy0 + 2 * pair + ro;
-                    if (gx < gx1 && oy < gy1)
-                        reinterpret_cast<uint2*>(plane)[2 * ((oy - by) * C::BW + (gx - bx)) + half] = res[ii];
+        for (int ii = 0; ii < C::kMaxItems; ii++) {
+            const int item = wave + ii * C::NW;
+            if (item < nitems) {
+                const int gx = gx0 + 16 * seg + j;
+                const int oy = gy0 + 2 * pair + ro;
+                if (gx < gx1 && oy < gy1) {
+                    const int base = ((gy0 + 2 * pair - by) * C::PW + (gx0 + 16 * seg - bx)) * 2;
+                    reinterpret_cast<uint2*>(plane)[base + lane_wr] = res[ii];
                 }
             }
+            seg += C::NW;
+            while (seg >= nseg) { seg -= nseg; pair++; }
+        }
+        __syncthreads();
+        if (border) {
+            fill_ring<TW, TH, NT>(plane, x0, y0, W, H, m);
             __syncthreads();
+        }
+    }
+
+    // ---- layer 8 on MFMA, ReLU, then the deconv 2x2 stride 2 (8 -> 1) in FP32 ----
+    {
+        const int gx0 = x0, gy0 = y0;
+        const int gx1 = min(x0 + TW, W), gy1 = min(y0 + TH, H);
+        const int nseg = (gx1 - gx0 + 15) >> 4;
+        const int nitems = nseg * ((gy1 - gy0 + 1) >> 1);
+        const uint4* al = wa + 7 * 3 * 64 + lane;
+        const v8 a0 = __builtin_bit_cast(v8, al[0]);
+        const v8 a1 = __builtin_bit_cast(v8, al[64]);
+        const v8 a2 = __builtin_bit_cast(v8, al[128]);
+        const float* bl = wts + offB(8) + 4 * half;
+        const float b0 = bl[0], b1 = bl[1], b2 = bl[2], b3 = bl[3];
+        // this lane's 4 channels x 4 output sub-pixels of the deconv, k2[c][q]
+        const float* k2 = wts + kOffK2 + 16 * half;
+        float kd[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) kd[c][q] = k2[c * 4 + q];
+        int pair = wave / nseg, seg = wave - (wave / nseg) * nseg;
+        for (int item = wave; item < nitems; item += 2 * C::NW) {
+            int pb = pair, sb = seg + C::NW;
+            while (sb >= nseg) { sb -= nseg; pb++; }
+            const bool hasB = item + C::NW < nitems;
+            const int baseA = (gy0 + 2 * pair - 1 - by) * C::PW + (gx0 + 16 * seg - 1 - bx);
+            const int baseB = hasB ? (gy0 + 2 * pb - 1 - by) * C::PW + (gx0 + 16 * sb - 1 - bx) : baseA;
+            const uint4* pA = plane + baseA + lane_rd;
+            const uint4* pB = plane + baseB + lane_rd;
+            const v8 qa0 = __builtin_bit_cast(v8, pA[0]), qb0 = __builtin_bit_cast(v8, pB[0]);
+            const v8 qa1 = __builtin_bit_cast(v8, pA[1]), qb1 = __builtin_bit_cast(v8, pB[1]);
+            const v8 qa2 = __builtin_bit_cast(v8, pA[2]), qb2 = __builtin_bit_cast(v8, pB[2]);
+            f32x4 acc[2] = { { 0.f, 0.f, 0.f, 0.f }, { 0.f, 0.f, 0.f, 0.f } };
+            acc[0] = M::mma(a0, qa0, acc[0]);
+            acc[1] = M::mma(a0, qb0, acc[1]);
+            acc[0] = M::mma(a1, qa1, acc[0]);
+            acc[1] = M::mma(a1, qb1, acc[1]);
+            acc[0] = M::mma(a2, qa2, acc[0]);
+            acc[1] = M::mma(a2, qb2, acc[1]);
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const float v0 = fmaxf(acc[u][0] + b0, 0.f), v1 = fmaxf(acc[u][1] + b1, 0.f);
+                const float v2 = fmaxf(acc[u][2] + b2, 0.f), v3 = fmaxf(acc[u][3] + b3, 0.f);
+                // this lane's 4 channels, then the partner lane (lane ^ 16) holds the other 4
+                float s[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) s[q] = fmaf(v3, kd[3][q], fmaf(v2, kd[2][q], fmaf(v1, kd[1][q], v0 * kd[0][q])));
+#pragma unroll
+                for (int q = 0; q < 4; q++) s[q] += __shfl_xor(s[q], 16);
+                const int gx = gx0 + 16 * (u ? sb : seg) + j;
+                const int oy = gy0 + 2 * (u ? pb : pair) + ro;
+                if ((u == 0 || hasB) && gx < gx1 && oy < gy1) {
+                    TOUT* orow = row_ptr<TOUT>(dst, dstride, 2 * oy + half);
+                    store_pair<TOUT>(orow + 2 * gx, half ? s[2] : s[0], half ? s[3] : s[1]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                seg += C::NW;
+                while (seg >= nseg) { seg -= nseg; pair++; }
+            }
         }
     }
 }
@@ -232,7 +337,7 @@ __global__ __launch_bounds__(NT) void acnet_mfma_fused(const void* __restrict__ 
                                                        const float* __restrict__ wts, const uint4* __restrict__ wa)
 {
     using C = MfmaTile<TW, TH, NT>;
-    __shared__ uint4 plane[C::BW * C::BH];        // 8 x 16-bit channels per pixel
+    __shared__ uint4 plane[C::PW * C::PH];        // 8 x 16-bit channels per pixel
     __shared__ float img[C::IW * C::IH];
     mfma_tile<T16, TIN, TOUT, TW, TH, NT>(src, sstride, W, H, dst, dstride, wts, wa,
                                           blockIdx.x * TW, blockIdx.y * TH, plane, img);
@@ -245,7 +350,7 @@ __global__ __launch_bounds__(NT) void acnet_mfma_batch(const BatchDesc* __restri
                                                        const float* __restrict__ wts, const uint4* __restrict__ wa)
 {
     using C = MfmaTile<TW, TH, NT>;
-    __shared__ uint4 plane[C::BW * C::BH];
+    __shared__ uint4 plane[C::PW * C::PH];
     __shared__ float img[C::IW * C::IH];
     __shared__ int s_tile;
     for (;;) {

@@ -5,7 +5,9 @@
 //   - every calling thread gets its own HIP stream (default priority), device buffers
 //     and error state, so one processor can serve many threads at once, as the
 //     reference's callers (CLI pools, VapourSynth fmParallel, the video filter) use it;
-//   - chained passes (factor 4, 8, ...) stay in device memory between passes; only the
+//   - the whole of process(src, dst, factor) runs on the device (processAll): colour
+//     split, the chained 2x passes (factor 4, 8, ...), the resize of Y by the
+//     non-power-of-two remainder and of UV by the full factor, and the merge.  Only the
 //     input and the final output cross PCIe.  Intermediate images keep the input's
 //     element type, as the reference's pass loop does (Processor.cpp:219-236).
 #include <hip/hip_runtime.h>
@@ -20,6 +22,7 @@
 
 #include "AC/Core/Processor.hpp"
 #include "achip.h"
+#include "acnet_kernels.hpp"
 #include "core_internal.hpp"
 
 namespace ac::core::hip
@@ -46,6 +49,7 @@ namespace ac::core::hip
         void process(const Image& src, Image& dst) override;
 
     protected:
+        bool processAll(const Image& src, Image& dst, double factor) override;
         bool processChain(const Image& src, Image& dst, int passes) override;
         void setError(const char* message) noexcept override;
 
@@ -53,13 +57,15 @@ namespace ac::core::hip
         struct ThreadState
         {
             hipStream_t stream = nullptr;
-            DeviceBuffer buf[3];
+            DeviceBuffer buf[8];
             std::string error;
         };
         ThreadState& local();
         bool fail(const std::string& what);
         bool ensure(DeviceBuffer& b, size_t bytes);
         bool run(const Image& src, Image& dst, int passes);
+        bool pipeline(const Image& src, Image& dst, double factor);
+        bool stream_ready(ThreadState& st);
 
     private:
         ACHipNet* net = nullptr;
@@ -222,6 +228,119 @@ void ac::core::hip::HIPProcessorACNetLegacy::process(const Image& src, Image& ds
 bool ac::core::hip::HIPProcessorACNetLegacy::processChain(const Image& src, Image& dst, const int passes)
 {
     return run(src, dst, passes);
+}
+
+bool ac::core::hip::HIPProcessorACNetLegacy::processAll(const Image& src, Image& dst, const double factor)
+{
+    pipeline(src, dst, factor);
+    return true;
+}
+
+bool ac::core::hip::HIPProcessorACNetLegacy::stream_ready(ThreadState& st)
+{
+    if (st.stream) return true;
+    hipError_t e = hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { st.stream = nullptr; return fail(std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+    return true;
+}
+
+// Processor.cpp:199-276 on the device.  Buffers: [0] input, [1] Y, [2] UV, [3]/[4] pass
+// ping-pong, [5] resized Y, [6] resized UV, [7] merged output.
+bool ac::core::hip::HIPProcessorACNetLegacy::pipeline(const Image& src, Image& dst, const double factor)
+{
+    if (!initError.empty()) return false;
+    auto& st = local();
+    st.error.clear();
+    const int C = src.channels();
+    if (src.empty()) return true;
+    if (C != 1 && C != 3 && C != 4) return fail("input must have 1, 3 or 4 channels");
+    if (!(factor > 0.0) || factor > 256.0) return fail("factor must be in (0, 256]");
+    const int power = factor > 2.0 ? detail::ceilLog2(factor) : 1;
+    const double fxy = factor / static_cast<double>(1 << power);
+    const int T = src.type(), es = src.elementSize();
+    const long w = src.width(), h = src.height();
+    const long pw = w << power, ph = h << power;
+    long fw, fh;
+    if (!dst.empty() && C == 1 && fxy != 1.0) { fw = dst.width(); fh = dst.height(); }     // resize(out, dst, 0, 0)
+    else if (fxy == 1.0) { fw = pw; fh = ph; }
+    else { fw = static_cast<long>(static_cast<double>(pw) * fxy); fh = static_cast<long>(static_cast<double>(ph) * fxy); }
+    if (pw > (1L << 30) || ph > (1L << 30) || fw <= 0 || fh <= 0 || fw > (1L << 30) || fh > (1L << 30))
+        return fail("output size out of range");
+    if (!dst.empty()) {
+        if (dst.width() != fw || dst.height() != fh || dst.channels() != C)
+            return fail("the preallocated destination has the wrong size or channel count");
+        if ((C > 1 || fxy != 1.0) && dst.type() != T)
+            return fail("a preallocated destination of another element type needs a 1-channel input and a power-of-two factor");
+    }
+    const int outT = dst.empty() ? T : dst.type();
+    CurrentDevice g(idx);
+    if (!stream_ready(st)) return false;
+    const hipStream_t s = st.stream;
+    auto pitch_of = [](long width, int bytes) { return static_cast<long>(alignUp(static_cast<size_t>(width) * bytes, 256)); };
+    auto hipok = [&](hipError_t e, const char* what) { return e == hipSuccess ? true : fail(std::string(what) + ": " + hipGetErrorString(e)); };
+
+    // upload
+    const long inPitch = pitch_of(w * C, es);
+    if (!ensure(st.buf[0], inPitch * h)) return false;
+    if (!hipok(hipMemcpy2DAsync(st.buf[0].ptr, inPitch, src.ptr(), src.stride(), w * C * es, h, hipMemcpyHostToDevice, s), "hipMemcpy2DAsync(H2D)"))
+        return false;
+    // colour split
+    void* y = st.buf[0].ptr;
+    long yPitch = inPitch;
+    const long uvPitch = pitch_of(w * (C - 1), es);
+    if (C > 1) {
+        yPitch = pitch_of(w, es);
+        if (!ensure(st.buf[1], yPitch * h) || !ensure(st.buf[2], uvPitch * h)) return false;
+        if (!hipok(achip::launch_color_split(T, C, st.buf[0].ptr, inPitch, static_cast<int>(w), static_cast<int>(h),
+                                             st.buf[1].ptr, yPitch, st.buf[2].ptr, uvPitch, s), "colour split")) return false;
+        y = st.buf[1].ptr;
+    }
+    // 2x passes on Y; the last writes the output type when nothing follows it
+    long cw = w, ch = h;
+    for (int i = 0; i < power; i++) {
+        const bool last = i == power - 1;
+        const int ot = (last && C == 1 && fxy == 1.0) ? outT : T;
+        const long op = pitch_of(2 * cw, ot & 0xff);
+        DeviceBuffer& ob = st.buf[3 + (i & 1)];
+        if (!ensure(ob, op * 2 * ch)) return false;
+        const int rc = achip_pass(net, precision, y, yPitch, T, static_cast<int>(cw), static_cast<int>(ch), ob.ptr, op, ot, s);
+        if (rc != ACHIP_OK) return fail(std::string("achip_pass: ") + achip_last_error());
+        y = ob.ptr;
+        yPitch = op;
+        cw *= 2;
+        ch *= 2;
+    }
+    int yT = (C == 1 && fxy == 1.0) ? outT : T;
+    // the remainder of a non-power-of-two factor (Catmull-Rom, as ImageResize.cpp defaults to)
+    if (fxy != 1.0) {
+        const long rp = pitch_of(fw, es);
+        if (!ensure(st.buf[5], rp * fh)) return false;
+        if (!hipok(achip::launch_resize(T, 1, 1, y, yPitch, static_cast<int>(cw), static_cast<int>(ch), st.buf[5].ptr, rp,
+                                        static_cast<int>(fw), static_cast<int>(fh), s), "resize")) return false;
+        y = st.buf[5].ptr;
+        yPitch = rp;
+    }
+    void* out = y;
+    long outPitch = yPitch;
+    if (C > 1) {
+        // UV by the full factor, then the merge
+        const long rp = pitch_of(fw * (C - 1), es);
+        const long mp = pitch_of(fw * C, es);
+        if (!ensure(st.buf[6], rp * fh) || !ensure(st.buf[7], mp * fh)) return false;
+        if (!hipok(achip::launch_resize(T, C - 1, 1, st.buf[2].ptr, uvPitch, static_cast<int>(w), static_cast<int>(h),
+                                        st.buf[6].ptr, rp, static_cast<int>(fw), static_cast<int>(fh), s), "resize")) return false;
+        if (!hipok(achip::launch_color_merge(T, C, y, yPitch, st.buf[6].ptr, rp, static_cast<int>(fw), static_cast<int>(fh),
+                                             st.buf[7].ptr, mp, s), "colour merge")) return false;
+        out = st.buf[7].ptr;
+        outPitch = mp;
+        yT = T;
+    }
+    // download
+    if (dst.empty()) dst.create(static_cast<int>(fw), static_cast<int>(fh), C, yT);
+    if (dst.empty()) return fail("cannot allocate the destination image");
+    if (!hipok(hipMemcpy2DAsync(dst.ptr(), dst.stride(), out, outPitch, fw * C * dst.elementSize(), fh, hipMemcpyDeviceToHost, s),
+               "hipMemcpy2DAsync(D2H)")) return false;
+    return hipok(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
 std::shared_ptr<ac::core::Processor> ac::core::detail::createHIP(const int device, const model::ACNetLegacy& model, const int precision)

@@ -112,6 +112,7 @@ namespace ac::core::detail
     private:
         void process(const Image&, Image&) override {}
         bool processChain(const Image&, Image&, int) override { return false; }
+        bool processAll(const Image&, Image&, double) override { return true; }
 
     private:
         int t;
@@ -135,6 +136,7 @@ ac::core::Image ac::core::Processor::process(const Image& src, const double fact
 void ac::core::Processor::process(const Image& src, Image& dst, const double factor)
 {
     if (src.empty()) return;
+    if (processAll(src, dst, factor)) return;
     const int power = factor > 2.0 ? detail::ceilLog2(factor) : 1;
     const double fxy = factor / static_cast<double>(1 << power);
 
@@ -193,6 +195,11 @@ bool ac::core::Processor::processChain(const Image& src, Image& dst, const int p
     if (dst.empty()) dst.create(out.width() * 2, out.height() * 2, 1, out.type());
     process(out, dst);
     return ok();
+}
+
+bool ac::core::Processor::processAll(const Image&, Image&, double)
+{
+    return false;
 }
 
 bool ac::core::Processor::colorSplit(const Image&, Image&, Image&)

@@ -97,6 +97,40 @@ class Net:
         return dst
 
 
+    def upscale_batch(self, srcs, dsts=None, precision="fp32", config=0, stream=None, workspace=None, out_dtype=None):
+        """2x passes of many images (same element types) in ONE persistent launch with a tile
+        work-queue (achip_pass_batch).  Returns the list of outputs."""
+        import torch
+        if not srcs:
+            return []
+        dev = srcs[0].device
+        if dsts is None:
+            dsts = [torch.empty((2 * s.shape[0], 2 * s.shape[1]), dtype=out_dtype or s.dtype, device=dev) for s in srcs]
+        st = elem_type(srcs[0].dtype)
+        dt = elem_type(dsts[0].dtype)
+        arr = (_native.ACHipImage * len(srcs))()
+        for i, (s, d) in enumerate(zip(srcs, dsts)):
+            if elem_type(s.dtype) != st or elem_type(d.dtype) != dt:
+                raise TypeError("a batch has one source and one destination element type")
+            if s.stride(1) != 1 or d.stride(1) != 1 or tuple(d.shape) != (2 * s.shape[0], 2 * s.shape[1]):
+                raise ValueError(f"image {i}: bad shape or layout")
+            arr[i] = _native.ACHipImage(s.data_ptr(), s.stride(0) * s.element_size(), d.data_ptr(),
+                                        d.stride(0) * d.element_size(), s.shape[1], s.shape[0])
+        L = _native.lib()
+        need = L.achip_batch_workspace_bytes(len(srcs))
+        if workspace is None:
+            workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        elif hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check(L.achip_pass_batch(self._h, prec, config, st, dt, arr, len(srcs), ctypes.c_void_p(workspace.data_ptr()),
+                                 workspace.numel(), ctypes.c_void_p(stream)), "achip_pass_batch")
+        self._keep = (workspace, arr)    # alive until the next call on this net
+        return dsts
+
+
 def configs(precision="fp32"):
     L = _native.lib()
     prec = PRECISIONS[precision]

@@ -3,7 +3,9 @@
 One step = one 2x pass over a batch of synthetic 1080p u8 luma frames already resident in
 HBM (two frames per weight set: GAN, HDN0, HDN1, HDN2, HDN3), written to resident 4K
 outputs.  Prints one JSON line (rank 0):
-  value      whole-job output Mpix/s of the fused FP32 kernel over the timed steps
+  value      whole-job output Mpix/s of the fused FP32 kernel over the timed steps (the
+             FP16-/BF16-MFMA forms are timed on the same frames and reported beside it,
+             with their error against the FP64 restatement)
   roofline   achieved FP32 TFLOP/s of the fused kernel (9424 algorithmic FLOP per input
              pixel x pixels per launch / mean launch time from HIP events on the launch
              stream) against the MI355X FP32 peak
@@ -25,6 +27,7 @@ sys.path.insert(0, ROOT)
 W_IN, H_IN = 1920, 1080
 FLOP_PER_INPUT_PX = 9424          # 2 x 4712 MAC per input pixel per 2x pass
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 peak
+PEAK_MFMA16_TFLOPS = 2500.0       # MI355X published dense FP16/BF16 MFMA peak
 FRAMES_PER_VARIANT = 2
 
 
@@ -57,13 +60,58 @@ def cpu_baseline(frames_u8, weights, min_seconds=10.0):
                       + f", OpenMP {threads} threads"}
 
 
+def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, stream):
+    """Time `steps` steps of one precision; returns (wall seconds, mean launch seconds)."""
+    import torch
+
+    def step():
+        for i, v in order:
+            nets[v].upscale(srcs[i], dsts[i], precision=prec, config=config, stream=stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
+
+
+def error_vs_fp64(nets, weights, prec):
+    """Max/mean abs error and PSNR of one 1920x1080 f32 frame (HDN0, f32 output) against
+    the FP64 restatement of the network."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    from anime4kcpp_amd import synth
+    src = synth.flat_shaded(7, W_IN, H_IN)
+    out = nets["hdn0"].upscale(torch.from_numpy(src).cuda(), precision=prec, out_dtype=torch.float32)
+    torch.cuda.synchronize()
+    ref = OL.oracle_pass_f64(src, *weights["hdn0"])
+    e = np.abs(out.cpu().numpy().astype(np.float64) - ref)
+    mse = float((e ** 2).mean())
+    return {"max_abs": float(e.max()), "mean_abs": float(e.mean()),
+            "psnr_db": round(10 * np.log10(1.0 / mse), 2) if mse > 0 else None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--config", type=int, default=None, help="fused-kernel tile configuration")
+    ap.add_argument("--no-error", action="store_true", help="skip the FP64 error report")
+    ap.add_argument("--precisions", default="fp32,fp16,bf16")
+    ap.add_argument("--config", type=int, default=None, help="kernel tile configuration")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -75,6 +123,12 @@ def main():
     from anime4kcpp_amd import hip, synth
 
     dev = torch.device("cuda:0")
+    if args.calib_copy_mib:
+        a = torch.empty(args.calib_copy_mib << 20, dtype=torch.uint8, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     weights = {v: hip.load_weights(v) for v in hip.VARIANTS}
     nets = {v: hip.Net(v, 0) for v in hip.VARIANTS}
     frames_np = [synth.flat_shaded(1000 + i, W_IN, H_IN, dtype=np.uint8)
@@ -83,64 +137,73 @@ def main():
     dsts = [torch.empty((2 * H_IN, 2 * W_IN), dtype=torch.uint8, device=dev) for _ in srcs]
     order = [(i, hip.VARIANTS[i % len(hip.VARIANTS)]) for i in range(len(srcs))]
     stream = torch.cuda.current_stream(dev)
+    out_px_per_step = len(srcs) * 4 * W_IN * H_IN
 
-    def step():
-        for i, v in order:
-            nets[v].upscale(srcs[i], dsts[i], config=args.config, stream=stream)
+    precs = [p for p in args.precisions.split(",") if p]
+    timings = {}
+    for prec in precs:
+        timings[prec] = time_precision(nets, srcs, dsts, order, prec, args.steps, args.warmup, args.config, stream)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = e0.elapsed_time(e1)
-
-    launches = args.steps * len(srcs)
-    out_px = launches * 4 * W_IN * H_IN
-    ms_per_step = wall * 1e3 / args.steps
-    value = out_px / 1e6 / wall
-    mean_launch_s = gpu_ms / 1e3 / launches
+    main_prec = precs[0]
+    wall, mean_launch_s = timings[main_prec]
+    value = out_px_per_step * args.steps / 1e6 / wall
     achieved = FLOP_PER_INPUT_PX * W_IN * H_IN / mean_launch_s / 1e12
+    kcfg = hip.configs(main_prec)[args.config or 0]
 
     result = {
-        "metric": "output Mpix/s, 1080p->4K 2x (fused FP32 kernel)",
+        "metric": "output Mpix/s, 1080p->4K 2x (FP32 vs FP16-MFMA)",
         "value": round(value, 1),
         "unit": "Mpix/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}[main_prec],
         "data": "synthetic",
         "config": {
-            "workload": "ACNetLegacy 2x pass, 1920x1080 u8 luma -> 3840x2160 u8, fused FP32 HIP kernel",
+            "workload": "ACNetLegacy 2x pass, 1920x1080 u8 luma -> 3840x2160 u8, one fused HIP kernel per frame",
+            "path": {"fp32": "fused FP32 kernel", "fp16": "FP16-MFMA fused kernel",
+                     "bf16": "BF16-MFMA fused kernel"}[main_prec],
             "frames_per_step": len(srcs),
             "weights": "GAN, HDN0, HDN1, HDN2, HDN3 (2 frames each)",
-            "content": "seeded flat-shaded polygons + Bezier strokes + noise (sigma 0.01)",
-            "kernel_config": hip.configs("fp32")[args.config or 0],
+            "content": "seeded flat-shaded polygons + Bezier strokes + noise (sigma 0.01), resident in HBM",
+            "kernel_config": kcfg,
         },
         "roofline": {
             "bound": "mfma",
-            "engine": "FP32 VALU (packed FP32 disabled; non-packed FMA caps at half this peak)",
+            "engine": ("FP32 VALU (packed FP32 disabled; non-packed FMA caps at half the FP32 peak)"
+                       if main_prec == "fp32" else f"{main_prec.upper()} MFMA, FP32 accumulate"),
             "achieved": round(achieved, 2),
-            "peak": PEAK_FP32_TFLOPS,
+            "peak": PEAK_FP32_TFLOPS if main_prec == "fp32" else PEAK_MFMA16_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "frac": round(achieved / (PEAK_FP32_TFLOPS if main_prec == "fp32" else PEAK_MFMA16_TFLOPS), 4),
             "traffic": None,
             "mean_launch_us": round(mean_launch_s * 1e6, 2),
             "flop_per_launch": FLOP_PER_INPUT_PX * W_IN * H_IN,
         },
     }
+    for prec in precs[1:]:
+        w2, ml2 = timings[prec]
+        v2 = out_px_per_step * args.steps / 1e6 / w2
+        ach2 = FLOP_PER_INPUT_PX * W_IN * H_IN / ml2 / 1e12
+        result[f"{prec}_mfma"] = {
+            "value": round(v2, 1), "unit": "Mpix/s", "vs_" + main_prec: round(v2 / value, 2),
+            "ms_per_step": round(w2 * 1e3 / args.steps, 4), "mean_launch_us": round(ml2 * 1e6, 2),
+            "achieved_tflops": round(ach2, 2), "peak_tflops": PEAK_MFMA16_TFLOPS,
+            "frac": round(ach2 / PEAK_MFMA16_TFLOPS, 4),
+            "kernel_config": hip.configs(prec)[args.config or 0],
+        }
+    if not args.no_error:
+        errs = {p: error_vs_fp64(nets, weights, p) for p in precs}
+        for p, e in errs.items():
+            key = "fp32_error_vs_fp64" if p == main_prec else None
+            if p == main_prec:
+                result["error_vs_fp64"] = e
+            else:
+                result[f"{p}_mfma"]["error_vs_fp64"] = e
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(frames_np, weights)
         result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)

@@ -41,6 +41,10 @@ private:
     AC_CORE_EXPORT virtual void process(const Image& src, Image& dst) = 0;
 
 protected:
+    // A processor that runs all of process(src, dst, factor) itself (colour split, passes,
+    // resize, merge) returns true; errors are reported through ok()/error().  The default
+    // returns false and the steps below run one at a time.
+    AC_CORE_EXPORT virtual bool processAll(const Image& src, Image& dst, double factor);
     // Steps of process(src, dst, factor) around the 2x passes.  The defaults run the
     // reference's sequence one pass at a time; a processor may keep chained passes and
     // colour work on its device instead.  Return false after recording an error.

@@ -63,7 +63,11 @@ ACHIP_API int achip_pass(ACHipNet* net, int precision,
                          const void* src, long src_stride, int src_type, int w, int h,
                          void* dst, long dst_stride, int dst_type, void* stream);
 
-/* The same with an explicit kernel configuration (autotuning and tests). */
+/* The same with an explicit kernel configuration (autotuning and tests).  config
+ * ACHIP_AUTO uses the autotuner's choice for the image's size class, or the default
+ * configuration when that class was not tuned (with ACHIP_AUTOTUNE=1 in the
+ * environment an untuned class is tuned on first use). */
+#define ACHIP_AUTO (-1)
 ACHIP_API int achip_pass_config(ACHipNet* net, int precision, int config,
                                 const void* src, long src_stride, int src_type, int w, int h,
                                 void* dst, long dst_stride, int dst_type, void* stream);
@@ -96,6 +100,14 @@ ACHIP_API int achip_pass_batch(ACHipNet* net, int precision, int config, int src
  * workgroup, pixels per thread run, LDS bytes.  Returns the count, or fills config i. */
 ACHIP_API int achip_config_count(int precision);
 ACHIP_API int achip_config_info(int precision, int i, int* tw, int* th, int* nt, int* p, int* lds_bytes);
+
+/* Autotuner (project-written; no counterpart in the reference): time every configuration
+ * of `precision` on w x h images (src_type -> dst_type) on the net's device, on temporary
+ * buffers, a few launches each, and remember the fastest for that size class (log2 of
+ * width and height).  Fills *best and, if ms is not NULL, the mean launch time of each
+ * configuration (achip_config_count(precision) floats). */
+ACHIP_API int achip_autotune(ACHipNet* net, int precision, int src_type, int dst_type, int w, int h,
+                             int* best, float* ms);
 
 /* Calling thread's last error message ("" if none). */
 ACHIP_API const char* achip_last_error(void);

@@ -356,7 +356,7 @@ int orc_acnet_pass_f64(const float* src, int w, int h, double* dst,
     return 0;
 }
 
-/* ---- colour split / merge (ImageProcess.cpp:38-61, :191-215) ----------------------- */
+/* ---- colour split / merge (ImageProcess.cpp:38-61, :114-139, :191-215, :276-308) ---- */
 
 int orc_rgb2yuv(const void* rgb, long rgb_stride, int w, int h, int channels, int type,
                 void* y, long y_stride, void* uv, long uv_stride)
@@ -370,6 +370,13 @@ int orc_rgb2yuv(const void* rgb, long rgb_stride, int w, int h, int channels, in
             unsigned char* yo = (unsigned char*)y + i * y_stride + (long)j * es;
             unsigned char* uo = (unsigned char*)uv + i * uv_stride + (long)j * uvc * es;
             float r = orc_to_float(in, type), g = orc_to_float(in + es, type), b = orc_to_float(in + 2 * es, type);
+            float alpha = 1.0f;
+            if (channels == 4) {   /* rgba2yuva premultiplies (ImageProcess.cpp:114-139) */
+                alpha = orc_to_float(in + 3 * es, type);
+                r = r * alpha;
+                g = g * alpha;
+                b = b * alpha;
+            }
             float a0 = 0.299f * r, a1 = 0.587f * g, a2 = 0.114f * b;
             float yy = (a0 + a1) + a2;
             float u = 0.564f * (b - yy) + 0.5f;
@@ -377,7 +384,7 @@ int orc_rgb2yuv(const void* rgb, long rgb_stride, int w, int h, int channels, in
             orc_from_float(yy, yo, type);
             orc_from_float(u, uo, type);
             orc_from_float(v, uo + es, type);
-            if (channels == 4) orc_from_float(orc_to_float(in + 3 * es, type), uo + 2 * es, type);
+            if (channels == 4) orc_from_float(alpha, uo + 2 * es, type);
         }
     return 0;
 }
@@ -399,73 +406,95 @@ int orc_yuv2rgb(const void* y, long y_stride, const void* uv, long uv_stride,
             float r = yy + 1.403f * v;
             float g = (yy - 0.344f * u) - 0.714f * v;
             float b = yy + 1.773f * u;
+            if (channels == 4) {   /* yuva2rgba un-premultiplies (ImageProcess.cpp:276-308) */
+                float alpha = orc_to_float(ui + 2 * es, type);
+                if (alpha > 1e-6f) { r = r / alpha; g = g / alpha; b = b / alpha; }
+                else r = g = b = 0.0f;
+                orc_from_float(alpha, out + 3 * es, type);
+            }
             orc_from_float(r, out, type);
             orc_from_float(g, out + es, type);
             orc_from_float(b, out + 2 * es, type);
-            if (channels == 4) orc_from_float(orc_to_float(ui + 2 * es, type), out + 3 * es, type);
         }
     return 0;
 }
 
-/* ---- 2x resize, closed-form taps (parity with stb_image_resize2 unpinned) ----------- */
+/* ---- resize (parity with stb_image_resize2 unpinned) -------------------------------- */
 
-/* Output pixel centre 2i+0.5 maps to source i + (-0.25 | +0.25).  Bilinear: taps
- * (0.25, 0.75) on (i-1, i) for even outputs and (0.75, 0.25) on (i, i+1) for odd.
- * Catmull-Rom (a = -0.5) at fractional offsets 0.75 / 0.25 from the left tap. */
-static void taps2x(int mode, int odd, int* off, float* wt, int* nt)
+/* Filter weight: mode 0 bilinear (support 1), 1 Catmull-Rom a = -0.5 (support 2). */
+static float filter_weight(int mode, float d)
 {
-    if (mode == 0) {
-        *nt = 2;
-        if (!odd) { off[0] = -1; off[1] = 0; wt[0] = 0.25f; wt[1] = 0.75f; }
-        else { off[0] = 0; off[1] = 1; wt[0] = 0.75f; wt[1] = 0.25f; }
-    } else {
-        *nt = 4;
-        /* t = 0.75 for even (between i-1 and i), t = 0.25 for odd (between i and i+1) */
-        static const float even_w[4] = { -0.0234375f, 0.2265625f, 0.8671875f, -0.0703125f };
-        static const float odd_w[4] = { -0.0703125f, 0.8671875f, 0.2265625f, -0.0234375f };
-        const float* src = odd ? odd_w : even_w;
-        int base = odd ? -1 : -2;
-        for (int t = 0; t < 4; t++) { off[t] = base + t; wt[t] = src[t]; }
-    }
+    d = fabsf(d);
+    if (mode == 0) return d < 1.0f ? 1.0f - d : 0.0f;
+    if (d < 1.0f) return (1.5f * d) * (d * d) + (-2.5f * (d * d) + 1.0f);
+    if (d < 2.0f) return (-0.5f * (d * (d * d)) + 2.5f * (d * d)) + (-4.0f * d + 2.0f);
+    return 0.0f;
 }
 
+#define ORC_MAX_TAPS 12
+/* Taps of output coordinate o: pixel-centre mapping, filter stretched by 1/scale when
+ * downscaling, weights normalised to sum 1. */
+static int resize_taps(int mode, int o, int n_in, int n_out, float* w, int* first)
+{
+    const float scale = (float)n_out / (float)n_in;
+    const float center = ((float)o + 0.5f) / scale - 0.5f;
+    const float support = (mode == 0 ? 1.0f : 2.0f) * (scale < 1.0f ? 1.0f / scale : 1.0f);
+    const float stretch = scale < 1.0f ? scale : 1.0f;
+    int lo = (int)ceilf(center - support);
+    int hi = (int)floorf(center + support);
+    if (hi - lo + 1 > ORC_MAX_TAPS) {
+        lo = (int)center - ORC_MAX_TAPS / 2 + 1;
+        hi = lo + ORC_MAX_TAPS - 1;
+    }
+    float sum = 0.0f;
+    int n = 0;
+    for (int i = lo; i <= hi; i++, n++) {
+        w[n] = filter_weight(mode, ((float)i - center) * stretch);
+        sum += w[n];
+    }
+    if (sum != 0.0f)
+        for (int i = 0; i < n; i++) w[i] /= sum;
+    *first = lo;
+    return n;
+}
+
+int orc_resize(const void* src, long src_stride, int sw, int sh, int channels, int type,
+               void* dst, long dst_stride, int dw, int dh, int mode)
+{
+    if (sw <= 0 || sh <= 0 || dw <= 0 || dh <= 0 || channels <= 0 || channels > 4) return -22;
+    const int es = elem_size(type);
+    for (int y = 0; y < dh; y++) {
+        float wy[ORC_MAX_TAPS], wx[ORC_MAX_TAPS];
+        int fy, fx;
+        const int ny = resize_taps(mode, y, sh, dh, wy, &fy);
+        for (int x = 0; x < dw; x++) {
+            const int nx = resize_taps(mode, x, sw, dw, wx, &fx);
+            float acc[4] = { 0, 0, 0, 0 };
+            for (int j = 0; j < ny; j++) {
+                const unsigned char* row = (const unsigned char*)src + (long)clampi(fy + j, 0, sh - 1) * src_stride;
+                float h[4] = { 0, 0, 0, 0 };
+                for (int i = 0; i < nx; i++) {
+                    const unsigned char* p = row + (long)clampi(fx + i, 0, sw - 1) * channels * es;
+                    for (int c = 0; c < channels; c++) {
+                        float m = wx[i] * orc_to_float(p + c * es, type);
+                        h[c] = h[c] + m;
+                    }
+                }
+                for (int c = 0; c < channels; c++) {
+                    float m = wy[j] * h[c];
+                    acc[c] = acc[c] + m;
+                }
+            }
+            for (int c = 0; c < channels; c++)
+                orc_from_float(acc[c], (unsigned char*)dst + (long)y * dst_stride + ((long)x * channels + c) * es, type);
+        }
+    }
+    return 0;
+}
+
+/* 2x closed form kept for the tests' tap check: even outputs sample i - 0.25, odd i + 0.25 */
 int orc_resize2x(const void* src, long src_stride, int w, int h, int channels, int type,
                  void* dst, long dst_stride, int mode)
 {
-    if (w <= 0 || h <= 0 || channels <= 0) return -22;
-    const int es = elem_size(type);
-    int W2 = 2 * w;
-    float* tmp = (float*)malloc(sizeof(float) * (size_t)W2 * h * channels);
-    if (!tmp) return -12;
-    int off[4], nt;
-    float wt[4];
-    for (int i = 0; i < h; i++)
-        for (int x = 0; x < W2; x++) {
-            taps2x(mode, x & 1, off, wt, &nt);
-            for (int c = 0; c < channels; c++) {
-                float acc = 0.0f;
-                for (int t = 0; t < nt; t++) {
-                    int sx = clampi((x >> 1) + off[t], 0, w - 1);
-                    const unsigned char* p = (const unsigned char*)src + i * src_stride + ((long)sx * channels + c) * es;
-                    float m = wt[t] * orc_to_float(p, type);
-                    acc = acc + m;
-                }
-                tmp[((size_t)i * W2 + x) * channels + c] = acc;
-            }
-        }
-    for (int y = 0; y < 2 * h; y++) {
-        taps2x(mode, y & 1, off, wt, &nt);
-        for (int x = 0; x < W2; x++)
-            for (int c = 0; c < channels; c++) {
-                float acc = 0.0f;
-                for (int t = 0; t < nt; t++) {
-                    int sy = clampi((y >> 1) + off[t], 0, h - 1);
-                    float m = wt[t] * tmp[((size_t)sy * W2 + x) * channels + c];
-                    acc = acc + m;
-                }
-                orc_from_float(acc, (unsigned char*)dst + y * dst_stride + ((long)x * channels + c) * es, type);
-            }
-    }
-    free(tmp);
-    return 0;
+    return orc_resize(src, src_stride, w, h, channels, type, dst, dst_stride, 2 * w, 2 * h, mode);
 }

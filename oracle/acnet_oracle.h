@@ -62,9 +62,12 @@ int orc_rgb2yuv(const void* rgb, long rgb_stride, int w, int h, int channels, in
 int orc_yuv2rgb(const void* y, long y_stride, const void* uv, long uv_stride,
                 int w, int h, int channels, int type, void* rgb, long rgb_stride);
 
-/* 2x separable resize with edge clamp and pixel-centre alignment, per channel.
- * mode 0: bilinear (taps 1/4, 3/4); mode 1: Catmull-Rom.  f32 math, output
- * quantised with fromFloat.  Parity with stb_image_resize2 is unpinned. */
+/* Separable resize with edge clamp and pixel-centre alignment, per channel, to dw x dh.
+ * mode 0: bilinear; mode 1: Catmull-Rom (a = -0.5); the filter is stretched by 1/scale
+ * when downscaling.  f32 math, horizontal then vertical, output through fromFloat.
+ * Parity with stb_image_resize2 is unpinned; pinned against closed-form 2x taps. */
+int orc_resize(const void* src, long src_stride, int sw, int sh, int channels, int type,
+               void* dst, long dst_stride, int dw, int dh, int mode);
 int orc_resize2x(const void* src, long src_stride, int w, int h, int channels, int type,
                  void* dst, long dst_stride, int mode);
 

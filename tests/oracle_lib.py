@@ -34,6 +34,8 @@ def orc():
         o.orc_rgb2yuv.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, L]
         o.orc_yuv2rgb.argtypes = [P, L, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L]
         o.orc_resize2x.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int]
+        o.orc_resize.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _orc = o
     return _orc
 

@@ -1,0 +1,142 @@
+"""Pins for the oracle's colour split/merge and resize (CPU).
+
+Colour (core/src/ImageProcess.cpp:38-61, :114-139, :191-215, :276-308): a numpy float32
+restatement with the same rounding order, exhaustive-ish over u8 triples, and the
+reference's own round-trip bound (tests/core/src/ImageProcessTest.cpp:52-79: gray
+within +-1).  Resize: stb_image_resize2 is absent, so parity is unpinned; the oracle is
+pinned to closed-form 2x Catmull-Rom / bilinear taps, partition of unity and exact
+reproduction of linear ramps in the interior."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+f32 = np.float32
+
+
+def np_from_float_u8(v):
+    s = np.where(v < 0, f32(0), np.where(v < 1, v, f32(1))).astype(np.float32)
+    return (s * f32(255) + f32(0.5)).astype(np.float32).astype(np.uint8)
+
+
+def np_rgb2yuv(rgb, alpha=None):
+    r, g, b = (rgb[..., i].astype(np.float32) / f32(255) for i in range(3))
+    if alpha is not None:
+        a = alpha.astype(np.float32) / f32(255)
+        r, g, b = (r * a).astype(np.float32), (g * a).astype(np.float32), (b * a).astype(np.float32)
+    y = ((f32(0.299) * r + f32(0.587) * g).astype(np.float32) + f32(0.114) * b).astype(np.float32)
+    u = (f32(0.564) * (b - y).astype(np.float32) + f32(0.5)).astype(np.float32)
+    v = (f32(0.713) * (r - y).astype(np.float32) + f32(0.5)).astype(np.float32)
+    return np_from_float_u8(y), np_from_float_u8(u), np_from_float_u8(v)
+
+
+def orc_split(rgb):
+    h, w, c = rgb.shape
+    rgb = np.ascontiguousarray(rgb)
+    y = np.zeros((h, w), np.uint8)
+    uv = np.zeros((h, w, c - 1), np.uint8)
+    assert OL.orc().orc_rgb2yuv(rgb.ctypes.data, rgb.strides[0], w, h, c, OL.U8, y.ctypes.data, y.strides[0],
+                                uv.ctypes.data, uv.strides[0]) == 0
+    return y, uv
+
+
+def orc_merge(y, uv):
+    h, w = y.shape
+    c = uv.shape[2] + 1
+    out = np.zeros((h, w, c), np.uint8)
+    assert OL.orc().orc_yuv2rgb(y.ctypes.data, y.strides[0], uv.ctypes.data, uv.strides[0], w, h, c, OL.U8,
+                                out.ctypes.data, out.strides[0]) == 0
+    return out
+
+
+def test_split_closed_form_u8():
+    vals = np.arange(0, 256, 15, dtype=np.uint8)
+    r, g, b = np.meshgrid(vals, vals, vals, indexing="ij")
+    rgb = np.stack([r, g, b], -1).reshape(-1, 1, 3)
+    rgb = np.ascontiguousarray(rgb.reshape(len(vals) ** 2, len(vals), 3))
+    y, uv = orc_split(rgb)
+    yy, uu, vv = np_rgb2yuv(rgb)
+    assert np.array_equal(y, yy) and np.array_equal(uv[..., 0], uu) and np.array_equal(uv[..., 1], vv)
+
+
+def test_split_rgba_premultiplies():
+    rng = np.random.default_rng(3)
+    rgba = rng.integers(0, 256, (17, 23, 4), dtype=np.uint8)
+    y, uva = orc_split(rgba)
+    yy, uu, vv = np_rgb2yuv(rgba[..., :3], rgba[..., 3])
+    assert np.array_equal(y, yy) and np.array_equal(uva[..., 0], uu) and np.array_equal(uva[..., 1], vv)
+    assert np.array_equal(uva[..., 2], rgba[..., 3])
+
+
+@pytest.mark.parametrize("c", [3, 4])
+def test_gray_round_trip_within_one(c):
+    for level in range(0, 256, 5):
+        img = np.full((4, 4, c), level, np.uint8)
+        if c == 4:
+            img[..., 3] = 255
+        y, uv = orc_split(img)
+        back = orc_merge(y, uv)
+        assert np.abs(back[..., :3].astype(int) - level).max() <= 1
+
+
+def resize(src, dw, dh, mode):
+    src = np.ascontiguousarray(src)
+    h, w = src.shape[:2]
+    c = 1 if src.ndim == 2 else src.shape[2]
+    dst = np.zeros((dh, dw) + (() if c == 1 else (c,)), src.dtype)
+    assert OL.orc().orc_resize(src.ctypes.data, src.strides[0], w, h, c, OL.TYPE_OF[src.dtype], dst.ctypes.data,
+                               dst.strides[0], dw, dh, mode) == 0
+    return dst
+
+
+def closed_2x(x, mode):
+    """2x: even outputs sample i - 1/4, odd i + 1/4; taps (1/4, 3/4) bilinear or
+    (-3/128, 29/128, 111/128, -9/128) Catmull-Rom, edge clamped, rows then columns."""
+    h, w = x.shape
+    if mode == 0:
+        even, odd = [(-1, 0.25), (0, 0.75)], [(0, 0.75), (1, 0.25)]
+    else:
+        even = [(-2, -0.0234375), (-1, 0.2265625), (0, 0.8671875), (1, -0.0703125)]
+        odd = [(-1, -0.0703125), (0, 0.8671875), (1, 0.2265625), (2, -0.0234375)]
+    tmp = np.zeros((h, 2 * w), np.float32)
+    for X in range(2 * w):
+        for off, wt in (odd if X & 1 else even):
+            tmp[:, X] = (tmp[:, X] + f32(wt) * x[:, min(max(X // 2 + off, 0), w - 1)]).astype(np.float32)
+    out = np.zeros((2 * h, 2 * w), np.float32)
+    for Y in range(2 * h):
+        for off, wt in (odd if Y & 1 else even):
+            out[Y] = (out[Y] + f32(wt) * tmp[min(max(Y // 2 + off, 0), h - 1)]).astype(np.float32)
+    return np.clip(out, 0, 1)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_resize_2x_closed_form(mode):
+    x = np.random.default_rng(5).random((9, 13), dtype=np.float32)
+    assert np.array_equal(resize(x, 26, 18, mode), closed_2x(x, mode))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("dw,dh", [(40, 30), (26, 18), (10, 9), (7, 5), (52, 36)])
+def test_resize_partition_of_unity_and_symmetry(mode, dw, dh):
+    const = np.full((9, 13), 0.4, np.float32)
+    assert np.abs(resize(const, dw, dh, mode) - 0.4).max() < 1e-6
+    # mirroring the input mirrors the output (pixel-centre alignment, symmetric filters)
+    x = np.random.default_rng(7).random((9, 13), dtype=np.float32)
+    a = resize(x, dw, dh, mode)
+    b = resize(np.ascontiguousarray(x[:, ::-1]), dw, dh, mode)[:, ::-1]
+    assert np.abs(a - b).max() < 4e-6   # taps summed in mirrored order
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("dw,dh", [(40, 30), (26, 18), (52, 36), (17, 11)])
+def test_resize_up_reproduces_ramps(mode, dw, dh):
+    """Upscaling interpolates: a horizontal ramp inside (0, 1) comes back exactly at every
+    output pixel whose taps stay inside the image."""
+    ramp = np.tile(np.linspace(0.1, 0.9, 13, dtype=np.float32), (9, 1))
+    out = resize(ramp, dw, dh, mode)
+    sx = (np.arange(dw) + 0.5) * 13 / dw - 0.5
+    want = 0.1 + 0.8 * sx / 12
+    inner = (sx > 1.0) & (sx < 11.0)
+    assert np.abs(out[:, inner] - want[inner]).max() < 2e-6
