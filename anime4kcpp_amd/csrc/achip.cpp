@@ -390,7 +390,7 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
                                net->wts, static_cast<hipStream_t>(stream));
     else
         e = achip::launch_mfma(precision == ACHIP_BF16, config, src_type, dst_type, src, src_stride, w, h, dst,
-                               dst_stride, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16,
+                               dst_stride, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16, net->wa32,
                                static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return ACHIP_OK;
@@ -522,7 +522,7 @@ int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int
         e = achip::launch_fp32_batch(config - 1, src_type, dst_type, ddesc, n, static_cast<int>(total), counter, net->wts, st);
     else
         e = achip::launch_mfma_batch(precision == ACHIP_BF16, config, src_type, dst_type, ddesc, n, static_cast<int>(total),
-                                     counter, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16, st);
+                                     counter, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16, net->wa32, st);
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
     return ACHIP_OK;
 }

@@ -39,11 +39,18 @@ constexpr int kMfma32L0Floats = 3 * 64;
 constexpr int kMfma32LayerFloats = 24 * 64;
 constexpr int kMfma32BlobFloats = kMfma32L0Floats + 8 * kMfma32LayerFloats;   // 12480
 
-// toFloat (core/internal/AC/Core/Internal/Util.hpp:49-61): unsigned -> v / max, correctly
-// rounded division as the host's IEEE divide.
+// toFloat (core/internal/AC/Core/Internal/Util.hpp:49-61): unsigned -> v / max, the
+// correctly rounded quotient as the host's IEEE divide.  Computed as q0 = v * RN(1/max),
+// one FMA residual and one FMA correction: for every one of the 256 (u8) and 65536 (u16)
+// inputs this equals RN(v / max) exactly, at a fraction of a full-precision divide.
+__device__ inline float div_exact(float v, float d, float r)
+{
+    const float q0 = __fmul_rn(v, r);
+    return __fmaf_rn(__fmaf_rn(-q0, d, v), r, q0);
+}
 template <typename T> __device__ inline float to_float(T v);
-template <> __device__ inline float to_float<uint8_t>(uint8_t v) { return __fdiv_rn(static_cast<float>(v), 255.0f); }
-template <> __device__ inline float to_float<uint16_t>(uint16_t v) { return __fdiv_rn(static_cast<float>(v), 65535.0f); }
+template <> __device__ inline float to_float<uint8_t>(uint8_t v) { return div_exact(static_cast<float>(v), 255.0f, 1.0f / 255.0f); }
+template <> __device__ inline float to_float<uint16_t>(uint16_t v) { return div_exact(static_cast<float>(v), 65535.0f, 1.0f / 65535.0f); }
 template <> __device__ inline float to_float<__half>(__half v) { return __half2float(v); }
 template <> __device__ inline float to_float<float>(float v) { return v; }
 

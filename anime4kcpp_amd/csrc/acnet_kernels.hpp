@@ -45,12 +45,13 @@ extern const int kFp32ConfigCount;
 hipError_t launch_fp32(int cfg, int stype, int dtype, const void* src, long sstride, int W, int H,
                        void* dst, long dstride, const float* wts, hipStream_t stream);
 
-// Fused pass with layers 1..8 on MFMA.  bf16 = 0: FP16 operands, 1: BF16 operands; wa is
-// the per-lane A-operand blob of that type (acnet_device.hpp layout).
+// Fused pass with layers 1..8 on 16-bit MFMA and layer 0 on FP32 MFMA.  bf16 = 0: FP16
+// operands, 1: BF16 operands; wa is the per-lane A-operand blob of that type, wa0 the
+// FP32 blob's layer-0 part (acnet_device.hpp layouts).
 extern const Fp32Config kMfmaConfigs[];
 extern const int kMfmaConfigCount;
 hipError_t launch_mfma(int bf16, int cfg, int stype, int dtype, const void* src, long sstride, int W, int H,
-                       void* dst, long dstride, const float* wts, const void* wa, hipStream_t stream);
+                       void* dst, long dstride, const float* wts, const void* wa, const float* wa0, hipStream_t stream);
 
 // Fused pass with every conv layer on FP32 MFMA (tile 64 x 32, 512 threads); wa32 is the
 // per-lane FP32 A-operand blob (acnet_device.hpp layout).
@@ -73,6 +74,6 @@ hipError_t launch_resize(int type, int channels, int mode, const void* src, long
 hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
                              const float* wts, hipStream_t stream);
 hipError_t launch_mfma_batch(int bf16, int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total,
-                             int* counter, const float* wts, const void* wa, hipStream_t stream);
+                             int* counter, const float* wts, const void* wa, const float* wa0, hipStream_t stream);
 
 } // namespace achip

@@ -131,6 +131,21 @@ class Net:
         return dsts
 
 
+    def autotune(self, w, h, precision="fp32", src_dtype=None, dst_dtype=None):
+        """Time every kernel configuration of `precision` on w x h images (achip_autotune) and
+        remember the fastest for that size class; returns (best, [ms per configuration])."""
+        import torch
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        st = elem_type(src_dtype or torch.uint8)
+        dt = elem_type(dst_dtype or src_dtype or torch.uint8)
+        L = _native.lib()
+        n = L.achip_config_count(prec)
+        best = ctypes.c_int(-1)
+        ms = (ctypes.c_float * max(n, 1))()
+        check(L.achip_autotune(self._h, prec, st, dt, w, h, ctypes.byref(best), ms), "achip_autotune")
+        return best.value, [ms[i] for i in range(n)]
+
+
 def configs(precision="fp32"):
     L = _native.lib()
     prec = PRECISIONS[precision]

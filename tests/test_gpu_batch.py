@@ -35,7 +35,7 @@ def test_batch_equals_alone(net, weights, cuda, prec, dtype):
     batch_out = net.upscale_batch(dsrc, precision=prec)
     torch.cuda.synchronize()
     for i, s in enumerate(dsrc):
-        alone = net.upscale(s, precision=prec)
+        alone = net.upscale(s, precision=prec, config=0)
         torch.cuda.synchronize()
         assert torch.equal(alone, batch_out[i]), f"image {i} {dims[i]} differs"
     if prec == "fp32":
@@ -81,3 +81,18 @@ def test_batch_rejects_bad_workspace(net, cuda):
     assert L.achip_pass_batch(net.handle, 0, 0, 1, 1, arr, 1, ctypes.c_void_p(host.data_ptr()), need, None) == -22
     assert L.achip_pass_batch(net.handle, 0, 0, 1, 1, arr, 0, None, 0, None) == -22
     torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_autotune_choice_is_used(net, cuda, prec):
+    """achip_autotune times every configuration, and achip_pass then runs the chosen one:
+    its output equals that configuration's output bit for bit."""
+    best, ms = net.autotune(200, 120, precision=prec, src_dtype=torch.uint8)
+    assert 0 <= best < len(ms) and all(t > 0 for t in ms)
+    assert ms[best] == min(ms)
+    src = torch.from_numpy(synth.flat_shaded(91, 200, 120, dtype=np.uint8)).to(cuda)
+    auto = net.upscale(src, precision=prec)
+    expl = net.upscale(src, precision=prec, config=best)
+    torch.cuda.synchronize()
+    assert torch.equal(auto, expl)
