@@ -31,6 +31,26 @@ class ACHipImage(ctypes.Structure):
                 ("w", c_int), ("h", c_int)]
 
 
+class ACHipRequest(ctypes.Structure):
+    """include/achip.h ACHipRequest"""
+    _fields_ = [("kind", c_int), ("width", c_int), ("height", c_int), ("factor", c_int), ("stream", c_int),
+                ("src", c_void_p * 3), ("src_stride", c_long * 3), ("dst", c_void_p * 3), ("dst_stride", c_long * 3)]
+
+
+class ACHipServerConfig(ctypes.Structure):
+    """include/achip.h ACHipServerConfig"""
+    _fields_ = [("precision", c_int), ("hbm_budget", ctypes.c_size_t), ("step_tiles", c_int), ("max_workgroups", c_int)]
+
+
+class ACHipServerStats(ctypes.Structure):
+    """include/achip.h ACHipServerStats"""
+    _fields_ = [("completed_frames", c_long), ("completed_oneoffs", c_long), ("failed", c_long),
+                ("frame_p50_ms", ctypes.c_double), ("frame_p99_ms", ctypes.c_double),
+                ("oneoff_p50_ms", ctypes.c_double), ("oneoff_p99_ms", ctypes.c_double),
+                ("output_mpix", ctypes.c_double), ("busy_s", ctypes.c_double),
+                ("steps", c_long), ("graph_replays", c_long)]
+
+
 class ACProcessor(ctypes.Structure):
     """binding/c/include/AC/Core/Processor.h:15-21"""
     _fields_ = [("device", c_int), ("type", c_char_p), ("model", c_char_p), ("hptr", c_void_p)]
@@ -49,6 +69,13 @@ SIGNATURES = {
     "achip_config_info": (c_int, [c_int, c_int] + [ctypes.POINTER(c_int)] * 5),
     "achip_last_error": (c_char_p, []),
     "achip_batch_workspace_bytes": (ctypes.c_size_t, [c_int]),
+    "achip_server_create": (c_int, [c_void_p, ctypes.POINTER(ACHipServerConfig), ctypes.POINTER(c_void_p)]),
+    "achip_server_destroy": (None, [c_void_p]),
+    "achip_server_submit": (c_int, [c_void_p, ctypes.POINTER(ACHipRequest), ctypes.POINTER(ctypes.c_ulonglong)]),
+    "achip_server_wait": (c_int, [c_void_p, ctypes.c_ulonglong, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]),
+    "achip_server_output": (c_int, [c_void_p, ctypes.c_ulonglong, c_int, ctypes.POINTER(c_void_p), ctypes.POINTER(c_long)]),
+    "achip_server_release": (c_int, [c_void_p, ctypes.c_ulonglong]),
+    "achip_server_stats": (c_int, [c_void_p, ctypes.POINTER(ACHipServerStats), c_int]),
     "achip_autotune": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_float)]),
     "achip_pass_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p]),
     # include/AC/Core/Image.h

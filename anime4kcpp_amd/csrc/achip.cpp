@@ -15,16 +15,9 @@
 #include <vector>
 
 #include "achip.h"
+#include "achip_internal.hpp"
 #include "acnet_device.hpp"
 #include "acnet_kernels.hpp"
-
-struct ACHipNet {
-    int device;
-    float* wts;      // device FP32 weight blob, acnet_device.hpp layout
-    void* wa16;      // MFMA A operands, FP16
-    void* wabf16;    // MFMA A operands, BF16
-    float* wa32;     // FP32 MFMA A operands
-};
 
 namespace {
 
@@ -221,7 +214,7 @@ std::tuple<int, int, int, int, int, int> tune_key(int device, int precision, int
 
 namespace achip {
 
-int persistent_grid(const void* kernel, int threads, int tiles)
+int persistent_grid(const void* kernel, int threads, int tiles, int max_wg)
 {
     static std::mutex mtx;
     static std::map<std::pair<const void*, int>, int> cache;   // (kernel, device) -> workgroups
@@ -241,7 +234,37 @@ int persistent_grid(const void* kernel, int threads, int tiles)
         std::lock_guard<std::mutex> lk(mtx);
         cache[{ kernel, dev }] = per_device;
     }
+    if (max_wg > 0) per_device = std::min(per_device, max_wg);
     return std::min(per_device, std::max(tiles, 1));
+}
+
+hipError_t launch_pass(const ACHipNet* net, int precision, int config, int stype, int dtype, const void* src,
+                       long sstride, int w, int h, void* dst, long dstride, hipStream_t stream)
+{
+    if (precision == ACHIP_FP32 && config == 0)
+        return launch_mfma32(stype, dtype, src, sstride, w, h, dst, dstride, net->wts, net->wa32, stream);
+    if (precision == ACHIP_FP32)
+        return launch_fp32(config - 1, stype, dtype, src, sstride, w, h, dst, dstride, net->wts, stream);
+    return launch_mfma(precision == ACHIP_BF16, config, stype, dtype, src, sstride, w, h, dst, dstride, net->wts,
+                       precision == ACHIP_BF16 ? net->wabf16 : net->wa16, net->wa32, stream);
+}
+
+hipError_t launch_batch(const ACHipNet* net, int precision, int config, int stype, int dtype, const BatchDesc* desc,
+                        int n, int total, int* counter, int max_wg, hipStream_t stream)
+{
+    if (precision == ACHIP_FP32 && config == 0)
+        return launch_mfma32_batch(stype, dtype, desc, n, total, counter, net->wts, net->wa32, max_wg, stream);
+    if (precision == ACHIP_FP32)
+        return launch_fp32_batch(config - 1, stype, dtype, desc, n, total, counter, net->wts, max_wg, stream);
+    return launch_mfma_batch(precision == ACHIP_BF16, config, stype, dtype, desc, n, total, counter, net->wts,
+                             precision == ACHIP_BF16 ? net->wabf16 : net->wa16, net->wa32, max_wg, stream);
+}
+
+int auto_config(const ACHipNet* net, int precision, int stype, int dtype, int w, int h)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mtx);
+    auto it = g_tune.find(tune_key(net->device, precision, stype, dtype, w, h));
+    return it != g_tune.end() ? it->second : 0;
 }
 
 } // namespace achip
@@ -363,35 +386,25 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
     if (precision != ACHIP_FP32 && precision != ACHIP_FP16 && precision != ACHIP_BF16)
         return fail(ACHIP_EINVAL, "precision %d not available", precision);
     if (config == ACHIP_AUTO && net) {
-        config = 0;
         bool found = false;
         {
             std::lock_guard<std::mutex> lk(g_tune_mtx);
-            auto it = g_tune.find(tune_key(net->device, precision, src_type, dst_type, w, h));
-            if (it != g_tune.end()) { config = it->second; found = true; }
+            found = g_tune.count(tune_key(net->device, precision, src_type, dst_type, w, h)) != 0;
         }
         const char* env = std::getenv("ACHIP_AUTOTUNE");
         if (!found && env && env[0] == '1') {
             int best = 0;
-            if (achip_autotune(net, precision, src_type, dst_type, w, h, &best, nullptr) == ACHIP_OK) config = best;
+            (void)achip_autotune(net, precision, src_type, dst_type, w, h, &best, nullptr);
         }
+        config = achip::auto_config(net, precision, src_type, dst_type, w, h);
     }
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
     int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
     if (r) return r;
     DeviceGuard g(net->device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
-    hipError_t e;
-    if (precision == ACHIP_FP32 && config == 0)
-        e = achip::launch_mfma32(src_type, dst_type, src, src_stride, w, h, dst, dst_stride, net->wts, net->wa32,
-                                 static_cast<hipStream_t>(stream));
-    else if (precision == ACHIP_FP32)
-        e = achip::launch_fp32(config - 1, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
-                               net->wts, static_cast<hipStream_t>(stream));
-    else
-        e = achip::launch_mfma(precision == ACHIP_BF16, config, src_type, dst_type, src, src_stride, w, h, dst,
-                               dst_stride, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16, net->wa32,
-                               static_cast<hipStream_t>(stream));
+    const hipError_t e = achip::launch_pass(net, precision, config, src_type, dst_type, src, src_stride, w, h, dst,
+                                            dst_stride, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return ACHIP_OK;
 }
@@ -502,7 +515,7 @@ int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int
         const size_t dbytes = static_cast<size_t>(2 * im.h - 1) * im.dst_stride + static_cast<size_t>(2 * im.w) * (dst_type & 0xff);
         if (ws < d0 + dbytes && d0 < ws + need) return fail(ACHIP_EINVAL, "image %d: dst overlaps the workspace", i);
         const int tx = (im.w + tw - 1) / tw, ty = (im.h + th - 1) / th;
-        desc[i] = achip::BatchDesc{ im.src, im.dst, im.src_stride, im.dst_stride, im.w, im.h, tx, static_cast<int>(total) };
+        desc[i] = achip::BatchDesc{ im.src, im.dst, im.src_stride, im.dst_stride, im.w, im.h, tx, static_cast<int>(total), 0 };
         total += static_cast<long>(tx) * ty;
         if (total > (1L << 30)) return fail(ACHIP_EINVAL, "batch too large (%ld tiles)", total);
     }
@@ -515,14 +528,7 @@ int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(batch descriptors)");
     auto* ddesc = reinterpret_cast<const achip::BatchDesc*>(static_cast<char*>(workspace) + kBatchHeader);
     int* counter = static_cast<int*>(workspace);
-    if (precision == ACHIP_FP32 && config == 0)
-        e = achip::launch_mfma32_batch(src_type, dst_type, ddesc, n, static_cast<int>(total), counter, net->wts,
-                                       net->wa32, st);
-    else if (precision == ACHIP_FP32)
-        e = achip::launch_fp32_batch(config - 1, src_type, dst_type, ddesc, n, static_cast<int>(total), counter, net->wts, st);
-    else
-        e = achip::launch_mfma_batch(precision == ACHIP_BF16, config, src_type, dst_type, ddesc, n, static_cast<int>(total),
-                                     counter, net->wts, precision == ACHIP_BF16 ? net->wabf16 : net->wa16, net->wa32, st);
+    e = achip::launch_batch(net, precision, config, src_type, dst_type, ddesc, n, static_cast<int>(total), counter, 0, st);
     if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
     return ACHIP_OK;
 }

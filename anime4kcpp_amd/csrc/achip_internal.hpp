@@ -1,0 +1,32 @@
+// Library-internal pieces shared by the device C ABI (achip.cpp) and the serving loop
+// (achip_server.cpp): the net object and unvalidated launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "achip.h"
+#include "acnet_kernels.hpp"
+
+struct ACHipNet {
+    int device;
+    float* wts;      // device FP32 weight blob, acnet_device.hpp layout
+    void* wa16;      // MFMA A operands, FP16
+    void* wabf16;    // MFMA A operands, BF16
+    float* wa32;     // FP32 MFMA A operands
+};
+
+namespace achip {
+
+// One 2x pass with an explicit configuration; arguments already validated, device current.
+hipError_t launch_pass(const ACHipNet* net, int precision, int config, int stype, int dtype, const void* src,
+                       long sstride, int w, int h, void* dst, long dstride, hipStream_t stream);
+
+// One persistent batched pass over `total` tiles described by n device descriptors;
+// counter (device int) must be zero when the kernel starts.  max_wg > 0 caps the grid.
+hipError_t launch_batch(const ACHipNet* net, int precision, int config, int stype, int dtype, const BatchDesc* desc,
+                        int n, int total, int* counter, int max_wg, hipStream_t stream);
+
+// The configuration ACHIP_AUTO resolves to for this shape (autotuner cache, else 0).
+int auto_config(const ACHipNet* net, int precision, int stype, int dtype, int w, int h);
+
+} // namespace achip

@@ -1,0 +1,792 @@
+// Serving loop of the HIP path (include/achip.h, achip_server_*).
+//
+// One scheduler thread per server.  Requests wait in arrival order until their device
+// buffers fit the HBM budget.  One-off requests then move through upload -> colour split
+// -> log2(factor) 2x passes -> chroma resize and merge.  The passes of all admitted one-offs
+// are batched STEP by STEP into one persistent launch with a tile work-queue: each step
+// takes, in admission order, tile-row ranges of the requests' current passes until the
+// step's tile budget is spent, so a big pass spreads over several steps and a request
+// admitted meanwhile joins the next step instead of waiting for the big one to finish.
+// A request contributes at most one pass to a step (its next pass reads this one's
+// output); all steps run on one stream, so stream order carries the dependencies.
+// Video-stream frames (fixed geometry per stream) take a separate path: the luma pass and
+// the two chroma resizes of a frame are captured once per stream into a hipGraph and
+// replayed on that stream's own HIP stream, overlapping the batched steps on the device.
+// Every kernel is one of the library's own (the same tiles and arithmetic as a request
+// processed alone), so results do not depend on what else is in flight.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "achip.h"
+#include "achip_internal.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+constexpr int kU8 = 0x001;
+constexpr int kMaxStreams = 64;
+constexpr int kRing = 4;              // descriptor workspaces (steps in flight <= kRing - 2)
+constexpr int kMaxStepItems = 512;    // descriptors per step
+constexpr size_t kBlock = 2u << 20;   // allocation granularity
+
+double ms_between(Clock::time_point a, Clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+size_t round_block(size_t v) { return (std::max<size_t>(v, 1) + kBlock - 1) / kBlock * kBlock; }
+
+// Caching device allocator: blocks rounded to 2 MiB, reused by exact rounded size.  Blocks
+// are handed back only after the work that used them has completed.
+class DevicePool {
+public:
+    ~DevicePool() { trim(0); }
+    hipError_t get(size_t bytes, void** out)
+    {
+        const size_t b = round_block(bytes);
+        auto it = cache_.find(b);
+        if (it != cache_.end()) {
+            *out = it->second;
+            cached_ -= b;
+            cache_.erase(it);
+            live_[*out] = b;
+            return hipSuccess;
+        }
+        hipError_t e = hipMalloc(out, b);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            trim(0);
+            e = hipMalloc(out, b);
+        }
+        if (e == hipSuccess) live_[*out] = b;
+        return e;
+    }
+    void put(void* p)
+    {
+        if (!p) return;
+        auto it = live_.find(p);
+        if (it == live_.end()) return;
+        cache_.emplace(it->second, p);
+        cached_ += it->second;
+        live_.erase(it);
+    }
+    void trim(size_t keep)
+    {
+        while (cached_ > keep && !cache_.empty()) {
+            auto it = std::prev(cache_.end());   // largest first
+            (void)hipFree(it->second);
+            cached_ -= it->first;
+            cache_.erase(it);
+        }
+    }
+    size_t cached() const { return cached_; }
+
+private:
+    std::unordered_map<void*, size_t> live_;
+    std::multimap<size_t, void*> cache_;
+    size_t cached_ = 0;
+};
+
+struct Plane {
+    void* ptr = nullptr;
+    long stride = 0;
+    int w = 0, h = 0, ch = 1;
+    size_t bytes() const { return static_cast<size_t>(stride) * h; }
+};
+
+enum class St { Pending, Uploading, Passing, Finishing, Done };
+
+struct Req {
+    unsigned long long ticket = 0;
+    ACHipRequest r{};
+    int passes = 0;
+    size_t need = 0;                  // bytes reserved against the budget
+    Clock::time_point t_submit, t_done;
+    St st = St::Pending;
+    int err = ACHIP_OK;
+    std::string msg;
+    // one-off buffers: in (RGB), luma[0..passes], uv / uvs (RGB) or u0 v0 u1 v1 (YUV420), out (RGB)
+    Plane in, luma[4], uv, uvs, u0, v0, u1, v1, out;
+    int pass = 0, next_row = 0;
+    hipEvent_t ev_up = nullptr;       // upload done
+    hipEvent_t ev_done = nullptr;     // everything done (own event, or a step's / a frame's)
+    bool own_done_event = false;
+    bool keep_output = false;         // dst all NULL: output stays until release
+    long long out_pixels = 0;
+};
+
+struct Slot {                          // one video stream
+    int w = 0, h = 0;
+    hipStream_t stream = nullptr;
+    Plane y0, u0, v0, y1, u1, v1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::deque<std::shared_ptr<Req>> inflight;
+};
+
+struct Step {
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    std::vector<std::shared_ptr<Req>> finishing;
+};
+
+double percentile(std::vector<double> v, double q)
+{
+    if (v.empty()) return 0.0;
+    const size_t k = std::min(v.size() - 1, static_cast<size_t>(q * (v.size() - 1) + 0.5));
+    std::nth_element(v.begin(), v.begin() + k, v.end());
+    return v[k];
+}
+
+} // namespace
+
+struct ACHipServer {
+    ACHipNet* net = nullptr;
+    ACHipServerConfig cfg{};
+    int config = 0, tw = 64, th = 64;
+
+    std::mutex mtx;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    std::thread thread;
+    unsigned long long next_ticket = 1;
+    std::deque<std::shared_ptr<Req>> pending;
+    std::unordered_map<unsigned long long, std::shared_ptr<Req>> tickets;
+    std::vector<std::shared_ptr<Req>> released;         // kept outputs to free (scheduler thread)
+    size_t reserved = 0;
+    int stream_w[kMaxStreams] = {}, stream_h[kMaxStreams] = {};
+    bool slot_reserved[kMaxStreams] = {};
+
+    // scheduler-thread state
+    DevicePool pool;
+    hipStream_t batch = nullptr, upload = nullptr, download = nullptr;
+    std::vector<std::shared_ptr<Req>> active;           // admitted one-offs, admission order
+    std::vector<std::shared_ptr<Req>> waiting_done;     // one-offs whose completion event is recorded
+    Slot slots[kMaxStreams];
+    Step steps[kRing];
+    int step_next = 0;
+    void* ws_dev[kRing] = {};
+    void* ws_host[kRing] = {};
+    size_t ws_bytes = 0;
+
+    // statistics (under mtx)
+    std::vector<double> lat_frame, lat_oneoff;
+    long failed = 0, nsteps = 0, replays = 0;
+    double out_mpix = 0.0;
+    bool have_first = false;
+    Clock::time_point first_submit, last_done;
+
+    void run();
+    void admit_locked(std::vector<std::shared_ptr<Req>>& frames, std::vector<std::shared_ptr<Req>>& oneoffs);
+    int start_frame(const std::shared_ptr<Req>& q);
+    int start_oneoff(const std::shared_ptr<Req>& q);
+    bool build_step();
+    void poll();
+    void complete(const std::shared_ptr<Req>& q, int err, const std::string& msg);
+    void free_req(Req& q);
+    int alloc(Plane& p, int w, int h, int ch);
+    bool busy_work() const;
+};
+
+namespace {
+
+// device bytes a request needs (frames of an existing stream need none)
+size_t request_bytes(const ACHipRequest& r, int passes)
+{
+    const size_t W = r.width, H = r.height, f = static_cast<size_t>(1) << passes;
+    size_t b = 0;
+    for (int k = 0; k <= passes; k++) b += round_block((W << k) * (H << k));
+    if (r.kind == ACHIP_REQ_RGB)
+        b += round_block(3 * W * H) + round_block(2 * W * H) + round_block(2 * f * W * f * H) + round_block(3 * f * W * f * H);
+    else if (r.kind == ACHIP_REQ_YUV420)
+        b += 2 * round_block(W / 2 * (H / 2)) + 2 * round_block(f * W / 2 * (f * H / 2));
+    return b;
+}
+
+int copy_plane(void* dst, long dstride, const void* src, long sstride, long row_bytes, int rows, hipStream_t st)
+{
+    hipError_t e;
+    if (dstride == row_bytes && sstride == row_bytes)   // dense planes: one linear copy
+        e = hipMemcpyAsync(dst, src, static_cast<size_t>(row_bytes) * rows, hipMemcpyDefault, st);
+    else
+        e = hipMemcpy2DAsync(dst, dstride, src, sstride, row_bytes, rows, hipMemcpyDefault, st);
+    return e == hipSuccess ? ACHIP_OK : ACHIP_EIO;
+}
+
+} // namespace
+
+int ACHipServer::alloc(Plane& p, int w, int h, int ch)
+{
+    p.w = w;
+    p.h = h;
+    p.ch = ch;
+    p.stride = static_cast<long>(w) * ch;
+    return pool.get(p.bytes(), &p.ptr) == hipSuccess ? ACHIP_OK : ACHIP_ENOMEM;
+}
+
+void ACHipServer::free_req(Req& q)
+{
+    for (Plane* p : { &q.in, &q.luma[0], &q.luma[1], &q.luma[2], &q.luma[3], &q.uv, &q.uvs, &q.u0, &q.v0, &q.u1, &q.v1, &q.out }) {
+        pool.put(p->ptr);
+        p->ptr = nullptr;
+    }
+    if (q.ev_up) (void)hipEventDestroy(q.ev_up);
+    if (q.own_done_event && q.ev_done) (void)hipEventDestroy(q.ev_done);
+    q.ev_up = q.ev_done = nullptr;
+    std::lock_guard<std::mutex> lk(mtx);
+    if (q.r.stream < 0) reserved -= q.need;
+    q.need = 0;
+}
+
+void ACHipServer::complete(const std::shared_ptr<Req>& q, int err, const std::string& msg)
+{
+    const auto now = Clock::now();
+    const bool frame = q->r.stream >= 0;
+    {
+        std::lock_guard<std::mutex> lk(mtx);
+        q->t_done = now;
+        q->err = err;
+        q->msg = msg;
+        q->st = St::Done;
+        last_done = now;
+        if (err) failed++;
+        else {
+            (frame ? lat_frame : lat_oneoff).push_back(ms_between(q->t_submit, now));
+            out_mpix += q->out_pixels * 1e-6;
+        }
+    }
+    if (frame || !q->keep_output || err) {
+        // frames keep no buffers of their own; one-offs copied out (or failed) free theirs
+        if (!frame) free_req(*q);
+        else if (q->own_done_event && q->ev_done) {
+            (void)hipEventDestroy(q->ev_done);
+            q->ev_done = nullptr;
+        }
+    }
+    cv_done.notify_all();
+}
+
+void ACHipServer::admit_locked(std::vector<std::shared_ptr<Req>>& frames, std::vector<std::shared_ptr<Req>>& oneoffs)
+{
+    bool blocked = false;    // one-offs are admitted strictly in order; frames may pass a blocked one
+    for (auto it = pending.begin(); it != pending.end();) {
+        const std::shared_ptr<Req>& q = *it;
+        if (q->r.stream >= 0) {
+            const size_t need = slot_reserved[q->r.stream] ? 0 : q->need;
+            if (need == 0 || reserved + need <= cfg.hbm_budget) {
+                reserved += need;
+                slot_reserved[q->r.stream] = true;
+                frames.push_back(q);
+                it = pending.erase(it);
+                continue;
+            }
+        } else if (!blocked) {
+            if (reserved + q->need <= cfg.hbm_budget) {
+                reserved += q->need;
+                oneoffs.push_back(q);
+                it = pending.erase(it);
+                continue;
+            }
+            blocked = true;
+        }
+        ++it;
+    }
+}
+
+int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
+{
+    const ACHipRequest& r = q->r;
+    Slot& s = slots[r.stream];
+    if (!s.w) {    // first frame of this stream: buffers, stream and graph
+        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return ACHIP_EIO;
+        int e = alloc(s.y0, r.width, r.height, 1);
+        if (!e) e = alloc(s.u0, r.width / 2, r.height / 2, 1);
+        if (!e) e = alloc(s.v0, r.width / 2, r.height / 2, 1);
+        if (!e) e = alloc(s.y1, 2 * r.width, 2 * r.height, 1);
+        if (!e) e = alloc(s.u1, r.width, r.height, 1);
+        if (!e) e = alloc(s.v1, r.width, r.height, 1);
+        if (e) return e;
+        s.w = r.width;
+        s.h = r.height;
+        const int pcfg = achip::auto_config(net, cfg.precision, kU8, kU8, s.w, s.h);
+        if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ACHIP_EIO;
+        hipError_t he = achip::launch_pass(net, cfg.precision, pcfg, kU8, kU8, s.y0.ptr, s.y0.stride, s.w, s.h, s.y1.ptr,
+                                           s.y1.stride, s.stream);
+        if (he == hipSuccess)
+            he = achip::launch_resize(kU8, 1, 0, s.u0.ptr, s.u0.stride, s.u0.w, s.u0.h, s.u1.ptr, s.u1.stride, s.u1.w, s.u1.h, s.stream);
+        if (he == hipSuccess)
+            he = achip::launch_resize(kU8, 1, 0, s.v0.ptr, s.v0.stride, s.v0.w, s.v0.h, s.v1.ptr, s.v1.stride, s.v1.w, s.v1.h, s.stream);
+        const hipError_t ce = hipStreamEndCapture(s.stream, &s.graph);
+        if (he != hipSuccess || ce != hipSuccess) return ACHIP_EIO;
+        if (hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0) != hipSuccess) return ACHIP_EIO;
+    }
+    int e = copy_plane(s.y0.ptr, s.y0.stride, r.src[0], r.src_stride[0], s.w, s.h, s.stream);
+    if (!e) e = copy_plane(s.u0.ptr, s.u0.stride, r.src[1], r.src_stride[1], s.w / 2, s.h / 2, s.stream);
+    if (!e) e = copy_plane(s.v0.ptr, s.v0.stride, r.src[2], r.src_stride[2], s.w / 2, s.h / 2, s.stream);
+    if (e) return e;
+    if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return ACHIP_EIO;
+    {
+        std::lock_guard<std::mutex> lk(mtx);
+        replays++;
+    }
+    if (r.dst[0]) {
+        e = copy_plane(r.dst[0], r.dst_stride[0], s.y1.ptr, s.y1.stride, s.y1.w, s.y1.h, s.stream);
+        if (!e) e = copy_plane(r.dst[1], r.dst_stride[1], s.u1.ptr, s.u1.stride, s.u1.w, s.u1.h, s.stream);
+        if (!e) e = copy_plane(r.dst[2], r.dst_stride[2], s.v1.ptr, s.v1.stride, s.v1.w, s.v1.h, s.stream);
+        if (e) return e;
+    }
+    if (hipEventCreateWithFlags(&q->ev_done, hipEventDisableTiming) != hipSuccess) return ACHIP_EIO;
+    q->own_done_event = true;
+    if (hipEventRecord(q->ev_done, s.stream) != hipSuccess) return ACHIP_EIO;
+    q->out_pixels = 4LL * s.w * s.h;
+    s.inflight.push_back(q);
+    return ACHIP_OK;
+}
+
+int ACHipServer::start_oneoff(const std::shared_ptr<Req>& q)
+{
+    const ACHipRequest& r = q->r;
+    const int W = r.width, H = r.height, P = q->passes, f = 1 << P;
+    int e = ACHIP_OK;
+    for (int k = 0; k <= P && !e; k++) e = alloc(q->luma[k], W << k, H << k, 1);
+    if (!e && r.kind == ACHIP_REQ_RGB) {
+        e = alloc(q->in, W, H, 3);
+        if (!e) e = alloc(q->uv, W, H, 2);
+        if (!e) e = alloc(q->uvs, f * W, f * H, 2);
+        if (!e) e = alloc(q->out, f * W, f * H, 3);
+    } else if (!e && r.kind == ACHIP_REQ_YUV420) {
+        e = alloc(q->u0, W / 2, H / 2, 1);
+        if (!e) e = alloc(q->v0, W / 2, H / 2, 1);
+        if (!e) e = alloc(q->u1, f * W / 2, f * H / 2, 1);
+        if (!e) e = alloc(q->v1, f * W / 2, f * H / 2, 1);
+    }
+    if (e) return e;
+    if (r.kind == ACHIP_REQ_RGB) e = copy_plane(q->in.ptr, q->in.stride, r.src[0], r.src_stride[0], 3L * W, H, upload);
+    else e = copy_plane(q->luma[0].ptr, q->luma[0].stride, r.src[0], r.src_stride[0], W, H, upload);
+    if (!e && r.kind == ACHIP_REQ_YUV420) {
+        e = copy_plane(q->u0.ptr, q->u0.stride, r.src[1], r.src_stride[1], W / 2, H / 2, upload);
+        if (!e) e = copy_plane(q->v0.ptr, q->v0.stride, r.src[2], r.src_stride[2], W / 2, H / 2, upload);
+    }
+    if (e) return e;
+    if (hipEventCreateWithFlags(&q->ev_up, hipEventDisableTiming) != hipSuccess) return ACHIP_EIO;
+    if (hipEventRecord(q->ev_up, upload) != hipSuccess) return ACHIP_EIO;
+    q->out_pixels = static_cast<long long>(f) * W * f * H;
+    q->st = St::Uploading;
+    active.push_back(q);
+    return ACHIP_OK;
+}
+
+// Gather tile-row ranges of the admitted one-offs' current passes into one batched launch,
+// then the finishing kernels of the requests whose last pass this step completes.
+bool ACHipServer::build_step()
+{
+    Step& step = steps[step_next];
+    if (step.busy) return false;
+    // at most kRing - 2 steps in flight: the one before must not be two steps behind
+    int inflight = 0;
+    for (const Step& s : steps) inflight += s.busy ? 1 : 0;
+    if (inflight >= kRing - 2) return false;
+
+    auto* hdr = static_cast<char*>(ws_host[step_next]);
+    auto* desc = reinterpret_cast<achip::BatchDesc*>(hdr + 256);
+    int n = 0;
+    long total = 0;
+    long budget = cfg.step_tiles;
+    std::vector<std::shared_ptr<Req>> finishing, members;
+    for (const std::shared_ptr<Req>& q : active) {
+        if (budget <= 0 || n >= kMaxStepItems) break;
+        if (q->st != St::Passing) continue;
+        members.push_back(q);
+        const Plane& a = q->luma[q->pass];
+        const Plane& b = q->luma[q->pass + 1];
+        const int tiles_x = (a.w + tw - 1) / tw, tiles_y = (a.h + th - 1) / th;
+        const int rows = std::min<long>(tiles_y - q->next_row, std::max<long>(1, budget / tiles_x));
+        desc[n++] = achip::BatchDesc{ a.ptr, b.ptr, a.stride, b.stride, a.w, a.h, tiles_x, static_cast<int>(total), q->next_row };
+        total += static_cast<long>(rows) * tiles_x;
+        budget -= static_cast<long>(rows) * tiles_x;
+        q->next_row += rows;
+        if (q->next_row == tiles_y) {
+            q->pass++;
+            q->next_row = 0;
+            if (q->pass == q->passes) {
+                q->st = St::Finishing;
+                finishing.push_back(q);
+            }
+        }
+    }
+    if (n == 0) return false;
+    std::memset(hdr, 0, 256);   // tile counter
+    const size_t bytes = 256 + static_cast<size_t>(n) * sizeof(achip::BatchDesc);
+    hipError_t e = hipMemcpyAsync(ws_dev[step_next], hdr, bytes, hipMemcpyHostToDevice, batch);
+    if (e == hipSuccess)
+        e = achip::launch_batch(net, cfg.precision, config, kU8, kU8,
+                                reinterpret_cast<const achip::BatchDesc*>(static_cast<char*>(ws_dev[step_next]) + 256), n,
+                                static_cast<int>(total), static_cast<int*>(ws_dev[step_next]), cfg.max_workgroups, batch);
+    for (const std::shared_ptr<Req>& q : finishing) {
+        if (e != hipSuccess) break;
+        const int P = q->passes;
+        const Plane& y = q->luma[P];
+        if (q->r.kind == ACHIP_REQ_RGB) {
+            e = achip::launch_resize(kU8, 2, 1, q->uv.ptr, q->uv.stride, q->uv.w, q->uv.h, q->uvs.ptr, q->uvs.stride, q->uvs.w,
+                                     q->uvs.h, batch);
+            if (e == hipSuccess)
+                e = achip::launch_color_merge(kU8, 3, y.ptr, y.stride, q->uvs.ptr, q->uvs.stride, y.w, y.h, q->out.ptr,
+                                              q->out.stride, batch);
+        } else if (q->r.kind == ACHIP_REQ_YUV420) {
+            e = achip::launch_resize(kU8, 1, 0, q->u0.ptr, q->u0.stride, q->u0.w, q->u0.h, q->u1.ptr, q->u1.stride, q->u1.w,
+                                     q->u1.h, batch);
+            if (e == hipSuccess)
+                e = achip::launch_resize(kU8, 1, 0, q->v0.ptr, q->v0.stride, q->v0.w, q->v0.h, q->v1.ptr, q->v1.stride,
+                                         q->v1.w, q->v1.h, batch);
+        }
+    }
+    if (e == hipSuccess) e = hipEventRecord(step.ev, batch);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(batch);   // nothing of this step may still use the buffers
+        for (const std::shared_ptr<Req>& q : members) complete(q, ACHIP_EIO, "step launch failed");
+        active.erase(std::remove_if(active.begin(), active.end(), [](const std::shared_ptr<Req>& q) { return q->st == St::Done; }),
+                     active.end());
+        return false;
+    }
+    step.busy = true;
+    step.finishing = std::move(finishing);
+    step_next = (step_next + 1) % kRing;
+    std::lock_guard<std::mutex> lk(mtx);
+    nsteps++;
+    return true;
+}
+
+void ACHipServer::poll()
+{
+    // uploads finished: colour split (RGB) and into the pass phase
+    for (const std::shared_ptr<Req>& q : active) {
+        if (q->st != St::Uploading || hipEventQuery(q->ev_up) != hipSuccess) continue;
+        if (q->r.kind == ACHIP_REQ_RGB &&
+            achip::launch_color_split(kU8, 3, q->in.ptr, q->in.stride, q->in.w, q->in.h, q->luma[0].ptr, q->luma[0].stride,
+                                      q->uv.ptr, q->uv.stride, batch) != hipSuccess) {
+            complete(q, ACHIP_EIO, "colour split launch failed");
+            continue;
+        }
+        q->st = St::Passing;
+    }
+    // steps finished: their finishing requests are done (or copy out)
+    for (Step& s : steps) {
+        if (!s.busy || hipEventQuery(s.ev) != hipSuccess) continue;
+        s.busy = false;
+        for (const std::shared_ptr<Req>& q : s.finishing) {
+            const ACHipRequest& r = q->r;
+            if (!r.dst[0]) {
+                q->keep_output = true;
+                complete(q, ACHIP_OK, "");
+                continue;
+            }
+            int e = ACHIP_OK;
+            const Plane& y = q->luma[q->passes];
+            if (r.kind == ACHIP_REQ_RGB) e = copy_plane(r.dst[0], r.dst_stride[0], q->out.ptr, q->out.stride, 3L * q->out.w, q->out.h, download);
+            else e = copy_plane(r.dst[0], r.dst_stride[0], y.ptr, y.stride, y.w, y.h, download);
+            if (!e && r.kind == ACHIP_REQ_YUV420) {
+                e = copy_plane(r.dst[1], r.dst_stride[1], q->u1.ptr, q->u1.stride, q->u1.w, q->u1.h, download);
+                if (!e) e = copy_plane(r.dst[2], r.dst_stride[2], q->v1.ptr, q->v1.stride, q->v1.w, q->v1.h, download);
+            }
+            if (!e && hipEventCreateWithFlags(&q->ev_done, hipEventDisableTiming) != hipSuccess) e = ACHIP_EIO;
+            if (!e) {
+                q->own_done_event = true;
+                if (hipEventRecord(q->ev_done, download) != hipSuccess) e = ACHIP_EIO;
+            }
+            if (e) complete(q, e, "output copy failed");
+            else waiting_done.push_back(q);
+        }
+        s.finishing.clear();
+    }
+    // copies out finished
+    for (auto it = waiting_done.begin(); it != waiting_done.end();) {
+        if (hipEventQuery((*it)->ev_done) == hipSuccess) {
+            complete(*it, ACHIP_OK, "");
+            it = waiting_done.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    active.erase(std::remove_if(active.begin(), active.end(), [](const std::shared_ptr<Req>& q) { return q->st == St::Done; }),
+                 active.end());
+    // frames, in order per stream
+    for (Slot& s : slots) {
+        while (!s.inflight.empty() && hipEventQuery(s.inflight.front()->ev_done) == hipSuccess) {
+            complete(s.inflight.front(), ACHIP_OK, "");
+            s.inflight.pop_front();
+        }
+    }
+}
+
+bool ACHipServer::busy_work() const
+{
+    if (!active.empty() || !waiting_done.empty()) return true;
+    for (const Step& s : steps)
+        if (s.busy) return true;
+    for (const Slot& s : slots)
+        if (!s.inflight.empty()) return true;
+    return false;
+}
+
+void ACHipServer::run()
+{
+    (void)hipSetDevice(net->device);
+    for (;;) {
+        std::vector<std::shared_ptr<Req>> frames, oneoffs, freed, refused;
+        {
+            std::unique_lock<std::mutex> lk(mtx);
+            if (!busy_work() && pending.empty() && released.empty()) {
+                if (stop) break;
+                cv_work.wait(lk, [&] { return stop || !pending.empty() || !released.empty(); });
+            }
+            freed.swap(released);
+            admit_locked(frames, oneoffs);
+            // shutting down with nothing in flight: what still cannot be admitted never will be
+            if (stop && frames.empty() && oneoffs.empty() && freed.empty() && !busy_work()) {
+                refused.assign(pending.begin(), pending.end());
+                pending.clear();
+            }
+        }
+        for (const std::shared_ptr<Req>& q : freed) free_req(*q);
+        for (const std::shared_ptr<Req>& q : refused) complete(q, ACHIP_ENOMEM, "not admitted before shutdown");
+        for (const std::shared_ptr<Req>& q : frames) {
+            const int e = start_frame(q);
+            if (e) complete(q, e, "frame launch failed");
+        }
+        for (const std::shared_ptr<Req>& q : oneoffs) {
+            const int e = start_oneoff(q);
+            if (e) complete(q, e, "request set-up failed");
+        }
+        poll();
+        bool launched = false;
+        while (build_step()) launched = true;
+        if (!launched && busy_work()) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+extern "C" {
+
+int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipServer** out)
+{
+    if (!net || !out) return ACHIP_EINVAL;
+    *out = nullptr;
+    ACHipServerConfig c = config ? *config : ACHipServerConfig{};
+    if (c.precision != ACHIP_FP32 && c.precision != ACHIP_FP16 && c.precision != ACHIP_BF16) return ACHIP_EINVAL;
+    if (c.hbm_budget == 0) c.hbm_budget = static_cast<size_t>(16) << 30;
+    if (c.step_tiles <= 0) c.step_tiles = c.precision == ACHIP_FP32 ? 2048 : 6144;
+    if (c.max_workgroups < 0) return ACHIP_EINVAL;
+    auto s = std::make_unique<ACHipServer>();
+    s->net = net;
+    s->cfg = c;
+    s->config = 0;
+    if (achip_config_info(c.precision, s->config, &s->tw, &s->th, nullptr, nullptr, nullptr) != ACHIP_OK) return ACHIP_EINVAL;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return ACHIP_EIO;
+    if (hipSetDevice(net->device) != hipSuccess) return ACHIP_ENODEV;
+    bool ok = hipStreamCreateWithFlags(&s->batch, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&s->download, hipStreamNonBlocking) == hipSuccess;
+    s->ws_bytes = 256 + static_cast<size_t>(kMaxStepItems) * sizeof(achip::BatchDesc);
+    for (int i = 0; i < kRing && ok; i++) {
+        ok = hipMalloc(&s->ws_dev[i], s->ws_bytes) == hipSuccess && hipHostMalloc(&s->ws_host[i], s->ws_bytes, 0) == hipSuccess &&
+             hipEventCreateWithFlags(&s->steps[i].ev, hipEventDisableTiming) == hipSuccess;
+    }
+    (void)hipSetDevice(dev);
+    if (!ok) {
+        (void)hipGetLastError();
+        achip_server_destroy(s.release());
+        return ACHIP_EIO;
+    }
+    ACHipServer* raw = s.release();
+    raw->thread = std::thread([raw] { raw->run(); });
+    *out = raw;
+    return ACHIP_OK;
+}
+
+void achip_server_destroy(ACHipServer* s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mtx);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->thread.joinable()) s->thread.join();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipSetDevice(s->net->device);
+    for (auto* st : { s->batch, s->upload, s->download })
+        if (st) (void)hipStreamSynchronize(st);
+    for (Slot& sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
+        if (sl.graph) (void)hipGraphDestroy(sl.graph);
+        for (Plane* p : { &sl.y0, &sl.u0, &sl.v0, &sl.y1, &sl.u1, &sl.v1 }) s->pool.put(p->ptr);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    for (auto& kv : s->tickets) s->free_req(*kv.second);
+    s->tickets.clear();
+    for (int i = 0; i < kRing; i++) {
+        if (s->ws_dev[i]) (void)hipFree(s->ws_dev[i]);
+        if (s->ws_host[i]) (void)hipHostFree(s->ws_host[i]);
+        if (s->steps[i].ev) (void)hipEventDestroy(s->steps[i].ev);
+    }
+    for (auto* st : { s->batch, s->upload, s->download })
+        if (st) (void)hipStreamDestroy(st);
+    s->pool.trim(0);
+    (void)hipSetDevice(dev);
+    delete s;
+}
+
+int achip_server_submit(ACHipServer* s, const ACHipRequest* r, unsigned long long* ticket)
+{
+    if (!s || !r || !ticket) return ACHIP_EINVAL;
+    if (r->kind < ACHIP_REQ_LUMA || r->kind > ACHIP_REQ_RGB) return ACHIP_EINVAL;
+    if (r->width <= 0 || r->height <= 0 || r->width > (1 << 15) || r->height > (1 << 15)) return ACHIP_EINVAL;
+    int passes = 0;
+    switch (r->factor) {
+    case 2: passes = 1; break;
+    case 4: passes = 2; break;
+    case 8: passes = 3; break;
+    default: return ACHIP_EINVAL;
+    }
+    if ((static_cast<long>(r->width) << passes) > (1 << 16) || (static_cast<long>(r->height) << passes) > (1 << 16))
+        return ACHIP_EINVAL;
+    const int planes = r->kind == ACHIP_REQ_YUV420 ? 3 : 1;
+    const int ch = r->kind == ACHIP_REQ_RGB ? 3 : 1;
+    if (r->kind == ACHIP_REQ_YUV420 && ((r->width | r->height) & 1)) return ACHIP_EINVAL;
+    for (int p = 0; p < planes; p++) {
+        const long row = p == 0 ? static_cast<long>(r->width) * ch : r->width / 2;
+        if (!r->src[p] || r->src_stride[p] < row) return ACHIP_EINVAL;
+    }
+    const bool has_dst = r->dst[0] != nullptr;
+    for (int p = 0; p < planes; p++) {
+        const long row = (p == 0 ? static_cast<long>(r->width) * ch : r->width / 2) << passes;
+        if (has_dst != (r->dst[p] != nullptr)) return ACHIP_EINVAL;
+        if (has_dst && r->dst_stride[p] < row) return ACHIP_EINVAL;
+    }
+    if (r->stream >= kMaxStreams || r->stream < -1) return ACHIP_EINVAL;
+    if (r->stream >= 0 && (r->kind != ACHIP_REQ_YUV420 || r->factor != 2)) return ACHIP_EINVAL;
+    auto q = std::make_shared<Req>();
+    q->r = *r;
+    q->passes = passes;
+    q->need = r->stream >= 0 ? request_bytes(*r, 1) : request_bytes(*r, passes);
+    if (q->need > s->cfg.hbm_budget) return ACHIP_ENOMEM;
+    std::lock_guard<std::mutex> lk(s->mtx);
+    if (s->stop) return ACHIP_EIO;
+    if (r->stream >= 0) {   // one geometry per stream
+        int& sw = s->stream_w[r->stream];
+        int& sh = s->stream_h[r->stream];
+        if (sw && (sw != r->width || sh != r->height)) return ACHIP_EINVAL;
+        sw = r->width;
+        sh = r->height;
+    }
+    q->ticket = s->next_ticket++;
+    q->t_submit = Clock::now();
+    if (!s->have_first) {
+        s->have_first = true;
+        s->first_submit = q->t_submit;
+    }
+    s->tickets[q->ticket] = q;
+    s->pending.push_back(q);
+    *ticket = q->ticket;
+    s->cv_work.notify_one();
+    return ACHIP_OK;
+}
+
+int achip_server_wait(ACHipServer* s, unsigned long long ticket, double timeout_ms, double* latency_ms)
+{
+    if (!s) return ACHIP_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mtx);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return ACHIP_EINVAL;
+    std::shared_ptr<Req> q = it->second;
+    auto done = [&] { return q->st == St::Done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, done);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done)) return ACHIP_EIO;
+    if (latency_ms) *latency_ms = ms_between(q->t_submit, q->t_done);
+    const int err = q->err;
+    if (!q->keep_output) s->tickets.erase(q->ticket);   // nothing left to read
+    return err;
+}
+
+int achip_server_output(ACHipServer* s, unsigned long long ticket, int plane, void** ptr, long* stride)
+{
+    if (!s || !ptr || !stride) return ACHIP_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mtx);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return ACHIP_EINVAL;
+    const Req& q = *it->second;
+    if (q.st != St::Done || !q.keep_output || q.err) return ACHIP_EINVAL;
+    const Plane* p = nullptr;
+    if (q.r.kind == ACHIP_REQ_RGB) p = plane == 0 ? &q.out : nullptr;
+    else if (q.r.kind == ACHIP_REQ_LUMA) p = plane == 0 ? &q.luma[q.passes] : nullptr;
+    else p = plane == 0 ? &q.luma[q.passes] : plane == 1 ? &q.u1 : plane == 2 ? &q.v1 : nullptr;
+    if (!p) return ACHIP_EINVAL;
+    *ptr = p->ptr;
+    *stride = p->stride;
+    return ACHIP_OK;
+}
+
+int achip_server_release(ACHipServer* s, unsigned long long ticket)
+{
+    if (!s) return ACHIP_EINVAL;
+    std::shared_ptr<Req> q;
+    {
+        std::lock_guard<std::mutex> lk(s->mtx);
+        auto it = s->tickets.find(ticket);
+        if (it == s->tickets.end()) return ACHIP_EINVAL;
+        if (it->second->st != St::Done) return ACHIP_EINVAL;
+        q = it->second;
+        s->tickets.erase(it);
+    }
+    // buffers go back to the pool on the scheduler thread, which owns it
+    if (q->keep_output) {
+        std::lock_guard<std::mutex> lk(s->mtx);
+        s->released.push_back(q);
+    }
+    s->cv_work.notify_one();
+    return ACHIP_OK;
+}
+
+int achip_server_stats(ACHipServer* s, ACHipServerStats* out, int reset)
+{
+    if (!s || !out) return ACHIP_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mtx);
+    out->completed_frames = static_cast<long>(s->lat_frame.size());
+    out->completed_oneoffs = static_cast<long>(s->lat_oneoff.size());
+    out->failed = s->failed;
+    out->frame_p50_ms = percentile(s->lat_frame, 0.50);
+    out->frame_p99_ms = percentile(s->lat_frame, 0.99);
+    out->oneoff_p50_ms = percentile(s->lat_oneoff, 0.50);
+    out->oneoff_p99_ms = percentile(s->lat_oneoff, 0.99);
+    out->output_mpix = s->out_mpix;
+    out->busy_s = s->have_first ? std::chrono::duration<double>(s->last_done - s->first_submit).count() : 0.0;
+    out->steps = s->nsteps;
+    out->graph_replays = s->replays;
+    if (reset) {
+        s->lat_frame.clear();
+        s->lat_oneoff.clear();
+        s->failed = 0;
+        s->out_mpix = 0.0;
+        s->nsteps = 0;
+        s->replays = 0;
+        s->have_first = false;
+    }
+    return ACHIP_OK;
+}
+
+} // extern "C"

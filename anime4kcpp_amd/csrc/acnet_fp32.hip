@@ -283,12 +283,13 @@ __global__ __launch_bounds__(NT) void acnet_fp32_batch(const BatchDesc* __restri
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
         __syncthreads();
-        const int t = s_tile;
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: keeps the tile state scalar
         if (t >= total) break;
         const BatchDesc d = find_tile(desc, n, t);
         const int local = t - d.first_tile;
-        const int ty = local / d.tiles_x;
-        const int tx = local - ty * d.tiles_x;
+        const int ly = local / d.tiles_x;
+        const int tx = local - ly * d.tiles_x;
+        const int ty = d.ty0 + ly;
         fp32_tile<TIN, TOUT, TW, TH, NT, P>(d.src, d.sstride, d.w, d.h, d.dst, d.dstride, wts, zero,
                                             tx * TW, ty * TH, fa, fb, img);
         __syncthreads();
@@ -308,11 +309,11 @@ static hipError_t launch_one(const void* src, long ss, int W, int H, void* dst, 
 }
 
 template <typename TIN, typename TOUT, int TW, int TH, int NT, int P>
-static hipError_t launch_batch_one(const BatchDesc* desc, int n, int total, int* counter, const float* wts,
+static hipError_t launch_batch_one(const BatchDesc* desc, int n, int total, int* counter, const float* wts, int max_wg,
                                    hipStream_t stream)
 {
     auto kern = acnet_fp32_batch<TIN, TOUT, TW, TH, NT, P>;
-    const int grid = persistent_grid(reinterpret_cast<const void*>(kern), NT, total);
+    const int grid = persistent_grid(reinterpret_cast<const void*>(kern), NT, total, max_wg);
     if (grid <= 0) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, stream, desc, n, total, counter, wts, 0);
     return hipGetLastError();
@@ -320,14 +321,14 @@ static hipError_t launch_batch_one(const BatchDesc* desc, int n, int total, int*
 
 template <int TW, int TH, int NT, int P>
 static hipError_t launch_batch_types(int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
-                                     const float* wts, hipStream_t stream)
+                                     const float* wts, int max_wg, hipStream_t stream)
 {
 #define ACHIP_OUT(TIN)                                                                                          \
     switch (dtype) {                                                                                            \
-    case U8: return launch_batch_one<TIN, uint8_t, TW, TH, NT, P>(desc, n, total, counter, wts, stream);        \
-    case U16: return launch_batch_one<TIN, uint16_t, TW, TH, NT, P>(desc, n, total, counter, wts, stream);      \
-    case F16: return launch_batch_one<TIN, __half, TW, TH, NT, P>(desc, n, total, counter, wts, stream);        \
-    case F32: return launch_batch_one<TIN, float, TW, TH, NT, P>(desc, n, total, counter, wts, stream);         \
+    case U8: return launch_batch_one<TIN, uint8_t, TW, TH, NT, P>(desc, n, total, counter, wts, max_wg, stream);        \
+    case U16: return launch_batch_one<TIN, uint16_t, TW, TH, NT, P>(desc, n, total, counter, wts, max_wg, stream);      \
+    case F16: return launch_batch_one<TIN, __half, TW, TH, NT, P>(desc, n, total, counter, wts, max_wg, stream);        \
+    case F32: return launch_batch_one<TIN, float, TW, TH, NT, P>(desc, n, total, counter, wts, max_wg, stream);         \
     default: return hipErrorInvalidValue;                                                                       \
     }
     switch (stype) {
@@ -369,11 +370,11 @@ const Fp32Config kFp32Configs[] = {
 const int kFp32ConfigCount = sizeof(kFp32Configs) / sizeof(kFp32Configs[0]);
 
 hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
-                             const float* wts, hipStream_t stream)
+                             const float* wts, int max_wg, hipStream_t stream)
 {
     switch (cfg) {
-    case 0: return launch_batch_types<64, 32, 512, 4>(stype, dtype, desc, n, total, counter, wts, stream);
-    case 1: return launch_batch_types<32, 32, 256, 4>(stype, dtype, desc, n, total, counter, wts, stream);
+    case 0: return launch_batch_types<64, 32, 512, 4>(stype, dtype, desc, n, total, counter, wts, max_wg, stream);
+    case 1: return launch_batch_types<32, 32, 256, 4>(stype, dtype, desc, n, total, counter, wts, max_wg, stream);
     default: return hipErrorInvalidValue;
     }
 }

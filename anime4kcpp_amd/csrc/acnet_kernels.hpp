@@ -6,13 +6,16 @@
 namespace achip {
 
 // One image of a batched launch (device memory).  Tiles of the batch are numbered image by
-// image: image i owns tiles [first_tile, first_tile + tiles_x * tiles_y).
+// image: image i owns tiles [first_tile, first_tile + tiles_x * rows), the tile rows ty0,
+// ty0 + 1, ... of the image (a launch may cover part of an image's rows; borders are
+// those of the whole w x h image).
 struct BatchDesc {
     const void* src;
     void* dst;
     long sstride, dstride;
     int w, h;
     int tiles_x, first_tile;
+    int ty0;
 };
 
 // The image that owns tile t (binary search over first_tile; desc sorted by it).
@@ -27,8 +30,9 @@ __device__ inline BatchDesc find_tile(const BatchDesc* __restrict__ desc, int n,
     return desc[lo];
 }
 
-// Workgroups of a persistent launch: resident workgroups per CU x CUs, at most `tiles`.
-int persistent_grid(const void* kernel, int threads, int tiles);
+// Workgroups of a persistent launch: resident workgroups per CU x CUs, at most `tiles`, and
+// at most max_wg when max_wg > 0 (leaves CUs to work on other streams).
+int persistent_grid(const void* kernel, int threads, int tiles, int max_wg);
 
 // One tile shape of the fused FP32 kernel: TW x TH input-resolution pixels per
 // workgroup, NT threads, P-pixel vertical runs per thread.
@@ -59,7 +63,7 @@ extern const int kMfma32LdsBytes;
 hipError_t launch_mfma32(int stype, int dtype, const void* src, long sstride, int W, int H,
                          void* dst, long dstride, const float* wts, const float* wa32, hipStream_t stream);
 hipError_t launch_mfma32_batch(int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
-                               const float* wts, const float* wa32, hipStream_t stream);
+                               const float* wts, const float* wa32, int max_wg, hipStream_t stream);
 
 // Colour split RGB(A) -> Y + UV(A), merge back, and resize (mode 0 bilinear, 1 Catmull-Rom);
 // interleaved channels, one element type, byte strides (acnet_color.hip).
@@ -72,8 +76,9 @@ hipError_t launch_resize(int type, int channels, int mode, const void* src, long
 
 // Persistent batched launches: `counter` (device int, zero at launch) is the tile queue.
 hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
-                             const float* wts, hipStream_t stream);
+                             const float* wts, int max_wg, hipStream_t stream);
 hipError_t launch_mfma_batch(int bf16, int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total,
-                             int* counter, const float* wts, const void* wa, const float* wa0, hipStream_t stream);
+                             int* counter, const float* wts, const void* wa, const float* wa0, int max_wg,
+                             hipStream_t stream);
 
 } // namespace achip

@@ -76,11 +76,15 @@ struct MfmaTile {
     // row past a layer's region.
     static constexpr int PW = 8 + 16 * kPh + 1;
     static constexpr int PH = BH + 1;
-    // f32 input tile, origin (x0 - 9, y0 - 9): plane-local (r, c) is tile (r + 1, c + 1)
+    // input tile in the source element type, origin (x0 - 9, y0 - 9): plane-local (r, c) is
+    // tile (r + 1, c + 1)
     static constexpr int IW = PW + 1, IH = TH + 18;
+    static constexpr int kStage = (IW * IH + NT - 1) / NT;          // input elements per thread
     static constexpr int kMaxPairs = ((BH + 1) / 2 + NW - 1) / NW;  // row pairs per wave and layer
-    static constexpr int lds_bytes = PW * PH * 16 + IW * IH * 4;
+    static constexpr int plane_bytes = PW * PH * 16;
+    static constexpr int lds_bytes(int elem) { return plane_bytes + (IW * IH * elem + 15) / 16 * 16; }
     static_assert(16 * kSeg8 + 8 <= PW - 1, "layer-8 reads stay in the plane");
+    static_assert(kPh % 2 == 1, "an odd unit stride keeps 16-lane LDS accesses conflict-free");
 };
 
 // Replicate ring.  After a layer of margin m has written its in-image region, copy the
@@ -124,7 +128,7 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
                                           void* __restrict__ dst, long dstride,
                                           const float* __restrict__ wts, const uint4* __restrict__ wa,
                                           const float* __restrict__ wa0,
-                                          int x0, int y0, uint4* __restrict__ plane, float* __restrict__ img)
+                                          int x0, int y0, uint4* __restrict__ plane, TIN* __restrict__ img)
 {
     using C = MfmaTile<TW, TH, NT>;
     using M = Mma<T16>;
@@ -145,25 +149,42 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
     if (border)
         for (int i = tid; i < C::PW * C::PH; i += NT) plane[i] = make_uint4(0u, 0u, 0u, 0u);
 
-    // ---- stage the input tile over its whole extent: toFloat, replicate clamp ----
+    // ---- stage the input tile over its whole extent (replicate clamp), all loads in
+    // flight before the first LDS store ----
     {
         constexpr int DX = NT % C::IW, DY = NT / C::IW;
         int iy = tid / C::IW, ix = tid - (tid / C::IW) * C::IW;
-        for (int i = tid; i < C::IW * C::IH; i += NT) {
-            const int gy = clampi(y0 - 9 + iy, 0, H - 1);
-            const int gx = clampi(x0 - 9 + ix, 0, W - 1);
-            img[i] = to_float<TIN>(row_ptr<TIN>(src, sstride, gy)[gx]);
+        TIN v[C::kStage];
+#pragma unroll
+        for (int k = 0; k < C::kStage; k++) {
+            if (k < C::kStage - 1 || tid + k * NT < C::IW * C::IH) {
+                const int gy = clampi(y0 - 9 + iy, 0, H - 1);
+                const int gx = clampi(x0 - 9 + ix, 0, W - 1);
+                v[k] = row_ptr<TIN>(src, sstride, gy)[gx];
+            }
             ix += DX;
             iy += DY;
             if (ix >= C::IW) { ix -= C::IW; iy++; }
         }
+#pragma unroll
+        for (int k = 0; k < C::kStage; k++)
+            if (k < C::kStage - 1 || tid + k * NT < C::IW * C::IH) img[tid + k * NT] = v[k];
     }
     __syncthreads();
 
     // ---- layers 0..7 on units, in place ----
+    // The next layer's A operands and bias are loaded at the end of each compute phase, so
+    // that their latency hides behind the barrier and the write-back.
     const int lane_px = g * C::PW + P * j;                  // B read (pixels), plane
     const int lane_im = g * C::IW + P * j;                  // B read, input tile
     const int lane_wr = (ro * C::PW + P * j) * 2 + half;    // write-back, 8-byte units
+    const float a0f = wa0[lane], a1f = wa0[64 + lane], a2f = wa0[128 + lane];
+    v8 a0 = {}, a1 = {}, a2 = {};
+    f32x4 bias;
+    {
+        const float* bl = wts + kOffB0 + 4 * half;
+        bias = (f32x4){ bl[0], bl[1], bl[2], bl[3] };
+    }
 #pragma nounroll
     for (int l = 0; l <= 7; l++) {
         const int m = 8 - l;
@@ -171,33 +192,26 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
         const int gx1 = min(x0 + TW + m, W), gy1 = min(y0 + TH + m, H);
         const int lx0 = gx0 - bx, ly0 = gy0 - by;
         const int npairs = (gy1 - gy0 + 1) >> 1;
-        const float* bl = wts + (l == 0 ? kOffB0 : offB(l)) + 4 * half;
-        const f32x4 bias = { bl[0], bl[1], bl[2], bl[3] };
         uint2 res[C::kMaxPairs][P];
         if (l == 0) {
-            const float a0 = wa0[lane], a1 = wa0[64 + lane], a2 = wa0[128 + lane];
 #pragma unroll
             for (int k = 0; k < C::kMaxPairs; k++) {
                 const int pair = wave + k * C::NW;
                 if (pair < npairs) {
-                    const float* p = img + (ly0 + 2 * pair) * C::IW + lx0 + lane_im;
+                    const TIN* p = img + (ly0 + 2 * pair) * C::IW + lx0 + lane_im;
                     float q[P + 2];
 #pragma unroll
-                    for (int d = 0; d < P + 2; d++) q[d] = p[d];
+                    for (int d = 0; d < P + 2; d++) q[d] = to_float<TIN>(p[d]);
 #pragma unroll
                     for (int o = 0; o < P; o++) {
-                        f32x4 acc = mma_f32(a0, q[o], bias);
-                        acc = mma_f32(a1, q[o + 1], acc);
-                        acc = mma_f32(a2, q[o + 2], acc);
+                        f32x4 acc = mma_f32(a0f, q[o], bias);
+                        acc = mma_f32(a1f, q[o + 1], acc);
+                        acc = mma_f32(a2f, q[o + 2], acc);
                         res[k][o] = make_uint2(relu_pack<T16>(acc[0], acc[1]), relu_pack<T16>(acc[2], acc[3]));
                     }
                 }
             }
         } else {
-            const uint4* al = wa + (l - 1) * 3 * 64 + lane;
-            const v8 a0 = __builtin_bit_cast(v8, al[0]);
-            const v8 a1 = __builtin_bit_cast(v8, al[64]);
-            const v8 a2 = __builtin_bit_cast(v8, al[128]);
 #pragma unroll
             for (int k = 0; k < C::kMaxPairs; k++) {
                 const int pair = wave + k * C::NW;
@@ -215,6 +229,14 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
                     }
                 }
             }
+        }
+        {   // operands of layer l + 1 (1..8)
+            const uint4* al = wa + l * 3 * 64 + lane;
+            a0 = __builtin_bit_cast(v8, al[0]);
+            a1 = __builtin_bit_cast(v8, al[64]);
+            a2 = __builtin_bit_cast(v8, al[128]);
+            const float* bl = wts + offB(l + 1) + 4 * half;
+            bias = (f32x4){ bl[0], bl[1], bl[2], bl[3] };
         }
         __syncthreads();
 #pragma unroll
@@ -239,12 +261,6 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
     {
         const int gx1 = min(x0 + TW, W), gy1 = min(y0 + TH, H);
         const int npairs = (gy1 - y0 + 1) >> 1;
-        const uint4* al = wa + 7 * 3 * 64 + lane;
-        const v8 a0 = __builtin_bit_cast(v8, al[0]);
-        const v8 a1 = __builtin_bit_cast(v8, al[64]);
-        const v8 a2 = __builtin_bit_cast(v8, al[128]);
-        const float* bl = wts + offB(8) + 4 * half;
-        const f32x4 bias = { bl[0], bl[1], bl[2], bl[3] };
         // this lane's 4 channels x 4 output sub-pixels of the deconv, k2[c][q]
         const float* k2 = wts + kOffK2 + 16 * half;
         float kd[4][4];
@@ -294,7 +310,7 @@ __global__ __launch_bounds__(NT) void acnet_mfma_fused(const void* __restrict__ 
 {
     using C = MfmaTile<TW, TH, NT>;
     __shared__ uint4 plane[C::PW * C::PH];        // 8 x 16-bit channels per pixel
-    __shared__ float img[C::IW * C::IH];
+    __shared__ TIN img[C::IW * C::IH];
     mfma_tile<T16, TIN, TOUT, TW, TH, NT>(src, sstride, W, H, dst, dstride, wts, wa, wa0,
                                           blockIdx.x * TW, blockIdx.y * TH, plane, img);
 }
@@ -308,17 +324,18 @@ __global__ __launch_bounds__(NT) void acnet_mfma_batch(const BatchDesc* __restri
 {
     using C = MfmaTile<TW, TH, NT>;
     __shared__ uint4 plane[C::PW * C::PH];
-    __shared__ float img[C::IW * C::IH];
+    __shared__ TIN img[C::IW * C::IH];
     __shared__ int s_tile;
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
         __syncthreads();
-        const int t = s_tile;
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: keeps the tile state scalar
         if (t >= total) break;
         const BatchDesc d = find_tile(desc, n, t);
         const int local = t - d.first_tile;
-        const int ty = local / d.tiles_x;
-        const int tx = local - ty * d.tiles_x;
+        const int ly = local / d.tiles_x;
+        const int tx = local - ly * d.tiles_x;
+        const int ty = d.ty0 + ly;
         mfma_tile<T16, TIN, TOUT, TW, TH, NT>(d.src, d.sstride, d.w, d.h, d.dst, d.dstride, wts, wa, wa0,
                                               tx * TW, ty * TH, plane, img);
         __syncthreads();
@@ -339,10 +356,10 @@ static hipError_t launch_one(const void* src, long ss, int W, int H, void* dst, 
 
 template <typename T16, typename TIN, typename TOUT, int TW, int TH, int NT>
 static hipError_t launch_batch_one(const BatchDesc* desc, int n, int total, int* counter, const float* wts,
-                                   const uint4* wa, const float* wa0, hipStream_t stream)
+                                   const uint4* wa, const float* wa0, int max_wg, hipStream_t stream)
 {
     auto kern = acnet_mfma_batch<T16, TIN, TOUT, TW, TH, NT>;
-    const int grid = persistent_grid(reinterpret_cast<const void*>(kern), NT, total);
+    const int grid = persistent_grid(reinterpret_cast<const void*>(kern), NT, total, max_wg);
     if (grid <= 0) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, stream, desc, n, total, counter, wts, wa, wa0);
     return hipGetLastError();
@@ -350,14 +367,14 @@ static hipError_t launch_batch_one(const BatchDesc* desc, int n, int total, int*
 
 template <typename T16, int TW, int TH, int NT>
 static hipError_t launch_batch_types(int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
-                                     const float* wts, const uint4* wa, const float* wa0, hipStream_t stream)
+                                     const float* wts, const uint4* wa, const float* wa0, int max_wg, hipStream_t stream)
 {
 #define ACHIP_OUT(TIN)                                                                                                \
     switch (dtype) {                                                                                                  \
-    case U8: return launch_batch_one<T16, TIN, uint8_t, TW, TH, NT>(desc, n, total, counter, wts, wa, wa0, stream);        \
-    case U16: return launch_batch_one<T16, TIN, uint16_t, TW, TH, NT>(desc, n, total, counter, wts, wa, wa0, stream);      \
-    case F16: return launch_batch_one<T16, TIN, __half, TW, TH, NT>(desc, n, total, counter, wts, wa, wa0, stream);        \
-    case F32: return launch_batch_one<T16, TIN, float, TW, TH, NT>(desc, n, total, counter, wts, wa, wa0, stream);         \
+    case U8: return launch_batch_one<T16, TIN, uint8_t, TW, TH, NT>(desc, n, total, counter, wts, wa, wa0, max_wg, stream);        \
+    case U16: return launch_batch_one<T16, TIN, uint16_t, TW, TH, NT>(desc, n, total, counter, wts, wa, wa0, max_wg, stream);      \
+    case F16: return launch_batch_one<T16, TIN, __half, TW, TH, NT>(desc, n, total, counter, wts, wa, wa0, max_wg, stream);        \
+    case F32: return launch_batch_one<T16, TIN, float, TW, TH, NT>(desc, n, total, counter, wts, wa, wa0, max_wg, stream);         \
     default: return hipErrorInvalidValue;                                                                             \
     }
     switch (stype) {
@@ -393,25 +410,26 @@ static hipError_t launch_types(int stype, int dtype, const void* src, long ss, i
 }
 
 const Fp32Config kMfmaConfigs[] = {
-    { 64, 64, 1024, 2, MfmaTile<64, 64, 1024>::lds_bytes },
-    { 64, 32, 512, 2, MfmaTile<64, 32, 512>::lds_bytes },
+    { 64, 64, 1024, 2, MfmaTile<64, 64, 1024>::lds_bytes(4) },   // LDS for f32 input; less for narrower types
+    { 64, 32, 512, 2, MfmaTile<64, 32, 512>::lds_bytes(4) },
 };
 const int kMfmaConfigCount = sizeof(kMfmaConfigs) / sizeof(kMfmaConfigs[0]);
 
 hipError_t launch_mfma_batch(int bf16, int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total,
-                             int* counter, const float* wts, const void* wa, const float* wa0, hipStream_t stream)
+                             int* counter, const float* wts, const void* wa, const float* wa0, int max_wg,
+                             hipStream_t stream)
 {
     const uint4* a = static_cast<const uint4*>(wa);
     if (bf16) {
         switch (cfg) {
-        case 0: return launch_batch_types<__bf16, 64, 64, 1024>(stype, dtype, desc, n, total, counter, wts, a, wa0, stream);
-        case 1: return launch_batch_types<__bf16, 64, 32, 512>(stype, dtype, desc, n, total, counter, wts, a, wa0, stream);
+        case 0: return launch_batch_types<__bf16, 64, 64, 1024>(stype, dtype, desc, n, total, counter, wts, a, wa0, max_wg, stream);
+        case 1: return launch_batch_types<__bf16, 64, 32, 512>(stype, dtype, desc, n, total, counter, wts, a, wa0, max_wg, stream);
         default: return hipErrorInvalidValue;
         }
     }
     switch (cfg) {
-    case 0: return launch_batch_types<_Float16, 64, 64, 1024>(stype, dtype, desc, n, total, counter, wts, a, wa0, stream);
-    case 1: return launch_batch_types<_Float16, 64, 32, 512>(stype, dtype, desc, n, total, counter, wts, a, wa0, stream);
+    case 0: return launch_batch_types<_Float16, 64, 64, 1024>(stype, dtype, desc, n, total, counter, wts, a, wa0, max_wg, stream);
+    case 1: return launch_batch_types<_Float16, 64, 32, 512>(stype, dtype, desc, n, total, counter, wts, a, wa0, max_wg, stream);
     default: return hipErrorInvalidValue;
     }
 }

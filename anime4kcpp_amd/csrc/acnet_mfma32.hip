@@ -21,15 +21,17 @@ template <int TW, int TH, int NT>
 struct Mfma32Tile {
     static constexpr int NW = NT / 64;
     static constexpr int BW = TW + 16, BH = TH + 16;
+    static constexpr int kSeg = (TW + 16 + 15) / 16;                // 16-pixel items per row pair, at most
     // padded so that 16-lane segments and row pairs may overhang a layer's region
     static constexpr int PW = 8 + 16 * ((TW + 14 + 15) / 16) + 1;
     static constexpr int PH = BH + 1;
-    // input tile, origin (x0 - 9, y0 - 9), padded the same way for layer 0's items
-    static constexpr int IW = 16 * ((TW + 16 + 15) / 16) + 2 + 8;
+    // input tile in the source element type, origin (x0 - 9, y0 - 9), padded the same way
+    // for layer 0's items
+    static constexpr int IW = 16 * kSeg + 2 + 8;
     static constexpr int IH = TH + 18 + 2;
-    static constexpr int items(int m) { return ((TW + 2 * m + 15) / 16) * ((TH + 2 * m + 1) / 2); }
-    static constexpr int kMaxItems = (items(7) + NW - 1) / NW;
-    static constexpr int lds_bytes = 2 * PW * PH * 16 + IW * IH * 4;
+    static constexpr int kStage = (IW * IH + NT - 1) / NT;          // input elements per thread
+    static constexpr int kMaxPairs = ((BH + 1) / 2 + NW - 1) / NW;  // row pairs per wave and layer
+    static constexpr int lds_bytes(int elem) { return 2 * PW * PH * 16 + (IW * IH * elem + 15) / 16 * 16; }
 };
 
 // Replicate ring and zeroed border tiles: see fill_ring in acnet_mfma.hip.
@@ -62,23 +64,36 @@ __device__ inline void fill_ring32(float4* __restrict__ pa, float4* __restrict__
 
 __device__ inline f32x4m mma32(float a, float b, f32x4m c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
-// the 24 MFMAs of one item on two accumulator chains; p: this lane's B row start
-__device__ __forceinline__ f32x4m item24(const float4* __restrict__ pa, const float4* __restrict__ pb, int p,
-                                         const float (&a)[24])
+// B operands of one item: the 8 channels of this lane's pixel in patch columns t = 0..2
+struct Item32 {
+    float4 lo[3], hi[3];
+};
+
+__device__ __forceinline__ Item32 load_item(const float4* __restrict__ pa, const float4* __restrict__ pb, int p)
 {
-    f32x4m c0 = { 0.f, 0.f, 0.f, 0.f }, c1 = { 0.f, 0.f, 0.f, 0.f };
+    Item32 q;
 #pragma unroll
     for (int t = 0; t < 3; t++) {
-        const float4 lo = pa[p + t];
-        const float4 hi = pb[p + t];
-        c0 = mma32(a[t * 8 + 0], lo.x, c0);
-        c1 = mma32(a[t * 8 + 1], lo.y, c1);
-        c0 = mma32(a[t * 8 + 2], lo.z, c0);
-        c1 = mma32(a[t * 8 + 3], lo.w, c1);
-        c0 = mma32(a[t * 8 + 4], hi.x, c0);
-        c1 = mma32(a[t * 8 + 5], hi.y, c1);
-        c0 = mma32(a[t * 8 + 6], hi.z, c0);
-        c1 = mma32(a[t * 8 + 7], hi.w, c1);
+        q.lo[t] = pa[p + t];
+        q.hi[t] = pb[p + t];
+    }
+    return q;
+}
+
+// the 24 MFMAs of one item on two accumulator chains, the first starting from the bias
+__device__ __forceinline__ f32x4m item24(const Item32& q, const float (&a)[24], f32x4m bias)
+{
+    f32x4m c0 = bias, c1 = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        c0 = mma32(a[t * 8 + 0], q.lo[t].x, c0);
+        c1 = mma32(a[t * 8 + 1], q.lo[t].y, c1);
+        c0 = mma32(a[t * 8 + 2], q.lo[t].z, c0);
+        c1 = mma32(a[t * 8 + 3], q.lo[t].w, c1);
+        c0 = mma32(a[t * 8 + 4], q.hi[t].x, c0);
+        c1 = mma32(a[t * 8 + 5], q.hi[t].y, c1);
+        c0 = mma32(a[t * 8 + 6], q.hi[t].z, c0);
+        c1 = mma32(a[t * 8 + 7], q.hi[t].w, c1);
     }
     return c0 + c1;
 }
@@ -88,7 +103,7 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
                                             void* __restrict__ dst, long dstride,
                                             const float* __restrict__ wts, const float* __restrict__ wa,
                                             int x0, int y0, float4* __restrict__ pa, float4* __restrict__ pb,
-                                            float* __restrict__ img)
+                                            TIN* __restrict__ img)
 {
     using C = Mfma32Tile<TW, TH, NT>;
     const int tid = threadIdx.x;
@@ -97,55 +112,83 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
     const int bx = x0 - 8, by = y0 - 8;
     const int j = lane & 15, g = lane >> 4, half = g & 1, ro = g >> 1;
 
-    if (x0 - 8 <= 0 || y0 - 8 <= 0 || x0 + TW + 8 >= W || y0 + TH + 8 >= H)
+    const bool border = x0 - 8 <= 0 || y0 - 8 <= 0 || x0 + TW + 8 >= W || y0 + TH + 8 >= H;
+    if (border)
         for (int i = tid; i < C::PW * C::PH; i += NT) {
             pa[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             pb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
-    // input tile with replicate clamp over its whole padded extent (every value finite)
-    for (int i = tid; i < C::IW * C::IH; i += NT) {
-        const int iy = i / C::IW;
-        const int ix = i - iy * C::IW;
-        const int gy = clampi(y0 - 9 + iy, 0, H - 1);
-        const int gx = clampi(x0 - 9 + ix, 0, W - 1);
-        img[i] = to_float<TIN>(row_ptr<TIN>(src, sstride, gy)[gx]);
+    // input tile with replicate clamp over its whole padded extent, all loads in flight
+    // before the first LDS store
+    {
+        constexpr int DX = NT % C::IW, DY = NT / C::IW;
+        int iy = tid / C::IW, ix = tid - (tid / C::IW) * C::IW;
+        TIN v[C::kStage];
+#pragma unroll
+        for (int k = 0; k < C::kStage; k++) {
+            if (k < C::kStage - 1 || tid + k * NT < C::IW * C::IH) {
+                const int gy = clampi(y0 - 9 + iy, 0, H - 1);
+                const int gx = clampi(x0 - 9 + ix, 0, W - 1);
+                v[k] = row_ptr<TIN>(src, sstride, gy)[gx];
+            }
+            ix += DX;
+            iy += DY;
+            if (ix >= C::IW) { ix -= C::IW; iy++; }
+        }
+#pragma unroll
+        for (int k = 0; k < C::kStage; k++)
+            if (k < C::kStage - 1 || tid + k * NT < C::IW * C::IH) img[tid + k * NT] = v[k];
     }
     __syncthreads();
+
+    // Each wave takes whole row pairs (pair = wave, wave + NW, ...) and their 16-pixel items
+    // left to right, loading the next item's operands before the current item's MFMAs.
+    // this lane's write target: channels 0-3 (pa) or 4-7 (pb) of pixel (column j, row ro)
+    float4* const pw = half ? pb : pa;
 
     // ---- layer 0: 3 MFMAs per item (K = 4 patch rows, one per patch column) ----
     {
         const int gx0 = max(x0 - 8, 0), gy0 = max(y0 - 8, 0);
         const int gx1 = min(x0 + TW + 8, W), gy1 = min(y0 + TH + 8, H);
         const int nseg = (gx1 - gx0 + 15) >> 4;
-        const int nitems = nseg * ((gy1 - gy0 + 1) >> 1);
+        const int npairs = (gy1 - gy0 + 1) >> 1;
         const float a0 = wa[lane], a1 = wa[64 + lane], a2 = wa[128 + lane];
         const float* bl = wts + kOffB0 + 4 * half;
-        const float b0 = bl[0], b1 = bl[1], b2 = bl[2], b3 = bl[3];
+        const f32x4m bias = { bl[0], bl[1], bl[2], bl[3] };
         const int lane_rd = g * C::IW + j;
-        int pair = wave / nseg, seg = wave - (wave / nseg) * nseg;
-        for (int item = wave; item < nitems; item += C::NW) {
+        for (int pair = wave; pair < npairs; pair += C::NW) {
             // img holds pixel (x, y) at ((y - y0 + 9) * IW + x - x0 + 9); B row y-1+g, column x+j+t-1
-            const int base = (gy0 + 2 * pair - 1 - (y0 - 9)) * C::IW + (gx0 + 16 * seg - 1 - (x0 - 9));
-            const float* p = img + base + lane_rd;
-            f32x4m acc = { 0.f, 0.f, 0.f, 0.f };
-            acc = mma32(a0, p[0], acc);
-            acc = mma32(a1, p[1], acc);
-            acc = mma32(a2, p[2], acc);
-            const int gx = gx0 + 16 * seg + j;
+            const TIN* p = img + (gy0 + 2 * pair - 1 - (y0 - 9)) * C::IW + (gx0 - 1 - (x0 - 9)) + lane_rd;
             const int oy = gy0 + 2 * pair + ro;
-            if (gx < gx1 && oy < gy1) {
-                const float4 v = make_float4(fmaxf(acc[0] + b0, 0.f), fmaxf(acc[1] + b1, 0.f),
-                                             fmaxf(acc[2] + b2, 0.f), fmaxf(acc[3] + b3, 0.f));
-                const int d = (oy - by) * C::PW + gx - bx;
-                if (half) pb[d] = v; else pa[d] = v;
+#pragma unroll
+            for (int s = 0; s < C::kSeg; s++) {
+                if (s < nseg) {
+                    f32x4m acc = mma32(a0, to_float<TIN>(p[16 * s]), bias);
+                    acc = mma32(a1, to_float<TIN>(p[16 * s + 1]), acc);
+                    acc = mma32(a2, to_float<TIN>(p[16 * s + 2]), acc);
+                    const int gx = gx0 + 16 * s + j;
+                    if (gx < gx1 && oy < gy1)
+                        pw[(oy - by) * C::PW + gx - bx] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f),
+                                                                      fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+                }
             }
-            seg += C::NW;
-            while (seg >= nseg) { seg -= nseg; pair++; }
         }
     }
+    // layer 1's operands
+    float a[24];
+    f32x4m bias;
+    {
+        const float* al = wa + kMfma32L0Floats + lane;
+#pragma unroll
+        for (int s = 0; s < 24; s++) a[s] = al[s * 64];
+        const float* bl = wts + offB(1) + 4 * half;
+        bias = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
+    }
     __syncthreads();
-    fill_ring32<TW, TH, NT>(pa, pb, x0, y0, W, H, 8);
-    __syncthreads();
+    if (border) {
+        fill_ring32<TW, TH, NT>(pa, pb, x0, y0, W, H, 8);
+        __syncthreads();
+    }
 
     const int lane_rd = g * C::PW + j;
     // ---- layers 1..7, in place ----
@@ -155,87 +198,91 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
         const int gx0 = max(x0 - m, 0), gy0 = max(y0 - m, 0);
         const int gx1 = min(x0 + TW + m, W), gy1 = min(y0 + TH + m, H);
         const int nseg = (gx1 - gx0 + 15) >> 4;
-        const int nitems = nseg * ((gy1 - gy0 + 1) >> 1);
-        float a[24];
-        const float* al = wa + 192 + (l - 1) * 24 * 64 + lane;
+        const int npairs = (gy1 - gy0 + 1) >> 1;
+        float4 res[C::kMaxPairs][C::kSeg];
 #pragma unroll
-        for (int s = 0; s < 24; s++) a[s] = al[s * 64];
-        const float* bl = wts + offB(l) + 4 * half;
-        const float b0 = bl[0], b1 = bl[1], b2 = bl[2], b3 = bl[3];
-
-        float4 res[C::kMaxItems];
-        int pair = wave / nseg, seg = wave - (wave / nseg) * nseg;
+        for (int k = 0; k < C::kMaxPairs; k++) {
+            const int pair = wave + k * C::NW;
+            if (pair < npairs) {
+                const int base = (gy0 + 2 * pair - 1 - by) * C::PW + (gx0 - 1 - bx) + lane_rd;
+                Item32 q = load_item(pa, pb, base);
 #pragma unroll
-        for (int ii = 0; ii < C::kMaxItems; ii++) {
-            const int item = wave + ii * C::NW;
-            if (item < nitems) {
-                const int base = (gy0 + 2 * pair - 1 - by) * C::PW + (gx0 + 16 * seg - 1 - bx);
-                const f32x4m acc = item24(pa, pb, base + lane_rd, a);
-                res[ii] = make_float4(fmaxf(acc[0] + b0, 0.f), fmaxf(acc[1] + b1, 0.f),
-                                      fmaxf(acc[2] + b2, 0.f), fmaxf(acc[3] + b3, 0.f));
-            }
-            seg += C::NW;
-            while (seg >= nseg) { seg -= nseg; pair++; }
-        }
-        __syncthreads();
-        pair = wave / nseg;
-        seg = wave - pair * nseg;
-#pragma unroll
-        for (int ii = 0; ii < C::kMaxItems; ii++) {
-            const int item = wave + ii * C::NW;
-            if (item < nitems) {
-                const int gx = gx0 + 16 * seg + j;
-                const int oy = gy0 + 2 * pair + ro;
-                if (gx < gx1 && oy < gy1) {
-                    const int d = (oy - by) * C::PW + gx - bx;
-                    if (half) pb[d] = res[ii]; else pa[d] = res[ii];
+                for (int s = 0; s < C::kSeg; s++) {
+                    if (s < nseg) {
+                        Item32 nq;
+                        if (s + 1 < nseg) nq = load_item(pa, pb, base + 16 * (s + 1));
+                        const f32x4m acc = item24(q, a, bias);
+                        res[k][s] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f),
+                                                fmaxf(acc[3], 0.f));
+                        q = nq;
+                    }
                 }
             }
-            seg += C::NW;
-            while (seg >= nseg) { seg -= nseg; pair++; }
+        }
+        {   // operands of layer l + 1
+            const float* al = wa + kMfma32L0Floats + l * kMfma32LayerFloats + lane;
+#pragma unroll
+            for (int s = 0; s < 24; s++) a[s] = al[s * 64];
+            const float* bl = wts + offB(l + 1) + 4 * half;
+            bias = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
         }
         __syncthreads();
-        fill_ring32<TW, TH, NT>(pa, pb, x0, y0, W, H, m);
+#pragma unroll
+        for (int k = 0; k < C::kMaxPairs; k++) {
+            const int pair = wave + k * C::NW;
+            const int oy = gy0 + 2 * pair + ro;
+            if (pair < npairs && oy < gy1) {
+                float4* w = pw + (oy - by) * C::PW + gx0 - bx + j;
+#pragma unroll
+                for (int s = 0; s < C::kSeg; s++)
+                    if (s < nseg && gx0 + 16 * s + j < gx1) w[16 * s] = res[k][s];
+            }
+        }
         __syncthreads();
+        if (border) {
+            fill_ring32<TW, TH, NT>(pa, pb, x0, y0, W, H, m);
+            __syncthreads();
+        }
     }
 
     // ---- layer 8, ReLU, deconv 2x2 stride 2 (8 -> 1) ----
     {
-        const int gx0 = x0, gy0 = y0;
         const int gx1 = min(x0 + TW, W), gy1 = min(y0 + TH, H);
-        const int nseg = (gx1 - gx0 + 15) >> 4;
-        const int nitems = nseg * ((gy1 - gy0 + 1) >> 1);
-        float a[24];
-        const float* al = wa + 192 + 7 * 24 * 64 + lane;
-#pragma unroll
-        for (int s = 0; s < 24; s++) a[s] = al[s * 64];
-        const float* bl = wts + offB(8) + 4 * half;
-        const float b0 = bl[0], b1 = bl[1], b2 = bl[2], b3 = bl[3];
+        const int nseg = (gx1 - x0 + 15) >> 4;
+        const int npairs = (gy1 - y0 + 1) >> 1;
         const float* k2 = wts + kOffK2 + 16 * half;
         float kd[4][4];
 #pragma unroll
         for (int c = 0; c < 4; c++)
 #pragma unroll
             for (int q = 0; q < 4; q++) kd[c][q] = k2[c * 4 + q];
-        int pair = wave / nseg, seg = wave - (wave / nseg) * nseg;
-        for (int item = wave; item < nitems; item += C::NW) {
-            const int base = (gy0 + 2 * pair - 1 - by) * C::PW + (gx0 + 16 * seg - 1 - bx);
-            const f32x4m acc = item24(pa, pb, base + lane_rd, a);
-            const float v0 = fmaxf(acc[0] + b0, 0.f), v1 = fmaxf(acc[1] + b1, 0.f);
-            const float v2 = fmaxf(acc[2] + b2, 0.f), v3 = fmaxf(acc[3] + b3, 0.f);
-            float s[4];
+        for (int pair = wave; pair < npairs; pair += C::NW) {
+            // region origin is plane-local (8, 8)
+            const int base = (7 + 2 * pair) * C::PW + 7 + lane_rd;
+            const int oy = y0 + 2 * pair + ro;
+            Item32 q = load_item(pa, pb, base);
 #pragma unroll
-            for (int q = 0; q < 4; q++) s[q] = fmaf(v3, kd[3][q], fmaf(v2, kd[2][q], fmaf(v1, kd[1][q], v0 * kd[0][q])));
+            for (int s = 0; s < (TW + 15) / 16; s++) {
+                if (s < nseg) {
+                    Item32 nq;
+                    if (s + 1 < nseg) nq = load_item(pa, pb, base + 16 * (s + 1));
+                    const f32x4m acc = item24(q, a, bias);
+                    const float v0 = fmaxf(acc[0], 0.f), v1 = fmaxf(acc[1], 0.f);
+                    const float v2 = fmaxf(acc[2], 0.f), v3 = fmaxf(acc[3], 0.f);
+                    float sum[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) s[q] += __shfl_xor(s[q], 16);
-            const int gx = gx0 + 16 * seg + j;
-            const int oy = gy0 + 2 * pair + ro;
-            if (gx < gx1 && oy < gy1) {
-                TOUT* orow = row_ptr<TOUT>(dst, dstride, 2 * oy + half);
-                store_pair<TOUT>(orow + 2 * gx, half ? s[2] : s[0], half ? s[3] : s[1]);
+                    for (int q4 = 0; q4 < 4; q4++)
+                        sum[q4] = fmaf(v3, kd[3][q4], fmaf(v2, kd[2][q4], fmaf(v1, kd[1][q4], v0 * kd[0][q4])));
+#pragma unroll
+                    for (int q4 = 0; q4 < 4; q4++) sum[q4] += __shfl_xor(sum[q4], 16);
+                    const int gx = x0 + 16 * s + j;
+                    if (gx < gx1 && oy < gy1) {
+                        TOUT* orow = row_ptr<TOUT>(dst, dstride, 2 * oy + half);
+                        store_pair<TOUT>(orow + 2 * gx, half ? sum[2] : sum[0], half ? sum[3] : sum[1]);
+                    }
+                    q = nq;
+                }
             }
-            seg += C::NW;
-            while (seg >= nseg) { seg -= nseg; pair++; }
         }
     }
 }
@@ -248,7 +295,7 @@ __global__ __launch_bounds__(NT) void acnet_mfma32_fused(const void* __restrict_
     using C = Mfma32Tile<TW, TH, NT>;
     __shared__ float4 pa[C::PW * C::PH];
     __shared__ float4 pb[C::PW * C::PH];
-    __shared__ float img[C::IW * C::IH];
+    __shared__ TIN img[C::IW * C::IH];
     mfma32_tile<TIN, TOUT, TW, TH, NT>(src, sstride, W, H, dst, dstride, wts, wa,
                                        blockIdx.x * TW, blockIdx.y * TH, pa, pb, img);
 }
@@ -261,17 +308,18 @@ __global__ __launch_bounds__(NT) void acnet_mfma32_batch(const BatchDesc* __rest
     using C = Mfma32Tile<TW, TH, NT>;
     __shared__ float4 pa[C::PW * C::PH];
     __shared__ float4 pb[C::PW * C::PH];
-    __shared__ float img[C::IW * C::IH];
+    __shared__ TIN img[C::IW * C::IH];
     __shared__ int s_tile;
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
         __syncthreads();
-        const int t = s_tile;
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: keeps the tile state scalar
         if (t >= total) break;
         const BatchDesc d = find_tile(desc, n, t);
         const int local = t - d.first_tile;
-        const int ty = local / d.tiles_x;
-        const int tx = local - ty * d.tiles_x;
+        const int ly = local / d.tiles_x;
+        const int tx = local - ly * d.tiles_x;
+        const int ty = d.ty0 + ly;
         mfma32_tile<TIN, TOUT, TW, TH, NT>(d.src, d.sstride, d.w, d.h, d.dst, d.dstride, wts, wa,
                                            tx * TW, ty * TH, pa, pb, img);
         __syncthreads();
@@ -292,10 +340,10 @@ static hipError_t launch_one(const void* src, long ss, int W, int H, void* dst, 
 
 template <typename TIN, typename TOUT, int TW, int TH, int NT>
 static hipError_t launch_batch_one(const BatchDesc* desc, int n, int total, int* counter, const float* wts,
-                                   const float* wa, hipStream_t stream)
+                                   const float* wa, int max_wg, hipStream_t stream)
 {
     auto kern = acnet_mfma32_batch<TIN, TOUT, TW, TH, NT>;
-    const int grid = persistent_grid(reinterpret_cast<const void*>(kern), NT, total);
+    const int grid = persistent_grid(reinterpret_cast<const void*>(kern), NT, total, max_wg);
     if (grid <= 0) return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), 0, stream, desc, n, total, counter, wts, wa);
     return hipGetLastError();
@@ -331,14 +379,14 @@ hipError_t launch_mfma32(int stype, int dtype, const void* src, long ss, int W, 
 }
 
 hipError_t launch_mfma32_batch(int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
-                               const float* wts, const float* wa, hipStream_t stream)
+                               const float* wts, const float* wa, int max_wg, hipStream_t stream)
 {
-#define CALL(TI, TO) launch_batch_one<TI, TO, 64, 32, 512>(desc, n, total, counter, wts, wa, stream)
+#define CALL(TI, TO) launch_batch_one<TI, TO, 64, 32, 512>(desc, n, total, counter, wts, wa, max_wg, stream)
     ACHIP_TYPES(CALL)
 #undef CALL
 }
 #undef ACHIP_TYPES
 
-const int kMfma32LdsBytes = Mfma32Tile<64, 32, 512>::lds_bytes;
+const int kMfma32LdsBytes = Mfma32Tile<64, 32, 512>::lds_bytes(4);   // f32 input; less for narrower types
 
 } // namespace achip

@@ -155,3 +155,95 @@ def configs(precision="fp32"):
         check(L.achip_config_info(prec, i, *[ctypes.byref(v) for v in vals]), "achip_config_info")
         out.append(dict(zip(("tw", "th", "nt", "p", "lds_bytes"), [v.value for v in vals])))
     return out
+
+
+REQ_LUMA, REQ_YUV420, REQ_RGB = 0, 1, 2
+
+
+def _addr(a):
+    return a.data_ptr() if hasattr(a, "data_ptr") else a.ctypes.data
+
+
+def _row_bytes(a):
+    return a.stride(0) * a.element_size() if hasattr(a, "data_ptr") else a.strides[0]
+
+
+class Server:
+    """Serving loop on one Net (achip_server_*): requests of mixed kind, size and factor,
+    admitted by an HBM budget, one-off passes batched step by step into persistent
+    launches, video-stream frames replayed from captured hipGraphs.  Planes are u8 numpy
+    arrays or torch tensors (host, page-locked preferred, or device)."""
+
+    def __init__(self, net, precision="fp16", hbm_budget=0, step_tiles=0, max_workgroups=0):
+        L = _native.lib()
+        cfg = _native.ACHipServerConfig(PRECISIONS[precision], int(hbm_budget), int(step_tiles), int(max_workgroups))
+        h = ctypes.c_void_p()
+        check(L.achip_server_create(net.handle, ctypes.byref(cfg), ctypes.byref(h)), "achip_server_create")
+        self._h = h
+        self.net = net
+        self.precision = precision
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _native.lib().achip_server_destroy(self._h)
+            self._h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, kind, planes, factor=2, stream=-1, dst=None):
+        """Queue one request; planes: [Y] (LUMA), [RGB H x W x 3] (RGB) or [Y, U, V] (YUV420);
+        dst: matching output planes, or None to keep the output in the server.  Returns the ticket."""
+        r = _native.ACHipRequest()
+        r.kind, r.factor, r.stream = kind, factor, stream
+        r.height, r.width = int(planes[0].shape[0]), int(planes[0].shape[1])
+        for i, p in enumerate(planes):
+            r.src[i] = _addr(p)
+            r.src_stride[i] = _row_bytes(p)
+        for i, p in enumerate(dst or []):
+            r.dst[i] = _addr(p)
+            r.dst_stride[i] = _row_bytes(p)
+        t = ctypes.c_ulonglong()
+        check(_native.lib().achip_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)), "achip_server_submit")
+        self._keep[t.value] = (planes, dst)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1.0):
+        """Block until done; returns the submit -> done latency in ms."""
+        lat = ctypes.c_double()
+        rc = _native.lib().achip_server_wait(self._h, ticket, float(timeout_ms), ctypes.byref(lat))
+        if rc == 0 and self._keep.get(ticket, (None, None))[1] is not None:
+            self._keep.pop(ticket, None)
+        check(rc, "achip_server_wait")
+        return lat.value
+
+    def output(self, ticket, plane=0):
+        ptr, stride = ctypes.c_void_p(), ctypes.c_long()
+        check(_native.lib().achip_server_output(self._h, ticket, plane, ctypes.byref(ptr), ctypes.byref(stride)),
+              "achip_server_output")
+        return ptr.value, stride.value
+
+    def fetch(self, ticket, plane, shape):
+        """Copy a kept output plane (u8, `shape`) from device memory into a new numpy array."""
+        ptr, stride = self.output(ticket, plane)
+        out = np.zeros(shape, np.uint8)
+        row = out.strides[0]
+        hip_rt = ctypes.CDLL("libamdhip64.so")
+        hip_rt.hipMemcpy2D.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int]
+        check(hip_rt.hipMemcpy2D(out.ctypes.data, row, ptr, stride, row, out.shape[0], 2), "hipMemcpy2D")   # D2H
+        return out
+
+    def release(self, ticket):
+        check(_native.lib().achip_server_release(self._h, ticket), "achip_server_release")
+        self._keep.pop(ticket, None)
+
+    def stats(self, reset=False):
+        s = _native.ACHipServerStats()
+        check(_native.lib().achip_server_stats(self._h, ctypes.byref(s), 1 if reset else 0), "achip_server_stats")
+        return {name: getattr(s, name) for name, _ in s._fields_}

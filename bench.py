@@ -101,6 +101,192 @@ def error_vs_fp64(nets, weights, prec):
             "psnr_db": round(10 * np.log10(1.0 / mse), 2) if mse > 0 else None}
 
 
+def batched_uneven(nets, prec, seed=2024, n=64):
+    """64 u8 images, sides uniform in [128, 2048] (seeded): one persistent launch with a tile
+    work-queue per configuration (the autotuner's sweep) against 64 separate launches."""
+    import numpy as np
+    import torch
+    from anime4kcpp_amd import hip, synth
+    dev = torch.device("cuda:0")
+    ws = synth.randint(seed, n, 128, 2048)
+    hs = synth.randint(seed + 1, n, 128, 2048)
+    srcs = [torch.from_numpy(synth.flat_shaded(700 + i, int(w), int(h), dtype=np.uint8)).to(dev) for i, (w, h) in enumerate(zip(ws, hs))]
+    dsts = [torch.empty((2 * s.shape[0], 2 * s.shape[1]), dtype=torch.uint8, device=dev) for s in srcs]
+    net = nets["hdn0"]
+    out_mpix = sum(d.numel() for d in dsts) / 1e6
+    from anime4kcpp_amd import _native
+    ws_t = torch.empty(_native.lib().achip_batch_workspace_bytes(n), dtype=torch.uint8, device=dev)
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    sweep = {}
+    for c in range(len(hip.configs(prec))):
+        sweep[c] = timed(lambda: net.upscale_batch(srcs, dsts, precision=prec, config=c, workspace=ws_t))
+    best = min(sweep, key=sweep.get)
+    sep = timed(lambda: [net.upscale(s, d, precision=prec, config=best) for s, d in zip(srcs, dsts)])
+    tb = sweep[best]
+    return {
+        "images": n, "input_mpix": round(sum(s.numel() for s in srcs) / 1e6, 2), "output_mpix": round(out_mpix, 2),
+        "one_launch": {"images_per_s": round(n / tb, 1), "mpix_per_s": round(out_mpix / tb, 1), "ms": round(tb * 1e3, 3)},
+        "separate_launches": {"images_per_s": round(n / sep, 1), "mpix_per_s": round(out_mpix / sep, 1), "ms": round(sep * 1e3, 3)},
+        "autotune_sweep_ms": {f"{hip.configs(prec)[c]['tw']}x{hip.configs(prec)[c]['th']}/{hip.configs(prec)[c]['nt']}thr"
+                              + ("/mfma" if prec != "fp32" or c == 0 else "/valu"): round(t * 1e3, 3) for c, t in sweep.items()},
+        "chosen_config": best,
+    }
+
+
+def precision_4k_x4(nets, weights, precs, crop=1024):
+    """3840x2160 f32 -> 15360x8640 f32 (two chained 2x passes) per precision: Mpix/s of the
+    two-pass chain, and each pass's error against the FP64 restatement on a crop (interior
+    only: the crop's own borders differ from the full image's)."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    from anime4kcpp_amd import synth
+    dev = torch.device("cuda:0")
+    net = nets["hdn0"]
+    k, b = weights["hdn0"]
+    src_np = synth.flat_shaded(44, 3840, 2160)
+    src = torch.from_numpy(src_np).to(dev)
+    mid = torch.empty((4320, 7680), dtype=torch.float32, device=dev)
+    out = torch.empty((8640, 15360), dtype=torch.float32, device=dev)
+    out_mpix = out.numel() / 1e6
+    res = {}
+    for prec in precs:
+        net.upscale(src, mid, precision=prec)
+        net.upscale(mid, out, precision=prec)
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net.upscale(src, mid, precision=prec)
+            net.upscale(mid, out, precision=prec)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        errs = {}
+        m = 12   # interior margin (input pixels) excluded from the crop comparison
+        for pi, (inp_np, dev_out) in enumerate(((src_np, mid), (None, out))):
+            if inp_np is None:
+                inp_np = mid.cpu().numpy()
+            y0, x0 = (inp_np.shape[0] - crop) // 2, (inp_np.shape[1] - crop) // 2
+            c = np.ascontiguousarray(inp_np[y0:y0 + crop, x0:x0 + crop])
+            ref = OL.oracle_pass_f64(c, k, b)[2 * m:-2 * m, 2 * m:-2 * m]
+            got = dev_out[2 * y0 + 2 * m:2 * (y0 + crop) - 2 * m, 2 * x0 + 2 * m:2 * (x0 + crop) - 2 * m].cpu().numpy()
+            e = np.abs(got.astype(np.float64) - ref)
+            mse = float((e ** 2).mean())
+            errs[f"pass{pi + 1}"] = {"max_abs": float(e.max()), "mean_abs": float(e.mean()),
+                                     "psnr_db": round(10 * np.log10(1.0 / mse), 2) if mse > 0 else None}
+        res[prec] = {"ms": round(t * 1e3, 3), "output_mpix_per_s": round(out_mpix / t, 1), "error_vs_fp64": errs}
+    base = res[precs[0]]["output_mpix_per_s"]
+    for prec in precs[1:]:
+        res[prec]["vs_" + precs[0]] = round(res[prec]["output_mpix_per_s"] / base, 2)
+    res["workload"] = f"3840x2160 f32 -> 7680x4320 -> 15360x8640 f32 (HDN0); error on a {crop}x{crop} crop per pass"
+    return res
+
+
+def serving(net, prec, loads, seconds, seed=5):
+    """Poisson arrivals at each offered load (requests/s): 60% frames of 8 concurrent 1080p
+    YUV420 u8 streams (Y through the network, UV bilinear 2x, hipGraph replays), 40% one-off
+    RGB u8 images with sides uniform in [128, 4096] and factor 2 or 4.  Inputs are
+    pre-generated in page-locked host memory and uploaded by the server; outputs stay in
+    device memory and are released on completion."""
+    import threading
+    import numpy as np
+    import torch
+    from anime4kcpp_amd import hip, synth
+    rng = np.random.default_rng(seed)
+    frames = []
+    for s in range(8):
+        y = torch.from_numpy(synth.flat_shaded(3000 + s, W_IN, H_IN, dtype=np.uint8)).pin_memory()
+        u = torch.from_numpy(synth.flat_shaded(3100 + s, W_IN // 2, H_IN // 2, dtype=np.uint8)).pin_memory()
+        v = torch.from_numpy(synth.flat_shaded(3200 + s, W_IN // 2, H_IN // 2, dtype=np.uint8)).pin_memory()
+        frames.append([y, u, v])
+    sides = synth.randint(seed, 24, 128, 4096), synth.randint(seed + 1, 24, 128, 4096)
+    pool = []
+    for i, (w, h) in enumerate(zip(*sides)):
+        g = synth.flat_shaded(3300 + i, int(w), int(h), dtype=np.uint8)
+        pool.append(torch.from_numpy(np.ascontiguousarray(np.stack([g, np.roll(g, 1, 0), np.roll(g, 1, 1)], 2))).pin_memory())
+    def pick():
+        if rng.random() < 0.6:
+            s = int(rng.integers(8))
+            return (hip.REQ_YUV420, frames[s], 2, s), False
+        return (hip.REQ_RGB, [pool[int(rng.integers(len(pool)))]], 2 if rng.random() < 0.5 else 4, -1), True
+
+    out = {}
+    # capacity: the same mix submitted all at once
+    with hip.Server(net, prec, hbm_budget=64 << 30) as srv:
+        reqs = [pick() for _ in range(120)]
+        ts = [(srv.submit(*r), keep) for r, keep in reqs]
+        for t, keep in ts:
+            srv.wait(t, 120000)
+            if keep:
+                srv.release(t)
+        st = srv.stats()
+        out["saturated"] = {"requests": len(reqs), "output_mpix_per_s": round(st["output_mpix"] / st["busy_s"], 1),
+                            "requests_per_s": round(len(reqs) / st["busy_s"], 1), "steps": st["steps"]}
+    for load in loads:
+        srv = hip.Server(net, prec, hbm_budget=64 << 30)
+        tickets = []
+        lock = threading.Lock()
+        done = threading.Event()
+
+        def waiter():
+            i = 0
+            while True:
+                with lock:
+                    t = tickets[i] if i < len(tickets) else None
+                if t is None:
+                    if done.is_set():
+                        with lock:
+                            if i >= len(tickets):
+                                return
+                    time.sleep(0.0005)
+                    continue
+                srv.wait(t[0], 120000)
+                if t[1]:
+                    srv.release(t[0])
+                i += 1
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t_next = time.perf_counter()
+        t_end = t_next + seconds
+        n_frame = n_one = 0
+        while True:
+            t_next += rng.exponential(1.0 / load)
+            if t_next > t_end:
+                break
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            r, keep = pick()
+            t = srv.submit(*r)
+            n_frame += 0 if keep else 1
+            n_one += 1 if keep else 0
+            with lock:
+                tickets.append((t, keep))
+        done.set()
+        th.join()
+        st = srv.stats()
+        srv.close()
+        out[str(load)] = {
+            "offered_req_per_s": load, "frames": n_frame, "oneoffs": n_one,
+            "frame_p50_ms": round(st["frame_p50_ms"], 3), "frame_p99_ms": round(st["frame_p99_ms"], 3),
+            "oneoff_p50_ms": round(st["oneoff_p50_ms"], 3), "oneoff_p99_ms": round(st["oneoff_p99_ms"], 3),
+            "output_mpix_per_s": round(st["output_mpix"] / st["busy_s"], 1) if st["busy_s"] > 0 else None,
+            "steps": st["steps"], "graph_replays": st["graph_replays"], "failed": st["failed"],
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +296,12 @@ def main():
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error report")
     ap.add_argument("--precisions", default="fp32,fp16,bf16")
     ap.add_argument("--config", type=int, default=None, help="kernel tile configuration")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batched measurement")
+    ap.add_argument("--no-4k", action="store_true", help="skip the 4K f32 x4 precision measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serving-loads", default="fp16:300,1200;fp32:60,240",
+                    help="offered loads (requests/s) per precision")
+    ap.add_argument("--serving-seconds", type=float, default=3.0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -174,7 +366,8 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "engine": ("FP32 VALU (packed FP32 disabled; non-packed FMA caps at half the FP32 peak)"
+            "engine": (("FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulate" if (args.config or 0) == 0 else
+                        "FP32 VALU (packed FP32 disabled; non-packed FMA caps at half the FP32 peak)")
                        if main_prec == "fp32" else f"{main_prec.upper()} MFMA, FP32 accumulate"),
             "achieved": round(achieved, 2),
             "peak": PEAK_FP32_TFLOPS if main_prec == "fp32" else PEAK_MFMA16_TFLOPS,
@@ -204,6 +397,15 @@ def main():
                 result["error_vs_fp64"] = e
             else:
                 result[f"{p}_mfma"]["error_vs_fp64"] = e
+    if not args.no_batch:
+        result["batched_uneven"] = {p: batched_uneven(nets, p) for p in precs[:2]}
+    if not args.no_4k:
+        result["precision_4k_x4"] = precision_4k_x4(nets, weights, precs)
+    if not args.no_serving:
+        result["serving"] = {}
+        for spec in args.serving_loads.split(";"):
+            p, loads = spec.split(":")
+            result["serving"][p] = serving(nets["hdn0"], p, [float(x) for x in loads.split(",")], args.serving_seconds)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(frames_np, weights)
         result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)

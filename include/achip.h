@@ -112,6 +112,79 @@ ACHIP_API int achip_autotune(ACHipNet* net, int precision, int src_type, int dst
 /* Calling thread's last error message ("" if none). */
 ACHIP_API const char* achip_last_error(void);
 
+/* ---- Serving loop -----------------------------------------------------------------------
+ * No single counterpart in the reference: it replaces the frame/image dispatch that the
+ * reference's callers do one processor call at a time per thread (video
+ * video/src/Filter.cpp:33-121 filterParallel, the CLI's per-image loop cli/src/Main.cpp:
+ * 91-199, both through Processor::process, core/src/processor/Processor.cpp:199-276).
+ *
+ * A server owns one scheduler thread.  Requests of mixed kind, size and factor are admitted
+ * in arrival order as long as their device buffers fit an HBM budget; admitted one-off
+ * requests have their 2x passes batched, step by step, into persistent launches with a
+ * tile work-queue (a big pass is spread over several steps by tile rows, so that newly
+ * admitted work joins within one step); video-stream frames (fixed geometry per stream)
+ * replay a captured hipGraph on the stream's own HIP stream and overlap with the batched
+ * work on the device.  All samples are u8.  Outputs are bit-identical to processing the
+ * request alone (same kernels, same tiles). */
+typedef struct ACHipServer ACHipServer;
+
+enum {
+    ACHIP_REQ_LUMA = 0,     /* 1 channel */
+    ACHIP_REQ_YUV420 = 1,   /* planes Y (W x H), U and V (W/2 x H/2); W, H even */
+    ACHIP_REQ_RGB = 2       /* 3 interleaved channels */
+};
+
+typedef struct ACHipRequest {
+    int kind;               /* ACHIP_REQ_* */
+    int width, height;      /* input size */
+    int factor;             /* 2, 4 or 8: log2(factor) chained 2x passes on luma */
+    int stream;             /* >= 0: frame of video stream `stream` (YUV420, factor 2, one geometry per
+                               stream, hipGraph path); -1: one-off */
+    const void* src[3];     /* planes: LUMA/RGB src[0]; YUV420 Y, U, V.  Host memory (page-locked for an
+                               asynchronous upload) or device memory */
+    long src_stride[3];     /* byte strides */
+    void* dst[3];           /* output planes (host or device), or all NULL to keep the output in the
+                               server until achip_server_release (read it with achip_server_output) */
+    long dst_stride[3];
+} ACHipRequest;
+
+/* Output geometry: LUMA fW x fH; RGB fW x fH x 3; YUV420 Y fW x fH, U/V fW/2 x fH/2.  Chroma of
+ * RGB requests is resized by Catmull-Rom (the reference's multi-channel path); chroma of
+ * YUV420 requests is resized bilinearly. */
+
+typedef struct ACHipServerConfig {
+    int precision;          /* ACHIP_FP32 / FP16 / BF16 */
+    size_t hbm_budget;      /* bytes of device memory for request buffers (0: 16 GiB) */
+    int step_tiles;         /* tile budget of one batched step (0: a default for the precision) */
+    int max_workgroups;     /* grid cap of a batched step (0: a default that leaves CUs for frames) */
+} ACHipServerConfig;
+
+typedef struct ACHipServerStats {
+    long completed_frames, completed_oneoffs, failed;
+    double frame_p50_ms, frame_p99_ms, oneoff_p50_ms, oneoff_p99_ms;   /* submit -> output ready */
+    double output_mpix;     /* output pixels of completed requests, millions */
+    double busy_s;          /* wall time from the first submit to the last completion */
+    long steps, graph_replays;
+} ACHipServerStats;
+
+ACHIP_API int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipServer** out);
+/* Waits for all admitted work, then frees everything. */
+ACHIP_API void achip_server_destroy(ACHipServer* server);
+/* Queue a request (the server copies the descriptor; buffers must stay valid until done).
+ * Fills *ticket.  Returns ACHIP_EINVAL for malformed requests and ACHIP_ENOMEM when the
+ * request alone exceeds the budget. */
+ACHIP_API int achip_server_submit(ACHipServer* server, const ACHipRequest* request, unsigned long long* ticket);
+/* Block until the request is done (timeout_ms < 0: no limit).  Returns ACHIP_OK, the
+ * request's error, or ACHIP_EIO on time-out; *latency_ms (optional) gets submit -> done. */
+ACHIP_API int achip_server_wait(ACHipServer* server, unsigned long long ticket, double timeout_ms, double* latency_ms);
+/* Device address and byte stride of output plane `plane` of a finished request kept in the
+ * server (dst all NULL), valid until achip_server_release. */
+ACHIP_API int achip_server_output(ACHipServer* server, unsigned long long ticket, int plane, void** ptr, long* stride);
+/* Forget a finished request and free what it holds. */
+ACHIP_API int achip_server_release(ACHipServer* server, unsigned long long ticket);
+/* Statistics since creation or the last reset (reset != 0 clears them after reading). */
+ACHIP_API int achip_server_stats(ACHipServer* server, ACHipServerStats* stats, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,11 +87,12 @@ def test_batch_rejects_bad_workspace(net, cuda):
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
 def test_autotune_choice_is_used(net, cuda, prec):
     """achip_autotune times every configuration, and achip_pass then runs the chosen one:
-    its output equals that configuration's output bit for bit."""
-    best, ms = net.autotune(200, 120, precision=prec, src_dtype=torch.uint8)
+    its output equals that configuration's output bit for bit.  (The size class, 2^11 x 2^5,
+    is one no other test uses: the choice is remembered for the whole process.)"""
+    best, ms = net.autotune(1500, 24, precision=prec, src_dtype=torch.uint8)
     assert 0 <= best < len(ms) and all(t > 0 for t in ms)
     assert ms[best] == min(ms)
-    src = torch.from_numpy(synth.flat_shaded(91, 200, 120, dtype=np.uint8)).to(cuda)
+    src = torch.from_numpy(synth.flat_shaded(91, 1500, 24, dtype=np.uint8)).to(cuda)
     auto = net.upscale(src, precision=prec)
     expl = net.upscale(src, precision=prec, config=best)
     torch.cuda.synchronize()

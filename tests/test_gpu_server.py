@@ -1,0 +1,175 @@
+"""Serving loop (achip_server_*): continuous batching of mixed requests, HBM-budget
+admission and hipGraph-replayed video frames.  Each request's output in a mixed load must
+equal, bit for bit, the same request processed alone -- and the drop-in processor's output
+(Processor::process, core/src/processor/Processor.cpp:199-276) for the one-offs -- and stay
+within the oracle tolerance."""
+import numpy as np
+import pytest
+import torch
+
+import oracle_lib as OL
+from anime4kcpp_amd import hip, synth
+from anime4kcpp_amd import processor as P
+from test_gpu_fp32 import check_int
+from test_gpu_pipeline import oracle_pipeline
+
+pytestmark = pytest.mark.gpu
+
+VARIANT = "hdn1"
+
+
+@pytest.fixture(scope="module")
+def net(cuda):
+    n = hip.Net(VARIANT, 0)
+    yield n
+    n.close()
+
+
+def rgb(seed, w, h):
+    return np.ascontiguousarray(np.stack([synth.flat_shaded(seed + c, w, h, dtype=np.uint8) for c in range(3)], axis=2))
+
+
+def yuv(seed, w, h):
+    return [synth.flat_shaded(seed, w, h, dtype=np.uint8), synth.flat_shaded(seed + 1, w // 2, h // 2, dtype=np.uint8),
+            synth.flat_shaded(seed + 2, w // 2, h // 2, dtype=np.uint8)]
+
+
+def make_requests():
+    """(kind, planes, factor, stream): one-offs of every kind and factor, interleaved with the
+    frames of two video streams."""
+    reqs = [
+        (hip.REQ_RGB, [rgb(10, 200, 150)], 2, -1),
+        (hip.REQ_LUMA, [synth.flat_shaded(20, 333, 211, dtype=np.uint8)], 4, -1),
+        (hip.REQ_YUV420, yuv(30, 256, 144), 4, -1),
+        (hip.REQ_RGB, [rgb(40, 97, 61)], 4, -1),
+        (hip.REQ_LUMA, [synth.flat_shaded(50, 1, 1, dtype=np.uint8)], 2, -1),
+        (hip.REQ_RGB, [rgb(60, 700, 390)], 2, -1),
+    ]
+    frames = [(hip.REQ_YUV420, yuv(100 + 10 * i + s, 320 if s == 0 else 192, 180 if s == 0 else 108), 2, s)
+              for i in range(4) for s in range(2)]
+    mix = []
+    for i in range(max(len(reqs), len(frames))):   # interleave
+        if i < len(frames):
+            mix.append(frames[i])
+        if i < len(reqs):
+            mix.append(reqs[i])
+    return mix
+
+
+def out_planes(kind, planes, factor):
+    h, w = planes[0].shape[:2]
+    if kind == hip.REQ_RGB:
+        return [np.zeros((factor * h, factor * w, 3), np.uint8)]
+    if kind == hip.REQ_LUMA:
+        return [np.zeros((factor * h, factor * w), np.uint8)]
+    return [np.zeros((factor * h, factor * w), np.uint8), np.zeros((factor * h // 2, factor * w // 2), np.uint8),
+            np.zeros((factor * h // 2, factor * w // 2), np.uint8)]
+
+
+def run(server, reqs, together):
+    outs = [out_planes(k, p, f) for k, p, f, _ in reqs]
+    if together:
+        tickets = [server.submit(k, p, f, s, outs[i]) for i, (k, p, f, s) in enumerate(reqs)]
+        for t in tickets:
+            server.wait(t, 60000)
+    else:
+        for i, (k, p, f, s) in enumerate(reqs):
+            server.wait(server.submit(k, p, f, s, outs[i]), 60000)
+    return outs
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_mix_equals_alone(net, prec):
+    reqs = make_requests()
+    with hip.Server(net, prec) as alone_srv:
+        alone = run(alone_srv, reqs, together=False)
+    # a small step budget spreads every pass over several steps, mixing requests in each
+    with hip.Server(net, prec, step_tiles=16) as mix_srv:
+        mixed = run(mix_srv, reqs, together=True)
+        st = mix_srv.stats()
+    assert st["failed"] == 0
+    assert st["completed_frames"] == 8 and st["completed_oneoffs"] == 6
+    assert st["graph_replays"] == 8
+    assert st["steps"] > 6
+    for i, (a, m) in enumerate(zip(alone, mixed)):
+        for pa, pm in zip(a, m):
+            assert np.array_equal(pa, pm), f"request {i} differs between alone and mixed"
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_oneoffs_equal_processor(net, weights, prec):
+    """RGB one-offs through the server equal Processor::process on the same input (same kernels)."""
+    proc = P.Processor("hip" if prec == "fp32" else "hip-fp16", 0, f"acnet-legacy-{VARIANT}")
+    assert proc.ok(), proc.error()
+    with hip.Server(net, prec, step_tiles=64) as srv:
+        for seed, (w, h), factor in [(7, (200, 150), 2), (8, (97, 61), 4)]:
+            src = rgb(seed, w, h)
+            out = out_planes(hip.REQ_RGB, [src], factor)
+            srv.wait(srv.submit(hip.REQ_RGB, [src], factor, -1, out), 60000)
+            rc, want = proc.process(src, factor)
+            assert rc == 0
+            assert np.array_equal(out[0], want)
+            if prec == "fp32":
+                k, b = weights[VARIANT]
+                check_int(out[0], oracle_pipeline(src, factor, k, b), frac=0.999, maxlsb=3)
+
+
+def test_frames_match_oracle(net, weights):
+    """Frame path (hipGraph replays): Y within the u8 tolerance of the oracle pass, U/V within
+    the same tolerance of the oracle's bilinear 2x resize (the device contracts the filter's
+    multiply-adds into FMAs, the oracle does not); repeated replays give identical results."""
+    k, b = weights[VARIANT]
+    y, u, v = yuv(500, 320, 180)
+    with hip.Server(net, "fp32") as srv:
+        outs = []
+        for _ in range(3):
+            o = out_planes(hip.REQ_YUV420, [y, u, v], 2)
+            srv.wait(srv.submit(hip.REQ_YUV420, [y, u, v], 2, 5, o), 60000)
+            outs.append(o)
+        assert srv.stats()["graph_replays"] == 3
+    for o in outs[1:]:
+        for a, c in zip(outs[0], o):
+            assert np.array_equal(a, c)
+    check_int(outs[0][0], OL.oracle_pass(y, k, b))
+    for src, got in ((u, outs[0][1]), (v, outs[0][2])):
+        h, w = src.shape
+        want = np.zeros((2 * h, 2 * w), np.uint8)
+        assert OL.orc().orc_resize(src.ctypes.data, src.strides[0], w, h, 1, OL.U8, want.ctypes.data, want.strides[0],
+                                   2 * w, 2 * h, 0) == 0
+        d = np.abs(got.astype(int) - want.astype(int))
+        print(f"chroma {src.shape}: {np.count_nonzero(d)} of {d.size} samples differ, max {d.max()}")
+        check_int(got, want)
+
+
+def test_budget_admission_and_device_outputs(net):
+    """A budget that holds one request at a time still completes every request in order of
+    admission; outputs kept in the server are read back from device memory."""
+    srcs = [synth.flat_shaded(900 + i, 640, 360, dtype=np.uint8) for i in range(4)]
+    alone = [net.upscale(torch.from_numpy(s).cuda()).cpu().numpy() for s in srcs]
+    budget = 4 << 20       # one request: 2 MiB-rounded blocks for 640x360 and 1280x720
+    with hip.Server(net, "fp32", hbm_budget=budget, step_tiles=32) as srv:
+        tickets = [srv.submit(hip.REQ_LUMA, [s], 2) for s in srcs]
+        for t, want in zip(tickets, alone):
+            srv.wait(t, 60000)
+            got = srv.fetch(t, 0, want.shape)
+            assert np.array_equal(got, want)
+            srv.release(t)     # frees the budget for the next request
+        st = srv.stats()
+    assert st["completed_oneoffs"] == 4 and st["failed"] == 0
+
+
+def test_invalid_requests(net):
+    y = synth.flat_shaded(1, 64, 48, dtype=np.uint8)
+    with hip.Server(net, "fp16", hbm_budget=64 << 20) as srv:
+        with pytest.raises(hip._native.AchipError):
+            srv.submit(hip.REQ_LUMA, [y], 3)                        # factor not a power of two
+        with pytest.raises(hip._native.AchipError):
+            srv.submit(hip.REQ_LUMA, [y], 2, stream=0)              # frames are YUV420
+        big = synth.flat_shaded(2, 4096, 4096, dtype=np.uint8)
+        with pytest.raises(hip._native.AchipError):
+            srv.submit(hip.REQ_LUMA, [big], 4)                      # alone exceeds the budget
+        f = yuv(3, 64, 48)
+        srv.wait(srv.submit(hip.REQ_YUV420, f, 2, 1, out_planes(hip.REQ_YUV420, f, 2)), 60000)
+        g = yuv(4, 96, 48)
+        with pytest.raises(hip._native.AchipError):
+            srv.submit(hip.REQ_YUV420, g, 2, 1)                     # a stream keeps its geometry
