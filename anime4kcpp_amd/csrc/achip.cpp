@@ -241,10 +241,10 @@ int persistent_grid(const void* kernel, int threads, int tiles, int max_wg)
 hipError_t launch_pass(const ACHipNet* net, int precision, int config, int stype, int dtype, const void* src,
                        long sstride, int w, int h, void* dst, long dstride, hipStream_t stream)
 {
-    if (precision == ACHIP_FP32 && config == 0)
-        return launch_mfma32(stype, dtype, src, sstride, w, h, dst, dstride, net->wts, net->wa32, stream);
+    if (precision == ACHIP_FP32 && config < kMfma32ConfigCount)
+        return launch_mfma32(config, stype, dtype, src, sstride, w, h, dst, dstride, net->wts, net->wa32, stream);
     if (precision == ACHIP_FP32)
-        return launch_fp32(config - 1, stype, dtype, src, sstride, w, h, dst, dstride, net->wts, stream);
+        return launch_fp32(config - kMfma32ConfigCount, stype, dtype, src, sstride, w, h, dst, dstride, net->wts, stream);
     return launch_mfma(precision == ACHIP_BF16, config, stype, dtype, src, sstride, w, h, dst, dstride, net->wts,
                        precision == ACHIP_BF16 ? net->wabf16 : net->wa16, net->wa32, stream);
 }
@@ -252,10 +252,11 @@ hipError_t launch_pass(const ACHipNet* net, int precision, int config, int stype
 hipError_t launch_batch(const ACHipNet* net, int precision, int config, int stype, int dtype, const BatchDesc* desc,
                         int n, int total, int* counter, int max_wg, hipStream_t stream)
 {
-    if (precision == ACHIP_FP32 && config == 0)
-        return launch_mfma32_batch(stype, dtype, desc, n, total, counter, net->wts, net->wa32, max_wg, stream);
+    if (precision == ACHIP_FP32 && config < kMfma32ConfigCount)
+        return launch_mfma32_batch(config, stype, dtype, desc, n, total, counter, net->wts, net->wa32, max_wg, stream);
     if (precision == ACHIP_FP32)
-        return launch_fp32_batch(config - 1, stype, dtype, desc, n, total, counter, net->wts, max_wg, stream);
+        return launch_fp32_batch(config - kMfma32ConfigCount, stype, dtype, desc, n, total, counter, net->wts, max_wg,
+                                 stream);
     return launch_mfma_batch(precision == ACHIP_BF16, config, stype, dtype, desc, n, total, counter, net->wts,
                              precision == ACHIP_BF16 ? net->wabf16 : net->wa16, net->wa32, max_wg, stream);
 }
@@ -358,7 +359,7 @@ int achip_net_device(const ACHipNet* net) { return net ? net->device : ACHIP_EIN
 int achip_config_count(int precision)
 {
     switch (precision) {
-    case ACHIP_FP32: return achip::kFp32ConfigCount + 1;   // the FP32 MFMA kernel, then the VALU ones
+    case ACHIP_FP32: return achip::kMfma32ConfigCount + achip::kFp32ConfigCount;   // FP32 MFMA, then VALU
     case ACHIP_FP16:
     case ACHIP_BF16: return achip::kMfmaConfigCount;
     default: return 0;
@@ -368,9 +369,9 @@ int achip_config_count(int precision)
 int achip_config_info(int precision, int i, int* tw, int* th, int* nt, int* p, int* lds_bytes)
 {
     if (i < 0 || i >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d for precision %d", i, precision);
-    const achip::Fp32Config mfma32{ 64, 32, 512, 2, achip::kMfma32LdsBytes };
-    const achip::Fp32Config& c = precision != ACHIP_FP32 ? achip::kMfmaConfigs[i]
-                               : (i == 0 ? mfma32 : achip::kFp32Configs[i - 1]);
+    const achip::Fp32Config& c = precision != ACHIP_FP32        ? achip::kMfmaConfigs[i]
+                               : i < achip::kMfma32ConfigCount ? achip::kMfma32Configs[i]
+                                                               : achip::kFp32Configs[i - achip::kMfma32ConfigCount];
     if (tw) *tw = c.tw;
     if (th) *th = c.th;
     if (nt) *nt = c.nt;

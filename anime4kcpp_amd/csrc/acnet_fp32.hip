@@ -262,8 +262,9 @@ __global__ __launch_bounds__(NT) void acnet_fp32_fused(const void* __restrict__ 
     __shared__ float4 fa[C::NPIX];
     __shared__ float4 fb[C::NPIX];
     __shared__ float img[C::IW * C::IH];
-    fp32_tile<TIN, TOUT, TW, TH, NT, P>(src, sstride, W, H, dst, dstride, wts, zero,
-                                        blockIdx.x * TW, blockIdx.y * TH, fa, fb, img);
+    const int t = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int ty = t / gridDim.x, tx = t - ty * gridDim.x;
+    fp32_tile<TIN, TOUT, TW, TH, NT, P>(src, sstride, W, H, dst, dstride, wts, zero, tx * TW, ty * TH, fa, fb, img);
 }
 
 // Persistent batched form: every workgroup takes the next tile of the batch from a

@@ -30,6 +30,18 @@ __device__ inline BatchDesc find_tile(const BatchDesc* __restrict__ desc, int n,
     return desc[lo];
 }
 
+// Tile of a non-persistent launch for workgroup b of T (row-major tile numbering).  The
+// hardware hands consecutive workgroups to the 8 XCDs in turn; this permutation gives the
+// workgroups that land on one XCD a contiguous band of tiles, so the input rows a band
+// shares (halo) are fetched into that XCD's L2 once instead of into every XCD's.
+__device__ inline int xcd_tile(int b, int T)
+{
+    constexpr int kXcd = 8;
+    const int q = T / kXcd, r = T - q * kXcd;
+    const int x = b % kXcd, i = b / kXcd;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // Workgroups of a persistent launch: resident workgroups per CU x CUs, at most `tiles`, and
 // at most max_wg when max_wg > 0 (leaves CUs to work on other streams).
 int persistent_grid(const void* kernel, int threads, int tiles, int max_wg);
@@ -57,12 +69,13 @@ extern const int kMfmaConfigCount;
 hipError_t launch_mfma(int bf16, int cfg, int stype, int dtype, const void* src, long sstride, int W, int H,
                        void* dst, long dstride, const float* wts, const void* wa, const float* wa0, hipStream_t stream);
 
-// Fused pass with every conv layer on FP32 MFMA (tile 64 x 32, 512 threads); wa32 is the
-// per-lane FP32 A-operand blob (acnet_device.hpp layout).
-extern const int kMfma32LdsBytes;
-hipError_t launch_mfma32(int stype, int dtype, const void* src, long sstride, int W, int H,
+// Fused pass with every conv layer on FP32 MFMA (tile 64 x 32; configuration 0: 1024
+// threads, 1: 512); wa32 is the per-lane FP32 A-operand blob (acnet_device.hpp layout).
+extern const Fp32Config kMfma32Configs[];
+extern const int kMfma32ConfigCount;
+hipError_t launch_mfma32(int cfg, int stype, int dtype, const void* src, long sstride, int W, int H,
                          void* dst, long dstride, const float* wts, const float* wa32, hipStream_t stream);
-hipError_t launch_mfma32_batch(int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
+hipError_t launch_mfma32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
                                const float* wts, const float* wa32, int max_wg, hipStream_t stream);
 
 // Colour split RGB(A) -> Y + UV(A), merge back, and resize (mode 0 bilinear, 1 Catmull-Rom);

@@ -311,8 +311,9 @@ __global__ __launch_bounds__(NT) void acnet_mfma_fused(const void* __restrict__ 
     using C = MfmaTile<TW, TH, NT>;
     __shared__ uint4 plane[C::PW * C::PH];        // 8 x 16-bit channels per pixel
     __shared__ TIN img[C::IW * C::IH];
-    mfma_tile<T16, TIN, TOUT, TW, TH, NT>(src, sstride, W, H, dst, dstride, wts, wa, wa0,
-                                          blockIdx.x * TW, blockIdx.y * TH, plane, img);
+    const int t = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int ty = t / gridDim.x, tx = t - ty * gridDim.x;
+    mfma_tile<T16, TIN, TOUT, TW, TH, NT>(src, sstride, W, H, dst, dstride, wts, wa, wa0, tx * TW, ty * TH, plane, img);
 }
 
 // Persistent batched form (see acnet_fp32_batch).
@@ -410,8 +411,8 @@ static hipError_t launch_types(int stype, int dtype, const void* src, long ss, i
 }
 
 const Fp32Config kMfmaConfigs[] = {
-    { 64, 64, 1024, 2, MfmaTile<64, 64, 1024>::lds_bytes(4) },   // LDS for f32 input; less for narrower types
-    { 64, 32, 512, 2, MfmaTile<64, 32, 512>::lds_bytes(4) },
+    { 64, 64, 1024, 0, MfmaTile<64, 64, 1024>::lds_bytes(4) },   // LDS for f32 input; less for narrower types
+    { 64, 32, 512, 0, MfmaTile<64, 32, 512>::lds_bytes(4) },
 };
 const int kMfmaConfigCount = sizeof(kMfmaConfigs) / sizeof(kMfmaConfigs[0]);
 

@@ -30,8 +30,35 @@ struct Mfma32Tile {
     static constexpr int IW = 16 * kSeg + 2 + 8;
     static constexpr int IH = TH + 18 + 2;
     static constexpr int kStage = (IW * IH + NT - 1) / NT;          // input elements per thread
-    static constexpr int kMaxPairs = ((BH + 1) / 2 + NW - 1) / NW;  // row pairs per wave and layer
+    static constexpr int items(int m) { return ((TW + 2 * m + 15) / 16) * ((TH + 2 * m + 1) / 2); }
+    static constexpr int kMaxItems = (items(7) + NW - 1) / NW;      // items per wave, layers 1..7
+    // with 512 threads (2 waves per SIMD) each wave loads its next item's operands before
+    // the current item's MFMAs; with 1024 the other waves hide the latency and the
+    // registers are needed elsewhere
+    static constexpr bool kPrefetch = NT <= 512;
     static constexpr int lds_bytes(int elem) { return 2 * PW * PH * 16 + (IW * IH * elem + 15) / 16 * 16; }
+};
+
+// The items of one wave in a layer: wave, wave + NW, ... over (row pair, 16-pixel segment)
+// in row-pair-major order, stepped without divisions.
+struct ItemWalk {
+    int pair, seg, dq, dr, nseg;
+    __device__ ItemWalk(int wave, int nw, int nseg_) : nseg(nseg_)
+    {
+        pair = wave / nseg;
+        seg = wave - pair * nseg;
+        dq = nw / nseg;
+        dr = nw - dq * nseg;
+    }
+    __device__ void next()
+    {
+        pair += dq;
+        seg += dr;
+        if (seg >= nseg) {
+            seg -= nseg;
+            pair++;
+        }
+    }
 };
 
 // Replicate ring and zeroed border tiles: see fill_ring in acnet_mfma.hip.
@@ -141,8 +168,7 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
     }
     __syncthreads();
 
-    // Each wave takes whole row pairs (pair = wave, wave + NW, ...) and their 16-pixel items
-    // left to right, loading the next item's operands before the current item's MFMAs.
+    // Items (16 pixels x 2 rows) are dealt to the waves round-robin (ItemWalk).
     // this lane's write target: channels 0-3 (pa) or 4-7 (pb) of pixel (column j, row ro)
     float4* const pw = half ? pb : pa;
 
@@ -151,27 +177,23 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
         const int gx0 = max(x0 - 8, 0), gy0 = max(y0 - 8, 0);
         const int gx1 = min(x0 + TW + 8, W), gy1 = min(y0 + TH + 8, H);
         const int nseg = (gx1 - gx0 + 15) >> 4;
-        const int npairs = (gy1 - gy0 + 1) >> 1;
+        const int nitems = nseg * ((gy1 - gy0 + 1) >> 1);
         const float a0 = wa[lane], a1 = wa[64 + lane], a2 = wa[128 + lane];
         const float* bl = wts + kOffB0 + 4 * half;
         const f32x4m bias = { bl[0], bl[1], bl[2], bl[3] };
         const int lane_rd = g * C::IW + j;
-        for (int pair = wave; pair < npairs; pair += C::NW) {
+        ItemWalk it(wave, C::NW, nseg);
+        for (int item = wave; item < nitems; item += C::NW, it.next()) {
             // img holds pixel (x, y) at ((y - y0 + 9) * IW + x - x0 + 9); B row y-1+g, column x+j+t-1
-            const TIN* p = img + (gy0 + 2 * pair - 1 - (y0 - 9)) * C::IW + (gx0 - 1 - (x0 - 9)) + lane_rd;
-            const int oy = gy0 + 2 * pair + ro;
-#pragma unroll
-            for (int s = 0; s < C::kSeg; s++) {
-                if (s < nseg) {
-                    f32x4m acc = mma32(a0, to_float<TIN>(p[16 * s]), bias);
-                    acc = mma32(a1, to_float<TIN>(p[16 * s + 1]), acc);
-                    acc = mma32(a2, to_float<TIN>(p[16 * s + 2]), acc);
-                    const int gx = gx0 + 16 * s + j;
-                    if (gx < gx1 && oy < gy1)
-                        pw[(oy - by) * C::PW + gx - bx] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f),
-                                                                      fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
-                }
-            }
+            const TIN* p = img + (gy0 + 2 * it.pair - 1 - (y0 - 9)) * C::IW + (gx0 + 16 * it.seg - 1 - (x0 - 9)) + lane_rd;
+            f32x4m acc = mma32(a0, to_float<TIN>(p[0]), bias);
+            acc = mma32(a1, to_float<TIN>(p[1]), acc);
+            acc = mma32(a2, to_float<TIN>(p[2]), acc);
+            const int gx = gx0 + 16 * it.seg + j;
+            const int oy = gy0 + 2 * it.pair + ro;
+            if (gx < gx1 && oy < gy1)
+                pw[(oy - by) * C::PW + gx - bx] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f),
+                                                              fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
         }
     }
     // layer 1's operands
@@ -198,24 +220,31 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
         const int gx0 = max(x0 - m, 0), gy0 = max(y0 - m, 0);
         const int gx1 = min(x0 + TW + m, W), gy1 = min(y0 + TH + m, H);
         const int nseg = (gx1 - gx0 + 15) >> 4;
-        const int npairs = (gy1 - gy0 + 1) >> 1;
-        float4 res[C::kMaxPairs][C::kSeg];
+        const int nitems = nseg * ((gy1 - gy0 + 1) >> 1);
+        const int base0 = (gy0 - 1 - by) * C::PW + (gx0 - 1 - bx) + lane_rd;
+        float4 res[C::kMaxItems];
+        {
+            ItemWalk it(wave, C::NW, nseg);
+            Item32 q;
+            if constexpr (C::kPrefetch)
+                if (wave < nitems) q = load_item(pa, pb, base0 + 2 * it.pair * C::PW + 16 * it.seg);
 #pragma unroll
-        for (int k = 0; k < C::kMaxPairs; k++) {
-            const int pair = wave + k * C::NW;
-            if (pair < npairs) {
-                const int base = (gy0 + 2 * pair - 1 - by) * C::PW + (gx0 - 1 - bx) + lane_rd;
-                Item32 q = load_item(pa, pb, base);
-#pragma unroll
-                for (int s = 0; s < C::kSeg; s++) {
-                    if (s < nseg) {
+            for (int ii = 0; ii < C::kMaxItems; ii++) {
+                const int item = wave + ii * C::NW;
+                if (item < nitems) {
+                    if constexpr (C::kPrefetch) {
+                        ItemWalk nx = it;
+                        nx.next();
                         Item32 nq;
-                        if (s + 1 < nseg) nq = load_item(pa, pb, base + 16 * (s + 1));
+                        if (item + C::NW < nitems) nq = load_item(pa, pb, base0 + 2 * nx.pair * C::PW + 16 * nx.seg);
                         const f32x4m acc = item24(q, a, bias);
-                        res[k][s] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f),
-                                                fmaxf(acc[3], 0.f));
+                        res[ii] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
                         q = nq;
+                    } else {
+                        const f32x4m acc = item24(load_item(pa, pb, base0 + 2 * it.pair * C::PW + 16 * it.seg), a, bias);
+                        res[ii] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
                     }
+                    it.next();
                 }
             }
         }
@@ -227,15 +256,17 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
             bias = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
         }
         __syncthreads();
+        {
+            ItemWalk it(wave, C::NW, nseg);
 #pragma unroll
-        for (int k = 0; k < C::kMaxPairs; k++) {
-            const int pair = wave + k * C::NW;
-            const int oy = gy0 + 2 * pair + ro;
-            if (pair < npairs && oy < gy1) {
-                float4* w = pw + (oy - by) * C::PW + gx0 - bx + j;
-#pragma unroll
-                for (int s = 0; s < C::kSeg; s++)
-                    if (s < nseg && gx0 + 16 * s + j < gx1) w[16 * s] = res[k][s];
+            for (int ii = 0; ii < C::kMaxItems; ii++) {
+                const int item = wave + ii * C::NW;
+                if (item < nitems) {
+                    const int gx = gx0 + 16 * it.seg + j;
+                    const int oy = gy0 + 2 * it.pair + ro;
+                    if (gx < gx1 && oy < gy1) pw[(oy - by) * C::PW + gx - bx] = res[ii];
+                    it.next();
+                }
             }
         }
         __syncthreads();
@@ -249,39 +280,31 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
     {
         const int gx1 = min(x0 + TW, W), gy1 = min(y0 + TH, H);
         const int nseg = (gx1 - x0 + 15) >> 4;
-        const int npairs = (gy1 - y0 + 1) >> 1;
+        const int nitems = nseg * ((gy1 - y0 + 1) >> 1);
         const float* k2 = wts + kOffK2 + 16 * half;
         float kd[4][4];
 #pragma unroll
         for (int c = 0; c < 4; c++)
 #pragma unroll
             for (int q = 0; q < 4; q++) kd[c][q] = k2[c * 4 + q];
-        for (int pair = wave; pair < npairs; pair += C::NW) {
-            // region origin is plane-local (8, 8)
-            const int base = (7 + 2 * pair) * C::PW + 7 + lane_rd;
-            const int oy = y0 + 2 * pair + ro;
-            Item32 q = load_item(pa, pb, base);
+        // region origin is plane-local (8, 8)
+        const int base0 = 7 * C::PW + 7 + lane_rd;
+        ItemWalk it(wave, C::NW, nseg);
+        for (int item = wave; item < nitems; item += C::NW, it.next()) {
+            const f32x4m acc = item24(load_item(pa, pb, base0 + 2 * it.pair * C::PW + 16 * it.seg), a, bias);
+            const float v0 = fmaxf(acc[0], 0.f), v1 = fmaxf(acc[1], 0.f);
+            const float v2 = fmaxf(acc[2], 0.f), v3 = fmaxf(acc[3], 0.f);
+            float sum[4];
 #pragma unroll
-            for (int s = 0; s < (TW + 15) / 16; s++) {
-                if (s < nseg) {
-                    Item32 nq;
-                    if (s + 1 < nseg) nq = load_item(pa, pb, base + 16 * (s + 1));
-                    const f32x4m acc = item24(q, a, bias);
-                    const float v0 = fmaxf(acc[0], 0.f), v1 = fmaxf(acc[1], 0.f);
-                    const float v2 = fmaxf(acc[2], 0.f), v3 = fmaxf(acc[3], 0.f);
-                    float sum[4];
+            for (int q4 = 0; q4 < 4; q4++)
+                sum[q4] = fmaf(v3, kd[3][q4], fmaf(v2, kd[2][q4], fmaf(v1, kd[1][q4], v0 * kd[0][q4])));
 #pragma unroll
-                    for (int q4 = 0; q4 < 4; q4++)
-                        sum[q4] = fmaf(v3, kd[3][q4], fmaf(v2, kd[2][q4], fmaf(v1, kd[1][q4], v0 * kd[0][q4])));
-#pragma unroll
-                    for (int q4 = 0; q4 < 4; q4++) sum[q4] += __shfl_xor(sum[q4], 16);
-                    const int gx = x0 + 16 * s + j;
-                    if (gx < gx1 && oy < gy1) {
-                        TOUT* orow = row_ptr<TOUT>(dst, dstride, 2 * oy + half);
-                        store_pair<TOUT>(orow + 2 * gx, half ? sum[2] : sum[0], half ? sum[3] : sum[1]);
-                    }
-                    q = nq;
-                }
+            for (int q4 = 0; q4 < 4; q4++) sum[q4] += __shfl_xor(sum[q4], 16);
+            const int gx = x0 + 16 * it.seg + j;
+            const int oy = y0 + 2 * it.pair + ro;
+            if (gx < gx1 && oy < gy1) {
+                TOUT* orow = row_ptr<TOUT>(dst, dstride, 2 * oy + half);
+                store_pair<TOUT>(orow + 2 * gx, half ? sum[2] : sum[0], half ? sum[3] : sum[1]);
             }
         }
     }
@@ -296,8 +319,9 @@ __global__ __launch_bounds__(NT) void acnet_mfma32_fused(const void* __restrict_
     __shared__ float4 pa[C::PW * C::PH];
     __shared__ float4 pb[C::PW * C::PH];
     __shared__ TIN img[C::IW * C::IH];
-    mfma32_tile<TIN, TOUT, TW, TH, NT>(src, sstride, W, H, dst, dstride, wts, wa,
-                                       blockIdx.x * TW, blockIdx.y * TH, pa, pb, img);
+    const int t = xcd_tile(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int ty = t / gridDim.x, tx = t - ty * gridDim.x;
+    mfma32_tile<TIN, TOUT, TW, TH, NT>(src, sstride, W, H, dst, dstride, wts, wa, tx * TW, ty * TH, pa, pb, img);
 }
 
 template <typename TIN, typename TOUT, int TW, int TH, int NT>
@@ -370,23 +394,38 @@ static hipError_t launch_batch_one(const BatchDesc* desc, int n, int total, int*
     default: return hipErrorInvalidValue;                                              \
     }
 
-hipError_t launch_mfma32(int stype, int dtype, const void* src, long ss, int W, int H,
+hipError_t launch_mfma32(int cfg, int stype, int dtype, const void* src, long ss, int W, int H,
                          void* dst, long ds, const float* wts, const float* wa, hipStream_t stream)
 {
+    if (cfg == 0) {
+#define CALL(TI, TO) launch_one<TI, TO, 64, 32, 1024>(src, ss, W, H, dst, ds, wts, wa, stream)
+        ACHIP_TYPES(CALL)
+#undef CALL
+    }
 #define CALL(TI, TO) launch_one<TI, TO, 64, 32, 512>(src, ss, W, H, dst, ds, wts, wa, stream)
     ACHIP_TYPES(CALL)
 #undef CALL
 }
 
-hipError_t launch_mfma32_batch(int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
+hipError_t launch_mfma32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
                                const float* wts, const float* wa, int max_wg, hipStream_t stream)
 {
+    if (cfg == 0) {
+#define CALL(TI, TO) launch_batch_one<TI, TO, 64, 32, 1024>(desc, n, total, counter, wts, wa, max_wg, stream)
+        ACHIP_TYPES(CALL)
+#undef CALL
+    }
 #define CALL(TI, TO) launch_batch_one<TI, TO, 64, 32, 512>(desc, n, total, counter, wts, wa, max_wg, stream)
     ACHIP_TYPES(CALL)
 #undef CALL
 }
 #undef ACHIP_TYPES
 
-const int kMfma32LdsBytes = Mfma32Tile<64, 32, 512>::lds_bytes(4);   // f32 input; less for narrower types
+// f32 input; less LDS for narrower types
+const Fp32Config kMfma32Configs[] = {
+    { 64, 32, 1024, 0, Mfma32Tile<64, 32, 1024>::lds_bytes(4) },
+    { 64, 32, 512, 0, Mfma32Tile<64, 32, 512>::lds_bytes(4) },
+};
+const int kMfma32ConfigCount = sizeof(kMfma32Configs) / sizeof(kMfma32Configs[0]);
 
 } // namespace achip

@@ -153,7 +153,7 @@ def configs(precision="fp32"):
     for i in range(L.achip_config_count(prec)):
         vals = [ctypes.c_int() for _ in range(5)]
         check(L.achip_config_info(prec, i, *[ctypes.byref(v) for v in vals]), "achip_config_info")
-        out.append(dict(zip(("tw", "th", "nt", "p", "lds_bytes"), [v.value for v in vals])))
+        out.append(dict(zip(("tw", "th", "nt", "p", "lds_bytes"), [v.value for v in vals])))   # p = 0: MFMA kernel
     return out
 
 

@@ -137,7 +137,8 @@ def batched_uneven(nets, prec, seed=2024, n=64):
         "one_launch": {"images_per_s": round(n / tb, 1), "mpix_per_s": round(out_mpix / tb, 1), "ms": round(tb * 1e3, 3)},
         "separate_launches": {"images_per_s": round(n / sep, 1), "mpix_per_s": round(out_mpix / sep, 1), "ms": round(sep * 1e3, 3)},
         "autotune_sweep_ms": {f"{hip.configs(prec)[c]['tw']}x{hip.configs(prec)[c]['th']}/{hip.configs(prec)[c]['nt']}thr"
-                              + ("/mfma" if prec != "fp32" or c == 0 else "/valu"): round(t * 1e3, 3) for c, t in sweep.items()},
+                              + ("/mfma" if hip.configs(prec)[c]["p"] == 0 else "/valu"): round(t * 1e3, 3)
+                              for c, t in sweep.items()},
         "chosen_config": best,
     }
 
@@ -366,7 +367,7 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "engine": (("FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulate" if (args.config or 0) == 0 else
+            "engine": (("FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulate" if kcfg["p"] == 0 else
                         "FP32 VALU (packed FP32 disabled; non-packed FMA caps at half the FP32 peak)")
                        if main_prec == "fp32" else f"{main_prec.upper()} MFMA, FP32 accumulate"),
             "achieved": round(achieved, 2),

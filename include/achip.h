@@ -31,7 +31,7 @@ extern "C" {
 #endif
 
 /* arithmetic of the 8->8 layers */
-#define ACHIP_FP32 0   /* fused FP32 VALU kernel */
+#define ACHIP_FP32 0   /* fused FP32 kernel: every conv on FP32 MFMA (default) or on FP32 VALU */
 #define ACHIP_FP16 1   /* FP16 MFMA implicit GEMM, FP32 accumulate */
 #define ACHIP_BF16 2   /* BF16 MFMA implicit GEMM, FP32 accumulate */
 
@@ -97,7 +97,9 @@ ACHIP_API int achip_pass_batch(ACHipNet* net, int precision, int config, int src
                                void* stream);
 
 /* Kernel configurations of a precision: tile width/height (input pixels), threads per
- * workgroup, pixels per thread run, LDS bytes.  Returns the count, or fills config i. */
+ * workgroup, pixels per thread run of the FP32 VALU kernel (0 for the MFMA kernels), LDS
+ * bytes for f32 input.  FP32: the FP32-MFMA kernel first (configuration 0 is the default),
+ * then the FP32 VALU kernel.  Returns the count, or fills config i. */
 ACHIP_API int achip_config_count(int precision);
 ACHIP_API int achip_config_info(int precision, int i, int* tw, int* th, int* nt, int* p, int* lds_bytes);
 
