@@ -117,8 +117,8 @@ struct Req {
     St st = St::Pending;
     int err = ACHIP_OK;
     std::string msg;
-    // one-off buffers: in (RGB), luma[0..passes], uv / uvs (RGB) or u0 v0 u1 v1 (YUV420), out (RGB)
-    Plane in, luma[4], uv, uvs, u0, v0, u1, v1, out;
+    // one-off buffers: in (RGB), luma[0..passes], uv (RGB) or u0 v0 u1 v1 (YUV420), out (RGB)
+    Plane in, luma[4], uv, u0, v0, u1, v1, out;
     int pass = 0, next_row = 0;
     hipEvent_t ev_up = nullptr;       // upload done
     hipEvent_t ev_done = nullptr;     // everything done (own event, or a step's / a frame's)
@@ -209,7 +209,7 @@ size_t request_bytes(const ACHipRequest& r, int passes)
     size_t b = 0;
     for (int k = 0; k <= passes; k++) b += round_block((W << k) * (H << k));
     if (r.kind == ACHIP_REQ_RGB)
-        b += round_block(3 * W * H) + round_block(2 * W * H) + round_block(2 * f * W * f * H) + round_block(3 * f * W * f * H);
+        b += round_block(3 * W * H) + round_block(2 * W * H) + round_block(3 * f * W * f * H);
     else if (r.kind == ACHIP_REQ_YUV420)
         b += 2 * round_block(W / 2 * (H / 2)) + 2 * round_block(f * W / 2 * (f * H / 2));
     return b;
@@ -238,7 +238,7 @@ int ACHipServer::alloc(Plane& p, int w, int h, int ch)
 
 void ACHipServer::free_req(Req& q)
 {
-    for (Plane* p : { &q.in, &q.luma[0], &q.luma[1], &q.luma[2], &q.luma[3], &q.uv, &q.uvs, &q.u0, &q.v0, &q.u1, &q.v1, &q.out }) {
+    for (Plane* p : { &q.in, &q.luma[0], &q.luma[1], &q.luma[2], &q.luma[3], &q.uv, &q.u0, &q.v0, &q.u1, &q.v1, &q.out }) {
         pool.put(p->ptr);
         p->ptr = nullptr;
     }
@@ -364,7 +364,6 @@ int ACHipServer::start_oneoff(const std::shared_ptr<Req>& q)
     if (!e && r.kind == ACHIP_REQ_RGB) {
         e = alloc(q->in, W, H, 3);
         if (!e) e = alloc(q->uv, W, H, 2);
-        if (!e) e = alloc(q->uvs, f * W, f * H, 2);
         if (!e) e = alloc(q->out, f * W, f * H, 3);
     } else if (!e && r.kind == ACHIP_REQ_YUV420) {
         e = alloc(q->u0, W / 2, H / 2, 1);
@@ -439,11 +438,8 @@ bool ACHipServer::build_step()
         const int P = q->passes;
         const Plane& y = q->luma[P];
         if (q->r.kind == ACHIP_REQ_RGB) {
-            e = achip::launch_resize(kU8, 2, 1, q->uv.ptr, q->uv.stride, q->uv.w, q->uv.h, q->uvs.ptr, q->uvs.stride, q->uvs.w,
-                                     q->uvs.h, batch);
-            if (e == hipSuccess)
-                e = achip::launch_color_merge(kU8, 3, y.ptr, y.stride, q->uvs.ptr, q->uvs.stride, y.w, y.h, q->out.ptr,
-                                              q->out.stride, batch);
+            e = achip::launch_resize_merge(kU8, 3, 1, y.ptr, y.stride, q->uv.ptr, q->uv.stride, q->uv.w, q->uv.h, y.w, y.h,
+                                           q->out.ptr, q->out.stride, batch);
         } else if (q->r.kind == ACHIP_REQ_YUV420) {
             e = achip::launch_resize(kU8, 1, 0, q->u0.ptr, q->u0.stride, q->u0.w, q->u0.h, q->u1.ptr, q->u1.stride, q->u1.w,
                                      q->u1.h, batch);

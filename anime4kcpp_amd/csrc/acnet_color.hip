@@ -114,36 +114,121 @@ __device__ inline int resize_taps(int mode, int o, int n_in, int n_out, float* w
     return n;
 }
 
-template <typename T, int C>
-__global__ __launch_bounds__(256) void resize_kernel(const void* __restrict__ src, long ss, int sw, int sh,
-                                                    void* __restrict__ dst, long ds, int dw, int dh, int mode)
+// A workgroup computes a 64 x 16 block of outputs (thread: one column, four rows).  The taps
+// of the block's 64 columns and 16 rows are computed once into LDS (resize_taps is the
+// costly part: filter, normalising division), then every output sums its taps in the order
+// above.
+constexpr int kRsW = 64, kRsH = 16;
+
+struct ResizeTaps {
+    float wx[kRsW][kMaxTaps];
+    float wy[kRsH][kMaxTaps];
+    int fx[kRsW], nx[kRsW], fy[kRsH], ny[kRsH];
+};
+
+__device__ inline void block_taps(ResizeTaps& t, int mode, int sw, int sh, int dw, int dh)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int yo = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= dw || yo >= dh) return;
-    float wx[kMaxTaps], wy[kMaxTaps];
-    int fx, fy;
-    const int nx = resize_taps(mode, x, sw, dw, wx, fx);
-    const int ny = resize_taps(mode, yo, sh, dh, wy, fy);
-    float acc[C];
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * kRsW, y0 = blockIdx.y * kRsH;
+    if (tid < kRsW) {
+        const int x = min(x0 + tid, dw - 1);
+        t.nx[tid] = resize_taps(mode, x, sw, dw, t.wx[tid], t.fx[tid]);
+    } else if (tid < kRsW + kRsH) {
+        const int r = tid - kRsW;
+        const int y = min(y0 + r, dh - 1);
+        t.ny[r] = resize_taps(mode, y, sh, dh, t.wy[r], t.fy[r]);
+    }
+    __syncthreads();
+}
+
+// resized value of channel block [C] at output (column tx, row ty) of the block
+template <typename T, int C>
+__device__ inline void resample(const ResizeTaps& t, const void* __restrict__ src, long ss, int sw, int sh, int tx,
+                                int ty, float (&acc)[C])
+{
 #pragma unroll
     for (int c = 0; c < C; c++) acc[c] = 0.0f;
-    for (int j = 0; j < ny; j++) {
-        const T* row = row_ptr<T>(src, ss, clampi(fy + j, 0, sh - 1));
+    const int nx = t.nx[tx], fx = t.fx[tx];
+    for (int j = 0; j < t.ny[ty]; j++) {
+        const T* row = row_ptr<T>(src, ss, clampi(t.fy[ty] + j, 0, sh - 1));
         float h[C];
 #pragma unroll
         for (int c = 0; c < C; c++) h[c] = 0.0f;
         for (int i = 0; i < nx; i++) {
             const T* p = row + static_cast<long>(clampi(fx + i, 0, sw - 1)) * C;
 #pragma unroll
-            for (int c = 0; c < C; c++) h[c] = __fadd_rn(h[c], __fmul_rn(wx[i], to_float<T>(p[c])));
+            for (int c = 0; c < C; c++) h[c] = __fadd_rn(h[c], __fmul_rn(t.wx[tx][i], to_float<T>(p[c])));
         }
 #pragma unroll
-        for (int c = 0; c < C; c++) acc[c] = __fadd_rn(acc[c], __fmul_rn(wy[j], h[c]));
+        for (int c = 0; c < C; c++) acc[c] = __fadd_rn(acc[c], __fmul_rn(t.wy[ty][j], h[c]));
     }
-    T* o = row_ptr<T>(dst, ds, yo) + static_cast<long>(x) * C;
+}
+
+template <typename T, int C>
+__global__ __launch_bounds__(256) void resize_kernel(const void* __restrict__ src, long ss, int sw, int sh,
+                                                    void* __restrict__ dst, long ds, int dw, int dh, int mode)
+{
+    __shared__ ResizeTaps t;
+    block_taps(t, mode, sw, sh, dw, dh);
+    const int tx = threadIdx.x & 63;
+    const int x = blockIdx.x * kRsW + tx;
+    if (x >= dw) return;
 #pragma unroll
-    for (int c = 0; c < C; c++) o[c] = from_float<T>(acc[c]);
+    for (int k = 0; k < kRsH / 4; k++) {
+        const int ty = (threadIdx.x >> 6) + 4 * k;
+        const int y = blockIdx.y * kRsH + ty;
+        if (y >= dh) break;
+        float acc[C];
+        resample<T, C>(t, src, ss, sw, sh, tx, ty, acc);
+        T* o = row_ptr<T>(dst, ds, y) + static_cast<long>(x) * C;
+#pragma unroll
+        for (int c = 0; c < C; c++) o[c] = from_float<T>(acc[c]);
+    }
+}
+
+// Chroma resize fused with the merge (Processor.cpp:258-274 with dst allocated: resize UV(A)
+// to the luma's size, then yuv2rgb): each output pixel's UV(A) is resampled, rounded to the
+// element type exactly as the resized image would hold it, and merged with Y at once, so
+// the full-size UV image is never written or read back.
+template <typename T, int C>
+__global__ __launch_bounds__(256) void resize_merge_kernel(const void* __restrict__ y, long ys, const void* __restrict__ uv,
+                                                          long uvs, int sw, int sh, int W, int H, void* __restrict__ dst,
+                                                          long ds, int mode)
+{
+    __shared__ ResizeTaps t;
+    block_taps(t, mode, sw, sh, W, H);
+    const int tx = threadIdx.x & 63;
+    const int x = blockIdx.x * kRsW + tx;
+    if (x >= W) return;
+#pragma unroll
+    for (int k = 0; k < kRsH / 4; k++) {
+        const int ty = (threadIdx.x >> 6) + 4 * k;
+        const int r0 = blockIdx.y * kRsH + ty;
+        if (r0 >= H) break;
+        float acc[C - 1];
+        resample<T, C - 1>(t, uv, uvs, sw, sh, tx, ty, acc);
+        const float yy = to_float<T>(row_ptr<T>(y, ys, r0)[x]);
+        const float u = __fsub_rn(to_float<T>(from_float<T>(acc[0])), 0.5f);
+        const float v = __fsub_rn(to_float<T>(from_float<T>(acc[1])), 0.5f);
+        float r = __fadd_rn(yy, __fmul_rn(1.403f, v));
+        float g = __fsub_rn(__fsub_rn(yy, __fmul_rn(0.344f, u)), __fmul_rn(0.714f, v));
+        float b = __fadd_rn(yy, __fmul_rn(1.773f, u));
+        T* o = row_ptr<T>(dst, ds, r0) + static_cast<long>(x) * C;
+        if (C == 4) {
+            const float a = to_float<T>(from_float<T>(acc[C - 2]));
+            if (a > 1e-6f) {
+                r = __fdiv_rn(r, a);
+                g = __fdiv_rn(g, a);
+                b = __fdiv_rn(b, a);
+            } else {
+                r = g = b = 0.0f;
+            }
+            o[3] = from_float<T>(a);
+        }
+        o[0] = from_float<T>(r);
+        o[1] = from_float<T>(g);
+        o[2] = from_float<T>(b);
+    }
 }
 
 template <typename T>
@@ -170,7 +255,7 @@ template <typename T>
 static hipError_t resize_t(int channels, int mode, const void* src, long ss, int sw, int sh, void* dst, long ds, int dw,
                            int dh, hipStream_t stream)
 {
-    const dim3 grid((dw + 63) / 64, (dh + 3) / 4), block(256);
+    const dim3 grid((dw + kRsW - 1) / kRsW, (dh + kRsH - 1) / kRsH), block(256);
     switch (channels) {
     case 1: hipLaunchKernelGGL((resize_kernel<T, 1>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode); break;
     case 2: hipLaunchKernelGGL((resize_kernel<T, 2>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode); break;
@@ -179,6 +264,32 @@ static hipError_t resize_t(int channels, int mode, const void* src, long ss, int
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t resize_merge_t(int channels, int mode, const void* y, long ys, const void* uv, long uvs, int sw, int sh,
+                                 int W, int H, void* dst, long ds, hipStream_t stream)
+{
+    const dim3 grid((W + kRsW - 1) / kRsW, (H + kRsH - 1) / kRsH), block(256);
+    if (channels == 4)
+        hipLaunchKernelGGL((resize_merge_kernel<T, 4>), grid, block, 0, stream, y, ys, uv, uvs, sw, sh, W, H, dst, ds, mode);
+    else if (channels == 3)
+        hipLaunchKernelGGL((resize_merge_kernel<T, 3>), grid, block, 0, stream, y, ys, uv, uvs, sw, sh, W, H, dst, ds, mode);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_resize_merge(int type, int channels, int mode, const void* y, long ys, const void* uv, long uvs, int sw,
+                               int sh, int W, int H, void* dst, long ds, hipStream_t stream)
+{
+    switch (type) {
+    case U8: return resize_merge_t<uint8_t>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, stream);
+    case U16: return resize_merge_t<uint16_t>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, stream);
+    case F16: return resize_merge_t<__half>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, stream);
+    case F32: return resize_merge_t<float>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, stream);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_color_split(int type, int channels, const void* src, long ss, int W, int H,

@@ -86,6 +86,10 @@ hipError_t launch_color_merge(int type, int channels, const void* y, long ys, co
                               void* dst, long ds, hipStream_t stream);
 hipError_t launch_resize(int type, int channels, int mode, const void* src, long ss, int sw, int sh,
                          void* dst, long ds, int dw, int dh, hipStream_t stream);
+// UV(A) (channels - 1 interleaved, sw x sh) resized to W x H, rounded to the element type,
+// and merged with Y (W x H) into RGB(A): launch_resize + launch_color_merge in one pass.
+hipError_t launch_resize_merge(int type, int channels, int mode, const void* y, long ys, const void* uv, long uvs, int sw,
+                               int sh, int W, int H, void* dst, long ds, hipStream_t stream);
 
 // Persistent batched launches: `counter` (device int, zero at launch) is the tile queue.
 hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,

@@ -323,14 +323,13 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pipeline(const Image& src, Image& d
     void* out = y;
     long outPitch = yPitch;
     if (C > 1) {
-        // UV by the full factor, then the merge
-        const long rp = pitch_of(fw * (C - 1), es);
+        // UV by the full factor and the merge, in one pass (the resized UV is rounded to the
+        // element type as the reference's intermediate image holds it)
         const long mp = pitch_of(fw * C, es);
-        if (!ensure(st.buf[6], rp * fh) || !ensure(st.buf[7], mp * fh)) return false;
-        if (!hipok(achip::launch_resize(T, C - 1, 1, st.buf[2].ptr, uvPitch, static_cast<int>(w), static_cast<int>(h),
-                                        st.buf[6].ptr, rp, static_cast<int>(fw), static_cast<int>(fh), s), "resize")) return false;
-        if (!hipok(achip::launch_color_merge(T, C, y, yPitch, st.buf[6].ptr, rp, static_cast<int>(fw), static_cast<int>(fh),
-                                             st.buf[7].ptr, mp, s), "colour merge")) return false;
+        if (!ensure(st.buf[7], mp * fh)) return false;
+        if (!hipok(achip::launch_resize_merge(T, C, 1, y, yPitch, st.buf[2].ptr, uvPitch, static_cast<int>(w), static_cast<int>(h),
+                                              static_cast<int>(fw), static_cast<int>(fh), st.buf[7].ptr, mp, s),
+                   "resize and merge")) return false;
         out = st.buf[7].ptr;
         outPitch = mp;
         yT = T;
