@@ -73,8 +73,10 @@ struct MfmaTile {
     static constexpr int kSeg8 = (TW + 15) / 16;                    // layer-8 items per row pair
     // Feature plane, origin (x0 - 8, y0 - 8), 8 x 16-bit channels per pixel.  A unit starts
     // at local column <= 8 and reads up to 16*kPh columns right of it; row pairs read one
-    // row past a layer's region.
-    static constexpr int PW = 8 + 16 * kPh + 1;
+    // row past a layer's region.  The pitch is a multiple of 16 pixels (256 bytes): the two
+    // 16-lane groups of a half-wave read rows one pitch apart, and 16-byte LDS reads only
+    // run at full rate when those addresses differ by a multiple of 256 bytes.
+    static constexpr int PW = (8 + 16 * kPh + 1 + 15) / 16 * 16;
     static constexpr int PH = BH + 1;
     // input tile in the source element type, origin (x0 - 9, y0 - 9): plane-local (r, c) is
     // tile (r + 1, c + 1)

@@ -17,13 +17,18 @@ namespace achip {
 
 typedef float f32x4m __attribute__((ext_vector_type(4)));
 
-template <int TW, int TH, int NT>
+// ES: element size of the source type (the staged input tile's bytes per pixel)
+template <int TW, int TH, int NT, int ES>
 struct Mfma32Tile {
     static constexpr int NW = NT / 64;
     static constexpr int BW = TW + 16, BH = TH + 16;
     static constexpr int kSeg = (TW + 16 + 15) / 16;                // 16-pixel items per row pair, at most
-    // padded so that 16-lane segments and row pairs may overhang a layer's region
-    static constexpr int PW = 8 + 16 * ((TW + 14 + 15) / 16) + 1;
+    // padded so that 16-lane segments and row pairs may overhang a layer's region; the two
+    // 16-lane groups of a half-wave read rows one pitch apart, and 16-byte LDS reads run at
+    // full rate only when that is a multiple of 256 bytes (16 pixels) -- which fits the LDS
+    // for inputs of up to 2 bytes per pixel
+    static constexpr int PWmin = 8 + 16 * ((TW + 14 + 15) / 16) + 1;
+    static constexpr int PW = ES <= 2 ? (PWmin + 15) / 16 * 16 : PWmin;
     static constexpr int PH = BH + 1;
     // input tile in the source element type, origin (x0 - 9, y0 - 9), padded the same way
     // for layer 0's items
@@ -62,10 +67,10 @@ struct ItemWalk {
 };
 
 // Replicate ring and zeroed border tiles: see fill_ring in acnet_mfma.hip.
-template <int TW, int TH, int NT>
+template <int TW, int TH, int NT, int ES>
 __device__ inline void fill_ring32(float4* __restrict__ pa, float4* __restrict__ pb, int x0, int y0, int W, int H, int m)
 {
-    using C = Mfma32Tile<TW, TH, NT>;
+    using C = Mfma32Tile<TW, TH, NT, ES>;
     const bool L = x0 - m <= -1, R = x0 + TW + m >= W + 1;
     const bool T = y0 - m <= -1, B = y0 + TH + m >= H + 1;
     if (!(L || R || T || B)) return;
@@ -132,7 +137,7 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
                                             int x0, int y0, float4* __restrict__ pa, float4* __restrict__ pb,
                                             TIN* __restrict__ img)
 {
-    using C = Mfma32Tile<TW, TH, NT>;
+    using C = Mfma32Tile<TW, TH, NT, sizeof(TIN)>;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -208,7 +213,7 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
     }
     __syncthreads();
     if (border) {
-        fill_ring32<TW, TH, NT>(pa, pb, x0, y0, W, H, 8);
+        fill_ring32<TW, TH, NT, sizeof(TIN)>(pa, pb, x0, y0, W, H, 8);
         __syncthreads();
     }
 
@@ -271,7 +276,7 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
         }
         __syncthreads();
         if (border) {
-            fill_ring32<TW, TH, NT>(pa, pb, x0, y0, W, H, m);
+            fill_ring32<TW, TH, NT, sizeof(TIN)>(pa, pb, x0, y0, W, H, m);
             __syncthreads();
         }
     }
@@ -315,7 +320,7 @@ __global__ __launch_bounds__(NT) void acnet_mfma32_fused(const void* __restrict_
                                                          void* __restrict__ dst, long dstride,
                                                          const float* __restrict__ wts, const float* __restrict__ wa)
 {
-    using C = Mfma32Tile<TW, TH, NT>;
+    using C = Mfma32Tile<TW, TH, NT, sizeof(TIN)>;
     __shared__ float4 pa[C::PW * C::PH];
     __shared__ float4 pb[C::PW * C::PH];
     __shared__ TIN img[C::IW * C::IH];
@@ -329,7 +334,7 @@ __global__ __launch_bounds__(NT) void acnet_mfma32_batch(const BatchDesc* __rest
                                                          int* __restrict__ counter,
                                                          const float* __restrict__ wts, const float* __restrict__ wa)
 {
-    using C = Mfma32Tile<TW, TH, NT>;
+    using C = Mfma32Tile<TW, TH, NT, sizeof(TIN)>;
     __shared__ float4 pa[C::PW * C::PH];
     __shared__ float4 pb[C::PW * C::PH];
     __shared__ TIN img[C::IW * C::IH];
@@ -423,8 +428,8 @@ hipError_t launch_mfma32_batch(int cfg, int stype, int dtype, const BatchDesc* d
 
 // f32 input; less LDS for narrower types
 const Fp32Config kMfma32Configs[] = {
-    { 64, 32, 1024, 0, Mfma32Tile<64, 32, 1024>::lds_bytes(4) },
-    { 64, 32, 512, 0, Mfma32Tile<64, 32, 512>::lds_bytes(4) },
+    { 64, 32, 1024, 0, Mfma32Tile<64, 32, 1024, 4>::lds_bytes(4) },
+    { 64, 32, 512, 0, Mfma32Tile<64, 32, 512, 4>::lds_bytes(4) },
 };
 const int kMfma32ConfigCount = sizeof(kMfma32Configs) / sizeof(kMfma32Configs[0]);
 

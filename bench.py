@@ -29,6 +29,10 @@ FLOP_PER_INPUT_PX = 9424          # 2 x 4712 MAC per input pixel per 2x pass
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 peak
 PEAK_MFMA16_TFLOPS = 2500.0       # MI355X published dense FP16/BF16 MFMA peak
 FRAMES_PER_VARIANT = 2
+# HBM bytes per 1080p u8 launch of the default configuration of each precision, measured with
+# rocprofv3 --pmc FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) against a 256 MiB copy
+# (profiles/acnet_fused_1080p_u8.md); counters cannot be read from inside this process.
+MEASURED_TRAFFIC = {"fp32": 11.24e6, "fp16": 11.08e6, "bf16": 11.08e6}
 
 
 def cpu_baseline(frames_u8, weights, min_seconds=10.0):
@@ -374,7 +378,9 @@ def main():
             "peak": PEAK_FP32_TFLOPS if main_prec == "fp32" else PEAK_MFMA16_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / (PEAK_FP32_TFLOPS if main_prec == "fp32" else PEAK_MFMA16_TFLOPS), 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get(main_prec) if not args.config else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE per launch, profiles/acnet_fused_1080p_u8.md",
+            "algorithmic_bytes": 5 * W_IN * H_IN,
             "mean_launch_us": round(mean_launch_s * 1e6, 2),
             "flop_per_launch": FLOP_PER_INPUT_PX * W_IN * H_IN,
         },
