@@ -89,6 +89,11 @@ SIGNATURES = {
     "ac_image_view": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage), c_int, c_int, c_int, c_int]),
     "ac_image_clone": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage)]),
     "ac_image_to": (c_int, [ctypes.POINTER(ACImage), c_void_p, c_int]),
+    "ac_resize": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage), ctypes.c_double, ctypes.c_double, c_int]),
+    "ac_rgb2yuv": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage)]),
+    "ac_rgba2yuva": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage)]),
+    "ac_yuv2rgb": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage)]),
+    "ac_yuva2rgba": (c_int, [ctypes.POINTER(ACImage), ctypes.POINTER(ACImage)]),
     # include/AC/Core/Processor.h
     "ac_processor_alloc": (ctypes.POINTER(ACProcessor), []),
     "ac_processor_free": (None, [ctypes.POINTER(ctypes.POINTER(ACProcessor))]),

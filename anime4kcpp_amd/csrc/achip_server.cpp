@@ -325,9 +325,9 @@ int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
         hipError_t he = achip::launch_pass(net, cfg.precision, pcfg, kU8, kU8, s.y0.ptr, s.y0.stride, s.w, s.h, s.y1.ptr,
                                            s.y1.stride, s.stream);
         if (he == hipSuccess)
-            he = achip::launch_resize(kU8, 1, 0, s.u0.ptr, s.u0.stride, s.u0.w, s.u0.h, s.u1.ptr, s.u1.stride, s.u1.w, s.u1.h, s.stream);
+            he = achip::launch_resize(kU8, 1, achip::kResizeBilinear, s.u0.ptr, s.u0.stride, s.u0.w, s.u0.h, s.u1.ptr, s.u1.stride, s.u1.w, s.u1.h, s.stream);
         if (he == hipSuccess)
-            he = achip::launch_resize(kU8, 1, 0, s.v0.ptr, s.v0.stride, s.v0.w, s.v0.h, s.v1.ptr, s.v1.stride, s.v1.w, s.v1.h, s.stream);
+            he = achip::launch_resize(kU8, 1, achip::kResizeBilinear, s.v0.ptr, s.v0.stride, s.v0.w, s.v0.h, s.v1.ptr, s.v1.stride, s.v1.w, s.v1.h, s.stream);
         const hipError_t ce = hipStreamEndCapture(s.stream, &s.graph);
         if (he != hipSuccess || ce != hipSuccess) return ACHIP_EIO;
         if (hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0) != hipSuccess) return ACHIP_EIO;
@@ -438,13 +438,13 @@ bool ACHipServer::build_step()
         const int P = q->passes;
         const Plane& y = q->luma[P];
         if (q->r.kind == ACHIP_REQ_RGB) {
-            e = achip::launch_resize_merge(kU8, 3, 1, y.ptr, y.stride, q->uv.ptr, q->uv.stride, q->uv.w, q->uv.h, y.w, y.h,
+            e = achip::launch_resize_merge(kU8, 3, achip::kResizeCatmullRom, y.ptr, y.stride, q->uv.ptr, q->uv.stride, q->uv.w, q->uv.h, y.w, y.h,
                                            q->out.ptr, q->out.stride, batch);
         } else if (q->r.kind == ACHIP_REQ_YUV420) {
-            e = achip::launch_resize(kU8, 1, 0, q->u0.ptr, q->u0.stride, q->u0.w, q->u0.h, q->u1.ptr, q->u1.stride, q->u1.w,
+            e = achip::launch_resize(kU8, 1, achip::kResizeBilinear, q->u0.ptr, q->u0.stride, q->u0.w, q->u0.h, q->u1.ptr, q->u1.stride, q->u1.w,
                                      q->u1.h, batch);
             if (e == hipSuccess)
-                e = achip::launch_resize(kU8, 1, 0, q->v0.ptr, q->v0.stride, q->v0.w, q->v0.h, q->v1.ptr, q->v1.stride,
+                e = achip::launch_resize(kU8, 1, achip::kResizeBilinear, q->v0.ptr, q->v0.stride, q->v0.w, q->v0.h, q->v1.ptr, q->v1.stride,
                                          q->v1.w, q->v1.h, batch);
         }
     }

@@ -1,15 +1,17 @@
-// Device kernels around the 2x passes: colour split and merge, and resize.  These are the
-// HBM-bound steps of Processor::process (core/src/processor/Processor.cpp:199-276) for
-// multi-channel input and non-power-of-two factors.
+// Device kernels around the 2x passes and behind the image helpers: colour conversion
+// and resize.  These are the HBM-bound steps of Processor::process
+// (core/src/processor/Processor.cpp:199-276) for multi-channel input and non-power-of-two
+// factors, and of ac::core::{rgb2yuv, rgba2yuva, yuv2rgb, yuva2rgba, resize}.
 //
-// Colour conversion follows core/src/ImageProcess.cpp:38-61 / :114-139 (split) and
-// :191-215 / :276-308 (merge): BT.601-like float constants, each product and sum rounded
-// separately in the order written there (no contraction), RGBA premultiplied by alpha
-// before the split and divided by it after the merge, every output through fromFloat.
+// Colour conversion follows core/src/ImageProcess.cpp:15-308: BT.601-like float constants,
+// each product and sum rounded separately in the order written there (no contraction),
+// RGBA premultiplied by alpha before the forward conversion and divided by it after the
+// inverse, every output through fromFloat.  One kernel serves the packed, two-plane and
+// per-channel forms: each component is read and written through a plane view.
 //
-// Resize replaces core/src/ImageResize.cpp:136-273 (stb_image_resize2): separable filter,
-// edge clamp, pixel-centre alignment (source x = (dst x + 0.5) / scale - 0.5); Catmull-Rom
-// (a = -0.5) or bilinear; when downscaling the filter is stretched by 1/scale and its
+// Resize replaces core/src/ImageResize.cpp:136-273 (stb_image_resize2 with the reference's
+// filter callbacks): separable filter, edge clamp, pixel-centre alignment (source
+// x = (dst x + 0.5) / scale - 0.5); when downscaling the filter is stretched by 1/scale;
 // weights normalised.  Horizontal taps are summed first for each source row, then the
 // rows vertically, in FP32.
 #include "acnet_device.hpp"
@@ -19,97 +21,195 @@ namespace achip {
 
 #pragma clang fp contract(off)
 
-template <typename T, int C>
-__global__ __launch_bounds__(256) void color_split(const void* __restrict__ src, long ss, int W, int H,
-                                                  void* __restrict__ y, long ys, void* __restrict__ uv, long uvs)
+// ---- colour -------------------------------------------------------------------------------
+
+// ImageProcess.cpp:26-28 (forward) and :186-188 (inverse; u, v already centred)
+__device__ inline void rgb_to_yuv(float r, float g, float b, float& y, float& u, float& v)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || r0 >= H) return;
-    const T* in = row_ptr<T>(src, ss, r0) + static_cast<long>(x) * C;
-    float r = to_float<T>(in[0]), g = to_float<T>(in[1]), b = to_float<T>(in[2]);
-    float a = 1.0f;
-    if (C == 4) {
-        a = to_float<T>(in[3]);
-        r = __fmul_rn(r, a);
-        g = __fmul_rn(g, a);
-        b = __fmul_rn(b, a);
+    y = __fadd_rn(__fadd_rn(__fmul_rn(0.299f, r), __fmul_rn(0.587f, g)), __fmul_rn(0.114f, b));
+    u = __fadd_rn(__fmul_rn(0.564f, __fsub_rn(b, y)), 0.5f);
+    v = __fadd_rn(__fmul_rn(0.713f, __fsub_rn(r, y)), 0.5f);
+}
+__device__ inline void yuv_to_rgb(float y, float u, float v, float& r, float& g, float& b)
+{
+    r = __fadd_rn(y, __fmul_rn(1.403f, v));
+    g = __fsub_rn(__fsub_rn(y, __fmul_rn(0.344f, u)), __fmul_rn(0.714f, v));
+    b = __fadd_rn(y, __fmul_rn(1.773f, u));
+}
+// yuva2rgba's un-premultiply (ImageProcess.cpp:290-296)
+__device__ inline void unpremultiply(float a, float& r, float& g, float& b)
+{
+    if (a > 1e-6f) {
+        r = __fdiv_rn(r, a);
+        g = __fdiv_rn(g, a);
+        b = __fdiv_rn(b, a);
+    } else {
+        r = g = b = 0.0f;
     }
-    const float yy = __fadd_rn(__fadd_rn(__fmul_rn(0.299f, r), __fmul_rn(0.587f, g)), __fmul_rn(0.114f, b));
-    const float u = __fadd_rn(__fmul_rn(0.564f, __fsub_rn(b, yy)), 0.5f);
-    const float v = __fadd_rn(__fmul_rn(0.713f, __fsub_rn(r, yy)), 0.5f);
-    row_ptr<T>(y, ys, r0)[x] = from_float<T>(yy);
-    T* o = row_ptr<T>(uv, uvs, r0) + static_cast<long>(x) * (C - 1);
-    o[0] = from_float<T>(u);
-    o[1] = from_float<T>(v);
-    if (C == 4) o[2] = from_float<T>(a);
 }
 
-template <typename T, int C>
-__global__ __launch_bounds__(256) void color_merge(const void* __restrict__ y, long ys, const void* __restrict__ uv, long uvs,
-                                                  int W, int H, void* __restrict__ dst, long ds)
+template <typename T>
+__device__ inline const T& comp_at(const PlaneView& p, int x, int y)
+{
+    return row_ptr<T>(p.ptr, p.pitch, y)[static_cast<long>(x) * p.step + p.off];
+}
+template <typename T>
+__device__ inline T& comp_at_mut(const PlaneView& p, int x, int y)
+{
+    return row_ptr<T>(p.ptr, p.pitch, y)[static_cast<long>(x) * p.step + p.off];
+}
+
+// One pixel per thread, 64 x 4 per workgroup.
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void colour_kernel(ColourPlanes pl, int W, int H)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int r0 = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= W || r0 >= H) return;
-    const T* ui = row_ptr<T>(uv, uvs, r0) + static_cast<long>(x) * (C - 1);
-    const float yy = to_float<T>(row_ptr<T>(y, ys, r0)[x]);
-    const float u = __fsub_rn(to_float<T>(ui[0]), 0.5f);
-    const float v = __fsub_rn(to_float<T>(ui[1]), 0.5f);
-    float r = __fadd_rn(yy, __fmul_rn(1.403f, v));
-    float g = __fsub_rn(__fsub_rn(yy, __fmul_rn(0.344f, u)), __fmul_rn(0.714f, v));
-    float b = __fadd_rn(yy, __fmul_rn(1.773f, u));
-    T* o = row_ptr<T>(dst, ds, r0) + static_cast<long>(x) * C;
-    if (C == 4) {
-        const float a = to_float<T>(ui[2]);
-        if (a > 1e-6f) {
-            r = __fdiv_rn(r, a);
-            g = __fdiv_rn(g, a);
-            b = __fdiv_rn(b, a);
-        } else {
-            r = g = b = 0.0f;
+    const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || yy >= H) return;
+    if constexpr (KIND == kRgb2Yuv || KIND == kRgba2Yuva) {
+        float r = to_float<T>(comp_at<T>(pl.in[0], x, yy));
+        float g = to_float<T>(comp_at<T>(pl.in[1], x, yy));
+        float b = to_float<T>(comp_at<T>(pl.in[2], x, yy));
+        float a = 1.0f;
+        if constexpr (KIND == kRgba2Yuva) {
+            a = to_float<T>(comp_at<T>(pl.in[3], x, yy));
+            r = __fmul_rn(r, a);
+            g = __fmul_rn(g, a);
+            b = __fmul_rn(b, a);
         }
-        o[3] = from_float<T>(a);
+        float y, u, v;
+        rgb_to_yuv(r, g, b, y, u, v);
+        comp_at_mut<T>(pl.out[0], x, yy) = from_float<T>(y);
+        comp_at_mut<T>(pl.out[1], x, yy) = from_float<T>(u);
+        comp_at_mut<T>(pl.out[2], x, yy) = from_float<T>(v);
+        if constexpr (KIND == kRgba2Yuva) comp_at_mut<T>(pl.out[3], x, yy) = from_float<T>(a);
+    } else {
+        const float y = to_float<T>(comp_at<T>(pl.in[0], x, yy));
+        const float u = __fsub_rn(to_float<T>(comp_at<T>(pl.in[1], x, yy)), 0.5f);
+        const float v = __fsub_rn(to_float<T>(comp_at<T>(pl.in[2], x, yy)), 0.5f);
+        float r, g, b;
+        yuv_to_rgb(y, u, v, r, g, b);
+        if constexpr (KIND == kYuva2Rgba) {
+            const float a = to_float<T>(comp_at<T>(pl.in[3], x, yy));
+            unpremultiply(a, r, g, b);
+            comp_at_mut<T>(pl.out[3], x, yy) = from_float<T>(a);
+        }
+        comp_at_mut<T>(pl.out[0], x, yy) = from_float<T>(r);
+        comp_at_mut<T>(pl.out[1], x, yy) = from_float<T>(g);
+        comp_at_mut<T>(pl.out[2], x, yy) = from_float<T>(b);
     }
-    o[0] = from_float<T>(r);
-    o[1] = from_float<T>(g);
-    o[2] = from_float<T>(b);
 }
 
-// filter kernels: mode 0 bilinear (support 1), 1 Catmull-Rom (support 2)
-__device__ inline float filter_weight(int mode, float d)
+// ---- resize filters -----------------------------------------------------------------------
+// Mode numbers are the reference's ResizeModes (core/include/AC/Core/Image.hpp:276-295);
+// the kernels are its filter callbacks (ImageResize.cpp:53-134, 193-263): the (b, c)
+// cubic family of Mitchell and Netravali, Lanczos, the spline16/36/64 piecewise cubics
+// and the tent, each evaluated in float as a Horner polynomial.
+
+__device__ inline float poly3(float x, float c0, float c1, float c2, float c3)
 {
-    d = fabsf(d);
-    if (mode == 0) return d < 1.0f ? 1.0f - d : 0.0f;
-    if (d < 1.0f) return __fadd_rn(__fmul_rn(__fmul_rn(1.5f, d), __fmul_rn(d, d)), __fadd_rn(__fmul_rn(-2.5f, __fmul_rn(d, d)), 1.0f));
-    if (d < 2.0f)
-        return __fadd_rn(__fadd_rn(__fmul_rn(-0.5f, __fmul_rn(d, __fmul_rn(d, d))), __fmul_rn(2.5f, __fmul_rn(d, d))),
-                         __fadd_rn(__fmul_rn(-4.0f, d), 2.0f));
+    return __fadd_rn(c0, __fmul_rn(x, __fadd_rn(c1, __fmul_rn(x, __fadd_rn(c2, __fmul_rn(x, c3))))));
+}
+
+// (b, c) cubic with b = bn/bd, c = cn/cd; coefficients rounded to float step by step
+struct CubicBC {
+    float p0, p2, p3, q0, q1, q2, q3;
+};
+__host__ __device__ constexpr CubicBC cubic_bc(int bn, int bd, int cn, int cd)
+{
+    const float b = static_cast<float>(bn) / static_cast<float>(bd);
+    const float c = static_cast<float>(cn) / static_cast<float>(cd);
+    return { (6.0f - 2.0f * b) / 6.0f,
+             (-18.0f + 12.0f * b + 6.0f * c) / 6.0f,
+             (12.0f - 9.0f * b - 6.0f * c) / 6.0f,
+             (8.0f * b + 24.0f * c) / 6.0f,
+             (-12.0f * b - 48.0f * c) / 6.0f,
+             (6.0f * b + 30.0f * c) / 6.0f,
+             (-b - 6.0f * c) / 6.0f };
+}
+__device__ inline float cubic(const CubicBC& k, float x)
+{
+    if (x < 1.0f) return poly3(x, k.p0, 0.0f, k.p2, k.p3);
+    if (x < 2.0f) return poly3(x, k.q0, k.q1, k.q2, k.q3);
     return 0.0f;
 }
 
-// Taps of one output coordinate: first source index and normalised weights (at most kMaxTaps).
-constexpr int kMaxTaps = 12;
-__device__ inline int resize_taps(int mode, int o, int n_in, int n_out, float* w, int& first)
+__device__ inline float sinc(float x)
+{
+    constexpr float pi = 3.14159265358979f;
+    return x == 0.0f ? 1.0f : __fdiv_rn(sinf(__fmul_rn(x, pi)), __fmul_rn(x, pi));
+}
+__device__ inline float lanczos(float x, float taps)
+{
+    return x < taps ? __fmul_rn(sinc(x), sinc(__fdiv_rn(x, taps))) : 0.0f;
+}
+
+// Spline16/36/64 (Panorama Tools' interpolating spline kernels), piece k on [k, k+1)
+__device__ inline float spline16(float x)
+{
+    if (x < 1.0f) return poly3(x, 1.0f, -1.0f / 5.0f, -9.0f / 5.0f, 1.0f);
+    if (x < 2.0f) return poly3(__fsub_rn(x, 1.0f), 0.0f, -7.0f / 15.0f, 4.0f / 5.0f, -1.0f / 3.0f);
+    return 0.0f;
+}
+__device__ inline float spline36(float x)
+{
+    if (x < 1.0f) return poly3(x, 1.0f, -3.0f / 209.0f, -453.0f / 209.0f, 13.0f / 11.0f);
+    if (x < 2.0f) return poly3(__fsub_rn(x, 1.0f), 0.0f, -156.0f / 209.0f, 270.0f / 209.0f, -6.0f / 11.0f);
+    if (x < 3.0f) return poly3(__fsub_rn(x, 2.0f), 0.0f, 26.0f / 209.0f, -45.0f / 209.0f, 1.0f / 11.0f);
+    return 0.0f;
+}
+__device__ inline float spline64(float x)
+{
+    if (x < 1.0f) return poly3(x, 1.0f, -3.0f / 2911.0f, -6387.0f / 2911.0f, 49.0f / 41.0f);
+    if (x < 2.0f) return poly3(__fsub_rn(x, 1.0f), 0.0f, -2328.0f / 2911.0f, 4032.0f / 2911.0f, -24.0f / 41.0f);
+    if (x < 3.0f) return poly3(__fsub_rn(x, 2.0f), 0.0f, 582.0f / 2911.0f, -1008.0f / 2911.0f, 6.0f / 41.0f);
+    if (x < 4.0f) return poly3(__fsub_rn(x, 3.0f), 0.0f, -97.0f / 2911.0f, 168.0f / 2911.0f, -1.0f / 41.0f);
+    return 0.0f;
+}
+
+__device__ inline float filter_weight(int mode, float d)
+{
+    const float x = fabsf(d);
+    switch (mode) {
+    case kResizePoint: return 1.0f;
+    case kResizeCatmullRom: { constexpr CubicBC k = cubic_bc(0, 1, 1, 2); return cubic(k, x); }
+    case kResizeMitchell: { constexpr CubicBC k = cubic_bc(1, 3, 1, 3); return cubic(k, x); }
+    case kResizeBicubic0_60: { constexpr CubicBC k = cubic_bc(0, 1, 3, 5); return cubic(k, x); }
+    case kResizeBicubic0_75: { constexpr CubicBC k = cubic_bc(0, 1, 3, 4); return cubic(k, x); }
+    case kResizeBicubic0_100: { constexpr CubicBC k = cubic_bc(0, 1, 1, 1); return cubic(k, x); }
+    case kResizeBicubic20_50: { constexpr CubicBC k = cubic_bc(1, 5, 1, 2); return cubic(k, x); }
+    case kResizeSoftCubic50: { constexpr CubicBC k = cubic_bc(1, 2, 1, 2); return cubic(k, x); }
+    case kResizeSoftCubic75: { constexpr CubicBC k = cubic_bc(3, 4, 1, 4); return cubic(k, x); }
+    case kResizeSoftCubic100: { constexpr CubicBC k = cubic_bc(1, 1, 0, 1); return cubic(k, x); }
+    case kResizeLanczos2: return lanczos(x, 2.0f);
+    case kResizeLanczos3: return lanczos(x, 3.0f);
+    case kResizeLanczos4: return lanczos(x, 4.0f);
+    case kResizeSpline16: return spline16(x);
+    case kResizeSpline36: return spline36(x);
+    case kResizeSpline64: return spline64(x);
+    default: return x < 1.0f ? __fsub_rn(1.0f, x) : 0.0f;   // bilinear
+    }
+}
+
+// Taps of one output coordinate: the source pixels whose centres lie in
+// (center - support, center + support], their filter weights normalised to sum 1.
+// At most kmax taps are written (the launcher sizes kmax from resize_max_taps).
+__device__ inline int resize_taps(int mode, int o, int n_in, int n_out, float* w, int kmax, int& first)
 {
     const float scale = static_cast<float>(n_out) / static_cast<float>(n_in);
-    const float center = (static_cast<float>(o) + 0.5f) / scale - 0.5f;
-    const float support = (mode == 0 ? 1.0f : 2.0f) * (scale < 1.0f ? 1.0f / scale : 1.0f);
+    const float center = __fsub_rn(__fdiv_rn(__fadd_rn(static_cast<float>(o), 0.5f), scale), 0.5f);
+    const float support = __fmul_rn(resize_support(mode), scale < 1.0f ? __fdiv_rn(1.0f, scale) : 1.0f);
     const float stretch = scale < 1.0f ? scale : 1.0f;
-    int lo = static_cast<int>(ceilf(center - support));
-    int hi = static_cast<int>(floorf(center + support));
-    if (hi - lo + 1 > kMaxTaps) {                 // only for scales below 1/3 (not used by the pipeline)
-        lo = static_cast<int>(center) - kMaxTaps / 2 + 1;
-        hi = lo + kMaxTaps - 1;
-    }
+    const int lo = static_cast<int>(floorf(__fsub_rn(center, support))) + 1;
+    const int hi = static_cast<int>(floorf(__fadd_rn(center, support)));
+    const int n = min(max(hi - lo + 1, 0), kmax);
     float sum = 0.0f;
-    int n = 0;
-    for (int i = lo; i <= hi; i++, n++) {
-        w[n] = filter_weight(mode, (static_cast<float>(i) - center) * stretch);
-        sum += w[n];
+    for (int i = 0; i < n; i++) {
+        w[i] = filter_weight(mode, __fmul_rn(__fsub_rn(static_cast<float>(lo + i), center), stretch));
+        sum = __fadd_rn(sum, w[i]);
     }
     if (sum != 0.0f)
-        for (int i = 0; i < n; i++) w[i] /= sum;
+        for (int i = 0; i < n; i++) w[i] = __fdiv_rn(w[i], sum);
     first = lo;
     return n;
 }
@@ -117,28 +217,40 @@ __device__ inline int resize_taps(int mode, int o, int n_in, int n_out, float* w
 // A workgroup computes a 64 x 16 block of outputs (thread: one column, four rows).  The taps
 // of the block's 64 columns and 16 rows are computed once into LDS (resize_taps is the
 // costly part: filter, normalising division), then every output sums its taps in the order
-// above.
+// above.  LDS: wx[64][kp], wy[16][kp] floats, then fx, nx[64], fy, ny[16] ints; kp = kmax | 1
+// (odd row pitch: the 64 columns' reads of one tap fall in different banks).
 constexpr int kRsW = 64, kRsH = 16;
 
 struct ResizeTaps {
-    float wx[kRsW][kMaxTaps];
-    float wy[kRsH][kMaxTaps];
-    int fx[kRsW], nx[kRsW], fy[kRsH], ny[kRsH];
+    float* wx;
+    float* wy;
+    int *fx, *nx, *fy, *ny;
+    int kp;
 };
 
-__device__ inline void block_taps(ResizeTaps& t, int mode, int sw, int sh, int dw, int dh)
+__device__ inline ResizeTaps block_taps(int mode, int sw, int sh, int dw, int dh, int kmax)
 {
+    extern __shared__ float smem[];
+    ResizeTaps t;
+    t.kp = kmax | 1;
+    t.wx = smem;
+    t.wy = smem + kRsW * t.kp;
+    t.fx = reinterpret_cast<int*>(t.wy + kRsH * t.kp);
+    t.nx = t.fx + kRsW;
+    t.fy = t.nx + kRsW;
+    t.ny = t.fy + kRsH;
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * kRsW, y0 = blockIdx.y * kRsH;
     if (tid < kRsW) {
         const int x = min(x0 + tid, dw - 1);
-        t.nx[tid] = resize_taps(mode, x, sw, dw, t.wx[tid], t.fx[tid]);
+        t.nx[tid] = resize_taps(mode, x, sw, dw, t.wx + tid * t.kp, kmax, t.fx[tid]);
     } else if (tid < kRsW + kRsH) {
         const int r = tid - kRsW;
         const int y = min(y0 + r, dh - 1);
-        t.ny[r] = resize_taps(mode, y, sh, dh, t.wy[r], t.fy[r]);
+        t.ny[r] = resize_taps(mode, y, sh, dh, t.wy + r * t.kp, kmax, t.fy[r]);
     }
     __syncthreads();
+    return t;
 }
 
 // resized value of channel block [C] at output (column tx, row ty) of the block
@@ -149,6 +261,8 @@ __device__ inline void resample(const ResizeTaps& t, const void* __restrict__ sr
 #pragma unroll
     for (int c = 0; c < C; c++) acc[c] = 0.0f;
     const int nx = t.nx[tx], fx = t.fx[tx];
+    const float* wx = t.wx + tx * t.kp;
+    const float* wy = t.wy + ty * t.kp;
     for (int j = 0; j < t.ny[ty]; j++) {
         const T* row = row_ptr<T>(src, ss, clampi(t.fy[ty] + j, 0, sh - 1));
         float h[C];
@@ -157,19 +271,18 @@ __device__ inline void resample(const ResizeTaps& t, const void* __restrict__ sr
         for (int i = 0; i < nx; i++) {
             const T* p = row + static_cast<long>(clampi(fx + i, 0, sw - 1)) * C;
 #pragma unroll
-            for (int c = 0; c < C; c++) h[c] = __fadd_rn(h[c], __fmul_rn(t.wx[tx][i], to_float<T>(p[c])));
+            for (int c = 0; c < C; c++) h[c] = __fadd_rn(h[c], __fmul_rn(wx[i], to_float<T>(p[c])));
         }
 #pragma unroll
-        for (int c = 0; c < C; c++) acc[c] = __fadd_rn(acc[c], __fmul_rn(t.wy[ty][j], h[c]));
+        for (int c = 0; c < C; c++) acc[c] = __fadd_rn(acc[c], __fmul_rn(wy[j], h[c]));
     }
 }
 
 template <typename T, int C>
 __global__ __launch_bounds__(256) void resize_kernel(const void* __restrict__ src, long ss, int sw, int sh,
-                                                    void* __restrict__ dst, long ds, int dw, int dh, int mode)
+                                                    void* __restrict__ dst, long ds, int dw, int dh, int mode, int kmax)
 {
-    __shared__ ResizeTaps t;
-    block_taps(t, mode, sw, sh, dw, dh);
+    const ResizeTaps t = block_taps(mode, sw, sh, dw, dh, kmax);
     const int tx = threadIdx.x & 63;
     const int x = blockIdx.x * kRsW + tx;
     if (x >= dw) return;
@@ -193,10 +306,9 @@ __global__ __launch_bounds__(256) void resize_kernel(const void* __restrict__ sr
 template <typename T, int C>
 __global__ __launch_bounds__(256) void resize_merge_kernel(const void* __restrict__ y, long ys, const void* __restrict__ uv,
                                                           long uvs, int sw, int sh, int W, int H, void* __restrict__ dst,
-                                                          long ds, int mode)
+                                                          long ds, int mode, int kmax)
 {
-    __shared__ ResizeTaps t;
-    block_taps(t, mode, sw, sh, W, H);
+    const ResizeTaps t = block_taps(mode, sw, sh, W, H, kmax);
     const int tx = threadIdx.x & 63;
     const int x = blockIdx.x * kRsW + tx;
     if (x >= W) return;
@@ -210,19 +322,12 @@ __global__ __launch_bounds__(256) void resize_merge_kernel(const void* __restric
         const float yy = to_float<T>(row_ptr<T>(y, ys, r0)[x]);
         const float u = __fsub_rn(to_float<T>(from_float<T>(acc[0])), 0.5f);
         const float v = __fsub_rn(to_float<T>(from_float<T>(acc[1])), 0.5f);
-        float r = __fadd_rn(yy, __fmul_rn(1.403f, v));
-        float g = __fsub_rn(__fsub_rn(yy, __fmul_rn(0.344f, u)), __fmul_rn(0.714f, v));
-        float b = __fadd_rn(yy, __fmul_rn(1.773f, u));
+        float r, g, b;
+        yuv_to_rgb(yy, u, v, r, g, b);
         T* o = row_ptr<T>(dst, ds, r0) + static_cast<long>(x) * C;
-        if (C == 4) {
+        if constexpr (C == 4) {
             const float a = to_float<T>(from_float<T>(acc[C - 2]));
-            if (a > 1e-6f) {
-                r = __fdiv_rn(r, a);
-                g = __fdiv_rn(g, a);
-                b = __fdiv_rn(b, a);
-            } else {
-                r = g = b = 0.0f;
-            }
+            unpremultiply(a, r, g, b);
             o[3] = from_float<T>(a);
         }
         o[0] = from_float<T>(r);
@@ -231,36 +336,99 @@ __global__ __launch_bounds__(256) void resize_merge_kernel(const void* __restric
     }
 }
 
-template <typename T>
-static hipError_t split_t(int channels, const void* src, long ss, int W, int H, void* y, long ys, void* uv, long uvs,
-                          hipStream_t stream)
+// ---- launchers ----------------------------------------------------------------------------
+
+int resize_max_taps(int mode, int n_in, int n_out)
+{
+    const float scale = static_cast<float>(n_out) / static_cast<float>(n_in);
+    const float support = resize_support(mode) * (scale < 1.0f ? 1.0f / scale : 1.0f);
+    return static_cast<int>(2.0f * support) + 2;
+}
+
+static size_t resize_lds_bytes(int kmax)
+{
+    return (static_cast<size_t>(kRsW + kRsH) * (kmax | 1) + 2 * (kRsW + kRsH)) * sizeof(float);
+}
+
+// Taps per output for this resize, or -1 when the tables would not fit in LDS.
+static int resize_kmax(int mode, int sw, int sh, int dw, int dh)
+{
+    const int k = max(resize_max_taps(mode, sw, dw), resize_max_taps(mode, sh, dh));
+    return k <= kResizeMaxTaps ? k : -1;
+}
+
+template <typename T, int KIND>
+static hipError_t colour_t(const ColourPlanes& pl, int W, int H, hipStream_t stream)
 {
     const dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
-    if (channels == 4) hipLaunchKernelGGL((color_split<T, 4>), grid, block, 0, stream, src, ss, W, H, y, ys, uv, uvs);
-    else hipLaunchKernelGGL((color_split<T, 3>), grid, block, 0, stream, src, ss, W, H, y, ys, uv, uvs);
+    hipLaunchKernelGGL((colour_kernel<T, KIND>), grid, block, 0, stream, pl, W, H);
     return hipGetLastError();
 }
 
 template <typename T>
-static hipError_t merge_t(int channels, const void* y, long ys, const void* uv, long uvs, int W, int H, void* dst, long ds,
-                          hipStream_t stream)
+static hipError_t colour_kind(int kind, const ColourPlanes& pl, int W, int H, hipStream_t stream)
 {
-    const dim3 grid((W + 63) / 64, (H + 3) / 4), block(256);
-    if (channels == 4) hipLaunchKernelGGL((color_merge<T, 4>), grid, block, 0, stream, y, ys, uv, uvs, W, H, dst, ds);
-    else hipLaunchKernelGGL((color_merge<T, 3>), grid, block, 0, stream, y, ys, uv, uvs, W, H, dst, ds);
-    return hipGetLastError();
+    switch (kind) {
+    case kRgb2Yuv: return colour_t<T, kRgb2Yuv>(pl, W, H, stream);
+    case kRgba2Yuva: return colour_t<T, kRgba2Yuva>(pl, W, H, stream);
+    case kYuv2Rgb: return colour_t<T, kYuv2Rgb>(pl, W, H, stream);
+    case kYuva2Rgba: return colour_t<T, kYuva2Rgba>(pl, W, H, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_colour(int type, int kind, const ColourPlanes& pl, int W, int H, hipStream_t stream)
+{
+    if (W <= 0 || H <= 0) return hipSuccess;
+    switch (type) {
+    case U8: return colour_kind<uint8_t>(kind, pl, W, H, stream);
+    case U16: return colour_kind<uint16_t>(kind, pl, W, H, stream);
+    case F16: return colour_kind<__half>(kind, pl, W, H, stream);
+    case F32: return colour_kind<float>(kind, pl, W, H, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Packed RGB(A) <-> Y plane + interleaved UV(A) plane: the colour kernel's views.
+static ColourPlanes split_planes(int channels, const void* rgb, long rs, void* y, long ys, void* uv, long uvs)
+{
+    ColourPlanes pl{};
+    for (int k = 0; k < channels; k++) pl.in[k] = { const_cast<void*>(rgb), rs, channels, k };
+    pl.out[0] = { y, ys, 1, 0 };
+    for (int k = 1; k < channels; k++) pl.out[k] = { uv, uvs, channels - 1, k - 1 };
+    return pl;
+}
+
+hipError_t launch_color_split(int type, int channels, const void* src, long ss, int W, int H,
+                              void* y, long ys, void* uv, long uvs, hipStream_t stream)
+{
+    if (channels != 3 && channels != 4) return hipErrorInvalidValue;
+    return launch_colour(type, channels == 4 ? kRgba2Yuva : kRgb2Yuv, split_planes(channels, src, ss, y, ys, uv, uvs), W, H,
+                         stream);
+}
+
+hipError_t launch_color_merge(int type, int channels, const void* y, long ys, const void* uv, long uvs, int W, int H,
+                              void* dst, long ds, hipStream_t stream)
+{
+    if (channels != 3 && channels != 4) return hipErrorInvalidValue;
+    ColourPlanes pl{};
+    pl.in[0] = { const_cast<void*>(y), ys, 1, 0 };
+    for (int k = 1; k < channels; k++) pl.in[k] = { const_cast<void*>(uv), uvs, channels - 1, k - 1 };
+    for (int k = 0; k < channels; k++) pl.out[k] = { dst, ds, channels, k };
+    return launch_colour(type, channels == 4 ? kYuva2Rgba : kYuv2Rgb, pl, W, H, stream);
 }
 
 template <typename T>
 static hipError_t resize_t(int channels, int mode, const void* src, long ss, int sw, int sh, void* dst, long ds, int dw,
-                           int dh, hipStream_t stream)
+                           int dh, int kmax, hipStream_t stream)
 {
     const dim3 grid((dw + kRsW - 1) / kRsW, (dh + kRsH - 1) / kRsH), block(256);
+    const size_t lds = resize_lds_bytes(kmax);
     switch (channels) {
-    case 1: hipLaunchKernelGGL((resize_kernel<T, 1>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode); break;
-    case 2: hipLaunchKernelGGL((resize_kernel<T, 2>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode); break;
-    case 3: hipLaunchKernelGGL((resize_kernel<T, 3>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode); break;
-    case 4: hipLaunchKernelGGL((resize_kernel<T, 4>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode); break;
+    case 1: hipLaunchKernelGGL((resize_kernel<T, 1>), grid, block, lds, stream, src, ss, sw, sh, dst, ds, dw, dh, mode, kmax); break;
+    case 2: hipLaunchKernelGGL((resize_kernel<T, 2>), grid, block, lds, stream, src, ss, sw, sh, dst, ds, dw, dh, mode, kmax); break;
+    case 3: hipLaunchKernelGGL((resize_kernel<T, 3>), grid, block, lds, stream, src, ss, sw, sh, dst, ds, dw, dh, mode, kmax); break;
+    case 4: hipLaunchKernelGGL((resize_kernel<T, 4>), grid, block, lds, stream, src, ss, sw, sh, dst, ds, dw, dh, mode, kmax); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -268,13 +436,14 @@ static hipError_t resize_t(int channels, int mode, const void* src, long ss, int
 
 template <typename T>
 static hipError_t resize_merge_t(int channels, int mode, const void* y, long ys, const void* uv, long uvs, int sw, int sh,
-                                 int W, int H, void* dst, long ds, hipStream_t stream)
+                                 int W, int H, void* dst, long ds, int kmax, hipStream_t stream)
 {
     const dim3 grid((W + kRsW - 1) / kRsW, (H + kRsH - 1) / kRsH), block(256);
+    const size_t lds = resize_lds_bytes(kmax);
     if (channels == 4)
-        hipLaunchKernelGGL((resize_merge_kernel<T, 4>), grid, block, 0, stream, y, ys, uv, uvs, sw, sh, W, H, dst, ds, mode);
+        hipLaunchKernelGGL((resize_merge_kernel<T, 4>), grid, block, lds, stream, y, ys, uv, uvs, sw, sh, W, H, dst, ds, mode, kmax);
     else if (channels == 3)
-        hipLaunchKernelGGL((resize_merge_kernel<T, 3>), grid, block, 0, stream, y, ys, uv, uvs, sw, sh, W, H, dst, ds, mode);
+        hipLaunchKernelGGL((resize_merge_kernel<T, 3>), grid, block, lds, stream, y, ys, uv, uvs, sw, sh, W, H, dst, ds, mode, kmax);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -283,35 +452,15 @@ static hipError_t resize_merge_t(int channels, int mode, const void* y, long ys,
 hipError_t launch_resize_merge(int type, int channels, int mode, const void* y, long ys, const void* uv, long uvs, int sw,
                                int sh, int W, int H, void* dst, long ds, hipStream_t stream)
 {
+    if (mode < 0 || mode >= kResizeModeCount || sw <= 0 || sh <= 0) return hipErrorInvalidValue;
+    if (W <= 0 || H <= 0) return hipSuccess;
+    const int kmax = resize_kmax(mode, sw, sh, W, H);
+    if (kmax < 0) return hipErrorInvalidValue;
     switch (type) {
-    case U8: return resize_merge_t<uint8_t>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, stream);
-    case U16: return resize_merge_t<uint16_t>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, stream);
-    case F16: return resize_merge_t<__half>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, stream);
-    case F32: return resize_merge_t<float>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, stream);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_color_split(int type, int channels, const void* src, long ss, int W, int H,
-                              void* y, long ys, void* uv, long uvs, hipStream_t stream)
-{
-    switch (type) {
-    case U8: return split_t<uint8_t>(channels, src, ss, W, H, y, ys, uv, uvs, stream);
-    case U16: return split_t<uint16_t>(channels, src, ss, W, H, y, ys, uv, uvs, stream);
-    case F16: return split_t<__half>(channels, src, ss, W, H, y, ys, uv, uvs, stream);
-    case F32: return split_t<float>(channels, src, ss, W, H, y, ys, uv, uvs, stream);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_color_merge(int type, int channels, const void* y, long ys, const void* uv, long uvs, int W, int H,
-                              void* dst, long ds, hipStream_t stream)
-{
-    switch (type) {
-    case U8: return merge_t<uint8_t>(channels, y, ys, uv, uvs, W, H, dst, ds, stream);
-    case U16: return merge_t<uint16_t>(channels, y, ys, uv, uvs, W, H, dst, ds, stream);
-    case F16: return merge_t<__half>(channels, y, ys, uv, uvs, W, H, dst, ds, stream);
-    case F32: return merge_t<float>(channels, y, ys, uv, uvs, W, H, dst, ds, stream);
+    case U8: return resize_merge_t<uint8_t>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, kmax, stream);
+    case U16: return resize_merge_t<uint16_t>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, kmax, stream);
+    case F16: return resize_merge_t<__half>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, kmax, stream);
+    case F32: return resize_merge_t<float>(channels, mode, y, ys, uv, uvs, sw, sh, W, H, dst, ds, kmax, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -319,11 +468,15 @@ hipError_t launch_color_merge(int type, int channels, const void* y, long ys, co
 hipError_t launch_resize(int type, int channels, int mode, const void* src, long ss, int sw, int sh,
                          void* dst, long ds, int dw, int dh, hipStream_t stream)
 {
+    if (mode < 0 || mode >= kResizeModeCount || sw <= 0 || sh <= 0) return hipErrorInvalidValue;
+    if (dw <= 0 || dh <= 0) return hipSuccess;
+    const int kmax = resize_kmax(mode, sw, sh, dw, dh);
+    if (kmax < 0) return hipErrorInvalidValue;
     switch (type) {
-    case U8: return resize_t<uint8_t>(channels, mode, src, ss, sw, sh, dst, ds, dw, dh, stream);
-    case U16: return resize_t<uint16_t>(channels, mode, src, ss, sw, sh, dst, ds, dw, dh, stream);
-    case F16: return resize_t<__half>(channels, mode, src, ss, sw, sh, dst, ds, dw, dh, stream);
-    case F32: return resize_t<float>(channels, mode, src, ss, sw, sh, dst, ds, dw, dh, stream);
+    case U8: return resize_t<uint8_t>(channels, mode, src, ss, sw, sh, dst, ds, dw, dh, kmax, stream);
+    case U16: return resize_t<uint16_t>(channels, mode, src, ss, sw, sh, dst, ds, dw, dh, kmax, stream);
+    case F16: return resize_t<__half>(channels, mode, src, ss, sw, sh, dst, ds, dw, dh, kmax, stream);
+    case F32: return resize_t<float>(channels, mode, src, ss, sw, sh, dst, ds, dw, dh, kmax, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -78,8 +78,49 @@ hipError_t launch_mfma32(int cfg, int stype, int dtype, const void* src, long ss
 hipError_t launch_mfma32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
                                const float* wts, const float* wa32, int max_wg, hipStream_t stream);
 
-// Colour split RGB(A) -> Y + UV(A), merge back, and resize (mode 0 bilinear, 1 Catmull-Rom);
-// interleaved channels, one element type, byte strides (acnet_color.hip).
+// ---- colour conversion and resize (acnet_color.hip) ----
+
+// One colour component of an image in device memory: component (x, y) is the element at
+// ptr + y * pitch bytes + (x * step + off) elements.
+struct PlaneView {
+    void* ptr;
+    long pitch;
+    int step, off;
+};
+// Components 0..3 in (R, G, B[, A] or Y, U, V[, A]) and out, in that order.
+struct ColourPlanes {
+    PlaneView in[4], out[4];
+};
+// ImageProcess.cpp: rgb2yuv, rgba2yuva, yuv2rgb, yuva2rgba
+enum ColourKind : int { kRgb2Yuv = 0, kRgba2Yuva = 1, kYuv2Rgb = 2, kYuva2Rgba = 3 };
+hipError_t launch_colour(int type, int kind, const ColourPlanes& planes, int W, int H, hipStream_t stream);
+
+// Resize modes: the reference's ResizeModes numbering (core/include/AC/Core/Image.hpp:276-295).
+enum ResizeMode : int {
+    kResizePoint = 0, kResizeCatmullRom, kResizeMitchell, kResizeBicubic0_60, kResizeBicubic0_75,
+    kResizeBicubic0_100, kResizeBicubic20_50, kResizeSoftCubic50, kResizeSoftCubic75, kResizeSoftCubic100,
+    kResizeLanczos2, kResizeLanczos3, kResizeLanczos4, kResizeSpline16, kResizeSpline36, kResizeSpline64,
+    kResizeBilinear, kResizeModeCount
+};
+// filter support radius in source pixels at scale 1 (ImageResize.cpp:193-263)
+__host__ __device__ inline float resize_support(int mode)
+{
+    switch (mode) {
+    case kResizePoint: return 0.5f;
+    case kResizeLanczos3: case kResizeSpline36: return 3.0f;
+    case kResizeLanczos4: case kResizeSpline64: return 4.0f;
+    case kResizeBilinear: return 1.0f;
+    default: return 2.0f;
+    }
+}
+// Upper bound of the taps of one output coordinate (source n_in -> n_out pixels); a resize
+// whose bound on either axis exceeds kResizeMaxTaps is rejected (downscales below about
+// 1/47 for support-2 filters, 1/23 for Lanczos4).
+constexpr int kResizeMaxTaps = 192;
+int resize_max_taps(int mode, int n_in, int n_out);
+
+// Colour split RGB(A) -> Y + UV(A) and merge back (interleaved channels, one element type,
+// byte strides), and resize; hipErrorInvalidValue for a mode, size or tap count out of range.
 hipError_t launch_color_split(int type, int channels, const void* src, long ss, int W, int H,
                               void* y, long ys, void* uv, long uvs, hipStream_t stream);
 hipError_t launch_color_merge(int type, int channels, const void* y, long ys, const void* uv, long uvs, int W, int H,
@@ -90,6 +131,23 @@ hipError_t launch_resize(int type, int channels, int mode, const void* src, long
 // and merged with Y (W x H) into RGB(A): launch_resize + launch_color_merge in one pass.
 hipError_t launch_resize_merge(int type, int channels, int mode, const void* y, long ys, const void* uv, long uvs, int sw,
                                int sh, int W, int H, void* dst, long ds, hipStream_t stream);
+
+// ---- element-wise image helpers (acnet_image.hip) ----
+// n = elements per row (width x channels); byte strides.
+// dst = fromFloat<dtype>(toFloat(src)) (ImageProcess.cpp:339-355)
+hipError_t launch_convert(int stype, int dtype, const void* src, long ss, void* dst, long ds, int n, int h,
+                          hipStream_t stream);
+// unsigned integer shift by `bits` (left when left != 0), truncated to the element type
+hipError_t launch_shift(int type, int left, int bits, const void* src, long ss, void* dst, long ds, int n, int h,
+                        hipStream_t stream);
+// pixelShuffle (ImageProcess.cpp:358-385): src sw x sh with sc = dc * up * up channels ->
+// dst (sw * up) x (sh * up) with dc channels
+hipError_t launch_pixel_shuffle(int stype, int dtype, const void* src, long ss, int sw, int sh, int sc, int up,
+                                void* dst, long ds, hipStream_t stream);
+// for every pixel of a w x h grid, copy len bytes from src + y*ss + x*spx + soff to
+// dst + y*ds + x*dpx + doff (channel extract / insert)
+hipError_t launch_copy_runs(const void* src, long ss, int spx, int soff, void* dst, long ds, int dpx, int doff, int len,
+                            int w, int h, hipStream_t stream);
 
 // Persistent batched launches: `counter` (device int, zero at launch) is the tile queue.
 hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,

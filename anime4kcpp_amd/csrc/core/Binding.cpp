@@ -8,6 +8,7 @@
 
 #include "AC/Core.h"
 #include "AC/Core.hpp"
+#include "core_internal.hpp"
 
 struct ACImageHandle
 {
@@ -106,6 +107,45 @@ int ac_image_to(const ACImage* const image, void* const data, const int stride)
     if (!(image && image->hptr && data)) return AC_ERROR(AC_EINVAL);
     image->hptr->object.to(data, stride);
     return AC_SUCCESS;
+}
+
+// Binding.cpp:125-159.  The reference returns AC_SUCCESS once its arguments are valid; here
+// the helpers run on the device and can fail, which returns AC_ERROR(AC_EPROCESSOR).
+namespace
+{
+    template <typename F>
+    int imageOp(ACImage* const out, F&& f)
+    {
+        (void)ac::core::detail::takeImageError();
+        f();
+        copyProp(out->hptr->object, out);
+        return ac::core::detail::takeImageError().empty() ? AC_SUCCESS : AC_ERROR(AC_EPROCESSOR);
+    }
+}
+int ac_resize(const ACImage* const src, ACImage* const dst, const double fx, const double fy, const int mode)
+{
+    if (!(src && src->hptr && dst && dst->hptr)) return AC_ERROR(AC_EINVAL);
+    return imageOp(dst, [&] { ac::core::resize(src->hptr->object, dst->hptr->object, fx, fy, mode); });
+}
+int ac_rgb2yuv(const ACImage* const rgb, ACImage* const yuv)
+{
+    if (!(rgb && rgb->hptr && yuv && yuv->hptr)) return AC_ERROR(AC_EINVAL);
+    return imageOp(yuv, [&] { ac::core::rgb2yuv(rgb->hptr->object, yuv->hptr->object); });
+}
+int ac_rgba2yuva(const ACImage* const rgba, ACImage* const yuva)
+{
+    if (!(rgba && rgba->hptr && yuva && yuva->hptr)) return AC_ERROR(AC_EINVAL);
+    return imageOp(yuva, [&] { ac::core::rgba2yuva(rgba->hptr->object, yuva->hptr->object); });
+}
+int ac_yuv2rgb(const ACImage* const yuv, ACImage* const rgb)
+{
+    if (!(yuv && yuv->hptr && rgb && rgb->hptr)) return AC_ERROR(AC_EINVAL);
+    return imageOp(rgb, [&] { ac::core::yuv2rgb(yuv->hptr->object, rgb->hptr->object); });
+}
+int ac_yuva2rgba(const ACImage* const yuva, ACImage* const rgba)
+{
+    if (!(yuva && yuva->hptr && rgba && rgba->hptr)) return AC_ERROR(AC_EINVAL);
+    return imageOp(rgba, [&] { ac::core::yuva2rgba(yuva->hptr->object, rgba->hptr->object); });
 }
 
 ACProcessor* ac_processor_alloc(void)

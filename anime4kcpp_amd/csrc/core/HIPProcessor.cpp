@@ -315,7 +315,7 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pipeline(const Image& src, Image& d
     if (fxy != 1.0) {
         const long rp = pitch_of(fw, es);
         if (!ensure(st.buf[5], rp * fh)) return false;
-        if (!hipok(achip::launch_resize(T, 1, 1, y, yPitch, static_cast<int>(cw), static_cast<int>(ch), st.buf[5].ptr, rp,
+        if (!hipok(achip::launch_resize(T, 1, achip::kResizeCatmullRom, y, yPitch, static_cast<int>(cw), static_cast<int>(ch), st.buf[5].ptr, rp,
                                         static_cast<int>(fw), static_cast<int>(fh), s), "resize")) return false;
         y = st.buf[5].ptr;
         yPitch = rp;
@@ -327,7 +327,7 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pipeline(const Image& src, Image& d
         // element type as the reference's intermediate image holds it)
         const long mp = pitch_of(fw * C, es);
         if (!ensure(st.buf[7], mp * fh)) return false;
-        if (!hipok(achip::launch_resize_merge(T, C, 1, y, yPitch, st.buf[2].ptr, uvPitch, static_cast<int>(w), static_cast<int>(h),
+        if (!hipok(achip::launch_resize_merge(T, C, achip::kResizeCatmullRom, y, yPitch, st.buf[2].ptr, uvPitch, static_cast<int>(w), static_cast<int>(h),
                                               static_cast<int>(fw), static_cast<int>(fh), st.buf[7].ptr, mp, s),
                    "resize and merge")) return false;
         out = st.buf[7].ptr;

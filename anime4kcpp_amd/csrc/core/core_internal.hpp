@@ -15,6 +15,10 @@ namespace ac::core::detail
     int ceilLog2(double v) noexcept;
     const char* typeNameOf(int type) noexcept;
 
+    // The calling thread's last image-helper failure (empty when the last helper call
+    // succeeded or none failed since the previous take); clears it.
+    std::string takeImageError() noexcept;
+
     // HIP processor for ACNetLegacy; precision 0 FP32, 1 FP16-MFMA, 2 BF16-MFMA
     std::shared_ptr<Processor> createHIP(int device, const model::ACNetLegacy& model, int precision);
 }

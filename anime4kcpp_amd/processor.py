@@ -120,3 +120,38 @@ def info(processor_type=3):
 
 def list_info():
     return _native.lib().ac_processor_list_info().decode()
+
+
+# ResizeModes (core/include/AC/Core/Image.hpp:272-291), as pyac's core.ResizeModes exports them
+(RESIZE_POINT, RESIZE_CATMULL_ROM, RESIZE_MITCHELL_NETRAVALI, RESIZE_BICUBIC_0_60, RESIZE_BICUBIC_0_75,
+ RESIZE_BICUBIC_0_100, RESIZE_BICUBIC_20_50, RESIZE_SOFTCUBIC50, RESIZE_SOFTCUBIC75, RESIZE_SOFTCUBIC100,
+ RESIZE_LANCZOS2, RESIZE_LANCZOS3, RESIZE_LANCZOS4, RESIZE_SPLINE16, RESIZE_SPLINE36, RESIZE_SPLINE64,
+ RESIZE_BILINEAR) = range(17)
+
+
+def resize(src, dsize=None, fx=0.0, fy=0.0, mode=RESIZE_BILINEAR):
+    """pyac's core.resize (binding/python/src/Binding.cpp:98-133) through ac_resize: the output
+    is dsize = (width, height) when given, else (int(w * fx), int(h * fy)); same dtype and
+    channels as src.  Runs on the current HIP device."""
+    a = np.asarray(src)
+    if a.ndim not in (2, 3):
+        raise ValueError("Incompatible dimension: expected 2 or 3.")
+    if a.dtype not in ELEM:
+        raise TypeError("Incompatible type: expected uint8, uint16, float16 or float32.")
+    h, w = a.shape[:2]
+    c = 1 if a.ndim == 2 else a.shape[2]
+    dw, dh = int(w * fx), int(h * fy)
+    if dsize is not None:
+        if len(dsize) != 2:
+            raise ValueError("dsize shouble be (width, height)")
+        dw, dh = int(dsize[0]), int(dsize[1])
+    out = np.empty((dh, dw) if c == 1 else (dh, dw, c), a.dtype)
+    if dw == w and dh == h:
+        out[...] = a   # ac_resize at the same size hands back src itself
+        return out
+    si = Image(a, mapped=True)
+    di = Image(out, mapped=True)
+    _native.check_ac(_native.lib().ac_resize(si._p, di._p, 0.0, 0.0, mode), "ac_resize")
+    if di.c.ptr != out.ctypes.data:
+        raise RuntimeError("ac_resize did not write the mapped output")
+    return out

@@ -81,6 +81,20 @@ AC_C_API int ac_image_clone(const ACImage* src, ACImage* dst);
 /* Binding.cpp:100-105 */
 AC_C_API int ac_image_to(const ACImage* image, void* data, int stride);
 
+/* Binding.cpp:125-159.  These run on the calling thread's current HIP device; a device
+ * failure returns AC_ERROR(AC_EPROCESSOR) (the reference's never fail once their arguments
+ * are valid).  mode is an ACResizeModes value. */
+/* Binding.cpp:125-131 */
+AC_C_API int ac_resize(const ACImage* src, ACImage* dst, double fx, double fy, int mode);
+/* Binding.cpp:132-138 */
+AC_C_API int ac_rgb2yuv(const ACImage* rgb, ACImage* yuv);
+/* Binding.cpp:139-145 */
+AC_C_API int ac_rgba2yuva(const ACImage* rgba, ACImage* yuva);
+/* Binding.cpp:146-152 */
+AC_C_API int ac_yuv2rgb(const ACImage* yuv, ACImage* rgb);
+/* Binding.cpp:153-159 */
+AC_C_API int ac_yuva2rgba(const ACImage* yuva, ACImage* rgba);
+
 static inline int ac_image_size(const ACImage* image) { return image->height * image->stride; }
 static inline int ac_image_element_size(const ACImage* image) { return image->element_type & 0xff; }
 static inline int ac_image_pixel_size(const ACImage* image) { return image->channels * ac_image_element_size(image); }

@@ -421,36 +421,117 @@ int orc_yuv2rgb(const void* y, long y_stride, const void* uv, long uv_stride,
 
 /* ---- resize (parity with stb_image_resize2 unpinned) -------------------------------- */
 
-/* Filter weight: mode 0 bilinear (support 1), 1 Catmull-Rom a = -0.5 (support 2). */
-static float filter_weight(int mode, float d)
+/* Filter kernels of ImageResize.cpp:53-134 and the mode switch of :193-263 (mode numbers are
+ * the ResizeModes of core/include/AC/Core/Image.hpp:272-291): the (b, c) cubic family of
+ * Mitchell and Netravali as Horner polynomials (:62-84), Lanczos (:85-91), the spline16/36/64
+ * piecewise cubics (:92-125) and the tent (:126-134).  Float throughout. */
+static float poly3(float x, float c0, float c1, float c2, float c3)
 {
-    d = fabsf(d);
-    if (mode == 0) return d < 1.0f ? 1.0f - d : 0.0f;
-    if (d < 1.0f) return (1.5f * d) * (d * d) + (-2.5f * (d * d) + 1.0f);
-    if (d < 2.0f) return (-0.5f * (d * (d * d)) + 2.5f * (d * d)) + (-4.0f * d + 2.0f);
+    return c0 + x * (c1 + x * (c2 + x * c3));
+}
+static float cubic_bc(float v, float b, float c)
+{
+    const float p0 = (6.0f - 2.0f * b) / 6.0f;
+    const float p1 = 0.0f;
+    const float p2 = (-18.0f + 12.0f * b + 6.0f * c) / 6.0f;
+    const float p3 = (12.0f - 9.0f * b - 6.0f * c) / 6.0f;
+    const float q0 = (8.0f * b + 24.0f * c) / 6.0f;
+    const float q1 = (-12.0f * b - 48.0f * c) / 6.0f;
+    const float q2 = (6.0f * b + 30.0f * c) / 6.0f;
+    const float q3 = (-b - 6.0f * c) / 6.0f;
+    const float x = fabsf(v);
+    if (x < 1.0f) return poly3(x, p0, p1, p2, p3);
+    if (x < 2.0f) return poly3(x, q0, q1, q2, q3);
+    return 0.0f;
+}
+static float sincf_(float x)
+{
+    const float pi = 3.14159265358979f;
+    return x == 0.0f ? 1.0f : sinf(x * pi) / (x * pi);
+}
+static float lanczos_(float v, float taps)
+{
+    const float x = fabsf(v);
+    return x < taps ? sincf_(x) * sincf_(x / taps) : 0.0f;
+}
+static float spline16_(float v)
+{
+    const float x = fabsf(v);
+    if (x < 1.0f) return poly3(x, 1.0f, -1.0f / 5.0f, -9.0f / 5.0f, 1.0f);
+    if (x < 2.0f) return poly3(x - 1.0f, 0.0f, -7.0f / 15.0f, 4.0f / 5.0f, -1.0f / 3.0f);
+    return 0.0f;
+}
+static float spline36_(float v)
+{
+    const float x = fabsf(v);
+    if (x < 1.0f) return poly3(x, 1.0f, -3.0f / 209.0f, -453.0f / 209.0f, 13.0f / 11.0f);
+    if (x < 2.0f) return poly3(x - 1.0f, 0.0f, -156.0f / 209.0f, 270.0f / 209.0f, -6.0f / 11.0f);
+    if (x < 3.0f) return poly3(x - 2.0f, 0.0f, 26.0f / 209.0f, -45.0f / 209.0f, 1.0f / 11.0f);
+    return 0.0f;
+}
+static float spline64_(float v)
+{
+    const float x = fabsf(v);
+    if (x < 1.0f) return poly3(x, 1.0f, -3.0f / 2911.0f, -6387.0f / 2911.0f, 49.0f / 41.0f);
+    if (x < 2.0f) return poly3(x - 1.0f, 0.0f, -2328.0f / 2911.0f, 4032.0f / 2911.0f, -24.0f / 41.0f);
+    if (x < 3.0f) return poly3(x - 2.0f, 0.0f, 582.0f / 2911.0f, -1008.0f / 2911.0f, 6.0f / 41.0f);
+    if (x < 4.0f) return poly3(x - 3.0f, 0.0f, -97.0f / 2911.0f, 168.0f / 2911.0f, -1.0f / 41.0f);
     return 0.0f;
 }
 
-#define ORC_MAX_TAPS 12
-/* Taps of output coordinate o: pixel-centre mapping, filter stretched by 1/scale when
- * downscaling, weights normalised to sum 1. */
+float orc_filter_weight(int mode, float x)
+{
+    switch (mode) {
+    case 0: return 1.0f;                                   /* POINT */
+    case 1: return cubic_bc(x, 0.0f, 1.0f / 2.0f);         /* CATMULL_ROM */
+    case 2: return cubic_bc(x, 1.0f / 3.0f, 1.0f / 3.0f);  /* MITCHELL_NETRAVALI */
+    case 3: return cubic_bc(x, 0.0f, 3.0f / 5.0f);         /* BICUBIC_0_60 */
+    case 4: return cubic_bc(x, 0.0f, 3.0f / 4.0f);         /* BICUBIC_0_75 */
+    case 5: return cubic_bc(x, 0.0f, 1.0f);                /* BICUBIC_0_100 */
+    case 6: return cubic_bc(x, 1.0f / 5.0f, 1.0f / 2.0f);  /* BICUBIC_20_50 */
+    case 7: return cubic_bc(x, 1.0f / 2.0f, 1.0f / 2.0f);  /* SOFTCUBIC50 */
+    case 8: return cubic_bc(x, 3.0f / 4.0f, 1.0f / 4.0f);  /* SOFTCUBIC75 */
+    case 9: return cubic_bc(x, 1.0f, 0.0f);                /* SOFTCUBIC100 */
+    case 10: return lanczos_(x, 2.0f);
+    case 11: return lanczos_(x, 3.0f);
+    case 12: return lanczos_(x, 4.0f);
+    case 13: return spline16_(x);
+    case 14: return spline36_(x);
+    case 15: return spline64_(x);
+    default: { const float a = fabsf(x); return a < 1.0f ? 1.0f - a : 0.0f; }   /* BILINEAR */
+    }
+}
+
+float orc_filter_support(int mode)
+{
+    switch (mode) {
+    case 0: return 0.5f;
+    case 11: case 14: return 3.0f;
+    case 12: case 15: return 4.0f;
+    case 16: return 1.0f;
+    default: return 2.0f;
+    }
+}
+
+/* Taps of output coordinate o: the source pixels whose centres lie in
+ * (center - support, center + support] (stb's pixel range), pixel-centre mapping, the
+ * filter stretched by 1/scale when downscaling, weights normalised to sum 1. */
+#define ORC_MAX_TAPS 256
 static int resize_taps(int mode, int o, int n_in, int n_out, float* w, int* first)
 {
     const float scale = (float)n_out / (float)n_in;
     const float center = ((float)o + 0.5f) / scale - 0.5f;
-    const float support = (mode == 0 ? 1.0f : 2.0f) * (scale < 1.0f ? 1.0f / scale : 1.0f);
+    const float support = orc_filter_support(mode) * (scale < 1.0f ? 1.0f / scale : 1.0f);
     const float stretch = scale < 1.0f ? scale : 1.0f;
-    int lo = (int)ceilf(center - support);
-    int hi = (int)floorf(center + support);
-    if (hi - lo + 1 > ORC_MAX_TAPS) {
-        lo = (int)center - ORC_MAX_TAPS / 2 + 1;
-        hi = lo + ORC_MAX_TAPS - 1;
-    }
+    const int lo = (int)floorf(center - support) + 1;
+    const int hi = (int)floorf(center + support);
+    int n = hi - lo + 1;
+    if (n < 0) n = 0;
+    if (n > ORC_MAX_TAPS) n = ORC_MAX_TAPS;
     float sum = 0.0f;
-    int n = 0;
-    for (int i = lo; i <= hi; i++, n++) {
-        w[n] = filter_weight(mode, ((float)i - center) * stretch);
-        sum += w[n];
+    for (int i = 0; i < n; i++) {
+        w[i] = orc_filter_weight(mode, ((float)(lo + i) - center) * stretch);
+        sum += w[i];
     }
     if (sum != 0.0f)
         for (int i = 0; i < n; i++) w[i] /= sum;
@@ -462,6 +543,7 @@ int orc_resize(const void* src, long src_stride, int sw, int sh, int channels, i
                void* dst, long dst_stride, int dw, int dh, int mode)
 {
     if (sw <= 0 || sh <= 0 || dw <= 0 || dh <= 0 || channels <= 0 || channels > 4) return -22;
+    if (mode < 0 || mode > 16) mode = 16;
     const int es = elem_size(type);
     for (int y = 0; y < dh; y++) {
         float wy[ORC_MAX_TAPS], wx[ORC_MAX_TAPS];

@@ -62,12 +62,17 @@ int orc_rgb2yuv(const void* rgb, long rgb_stride, int w, int h, int channels, in
 int orc_yuv2rgb(const void* y, long y_stride, const void* uv, long uv_stride,
                 int w, int h, int channels, int type, void* rgb, long rgb_stride);
 
-/* Separable resize with edge clamp and pixel-centre alignment, per channel, to dw x dh.
- * mode 0: bilinear; mode 1: Catmull-Rom (a = -0.5); the filter is stretched by 1/scale
- * when downscaling.  f32 math, horizontal then vertical, output through fromFloat.
- * Parity with stb_image_resize2 is unpinned; pinned against closed-form 2x taps. */
+/* Separable resize with edge clamp and pixel-centre alignment, per channel, to dw x dh,
+ * with the filter of ResizeModes `mode` (core/include/AC/Core/Image.hpp:272-291; out of range
+ * means bilinear, as ImageResize.cpp:260 does); the filter is stretched by 1/scale when
+ * downscaling.  f32 math, horizontal then vertical, output through fromFloat.
+ * Parity with stb_image_resize2 is unpinned; pinned against closed-form 2x taps, the
+ * filters' published forms and their interpolation / partition-of-unity properties. */
 int orc_resize(const void* src, long src_stride, int sw, int sh, int channels, int type,
                void* dst, long dst_stride, int dw, int dh, int mode);
+/* the filter kernel of `mode` at distance x, and its support radius */
+float orc_filter_weight(int mode, float x);
+float orc_filter_support(int mode);
 int orc_resize2x(const void* src, long src_stride, int w, int h, int channels, int type,
                  void* dst, long dst_stride, int mode);
 

@@ -13,6 +13,16 @@ U8, U16, F16, F32 = 0x001, 0x002, 0x202, 0x204
 NP_TYPE = {U8: np.uint8, U16: np.uint16, F16: np.float16, F32: np.float32}
 TYPE_OF = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16, np.dtype(np.float16): F16, np.dtype(np.float32): F32}
 ORDER_FMA, ORDER_GENERIC = 0, 1
+# ResizeModes (core/include/AC/Core/Image.hpp:272-291)
+RESIZE_MODES = ["POINT", "CATMULL_ROM", "MITCHELL_NETRAVALI", "BICUBIC_0_60", "BICUBIC_0_75", "BICUBIC_0_100",
+                "BICUBIC_20_50", "SOFTCUBIC50", "SOFTCUBIC75", "SOFTCUBIC100", "LANCZOS2", "LANCZOS3", "LANCZOS4",
+                "SPLINE16", "SPLINE36", "SPLINE64", "BILINEAR"]
+RESIZE_POINT, RESIZE_CATMULL_ROM, RESIZE_LANCZOS4, RESIZE_SPLINE36, RESIZE_BILINEAR = 0, 1, 12, 14, 16
+# (b, c) of the cubic modes
+RESIZE_BC = {1: (0, 1 / 2), 2: (1 / 3, 1 / 3), 3: (0, 3 / 5), 4: (0, 3 / 4), 5: (0, 1), 6: (1 / 5, 1 / 2),
+             7: (1 / 2, 1 / 2), 8: (3 / 4, 1 / 4), 9: (1, 0)}
+RESIZE_LANCZOS = {10: 2, 11: 3, 12: 4}
+RESIZE_SPLINE = {13: 2, 14: 3, 15: 4}
 
 _orc = None
 _ref = None
@@ -36,6 +46,10 @@ def orc():
         o.orc_resize2x.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int]
         o.orc_resize.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        o.orc_filter_weight.argtypes = [ctypes.c_int, ctypes.c_float]
+        o.orc_filter_weight.restype = ctypes.c_float
+        o.orc_filter_support.argtypes = [ctypes.c_int]
+        o.orc_filter_support.restype = ctypes.c_float
         _orc = o
     return _orc
 
@@ -98,3 +112,14 @@ def to_float(src):
     if src.dtype == np.uint16:
         return (src.astype(np.float32) / np.float32(65535.0)).astype(np.float32)
     return src.astype(np.float32)
+
+
+def resize(src, dw, dh, mode):
+    """The oracle's resize of an H x W (x C) array to dw x dh."""
+    src = np.ascontiguousarray(src)
+    h, w = src.shape[:2]
+    c = 1 if src.ndim == 2 else src.shape[2]
+    dst = np.zeros((dh, dw) + (() if c == 1 else (c,)), src.dtype)
+    assert orc().orc_resize(src.ctypes.data, src.strides[0], w, h, c, TYPE_OF[src.dtype], dst.ctypes.data,
+                            dst.strides[0], dw, dh, mode) == 0
+    return dst

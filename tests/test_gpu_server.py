@@ -135,7 +135,7 @@ def test_frames_match_oracle(net, weights):
         h, w = src.shape
         want = np.zeros((2 * h, 2 * w), np.uint8)
         assert OL.orc().orc_resize(src.ctypes.data, src.strides[0], w, h, 1, OL.U8, want.ctypes.data, want.strides[0],
-                                   2 * w, 2 * h, 0) == 0
+                                   2 * w, 2 * h, OL.RESIZE_BILINEAR) == 0
         d = np.abs(got.astype(int) - want.astype(int))
         assert np.array_equal(got, want), f"{np.count_nonzero(d)} of {d.size} chroma samples differ, max {d.max()}"
 

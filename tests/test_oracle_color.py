@@ -95,7 +95,7 @@ def closed_2x(x, mode):
     """2x: even outputs sample i - 1/4, odd i + 1/4; taps (1/4, 3/4) bilinear or
     (-3/128, 29/128, 111/128, -9/128) Catmull-Rom, edge clamped, rows then columns."""
     h, w = x.shape
-    if mode == 0:
+    if mode == OL.RESIZE_BILINEAR:
         even, odd = [(-1, 0.25), (0, 0.75)], [(0, 0.75), (1, 0.25)]
     else:
         even = [(-2, -0.0234375), (-1, 0.2265625), (0, 0.8671875), (1, -0.0703125)]
@@ -111,13 +111,13 @@ def closed_2x(x, mode):
     return np.clip(out, 0, 1)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [OL.RESIZE_BILINEAR, OL.RESIZE_CATMULL_ROM])
 def test_resize_2x_closed_form(mode):
     x = np.random.default_rng(5).random((9, 13), dtype=np.float32)
     assert np.array_equal(resize(x, 26, 18, mode), closed_2x(x, mode))
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [OL.RESIZE_BILINEAR, OL.RESIZE_CATMULL_ROM])
 @pytest.mark.parametrize("dw,dh", [(40, 30), (26, 18), (10, 9), (7, 5), (52, 36)])
 def test_resize_partition_of_unity_and_symmetry(mode, dw, dh):
     const = np.full((9, 13), 0.4, np.float32)
@@ -129,7 +129,7 @@ def test_resize_partition_of_unity_and_symmetry(mode, dw, dh):
     assert np.abs(a - b).max() < 4e-6   # taps summed in mirrored order
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [OL.RESIZE_BILINEAR, OL.RESIZE_CATMULL_ROM])
 @pytest.mark.parametrize("dw,dh", [(40, 30), (26, 18), (52, 36), (17, 11)])
 def test_resize_up_reproduces_ramps(mode, dw, dh):
     """Upscaling interpolates: a horizontal ramp inside (0, 1) comes back exactly at every
@@ -140,3 +140,92 @@ def test_resize_up_reproduces_ramps(mode, dw, dh):
     want = 0.1 + 0.8 * sx / 12
     inner = (sx > 1.0) & (sx < 11.0)
     assert np.abs(out[:, inner] - want[inner]).max() < 2e-6
+
+
+# ---- every ResizeModes filter (ImageResize.cpp:53-134, 193-263) ----
+
+def fw(mode, x):
+    return float(OL.orc().orc_filter_weight(mode, float(x)))
+
+
+XS = np.linspace(-4.5, 4.5, 721)
+
+
+@pytest.mark.parametrize("mode", sorted(OL.RESIZE_BC))
+def test_cubic_filters_match_mitchell_netravali(mode):
+    """The (b, c) family as Mitchell and Netravali publish it (Reconstruction Filters in
+    Computer Graphics, 1988, eq. 8), in float64, against the oracle's Horner form."""
+    B, C = OL.RESIZE_BC[mode]
+
+    def k(x):
+        x = abs(x)
+        if x < 1:
+            return ((12 - 9 * B - 6 * C) * x ** 3 + (-18 + 12 * B + 6 * C) * x ** 2 + (6 - 2 * B)) / 6
+        if x < 2:
+            return ((-B - 6 * C) * x ** 3 + (6 * B + 30 * C) * x ** 2 + (-12 * B - 48 * C) * x + (8 * B + 24 * C)) / 6
+        return 0.0
+
+    assert max(abs(fw(mode, x) - k(x)) for x in XS) < 2e-6
+    assert abs(fw(mode, 0) - (1 - B / 3)) < 1e-6 and abs(fw(mode, 1) - B / 6) < 1e-6 and fw(mode, 2) == 0
+
+
+@pytest.mark.parametrize("mode", sorted(OL.RESIZE_LANCZOS))
+def test_lanczos_matches_numpy_sinc(mode):
+    a = OL.RESIZE_LANCZOS[mode]
+    want = np.where(np.abs(XS) < a, np.sinc(XS) * np.sinc(XS / a), 0.0)
+    got = np.array([fw(mode, x) for x in XS])
+    assert np.abs(got - want).max() < 2e-6
+    assert OL.orc().orc_filter_support(mode) == a
+
+
+@pytest.mark.parametrize("mode", sorted(OL.RESIZE_SPLINE))
+def test_spline_kernels_interpolate_and_reproduce_lines(mode):
+    """Spline16/36/64 are interpolating piecewise cubics: k(0) = 1, k(n) = 0, continuous at
+    the knots, and at every phase the integer shifts sum to 1 and reproduce a line."""
+    s = OL.RESIZE_SPLINE[mode]
+    assert OL.orc().orc_filter_support(mode) == s
+    assert fw(mode, 0) == 1.0
+    for n in range(1, s + 1):
+        assert abs(fw(mode, n)) < 1e-6 and abs(fw(mode, -n)) < 1e-6
+        assert abs(fw(mode, n - 1e-4) - fw(mode, n + 1e-4)) < 1e-3
+    for t in np.linspace(0, 1, 17):
+        taps = [(n, fw(mode, t - n)) for n in range(-s, s + 2)]
+        assert abs(sum(w for _, w in taps) - 1) < 2e-6
+        assert abs(sum(w * n for n, w in taps) - t) < 2e-6
+
+
+def test_bilinear_and_point_kernels():
+    for x in XS:
+        assert abs(fw(OL.RESIZE_BILINEAR, x) - max(0.0, 1 - abs(x))) < 1e-7
+        assert fw(OL.RESIZE_POINT, x) == 1.0
+    assert OL.orc().orc_filter_support(OL.RESIZE_POINT) == 0.5
+    assert OL.orc().orc_filter_support(OL.RESIZE_BILINEAR) == 1.0
+
+
+def test_point_sampling_known_answer():
+    """tests/core/src/ImageResizeTest.cpp:46-78: 2x2 -> 4x4 RESIZE_POINT repeats pixels."""
+    src = np.array([[10, 20], [30, 40]], np.uint8)
+    want = np.array([[10, 10, 20, 20], [10, 10, 20, 20], [30, 30, 40, 40], [30, 30, 40, 40]], np.uint8)
+    assert np.array_equal(OL.resize(src, 4, 4, OL.RESIZE_POINT), want)
+
+
+@pytest.mark.parametrize("mode", range(17))
+@pytest.mark.parametrize("dw,dh", [(39, 27), (7, 5), (4, 3)])
+def test_every_mode_keeps_constants_and_mirrors(mode, dw, dh):
+    const = np.full((9, 13), 0.4, np.float32)
+    assert np.abs(OL.resize(const, dw, dh, mode) - 0.4).max() < 2e-6
+    x = np.random.default_rng(11).random((9, 13), dtype=np.float32)
+    a = OL.resize(x, dw, dh, mode)
+    b = OL.resize(np.ascontiguousarray(x[:, ::-1]), dw, dh, mode)[:, ::-1]
+    if mode == OL.RESIZE_POINT:
+        return   # ties (centre exactly between two pixels) go to the upper one: not mirror-symmetric
+    assert np.abs(a - b).max() < 5e-6
+
+
+@pytest.mark.parametrize("mode", [m for m in range(17) if m not in (2, 6, 7, 8, 9)])
+def test_interpolating_modes_pass_samples_through_at_3x(mode):
+    """At 3x every third output pixel sits on a source centre; filters with k(0) = 1 and
+    k(n) = 0 (b = 0 cubics, Lanczos, splines, tent, point) reproduce the source there."""
+    x = np.random.default_rng(13).random((7, 9), dtype=np.float32)
+    out = OL.resize(x, 27, 21, mode)
+    assert np.abs(out[1::3, 1::3] - x).max() < 1e-6
