@@ -31,10 +31,17 @@ class ACHipImage(ctypes.Structure):
                 ("w", c_int), ("h", c_int)]
 
 
+class ACHipFrameFormat(ctypes.Structure):
+    """include/achip.h ACHipFrameFormat"""
+    _fields_ = [("planes", c_int), ("chroma_shift_x", c_int), ("chroma_shift_y", c_int), ("element_type", c_int),
+                ("shift", c_int), ("chroma_filter", c_int)]
+
+
 class ACHipRequest(ctypes.Structure):
     """include/achip.h ACHipRequest"""
     _fields_ = [("kind", c_int), ("width", c_int), ("height", c_int), ("factor", c_int), ("stream", c_int),
-                ("src", c_void_p * 3), ("src_stride", c_long * 3), ("dst", c_void_p * 3), ("dst_stride", c_long * 3)]
+                ("src", c_void_p * 3), ("src_stride", c_long * 3), ("dst", c_void_p * 3), ("dst_stride", c_long * 3),
+                ("format", ACHipFrameFormat)]
 
 
 class ACHipServerConfig(ctypes.Structure):

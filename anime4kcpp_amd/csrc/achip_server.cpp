@@ -9,9 +9,13 @@
 // admitted meanwhile joins the next step instead of waiting for the big one to finish.
 // A request contributes at most one pass to a step (its next pass reads this one's
 // output); all steps run on one stream, so stream order carries the dependencies.
-// Video-stream frames (fixed geometry per stream) take a separate path: the luma pass and
-// the two chroma resizes of a frame are captured once per stream into a hipGraph and
-// replayed on that stream's own HIP stream, overlapping the batched steps on the device.
+// Video-stream frames (fixed geometry and format per stream) take a separate path: the luma
+// shifts and pass and the chroma resizes of a frame are captured once per stream into a
+// hipGraph and replayed on that stream's own HIP stream, overlapping the batched steps.
+// Luma, YUV420 and the general YUV kind share one path: a frame format (planes, chroma
+// subsampling, 8- or 16-bit words, LSB shift, chroma filter) as the reference's video
+// module describes frames (video/src/Pipeline.cpp:495-566, cli/src/Main.cpp:183-205).
+// u8 and u16 passes of one step go into separate persistent launches.
 // Every kernel is one of the library's own (the same tiles and arithmetic as a request
 // processed alone), so results do not depend on what else is in flight.
 #include <hip/hip_runtime.h>
@@ -36,6 +40,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr int kU8 = 0x001;
+constexpr int kU16 = 0x002;
 constexpr int kMaxStreams = 64;
 constexpr int kRing = 4;              // descriptor workspaces (steps in flight <= kRing - 2)
 constexpr int kMaxStepItems = 512;    // descriptors per step
@@ -102,23 +107,50 @@ private:
 struct Plane {
     void* ptr = nullptr;
     long stride = 0;
-    int w = 0, h = 0, ch = 1;
+    int w = 0, h = 0, ch = 1, es = 1;
     size_t bytes() const { return static_cast<size_t>(stride) * h; }
+    long row() const { return static_cast<long>(w) * ch * es; }
 };
+
+// chroma planes of a frame format: count, size and interleaved channels
+struct Chroma {
+    int n = 0, w = 0, h = 0, ch = 1;
+};
+Chroma chroma_of(const ACHipFrameFormat& f, int W, int H)
+{
+    Chroma c;
+    if (f.planes <= 1) return c;
+    c.n = f.planes - 1;
+    c.w = W >> f.chroma_shift_x;
+    c.h = H >> f.chroma_shift_y;
+    c.ch = f.planes == 2 ? 2 : 1;
+    return c;
+}
+int chroma_mode(const ACHipFrameFormat& f)
+{
+    return f.chroma_filter == ACHIP_CHROMA_CATMULL_ROM ? achip::kResizeCatmullRom : achip::kResizeBilinear;
+}
+bool same_format(const ACHipFrameFormat& a, const ACHipFrameFormat& b)
+{
+    return a.planes == b.planes && a.chroma_shift_x == b.chroma_shift_x && a.chroma_shift_y == b.chroma_shift_y &&
+           a.element_type == b.element_type && a.shift == b.shift && a.chroma_filter == b.chroma_filter;
+}
 
 enum class St { Pending, Uploading, Passing, Finishing, Done };
 
 struct Req {
     unsigned long long ticket = 0;
     ACHipRequest r{};
+    ACHipFrameFormat fmt{};           // luma/YUV kinds: the normalised frame format
+    int type = kU8;                   // element type of every plane
     int passes = 0;
     size_t need = 0;                  // bytes reserved against the budget
     Clock::time_point t_submit, t_done;
     St st = St::Pending;
     int err = ACHIP_OK;
     std::string msg;
-    // one-off buffers: in (RGB), luma[0..passes], uv (RGB) or u0 v0 u1 v1 (YUV420), out (RGB)
-    Plane in, luma[4], uv, u0, v0, u1, v1, out;
+    // one-off buffers: in (RGB), luma[0..passes], uv (RGB), chroma in c0 / out c1 (YUV), out (RGB)
+    Plane in, luma[4], uv, c0[2], c1[2], out;
     int pass = 0, next_row = 0;
     hipEvent_t ev_up = nullptr;       // upload done
     hipEvent_t ev_done = nullptr;     // everything done (own event, or a step's / a frame's)
@@ -130,7 +162,7 @@ struct Req {
 struct Slot {                          // one video stream
     int w = 0, h = 0;
     hipStream_t stream = nullptr;
-    Plane y0, u0, v0, y1, u1, v1;
+    Plane y0, c0[2], y1, c1[2];
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::deque<std::shared_ptr<Req>> inflight;
@@ -167,6 +199,7 @@ struct ACHipServer {
     std::vector<std::shared_ptr<Req>> released;         // kept outputs to free (scheduler thread)
     size_t reserved = 0;
     int stream_w[kMaxStreams] = {}, stream_h[kMaxStreams] = {};
+    ACHipFrameFormat stream_fmt[kMaxStreams] = {};
     bool slot_reserved[kMaxStreams] = {};
 
     // scheduler-thread state
@@ -196,22 +229,25 @@ struct ACHipServer {
     void poll();
     void complete(const std::shared_ptr<Req>& q, int err, const std::string& msg);
     void free_req(Req& q);
-    int alloc(Plane& p, int w, int h, int ch);
+    int alloc(Plane& p, int w, int h, int ch, int es = 1);
     bool busy_work() const;
 };
 
 namespace {
 
 // device bytes a request needs (frames of an existing stream need none)
-size_t request_bytes(const ACHipRequest& r, int passes)
+size_t request_bytes(const ACHipRequest& r, const ACHipFrameFormat& fmt, int type, int passes)
 {
-    const size_t W = r.width, H = r.height, f = static_cast<size_t>(1) << passes;
+    const size_t W = r.width, H = r.height, f = static_cast<size_t>(1) << passes, es = type & 0xff;
     size_t b = 0;
-    for (int k = 0; k <= passes; k++) b += round_block((W << k) * (H << k));
-    if (r.kind == ACHIP_REQ_RGB)
+    for (int k = 0; k <= passes; k++) b += round_block((W << k) * (H << k) * es);
+    if (r.kind == ACHIP_REQ_RGB) {
         b += round_block(3 * W * H) + round_block(2 * W * H) + round_block(3 * f * W * f * H);
-    else if (r.kind == ACHIP_REQ_YUV420)
-        b += 2 * round_block(W / 2 * (H / 2)) + 2 * round_block(f * W / 2 * (f * H / 2));
+    } else {
+        const Chroma c = chroma_of(fmt, r.width, r.height);
+        const size_t cw = c.w, chh = c.h, cc = c.ch;
+        b += c.n * (round_block(cw * chh * cc * es) + round_block(f * cw * f * chh * cc * es));
+    }
     return b;
 }
 
@@ -227,18 +263,19 @@ int copy_plane(void* dst, long dstride, const void* src, long sstride, long row_
 
 } // namespace
 
-int ACHipServer::alloc(Plane& p, int w, int h, int ch)
+int ACHipServer::alloc(Plane& p, int w, int h, int ch, int es)
 {
     p.w = w;
     p.h = h;
     p.ch = ch;
-    p.stride = static_cast<long>(w) * ch;
+    p.es = es;
+    p.stride = p.row();
     return pool.get(p.bytes(), &p.ptr) == hipSuccess ? ACHIP_OK : ACHIP_ENOMEM;
 }
 
 void ACHipServer::free_req(Req& q)
 {
-    for (Plane* p : { &q.in, &q.luma[0], &q.luma[1], &q.luma[2], &q.luma[3], &q.uv, &q.u0, &q.v0, &q.u1, &q.v1, &q.out }) {
+    for (Plane* p : { &q.in, &q.luma[0], &q.luma[1], &q.luma[2], &q.luma[3], &q.uv, &q.c0[0], &q.c0[1], &q.c1[0], &q.c1[1], &q.out }) {
         pool.put(p->ptr);
         p->ptr = nullptr;
     }
@@ -308,33 +345,40 @@ void ACHipServer::admit_locked(std::vector<std::shared_ptr<Req>>& frames, std::v
 int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
 {
     const ACHipRequest& r = q->r;
+    const ACHipFrameFormat& f = q->fmt;
+    const int T = q->type, es = T & 0xff;
+    const Chroma c = chroma_of(f, r.width, r.height);
     Slot& s = slots[r.stream];
     if (!s.w) {    // first frame of this stream: buffers, stream and graph
         if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return ACHIP_EIO;
-        int e = alloc(s.y0, r.width, r.height, 1);
-        if (!e) e = alloc(s.u0, r.width / 2, r.height / 2, 1);
-        if (!e) e = alloc(s.v0, r.width / 2, r.height / 2, 1);
-        if (!e) e = alloc(s.y1, 2 * r.width, 2 * r.height, 1);
-        if (!e) e = alloc(s.u1, r.width, r.height, 1);
-        if (!e) e = alloc(s.v1, r.width, r.height, 1);
+        int e = alloc(s.y0, r.width, r.height, 1, es);
+        if (!e) e = alloc(s.y1, 2 * r.width, 2 * r.height, 1, es);
+        for (int i = 0; i < c.n && !e; i++) {
+            e = alloc(s.c0[i], c.w, c.h, c.ch, es);
+            if (!e) e = alloc(s.c1[i], 2 * c.w, 2 * c.h, c.ch, es);
+        }
         if (e) return e;
         s.w = r.width;
         s.h = r.height;
-        const int pcfg = achip::auto_config(net, cfg.precision, kU8, kU8, s.w, s.h);
+        const int pcfg = achip::auto_config(net, cfg.precision, T, T, s.w, s.h);
         if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ACHIP_EIO;
-        hipError_t he = achip::launch_pass(net, cfg.precision, pcfg, kU8, kU8, s.y0.ptr, s.y0.stride, s.w, s.h, s.y1.ptr,
-                                           s.y1.stride, s.stream);
+        hipError_t he = hipSuccess;
+        if (f.shift) he = achip::launch_shift(T, 1, f.shift, s.y0.ptr, s.y0.stride, s.y0.ptr, s.y0.stride, s.w, s.h, s.stream);
         if (he == hipSuccess)
-            he = achip::launch_resize(kU8, 1, achip::kResizeBilinear, s.u0.ptr, s.u0.stride, s.u0.w, s.u0.h, s.u1.ptr, s.u1.stride, s.u1.w, s.u1.h, s.stream);
-        if (he == hipSuccess)
-            he = achip::launch_resize(kU8, 1, achip::kResizeBilinear, s.v0.ptr, s.v0.stride, s.v0.w, s.v0.h, s.v1.ptr, s.v1.stride, s.v1.w, s.v1.h, s.stream);
+            he = achip::launch_pass(net, cfg.precision, pcfg, T, T, s.y0.ptr, s.y0.stride, s.w, s.h, s.y1.ptr, s.y1.stride,
+                                    s.stream);
+        if (he == hipSuccess && f.shift)
+            he = achip::launch_shift(T, 0, f.shift, s.y1.ptr, s.y1.stride, s.y1.ptr, s.y1.stride, s.y1.w, s.y1.h, s.stream);
+        for (int i = 0; i < c.n && he == hipSuccess; i++)
+            he = achip::launch_resize(T, c.ch, chroma_mode(f), s.c0[i].ptr, s.c0[i].stride, s.c0[i].w, s.c0[i].h, s.c1[i].ptr,
+                                      s.c1[i].stride, s.c1[i].w, s.c1[i].h, s.stream);
         const hipError_t ce = hipStreamEndCapture(s.stream, &s.graph);
         if (he != hipSuccess || ce != hipSuccess) return ACHIP_EIO;
         if (hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0) != hipSuccess) return ACHIP_EIO;
     }
-    int e = copy_plane(s.y0.ptr, s.y0.stride, r.src[0], r.src_stride[0], s.w, s.h, s.stream);
-    if (!e) e = copy_plane(s.u0.ptr, s.u0.stride, r.src[1], r.src_stride[1], s.w / 2, s.h / 2, s.stream);
-    if (!e) e = copy_plane(s.v0.ptr, s.v0.stride, r.src[2], r.src_stride[2], s.w / 2, s.h / 2, s.stream);
+    int e = copy_plane(s.y0.ptr, s.y0.stride, r.src[0], r.src_stride[0], s.y0.row(), s.h, s.stream);
+    for (int i = 0; i < c.n && !e; i++)
+        e = copy_plane(s.c0[i].ptr, s.c0[i].stride, r.src[1 + i], r.src_stride[1 + i], s.c0[i].row(), s.c0[i].h, s.stream);
     if (e) return e;
     if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return ACHIP_EIO;
     {
@@ -342,9 +386,9 @@ int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
         replays++;
     }
     if (r.dst[0]) {
-        e = copy_plane(r.dst[0], r.dst_stride[0], s.y1.ptr, s.y1.stride, s.y1.w, s.y1.h, s.stream);
-        if (!e) e = copy_plane(r.dst[1], r.dst_stride[1], s.u1.ptr, s.u1.stride, s.u1.w, s.u1.h, s.stream);
-        if (!e) e = copy_plane(r.dst[2], r.dst_stride[2], s.v1.ptr, s.v1.stride, s.v1.w, s.v1.h, s.stream);
+        e = copy_plane(r.dst[0], r.dst_stride[0], s.y1.ptr, s.y1.stride, s.y1.row(), s.y1.h, s.stream);
+        for (int i = 0; i < c.n && !e; i++)
+            e = copy_plane(r.dst[1 + i], r.dst_stride[1 + i], s.c1[i].ptr, s.c1[i].stride, s.c1[i].row(), s.c1[i].h, s.stream);
         if (e) return e;
     }
     if (hipEventCreateWithFlags(&q->ev_done, hipEventDisableTiming) != hipSuccess) return ACHIP_EIO;
@@ -358,26 +402,24 @@ int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
 int ACHipServer::start_oneoff(const std::shared_ptr<Req>& q)
 {
     const ACHipRequest& r = q->r;
-    const int W = r.width, H = r.height, P = q->passes, f = 1 << P;
+    const int W = r.width, H = r.height, P = q->passes, f = 1 << P, es = q->type & 0xff;
     int e = ACHIP_OK;
-    for (int k = 0; k <= P && !e; k++) e = alloc(q->luma[k], W << k, H << k, 1);
+    for (int k = 0; k <= P && !e; k++) e = alloc(q->luma[k], W << k, H << k, 1, es);
+    const Chroma c = r.kind == ACHIP_REQ_RGB ? Chroma{} : chroma_of(q->fmt, W, H);
     if (!e && r.kind == ACHIP_REQ_RGB) {
         e = alloc(q->in, W, H, 3);
         if (!e) e = alloc(q->uv, W, H, 2);
         if (!e) e = alloc(q->out, f * W, f * H, 3);
-    } else if (!e && r.kind == ACHIP_REQ_YUV420) {
-        e = alloc(q->u0, W / 2, H / 2, 1);
-        if (!e) e = alloc(q->v0, W / 2, H / 2, 1);
-        if (!e) e = alloc(q->u1, f * W / 2, f * H / 2, 1);
-        if (!e) e = alloc(q->v1, f * W / 2, f * H / 2, 1);
+    }
+    for (int i = 0; i < c.n && !e; i++) {
+        e = alloc(q->c0[i], c.w, c.h, c.ch, es);
+        if (!e) e = alloc(q->c1[i], f * c.w, f * c.h, c.ch, es);
     }
     if (e) return e;
-    if (r.kind == ACHIP_REQ_RGB) e = copy_plane(q->in.ptr, q->in.stride, r.src[0], r.src_stride[0], 3L * W, H, upload);
-    else e = copy_plane(q->luma[0].ptr, q->luma[0].stride, r.src[0], r.src_stride[0], W, H, upload);
-    if (!e && r.kind == ACHIP_REQ_YUV420) {
-        e = copy_plane(q->u0.ptr, q->u0.stride, r.src[1], r.src_stride[1], W / 2, H / 2, upload);
-        if (!e) e = copy_plane(q->v0.ptr, q->v0.stride, r.src[2], r.src_stride[2], W / 2, H / 2, upload);
-    }
+    if (r.kind == ACHIP_REQ_RGB) e = copy_plane(q->in.ptr, q->in.stride, r.src[0], r.src_stride[0], q->in.row(), H, upload);
+    else e = copy_plane(q->luma[0].ptr, q->luma[0].stride, r.src[0], r.src_stride[0], q->luma[0].row(), H, upload);
+    for (int i = 0; i < c.n && !e; i++)
+        e = copy_plane(q->c0[i].ptr, q->c0[i].stride, r.src[1 + i], r.src_stride[1 + i], q->c0[i].row(), c.h, upload);
     if (e) return e;
     if (hipEventCreateWithFlags(&q->ev_up, hipEventDisableTiming) != hipSuccess) return ACHIP_EIO;
     if (hipEventRecord(q->ev_up, upload) != hipSuccess) return ACHIP_EIO;
@@ -398,22 +440,28 @@ bool ACHipServer::build_step()
     for (const Step& s : steps) inflight += s.busy ? 1 : 0;
     if (inflight >= kRing - 2) return false;
 
+    // descriptors of u8 passes first, then u16 passes: one persistent launch per element
+    // type, each with its own tile counter (bytes 0 and 64 of the header) and tile numbering
     auto* hdr = static_cast<char*>(ws_host[step_next]);
     auto* desc = reinterpret_cast<achip::BatchDesc*>(hdr + 256);
+    std::vector<achip::BatchDesc> d16;
     int n = 0;
-    long total = 0;
+    long total = 0, total16 = 0;
     long budget = cfg.step_tiles;
     std::vector<std::shared_ptr<Req>> finishing, members;
     for (const std::shared_ptr<Req>& q : active) {
-        if (budget <= 0 || n >= kMaxStepItems) break;
+        if (budget <= 0 || n + static_cast<int>(d16.size()) >= kMaxStepItems) break;
         if (q->st != St::Passing) continue;
         members.push_back(q);
         const Plane& a = q->luma[q->pass];
         const Plane& b = q->luma[q->pass + 1];
         const int tiles_x = (a.w + tw - 1) / tw, tiles_y = (a.h + th - 1) / th;
         const int rows = std::min<long>(tiles_y - q->next_row, std::max<long>(1, budget / tiles_x));
-        desc[n++] = achip::BatchDesc{ a.ptr, b.ptr, a.stride, b.stride, a.w, a.h, tiles_x, static_cast<int>(total), q->next_row };
-        total += static_cast<long>(rows) * tiles_x;
+        long& tot = q->type == kU16 ? total16 : total;
+        const achip::BatchDesc d{ a.ptr, b.ptr, a.stride, b.stride, a.w, a.h, tiles_x, static_cast<int>(tot), q->next_row };
+        if (q->type == kU16) d16.push_back(d);
+        else desc[n++] = d;
+        tot += static_cast<long>(rows) * tiles_x;
         budget -= static_cast<long>(rows) * tiles_x;
         q->next_row += rows;
         if (q->next_row == tiles_y) {
@@ -425,28 +473,34 @@ bool ACHipServer::build_step()
             }
         }
     }
-    if (n == 0) return false;
-    std::memset(hdr, 0, 256);   // tile counter
-    const size_t bytes = 256 + static_cast<size_t>(n) * sizeof(achip::BatchDesc);
-    hipError_t e = hipMemcpyAsync(ws_dev[step_next], hdr, bytes, hipMemcpyHostToDevice, batch);
-    if (e == hipSuccess)
-        e = achip::launch_batch(net, cfg.precision, config, kU8, kU8,
-                                reinterpret_cast<const achip::BatchDesc*>(static_cast<char*>(ws_dev[step_next]) + 256), n,
-                                static_cast<int>(total), static_cast<int*>(ws_dev[step_next]), cfg.max_workgroups, batch);
+    const int n16 = static_cast<int>(d16.size());
+    if (n + n16 == 0) return false;
+    std::memcpy(desc + n, d16.data(), static_cast<size_t>(n16) * sizeof(achip::BatchDesc));
+    std::memset(hdr, 0, 256);   // tile counters
+    const size_t bytes = 256 + static_cast<size_t>(n + n16) * sizeof(achip::BatchDesc);
+    char* ws = static_cast<char*>(ws_dev[step_next]);
+    const auto* ddesc = reinterpret_cast<const achip::BatchDesc*>(ws + 256);
+    hipError_t e = hipMemcpyAsync(ws, hdr, bytes, hipMemcpyHostToDevice, batch);
+    if (e == hipSuccess && n)
+        e = achip::launch_batch(net, cfg.precision, config, kU8, kU8, ddesc, n, static_cast<int>(total), reinterpret_cast<int*>(ws),
+                                cfg.max_workgroups, batch);
+    if (e == hipSuccess && n16)
+        e = achip::launch_batch(net, cfg.precision, config, kU16, kU16, ddesc + n, n16, static_cast<int>(total16),
+                                reinterpret_cast<int*>(ws + 64), cfg.max_workgroups, batch);
     for (const std::shared_ptr<Req>& q : finishing) {
         if (e != hipSuccess) break;
-        const int P = q->passes;
+        const int P = q->passes, T = q->type;
         const Plane& y = q->luma[P];
         if (q->r.kind == ACHIP_REQ_RGB) {
-            e = achip::launch_resize_merge(kU8, 3, achip::kResizeCatmullRom, y.ptr, y.stride, q->uv.ptr, q->uv.stride, q->uv.w, q->uv.h, y.w, y.h,
-                                           q->out.ptr, q->out.stride, batch);
-        } else if (q->r.kind == ACHIP_REQ_YUV420) {
-            e = achip::launch_resize(kU8, 1, achip::kResizeBilinear, q->u0.ptr, q->u0.stride, q->u0.w, q->u0.h, q->u1.ptr, q->u1.stride, q->u1.w,
-                                     q->u1.h, batch);
-            if (e == hipSuccess)
-                e = achip::launch_resize(kU8, 1, achip::kResizeBilinear, q->v0.ptr, q->v0.stride, q->v0.w, q->v0.h, q->v1.ptr, q->v1.stride,
-                                         q->v1.w, q->v1.h, batch);
+            e = achip::launch_resize_merge(kU8, 3, achip::kResizeCatmullRom, y.ptr, y.stride, q->uv.ptr, q->uv.stride, q->uv.w,
+                                           q->uv.h, y.w, y.h, q->out.ptr, q->out.stride, batch);
+            continue;
         }
+        if (q->fmt.shift) e = achip::launch_shift(T, 0, q->fmt.shift, y.ptr, y.stride, y.ptr, y.stride, y.w, y.h, batch);
+        const Chroma c = chroma_of(q->fmt, q->r.width, q->r.height);
+        for (int i = 0; i < c.n && e == hipSuccess; i++)
+            e = achip::launch_resize(T, c.ch, chroma_mode(q->fmt), q->c0[i].ptr, q->c0[i].stride, q->c0[i].w, q->c0[i].h,
+                                     q->c1[i].ptr, q->c1[i].stride, q->c1[i].w, q->c1[i].h, batch);
     }
     if (e == hipSuccess) e = hipEventRecord(step.ev, batch);
     if (e != hipSuccess) {
@@ -476,6 +530,12 @@ void ACHipServer::poll()
             complete(q, ACHIP_EIO, "colour split launch failed");
             continue;
         }
+        const Plane& y0 = q->luma[0];
+        if (q->r.kind != ACHIP_REQ_RGB && q->fmt.shift &&
+            achip::launch_shift(q->type, 1, q->fmt.shift, y0.ptr, y0.stride, y0.ptr, y0.stride, y0.w, y0.h, batch) != hipSuccess) {
+            complete(q, ACHIP_EIO, "luma shift launch failed");
+            continue;
+        }
         q->st = St::Passing;
     }
     // steps finished: their finishing requests are done (or copy out)
@@ -491,12 +551,11 @@ void ACHipServer::poll()
             }
             int e = ACHIP_OK;
             const Plane& y = q->luma[q->passes];
-            if (r.kind == ACHIP_REQ_RGB) e = copy_plane(r.dst[0], r.dst_stride[0], q->out.ptr, q->out.stride, 3L * q->out.w, q->out.h, download);
-            else e = copy_plane(r.dst[0], r.dst_stride[0], y.ptr, y.stride, y.w, y.h, download);
-            if (!e && r.kind == ACHIP_REQ_YUV420) {
-                e = copy_plane(r.dst[1], r.dst_stride[1], q->u1.ptr, q->u1.stride, q->u1.w, q->u1.h, download);
-                if (!e) e = copy_plane(r.dst[2], r.dst_stride[2], q->v1.ptr, q->v1.stride, q->v1.w, q->v1.h, download);
-            }
+            if (r.kind == ACHIP_REQ_RGB) e = copy_plane(r.dst[0], r.dst_stride[0], q->out.ptr, q->out.stride, q->out.row(), q->out.h, download);
+            else e = copy_plane(r.dst[0], r.dst_stride[0], y.ptr, y.stride, y.row(), y.h, download);
+            const int nc = r.kind == ACHIP_REQ_RGB ? 0 : chroma_of(q->fmt, r.width, r.height).n;
+            for (int i = 0; i < nc && !e; i++)
+                e = copy_plane(r.dst[1 + i], r.dst_stride[1 + i], q->c1[i].ptr, q->c1[i].stride, q->c1[i].row(), q->c1[i].h, download);
             if (!e && hipEventCreateWithFlags(&q->ev_done, hipEventDisableTiming) != hipSuccess) e = ACHIP_EIO;
             if (!e) {
                 q->own_done_event = true;
@@ -630,7 +689,7 @@ void achip_server_destroy(ACHipServer* s)
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
         if (sl.graph) (void)hipGraphDestroy(sl.graph);
-        for (Plane* p : { &sl.y0, &sl.u0, &sl.v0, &sl.y1, &sl.u1, &sl.v1 }) s->pool.put(p->ptr);
+        for (Plane* p : { &sl.y0, &sl.c0[0], &sl.c0[1], &sl.y1, &sl.c1[0], &sl.c1[1] }) s->pool.put(p->ptr);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     for (auto& kv : s->tickets) s->free_req(*kv.second);
@@ -650,7 +709,7 @@ void achip_server_destroy(ACHipServer* s)
 int achip_server_submit(ACHipServer* s, const ACHipRequest* r, unsigned long long* ticket)
 {
     if (!s || !r || !ticket) return ACHIP_EINVAL;
-    if (r->kind < ACHIP_REQ_LUMA || r->kind > ACHIP_REQ_RGB) return ACHIP_EINVAL;
+    if (r->kind < ACHIP_REQ_LUMA || r->kind > ACHIP_REQ_YUV) return ACHIP_EINVAL;
     if (r->width <= 0 || r->height <= 0 || r->width > (1 << 15) || r->height > (1 << 15)) return ACHIP_EINVAL;
     int passes = 0;
     switch (r->factor) {
@@ -661,34 +720,52 @@ int achip_server_submit(ACHipServer* s, const ACHipRequest* r, unsigned long lon
     }
     if ((static_cast<long>(r->width) << passes) > (1 << 16) || (static_cast<long>(r->height) << passes) > (1 << 16))
         return ACHIP_EINVAL;
-    const int planes = r->kind == ACHIP_REQ_YUV420 ? 3 : 1;
-    const int ch = r->kind == ACHIP_REQ_RGB ? 3 : 1;
-    if (r->kind == ACHIP_REQ_YUV420 && ((r->width | r->height) & 1)) return ACHIP_EINVAL;
-    for (int p = 0; p < planes; p++) {
-        const long row = p == 0 ? static_cast<long>(r->width) * ch : r->width / 2;
-        if (!r->src[p] || r->src_stride[p] < row) return ACHIP_EINVAL;
+    // the frame format: LUMA and YUV420 are shorthands of the general kind
+    ACHipFrameFormat f{};
+    if (r->kind == ACHIP_REQ_LUMA) f = ACHipFrameFormat{ 1, 0, 0, kU8, 0, ACHIP_CHROMA_BILINEAR };
+    else if (r->kind == ACHIP_REQ_YUV420) f = ACHipFrameFormat{ 3, 1, 1, kU8, 0, ACHIP_CHROMA_BILINEAR };
+    else if (r->kind == ACHIP_REQ_YUV) f = r->format;
+    const bool rgb = r->kind == ACHIP_REQ_RGB;
+    const int type = rgb ? kU8 : f.element_type;
+    if (!rgb) {
+        if (f.planes < 1 || f.planes > 3) return ACHIP_EINVAL;
+        if (f.element_type != kU8 && f.element_type != kU16) return ACHIP_EINVAL;
+        if (f.shift < 0 || f.shift > (f.element_type == kU8 ? 0 : 15)) return ACHIP_EINVAL;
+        if (f.planes == 1) f.chroma_shift_x = f.chroma_shift_y = 0, f.chroma_filter = ACHIP_CHROMA_BILINEAR;
+        if (f.chroma_shift_x < 0 || f.chroma_shift_x > 1 || f.chroma_shift_y < 0 || f.chroma_shift_y > 1) return ACHIP_EINVAL;
+        if (f.chroma_filter != ACHIP_CHROMA_BILINEAR && f.chroma_filter != ACHIP_CHROMA_CATMULL_ROM) return ACHIP_EINVAL;
+        if ((r->width & ((1 << f.chroma_shift_x) - 1)) || (r->height & ((1 << f.chroma_shift_y) - 1))) return ACHIP_EINVAL;
     }
+    const int planes = rgb ? 1 : f.planes;
+    const long es = type & 0xff;
+    const Chroma c = rgb ? Chroma{} : chroma_of(f, r->width, r->height);
+    auto row_of = [&](int p) { return p == 0 ? static_cast<long>(r->width) * (rgb ? 3 : 1) * es : static_cast<long>(c.w) * c.ch * es; };
+    for (int p = 0; p < planes; p++)
+        if (!r->src[p] || r->src_stride[p] < row_of(p)) return ACHIP_EINVAL;
     const bool has_dst = r->dst[0] != nullptr;
     for (int p = 0; p < planes; p++) {
-        const long row = (p == 0 ? static_cast<long>(r->width) * ch : r->width / 2) << passes;
         if (has_dst != (r->dst[p] != nullptr)) return ACHIP_EINVAL;
-        if (has_dst && r->dst_stride[p] < row) return ACHIP_EINVAL;
+        if (has_dst && r->dst_stride[p] < (row_of(p) << passes)) return ACHIP_EINVAL;
     }
     if (r->stream >= kMaxStreams || r->stream < -1) return ACHIP_EINVAL;
-    if (r->stream >= 0 && (r->kind != ACHIP_REQ_YUV420 || r->factor != 2)) return ACHIP_EINVAL;
+    if (r->stream >= 0 && (rgb || r->factor != 2)) return ACHIP_EINVAL;
     auto q = std::make_shared<Req>();
     q->r = *r;
+    q->fmt = f;
+    q->type = type;
     q->passes = passes;
-    q->need = r->stream >= 0 ? request_bytes(*r, 1) : request_bytes(*r, passes);
+    q->need = r->stream >= 0 ? request_bytes(*r, f, type, 1) : request_bytes(*r, f, type, passes);
     if (q->need > s->cfg.hbm_budget) return ACHIP_ENOMEM;
     std::lock_guard<std::mutex> lk(s->mtx);
     if (s->stop) return ACHIP_EIO;
-    if (r->stream >= 0) {   // one geometry per stream
+    if (r->stream >= 0) {   // one geometry and format per stream
         int& sw = s->stream_w[r->stream];
         int& sh = s->stream_h[r->stream];
-        if (sw && (sw != r->width || sh != r->height)) return ACHIP_EINVAL;
+        ACHipFrameFormat& sf = s->stream_fmt[r->stream];
+        if (sw && (sw != r->width || sh != r->height || !same_format(sf, f))) return ACHIP_EINVAL;
         sw = r->width;
         sh = r->height;
+        sf = f;
     }
     q->ticket = s->next_ticket++;
     q->t_submit = Clock::now();
@@ -729,8 +806,8 @@ int achip_server_output(ACHipServer* s, unsigned long long ticket, int plane, vo
     if (q.st != St::Done || !q.keep_output || q.err) return ACHIP_EINVAL;
     const Plane* p = nullptr;
     if (q.r.kind == ACHIP_REQ_RGB) p = plane == 0 ? &q.out : nullptr;
-    else if (q.r.kind == ACHIP_REQ_LUMA) p = plane == 0 ? &q.luma[q.passes] : nullptr;
-    else p = plane == 0 ? &q.luma[q.passes] : plane == 1 ? &q.u1 : plane == 2 ? &q.v1 : nullptr;
+    else if (plane == 0) p = &q.luma[q.passes];
+    else if (plane > 0 && plane <= chroma_of(q.fmt, q.r.width, q.r.height).n) p = &q.c1[plane - 1];
     if (!p) return ACHIP_EINVAL;
     *ptr = p->ptr;
     *stride = p->stride;

@@ -29,8 +29,9 @@ __global__ __launch_bounds__(256) void convert_kernel(const void* __restrict__ s
 
 // a << n / a >> n on the integer promoted from an unsigned element (ImageProcess.cpp:
 // 464-479, 515-530), stored back truncated to the element type; shifts of 32 or more give 0.
+// src and dst may be the same buffer (each element is read before it is written).
 template <typename T>
-__global__ __launch_bounds__(256) void shift_kernel(const void* __restrict__ src, long ss, void* __restrict__ dst, long ds,
+__global__ __launch_bounds__(256) void shift_kernel(const void* src, long ss, void* dst, long ds,
                                                    int n, int h, int left, int bits)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;

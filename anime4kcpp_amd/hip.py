@@ -157,7 +157,35 @@ def configs(precision="fp32"):
     return out
 
 
-REQ_LUMA, REQ_YUV420, REQ_RGB = 0, 1, 2
+REQ_LUMA, REQ_YUV420, REQ_RGB, REQ_YUV = 0, 1, 2, 3
+CHROMA_BILINEAR, CHROMA_CATMULL_ROM = 0, 1
+
+
+class FrameFormat:
+    """ACHipFrameFormat: plane layout of a REQ_YUV frame (the reference video module's formats)."""
+
+    def __init__(self, planes, chroma_shift_x=1, chroma_shift_y=1, element_type=0x001, shift=0,
+                 chroma_filter=CHROMA_BILINEAR):
+        self.planes, self.chroma_shift_x, self.chroma_shift_y = planes, chroma_shift_x, chroma_shift_y
+        self.element_type, self.shift, self.chroma_filter = element_type, shift, chroma_filter
+
+    @property
+    def dtype(self):
+        return np.uint16 if self.element_type == 0x002 else np.uint8
+
+
+# common layouts: (planes, chroma x shift, chroma y shift, element type, luma LSB shift)
+FORMATS = {
+    "gray8": (1, 0, 0, 0x001, 0), "gray10le": (1, 0, 0, 0x002, 6), "gray16": (1, 0, 0, 0x002, 0),
+    "yuv420p": (3, 1, 1, 0x001, 0), "yuv422p": (3, 1, 0, 0x001, 0), "yuv444p": (3, 0, 0, 0x001, 0),
+    "yuv420p10le": (3, 1, 1, 0x002, 6), "yuv420p12le": (3, 1, 1, 0x002, 4), "yuv444p16": (3, 0, 0, 0x002, 0),
+    "nv12": (2, 1, 1, 0x001, 0), "nv16": (2, 1, 0, 0x001, 0), "nv24": (2, 0, 0, 0x001, 0),
+    "p010": (2, 1, 1, 0x002, 0), "p016": (2, 1, 1, 0x002, 0), "p210": (2, 1, 0, 0x002, 0),
+}
+
+
+def frame_format(name, chroma_filter=CHROMA_BILINEAR):
+    return FrameFormat(*FORMATS[name], chroma_filter=chroma_filter)
 
 
 def _addr(a):
@@ -196,9 +224,10 @@ class Server:
     def __exit__(self, *exc):
         self.close()
 
-    def submit(self, kind, planes, factor=2, stream=-1, dst=None):
-        """Queue one request; planes: [Y] (LUMA), [RGB H x W x 3] (RGB) or [Y, U, V] (YUV420);
-        dst: matching output planes, or None to keep the output in the server.  Returns the ticket."""
+    def submit(self, kind, planes, factor=2, stream=-1, dst=None, fmt=None):
+        """Queue one request; planes: [Y] (LUMA), [RGB H x W x 3] (RGB), [Y, U, V] (YUV420) or
+        as `fmt` (a FrameFormat, kind REQ_YUV) lays them out; dst: matching output planes, or
+        None to keep the output in the server.  Returns the ticket."""
         r = _native.ACHipRequest()
         r.kind, r.factor, r.stream = kind, factor, stream
         r.height, r.width = int(planes[0].shape[0]), int(planes[0].shape[1])
@@ -208,6 +237,9 @@ class Server:
         for i, p in enumerate(dst or []):
             r.dst[i] = _addr(p)
             r.dst_stride[i] = _row_bytes(p)
+        if fmt is not None:
+            r.format = _native.ACHipFrameFormat(fmt.planes, fmt.chroma_shift_x, fmt.chroma_shift_y, fmt.element_type,
+                                                fmt.shift, fmt.chroma_filter)
         t = ctypes.c_ulonglong()
         check(_native.lib().achip_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)), "achip_server_submit")
         self._keep[t.value] = (planes, dst)
@@ -228,10 +260,10 @@ class Server:
               "achip_server_output")
         return ptr.value, stride.value
 
-    def fetch(self, ticket, plane, shape):
-        """Copy a kept output plane (u8, `shape`) from device memory into a new numpy array."""
+    def fetch(self, ticket, plane, shape, dtype=np.uint8):
+        """Copy a kept output plane (`shape`, `dtype`) from device memory into a new numpy array."""
         ptr, stride = self.output(ticket, plane)
-        out = np.zeros(shape, np.uint8)
+        out = np.zeros(shape, dtype)
         row = out.strides[0]
         hip_rt = ctypes.CDLL("libamdhip64.so")
         hip_rt.hipMemcpy2D.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,

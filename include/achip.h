@@ -126,33 +126,53 @@ ACHIP_API const char* achip_last_error(void);
  * tile work-queue (a big pass is spread over several steps by tile rows, so that newly
  * admitted work joins within one step); video-stream frames (fixed geometry per stream)
  * replay a captured hipGraph on the stream's own HIP stream and overlap with the batched
- * work on the device.  All samples are u8.  Outputs are bit-identical to processing the
- * request alone (same kernels, same tiles). */
+ * work on the device.  Outputs are bit-identical to processing the request alone (same
+ * kernels, same tiles). */
 typedef struct ACHipServer ACHipServer;
 
 enum {
-    ACHIP_REQ_LUMA = 0,     /* 1 channel */
-    ACHIP_REQ_YUV420 = 1,   /* planes Y (W x H), U and V (W/2 x H/2); W, H even */
-    ACHIP_REQ_RGB = 2       /* 3 interleaved channels */
+    ACHIP_REQ_LUMA = 0,     /* 1 channel u8 (ACHIP_REQ_YUV with the format { 1, 0, 0, 0x001, 0, 0 }) */
+    ACHIP_REQ_YUV420 = 1,   /* u8 planes Y (W x H), U and V (W/2 x H/2), W and H even; chroma bilinear
+                               (ACHIP_REQ_YUV with { 3, 1, 1, 0x001, 0, ACHIP_CHROMA_BILINEAR }) */
+    ACHIP_REQ_RGB = 2,      /* 3 interleaved u8 channels */
+    ACHIP_REQ_YUV = 3       /* a video frame laid out as `format` says */
 };
+
+enum { ACHIP_CHROMA_BILINEAR = 0, ACHIP_CHROMA_CATMULL_ROM = 1 };
+
+/* Plane layout of an ACHIP_REQ_YUV frame: the formats of the reference's video module
+ * (video/src/Pipeline.cpp:495-566) and the handling of its frame callback (cli/src/Main.cpp:
+ * 183-205: luma shifted up by `shift` bits, through the passes, shifted back; chroma planes
+ * resized to the output size).  Examples:
+ *   gray8 { 1, 0, 0, 0x001, 0, - }      gray10le { 1, 0, 0, 0x002, 6, - }
+ *   yuv420p { 3, 1, 1, 0x001, 0, f }    yuv420p10le { 3, 1, 1, 0x002, 6, f }   yuv444p16 { 3, 0, 0, 0x002, 0, f }
+ *   nv12 { 2, 1, 1, 0x001, 0, f }       p010 { 2, 1, 1, 0x002, 0, f }          nv16 { 2, 1, 0, 0x001, 0, f } */
+typedef struct ACHipFrameFormat {
+    int planes;             /* 1: Y only; 2: Y + one plane of interleaved U, V; 3: Y, U, V */
+    int chroma_shift_x;     /* log2 horizontal chroma subsampling: 1 (4:2:0, 4:2:2) or 0 (4:4:4) */
+    int chroma_shift_y;     /* log2 vertical chroma subsampling: 1 (4:2:0) or 0 */
+    int element_type;       /* 0x001: 8-bit samples; 0x002: 16-bit words (10/12/16-bit video) */
+    int shift;              /* luma bits below the top of the word (LSB-aligned 10/12-bit: 6/4); 0 else */
+    int chroma_filter;      /* ACHIP_CHROMA_BILINEAR or ACHIP_CHROMA_CATMULL_ROM (the reference CLI's) */
+} ACHipFrameFormat;
 
 typedef struct ACHipRequest {
     int kind;               /* ACHIP_REQ_* */
-    int width, height;      /* input size */
+    int width, height;      /* input size (luma) */
     int factor;             /* 2, 4 or 8: log2(factor) chained 2x passes on luma */
-    int stream;             /* >= 0: frame of video stream `stream` (YUV420, factor 2, one geometry per
-                               stream, hipGraph path); -1: one-off */
-    const void* src[3];     /* planes: LUMA/RGB src[0]; YUV420 Y, U, V.  Host memory (page-locked for an
-                               asynchronous upload) or device memory */
+    int stream;             /* >= 0: frame of video stream `stream` (LUMA/YUV420/YUV, factor 2, one
+                               geometry and format per stream, hipGraph path); -1: one-off */
+    const void* src[3];     /* planes: LUMA/RGB src[0]; YUV420 Y, U, V; YUV as `format` says.  Host memory
+                               (page-locked for an asynchronous upload) or device memory */
     long src_stride[3];     /* byte strides */
     void* dst[3];           /* output planes (host or device), or all NULL to keep the output in the
                                server until achip_server_release (read it with achip_server_output) */
     long dst_stride[3];
+    ACHipFrameFormat format;   /* ACHIP_REQ_YUV only */
 } ACHipRequest;
 
-/* Output geometry: LUMA fW x fH; RGB fW x fH x 3; YUV420 Y fW x fH, U/V fW/2 x fH/2.  Chroma of
- * RGB requests is resized by Catmull-Rom (the reference's multi-channel path); chroma of
- * YUV420 requests is resized bilinearly. */
+/* Output geometry: luma fW x fH; RGB fW x fH x 3; chroma planes f times their input size.
+ * Chroma of RGB requests is resized by Catmull-Rom (the reference's multi-channel path). */
 
 typedef struct ACHipServerConfig {
     int precision;          /* ACHIP_FP32 / FP16 / BF16 */

@@ -163,7 +163,7 @@ def test_invalid_requests(net):
         with pytest.raises(hip._native.AchipError):
             srv.submit(hip.REQ_LUMA, [y], 3)                        # factor not a power of two
         with pytest.raises(hip._native.AchipError):
-            srv.submit(hip.REQ_LUMA, [y], 2, stream=0)              # frames are YUV420
+            srv.submit(hip.REQ_RGB, [rgb(5, 64, 48)], 2, stream=0)  # frames are luma / YUV
         big = synth.flat_shaded(2, 4096, 4096, dtype=np.uint8)
         with pytest.raises(hip._native.AchipError):
             srv.submit(hip.REQ_LUMA, [big], 4)                      # alone exceeds the budget
