@@ -65,6 +65,11 @@ class ACProcessor(ctypes.Structure):
 
 # name -> (restype, argtypes); the C-ABI surface this package relies on
 SIGNATURES = {
+    "achip_rgb2yuv": (c_int, [c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p]),
+    "achip_yuv2rgb": (c_int, [c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_void_p]),
+    "achip_resize": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p]),
+    "achip_resize_yuv2rgb": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_int, c_int, c_int, c_int,
+                                     c_void_p, c_long, c_void_p]),
     # include/achip.h
     "achip_device_count": (c_int, []),
     "achip_net_create": (c_int, [c_int, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),

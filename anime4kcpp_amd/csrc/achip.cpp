@@ -410,6 +410,89 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
     return ACHIP_OK;
 }
 
+// ---- device-memory colour and resize stages ----
+
+namespace {
+// a plane of w x h pixels of `comps` elements: stride, alignment and allocation on `device`
+int check_plane(const void* p, long stride, int w, int h, int comps, int type, int device, const char* name)
+{
+    const int es = elem_size(type);
+    if (!es) return fail(ACHIP_EINVAL, "unsupported element type 0x%x", type);
+    if (!p) return fail(ACHIP_EINVAL, "%s is NULL", name);
+    if (w <= 0 || h <= 0 || w > (1 << 29) || h > (1 << 29)) return fail(ACHIP_EINVAL, "%s: bad size %dx%d", name, w, h);
+    const long row = static_cast<long>(w) * comps * es;
+    if (stride < row || stride % es) return fail(ACHIP_EINVAL, "%s: stride %ld invalid for %ld-byte rows", name, stride, row);
+    if (reinterpret_cast<uintptr_t>(p) % es) return fail(ACHIP_EINVAL, "%s: pointer not aligned to its element size", name);
+    return check_device_buffer(p, static_cast<size_t>(h - 1) * stride + row, device, name);
+}
+int current_device(int* dev)
+{
+    const hipError_t e = hipGetDevice(dev);
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "hipGetDevice");
+}
+} // namespace
+
+int achip_rgb2yuv(int type, int channels, const void* rgb, long rgb_stride, int w, int h, void* y, long y_stride,
+                  void* uv, long uv_stride, void* stream)
+{
+    if (channels != 3 && channels != 4) return fail(ACHIP_EINVAL, "channels must be 3 or 4");
+    int dev = 0, r = current_device(&dev);
+    if (!r) r = check_plane(rgb, rgb_stride, w, h, channels, type, dev, "rgb");
+    if (!r) r = check_plane(y, y_stride, w, h, 1, type, dev, "y");
+    if (!r) r = check_plane(uv, uv_stride, w, h, channels - 1, type, dev, "uv");
+    if (r) return r;
+    const hipError_t e = achip::launch_color_split(type, channels, rgb, rgb_stride, w, h, y, y_stride, uv, uv_stride,
+                                                   static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "colour kernel");
+}
+
+int achip_yuv2rgb(int type, int channels, const void* y, long y_stride, const void* uv, long uv_stride, int w, int h,
+                  void* rgb, long rgb_stride, void* stream)
+{
+    if (channels != 3 && channels != 4) return fail(ACHIP_EINVAL, "channels must be 3 or 4");
+    int dev = 0, r = current_device(&dev);
+    if (!r) r = check_plane(y, y_stride, w, h, 1, type, dev, "y");
+    if (!r) r = check_plane(uv, uv_stride, w, h, channels - 1, type, dev, "uv");
+    if (!r) r = check_plane(rgb, rgb_stride, w, h, channels, type, dev, "rgb");
+    if (r) return r;
+    const hipError_t e = achip::launch_color_merge(type, channels, y, y_stride, uv, uv_stride, w, h, rgb, rgb_stride,
+                                                   static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "colour kernel");
+}
+
+int achip_resize(int type, int channels, int mode, const void* src, long src_stride, int sw, int sh, void* dst,
+                 long dst_stride, int dw, int dh, void* stream)
+{
+    if (channels < 1 || channels > 4) return fail(ACHIP_EINVAL, "channels must be 1 to 4");
+    if (mode < 0 || mode >= achip::kResizeModeCount) return fail(ACHIP_EINVAL, "resize mode %d unknown", mode);
+    int dev = 0, r = current_device(&dev);
+    if (!r) r = check_plane(src, src_stride, sw, sh, channels, type, dev, "src");
+    if (!r) r = check_plane(dst, dst_stride, dw, dh, channels, type, dev, "dst");
+    if (r) return r;
+    if (std::max(achip::resize_max_taps(mode, sw, dw), achip::resize_max_taps(mode, sh, dh)) > achip::kResizeMaxTaps)
+        return fail(ACHIP_EINVAL, "downscale %dx%d -> %dx%d needs more than %d filter taps", sw, sh, dw, dh, achip::kResizeMaxTaps);
+    const hipError_t e = achip::launch_resize(type, channels, mode, src, src_stride, sw, sh, dst, dst_stride, dw, dh,
+                                              static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "resize kernel");
+}
+
+int achip_resize_yuv2rgb(int type, int channels, int mode, const void* y, long y_stride, const void* uv, long uv_stride,
+                         int uv_w, int uv_h, int w, int h, void* rgb, long rgb_stride, void* stream)
+{
+    if (channels != 3 && channels != 4) return fail(ACHIP_EINVAL, "channels must be 3 or 4");
+    if (mode < 0 || mode >= achip::kResizeModeCount) return fail(ACHIP_EINVAL, "resize mode %d unknown", mode);
+    int dev = 0, r = current_device(&dev);
+    if (!r) r = check_plane(y, y_stride, w, h, 1, type, dev, "y");
+    if (!r) r = check_plane(uv, uv_stride, uv_w, uv_h, channels - 1, type, dev, "uv");
+    if (!r) r = check_plane(rgb, rgb_stride, w, h, channels, type, dev, "rgb");
+    if (r) return r;
+    if (std::max(achip::resize_max_taps(mode, uv_w, w), achip::resize_max_taps(mode, uv_h, h)) > achip::kResizeMaxTaps)
+        return fail(ACHIP_EINVAL, "chroma downscale needs more than %d filter taps", achip::kResizeMaxTaps);
+    const hipError_t e = achip::launch_resize_merge(type, channels, mode, y, y_stride, uv, uv_stride, uv_w, uv_h, w, h,
+                                                    rgb, rgb_stride, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "resize kernel");
+}
+
 int achip_pass(ACHipNet* net, int precision, const void* src, long src_stride, int src_type, int w, int h,
                void* dst, long dst_stride, int dst_type, void* stream)
 {

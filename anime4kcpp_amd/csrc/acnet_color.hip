@@ -336,6 +336,145 @@ __global__ __launch_bounds__(256) void resize_merge_kernel(const void* __restric
     }
 }
 
+// ---- exact 2x / 4x upscale ---------------------------------------------------------------
+// When dst is exactly K = 2 or 4 times src on both axes, every output of phase p = x mod K
+// has the same normalised taps, shifted by x div K: the centre (x + 0.5) / K - 0.5 and every
+// tap distance are exact in float, so resize_taps gives bit-identical weights for all of them.
+// A workgroup then stages its source region once in LDS as float, runs the horizontal taps
+// for every staged row, then the vertical taps -- the same sums in the same order as
+// resample(), so the results equal the general kernel's exactly, with each source pixel
+// loaded and converted once instead of once per tap.  Taps of an upscale reach at most
+// 4 source pixels from x div K (support <= 4); the staged region has kUpR on each side.
+constexpr int kUpW = 64, kUpH = 32, kUpR = 5, kUpMaxTaps = 12;
+
+template <int K>
+struct UpTile {
+    static constexpr int IW = kUpW / K + 2 * kUpR;   // staged source columns
+    static constexpr int IH = kUpH / K + 2 * kUpR;   // staged source rows
+};
+
+struct UpTaps {
+    float w[2][4][kUpMaxTaps];   // [axis][phase][tap]
+    int off[2][4], n[2][4];      // first tap relative to x div K, tap count
+};
+
+template <int K>
+__device__ inline void up_taps(UpTaps& t, int mode, int sw, int sh, int dw, int dh)
+{
+    const int tid = threadIdx.x;
+    if (tid < 2 * K) {
+        const int axis = tid / K, p = tid % K;
+        int first = 0;
+        t.n[axis][p] = resize_taps(mode, p, axis ? sh : sw, axis ? dh : dw, t.w[axis][p], kUpMaxTaps, first);
+        t.off[axis][p] = first;   // output p has source index 0 as its x div K
+    }
+}
+
+// the staged region's source rows/columns -> LDS index, kept inside the region
+__device__ inline int up_local(int first, int base, int size) { return clampi(first - base, 0, size - 1); }
+
+// One pass over a K-times upscale: channels [C] of src into acc for outputs of this tile;
+// MERGE: the resampled chroma is rounded to T and merged with Y into RGB(A) (C = chroma
+// channels, the output has C + 1).
+template <typename T, int C, int K, bool MERGE>
+__global__ __launch_bounds__(256) void resize_up_kernel(const void* __restrict__ src, long ss, int sw, int sh,
+                                                       void* __restrict__ dst, long ds, int dw, int dh, int mode,
+                                                       const void* __restrict__ y, long ys)
+{
+    using Tile = UpTile<K>;
+    __shared__ UpTaps taps;
+    __shared__ float in[Tile::IH][Tile::IW][C];
+    __shared__ float hs[Tile::IH][kUpW][C];
+    const int tid = threadIdx.x;
+    const int ox0 = blockIdx.x * kUpW, oy0 = blockIdx.y * kUpH;
+    const int ix0 = ox0 / K - kUpR, iy0 = oy0 / K - kUpR;   // staged region origin (source pixels)
+    up_taps<K>(taps, mode, sw, sh, dw, dh);
+    for (int e = tid; e < Tile::IH * Tile::IW; e += 256) {
+        const int r = e / Tile::IW, c = e - r * Tile::IW;
+        const T* p = row_ptr<T>(src, ss, clampi(iy0 + r, 0, sh - 1)) + static_cast<long>(clampi(ix0 + c, 0, sw - 1)) * C;
+#pragma unroll
+        for (int ch = 0; ch < C; ch++) in[r][c][ch] = to_float<T>(p[ch]);
+    }
+    __syncthreads();
+    // horizontal taps of every staged row for the tile's 64 output columns
+    {
+        const int X = tid & 63, x = ox0 + X, p = x % K;
+        const int nx = taps.n[0][p];
+        const int base = up_local(x / K + taps.off[0][p], ix0, Tile::IW);
+        for (int r = tid >> 6; r < Tile::IH; r += 4) {
+            float h[C];
+#pragma unroll
+            for (int ch = 0; ch < C; ch++) h[ch] = 0.0f;
+            for (int i = 0; i < nx; i++) {
+                const int c = min(base + i, Tile::IW - 1);
+#pragma unroll
+                for (int ch = 0; ch < C; ch++) h[ch] = __fadd_rn(h[ch], __fmul_rn(taps.w[0][p][i], in[r][c][ch]));
+            }
+#pragma unroll
+            for (int ch = 0; ch < C; ch++) hs[r][X][ch] = h[ch];
+        }
+    }
+    __syncthreads();
+    // vertical taps: thread = one output column, rows tid/64, +4, ...
+    const int X = tid & 63, x = ox0 + X;
+    if (x >= dw) return;
+    for (int Y = tid >> 6; Y < kUpH; Y += 4) {
+        const int yy = oy0 + Y;
+        if (yy >= dh) break;
+        const int p = yy % K, ny = taps.n[1][p];
+        const int base = up_local(yy / K + taps.off[1][p], iy0, Tile::IH);
+        float acc[C];
+#pragma unroll
+        for (int ch = 0; ch < C; ch++) acc[ch] = 0.0f;
+        for (int j = 0; j < ny; j++) {
+            const int r = min(base + j, Tile::IH - 1);
+#pragma unroll
+            for (int ch = 0; ch < C; ch++) acc[ch] = __fadd_rn(acc[ch], __fmul_rn(taps.w[1][p][j], hs[r][X][ch]));
+        }
+        if constexpr (!MERGE) {
+            T* o = row_ptr<T>(dst, ds, yy) + static_cast<long>(x) * C;
+#pragma unroll
+            for (int ch = 0; ch < C; ch++) o[ch] = from_float<T>(acc[ch]);
+        } else {
+            const float lum = to_float<T>(row_ptr<T>(y, ys, yy)[x]);
+            const float u = __fsub_rn(to_float<T>(from_float<T>(acc[0])), 0.5f);
+            const float v = __fsub_rn(to_float<T>(from_float<T>(acc[1])), 0.5f);
+            float r, g, b;
+            yuv_to_rgb(lum, u, v, r, g, b);
+            T* o = row_ptr<T>(dst, ds, yy) + static_cast<long>(x) * (C + 1);
+            if constexpr (C == 3) {
+                const float a = to_float<T>(from_float<T>(acc[2]));
+                unpremultiply(a, r, g, b);
+                o[3] = from_float<T>(a);
+            }
+            o[0] = from_float<T>(r);
+            o[1] = from_float<T>(g);
+            o[2] = from_float<T>(b);
+        }
+    }
+}
+
+// K when dst is exactly 2 or 4 times src on both axes (and small enough for exact centres)
+static int upscale_factor(int sw, int sh, int dw, int dh)
+{
+    if (sw > (1 << 20) || sh > (1 << 20)) return 0;
+    for (int k : { 2, 4 })
+        if (dw == k * sw && dh == k * sh) return k;
+    return 0;
+}
+
+template <typename T, int C, bool MERGE>
+static hipError_t resize_up_t(int k, int mode, const void* src, long ss, int sw, int sh, void* dst, long ds, int dw, int dh,
+                              const void* y, long ys, hipStream_t stream)
+{
+    const dim3 grid((dw + kUpW - 1) / kUpW, (dh + kUpH - 1) / kUpH), block(256);
+    if (k == 2)
+        hipLaunchKernelGGL((resize_up_kernel<T, C, 2, MERGE>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode, y, ys);
+    else
+        hipLaunchKernelGGL((resize_up_kernel<T, C, 4, MERGE>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode, y, ys);
+    return hipGetLastError();
+}
+
 // ---- launchers ----------------------------------------------------------------------------
 
 int resize_max_taps(int mode, int n_in, int n_out)
@@ -422,6 +561,15 @@ template <typename T>
 static hipError_t resize_t(int channels, int mode, const void* src, long ss, int sw, int sh, void* dst, long ds, int dw,
                            int dh, int kmax, hipStream_t stream)
 {
+    if (const int k = upscale_factor(sw, sh, dw, dh)) {
+        switch (channels) {
+        case 1: return resize_up_t<T, 1, false>(k, mode, src, ss, sw, sh, dst, ds, dw, dh, nullptr, 0, stream);
+        case 2: return resize_up_t<T, 2, false>(k, mode, src, ss, sw, sh, dst, ds, dw, dh, nullptr, 0, stream);
+        case 3: return resize_up_t<T, 3, false>(k, mode, src, ss, sw, sh, dst, ds, dw, dh, nullptr, 0, stream);
+        case 4: return resize_up_t<T, 4, false>(k, mode, src, ss, sw, sh, dst, ds, dw, dh, nullptr, 0, stream);
+        default: return hipErrorInvalidValue;
+        }
+    }
     const dim3 grid((dw + kRsW - 1) / kRsW, (dh + kRsH - 1) / kRsH), block(256);
     const size_t lds = resize_lds_bytes(kmax);
     switch (channels) {
@@ -438,6 +586,11 @@ template <typename T>
 static hipError_t resize_merge_t(int channels, int mode, const void* y, long ys, const void* uv, long uvs, int sw, int sh,
                                  int W, int H, void* dst, long ds, int kmax, hipStream_t stream)
 {
+    if (const int k = upscale_factor(sw, sh, W, H)) {
+        if (channels == 4) return resize_up_t<T, 3, true>(k, mode, uv, uvs, sw, sh, dst, ds, W, H, y, ys, stream);
+        if (channels == 3) return resize_up_t<T, 2, true>(k, mode, uv, uvs, sw, sh, dst, ds, W, H, y, ys, stream);
+        return hipErrorInvalidValue;
+    }
     const dim3 grid((W + kRsW - 1) / kRsW, (H + kRsH - 1) / kRsH), block(256);
     const size_t lds = resize_lds_bytes(kmax);
     if (channels == 4)

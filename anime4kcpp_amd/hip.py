@@ -279,3 +279,69 @@ class Server:
         s = _native.ACHipServerStats()
         check(_native.lib().achip_server_stats(self._h, ctypes.byref(s), 1 if reset else 0), "achip_server_stats")
         return {name: getattr(s, name) for name, _ in s._fields_}
+
+
+# ---- device-memory colour and resize stages (achip_rgb2yuv / yuv2rgb / resize / resize_yuv2rgb) ----
+
+def _stream_of(t, stream):
+    import torch
+    if stream is None:
+        return torch.cuda.current_stream(t.device).cuda_stream
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+
+
+def _check_hw(t, c):
+    if not t.is_cuda or t.stride(-1) != 1 or (t.dim() == 3 and t.stride(1) != t.shape[2]):
+        raise ValueError("planes must be CUDA tensors with packed pixels")
+    if (t.dim() == 3 and t.shape[2] != c) or (t.dim() == 2 and c != 1):
+        raise ValueError(f"expected {c} channels")
+
+
+def rgb2yuv(rgb, y=None, uv=None, stream=None):
+    """RGB(A) H x W x C tensor -> (Y H x W, UV(A) H x W x (C-1)) on the device."""
+    import torch
+    h, w, c = rgb.shape
+    y = torch.empty((h, w), dtype=rgb.dtype, device=rgb.device) if y is None else y
+    uv = torch.empty((h, w, c - 1), dtype=rgb.dtype, device=rgb.device) if uv is None else uv
+    _check_hw(rgb, c), _check_hw(y, 1), _check_hw(uv, c - 1)
+    check(_native.lib().achip_rgb2yuv(elem_type(rgb.dtype), c, rgb.data_ptr(), _row_bytes(rgb), w, h, y.data_ptr(),
+                                      _row_bytes(y), uv.data_ptr(), _row_bytes(uv), _stream_of(rgb, stream)), "achip_rgb2yuv")
+    return y, uv
+
+
+def yuv2rgb(y, uv, out=None, stream=None):
+    import torch
+    h, w = y.shape
+    c = uv.shape[2] + 1
+    out = torch.empty((h, w, c), dtype=y.dtype, device=y.device) if out is None else out
+    _check_hw(y, 1), _check_hw(uv, c - 1), _check_hw(out, c)
+    check(_native.lib().achip_yuv2rgb(elem_type(y.dtype), c, y.data_ptr(), _row_bytes(y), uv.data_ptr(), _row_bytes(uv),
+                                      w, h, out.data_ptr(), _row_bytes(out), _stream_of(y, stream)), "achip_yuv2rgb")
+    return out
+
+
+def resize(src, dsize, mode=1, out=None, stream=None):
+    """Resize an H x W (x C) device tensor to dsize = (width, height); mode: ResizeModes."""
+    import torch
+    h, w = src.shape[:2]
+    c = 1 if src.dim() == 2 else src.shape[2]
+    dw, dh = dsize
+    if out is None:
+        out = torch.empty((dh, dw) + ((c,) if src.dim() == 3 else ()), dtype=src.dtype, device=src.device)
+    _check_hw(src, c), _check_hw(out, c)
+    check(_native.lib().achip_resize(elem_type(src.dtype), c, mode, src.data_ptr(), _row_bytes(src), w, h, out.data_ptr(),
+                                     _row_bytes(out), dw, dh, _stream_of(src, stream)), "achip_resize")
+    return out
+
+
+def resize_yuv2rgb(y, uv, mode=1, out=None, stream=None):
+    """Chroma UV(A) (any size) resized to Y's size and merged with Y into RGB(A) in one pass."""
+    import torch
+    h, w = y.shape
+    uh, uw, cc = uv.shape
+    out = torch.empty((h, w, cc + 1), dtype=y.dtype, device=y.device) if out is None else out
+    _check_hw(y, 1), _check_hw(uv, cc), _check_hw(out, cc + 1)
+    check(_native.lib().achip_resize_yuv2rgb(elem_type(y.dtype), cc + 1, mode, y.data_ptr(), _row_bytes(y), uv.data_ptr(),
+                                             _row_bytes(uv), uw, uh, w, h, out.data_ptr(), _row_bytes(out),
+                                             _stream_of(y, stream)), "achip_resize_yuv2rgb")
+    return out

@@ -28,6 +28,8 @@ W_IN, H_IN = 1920, 1080
 FLOP_PER_INPUT_PX = 9424          # 2 x 4712 MAC per input pixel per 2x pass
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 peak
 PEAK_MFMA16_TFLOPS = 2500.0       # MI355X published dense FP16/BF16 MFMA peak
+PEAK_HBM_GBS = 8000.0             # MI355X published HBM3E bandwidth
+PEAK_PCIE_GBS = 64.0              # PCIe 5.0 x16, one direction
 FRAMES_PER_VARIANT = 2
 # HBM bytes per 1080p u8 launch of the default configuration of each precision, measured with
 # rocprofv3 --pmc FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) against a 256 MiB copy
@@ -292,6 +294,59 @@ def serving(net, prec, loads, seconds, seed=5):
     return out
 
 
+def stages():
+    """HBM-bound stages around the passes (colour split, chroma resize + merge, YUV420 chroma
+    2x) and the PCIe copies of a frame, timed with HIP events on device-resident planes.
+    Algorithmic bytes: every input element read once, every output element written once."""
+    import torch
+    from anime4kcpp_amd import hip
+    dev = torch.device("cuda:0")
+
+    def timeit(fn, n=20):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(n):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / n / 1e3   # s
+
+    def entry(sec, nbytes, peak, what):
+        gbs = nbytes / sec / 1e9
+        return {"us": round(sec * 1e6, 2), "bytes": nbytes, "gb_per_s": round(gbs, 1), "peak_gb_per_s": peak,
+                "frac": round(gbs / peak, 4), "workload": what}
+
+    out = {}
+    W, H = 4096, 4096
+    rgb = torch.randint(0, 256, (H, W, 3), dtype=torch.uint8, device=dev)
+    y, uv = hip.rgb2yuv(rgb)
+    out["colour_split_u8"] = entry(timeit(lambda: hip.rgb2yuv(rgb, y, uv)), 6 * W * H, PEAK_HBM_GBS,
+                                   "RGB u8 4096x4096 -> Y + UV (ImageProcess.cpp rgb2yuv)")
+    uvh = uv[: H // 2, : W // 2].contiguous()
+    o = torch.empty_like(rgb)
+    out["chroma_resize_merge_u8"] = entry(timeit(lambda: hip.resize_yuv2rgb(y, uvh, 1, o)),
+                                          2 * (W // 2) * (H // 2) + 4 * W * H, PEAK_HBM_GBS,
+                                          "UV 2048x2048 Catmull-Rom 2x + Y 4096x4096 -> RGB u8 (RGB one-off finish)")
+    cu = torch.randint(0, 256, (540, 960), dtype=torch.uint8, device=dev)
+    c2 = torch.empty((1080, 1920), dtype=torch.uint8, device=dev)
+    out["yuv420_chroma_2x_u8"] = entry(timeit(lambda: hip.resize(cu, (1920, 1080), 16, c2)), 960 * 540 + 1920 * 1080,
+                                       PEAK_HBM_GBS, "U or V 960x540 bilinear 2x (1080p YUV420 frame)")
+    frame_in = 1920 * 1080 * 3 // 2
+    frame_out = 3840 * 2160 * 3 // 2
+    hin = torch.empty(frame_in, dtype=torch.uint8, pin_memory=True)
+    din = torch.empty(frame_in, dtype=torch.uint8, device=dev)
+    hout = torch.empty(frame_out, dtype=torch.uint8, pin_memory=True)
+    dout = torch.empty(frame_out, dtype=torch.uint8, device=dev)
+    out["upload_frame"] = entry(timeit(lambda: din.copy_(hin, non_blocking=True)), frame_in, PEAK_PCIE_GBS,
+                                "1080p YUV420 u8 frame, page-locked host -> HBM")
+    out["download_frame"] = entry(timeit(lambda: hout.copy_(dout, non_blocking=True)), frame_out, PEAK_PCIE_GBS,
+                                  "4K YUV420 u8 frame, HBM -> page-locked host")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +359,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batched measurement")
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K f32 x4 precision measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-stages", action="store_true", help="skip the colour / resize / PCIe stage measurement")
     ap.add_argument("--serving-loads", default="fp16:300,1200;fp32:60,240",
                     help="offered loads (requests/s) per precision")
     ap.add_argument("--serving-seconds", type=float, default=3.0)
@@ -408,6 +464,8 @@ def main():
         result["batched_uneven"] = {p: batched_uneven(nets, p) for p in precs[:2]}
     if not args.no_4k:
         result["precision_4k_x4"] = precision_4k_x4(nets, weights, precs)
+    if not args.no_stages:
+        result["stages"] = stages()
     if not args.no_serving:
         result["serving"] = {}
         for spec in args.serving_loads.split(";"):

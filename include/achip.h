@@ -114,6 +114,28 @@ ACHIP_API int achip_autotune(ACHipNet* net, int precision, int src_type, int dst
 /* Calling thread's last error message ("" if none). */
 ACHIP_API const char* achip_last_error(void);
 
+/* ---- Device-memory colour and resize stages ----
+ * The steps of Processor::process around the 2x passes (core/src/processor/Processor.cpp:
+ * 207-213, 242-274) on device pointers of the current HIP device: the colour conversions of
+ * core/src/ImageProcess.cpp:15-308 and the resize of core/src/ImageResize.cpp:136-273.  The
+ * reference only has host-image forms of these (ac::core::rgb2yuv, resize; declared in
+ * include/AC/Core/Image.hpp here); these take HBM-resident planes, for callers whose frames
+ * stay on the device.  All planes of one element type (0x001 u8, 0x002 u16, 0x202 f16,
+ * 0x204 f32), byte strides, asynchronous on `stream`.  channels: 3 (RGB) or 4 (RGBA,
+ * premultiplied as the reference does); the UV plane interleaves channels - 1 components. */
+ACHIP_API int achip_rgb2yuv(int type, int channels, const void* rgb, long rgb_stride, int w, int h,
+                            void* y, long y_stride, void* uv, long uv_stride, void* stream);
+ACHIP_API int achip_yuv2rgb(int type, int channels, const void* y, long y_stride, const void* uv, long uv_stride,
+                            int w, int h, void* rgb, long rgb_stride, void* stream);
+/* resize of a 1-4 channel plane (mode: ResizeModes of core/include/AC/Core/Image.hpp:272-291) */
+ACHIP_API int achip_resize(int type, int channels, int mode, const void* src, long src_stride, int sw, int sh,
+                           void* dst, long dst_stride, int dw, int dh, void* stream);
+/* UV(A) (uv_w x uv_h) resized to w x h and merged with Y (w x h) into RGB(A) in one pass:
+ * achip_resize then achip_yuv2rgb, with the resized chroma rounded to the element type */
+ACHIP_API int achip_resize_yuv2rgb(int type, int channels, int mode, const void* y, long y_stride, const void* uv,
+                                   long uv_stride, int uv_w, int uv_h, int w, int h, void* rgb, long rgb_stride,
+                                   void* stream);
+
 /* ---- Serving loop -----------------------------------------------------------------------
  * No single counterpart in the reference: it replaces the frame/image dispatch that the
  * reference's callers do one processor call at a time per thread (video

@@ -41,7 +41,8 @@ def compare_resize(got, want, lanczos):
         assert np.abs(got.astype(np.float64) - want.astype(np.float64)).max() <= 1e-6 + 1e-3 * (got.dtype == np.float16)
 
 
-SIZES = [(40, 26), (10, 7), (33, 21), (6, 19), (101, 13)]   # from a 23 x 17 source: up, down, odd, mixed
+# from a 23 x 17 source: up, down, odd, mixed, and exactly 2x and 4x (the staged upscale kernel)
+SIZES = [(40, 26), (10, 7), (33, 21), (6, 19), (101, 13), (46, 34), (92, 68)]
 
 
 @pytest.mark.parametrize("mode", range(17), ids=OL.RESIZE_MODES)
@@ -62,8 +63,18 @@ def test_resize_matches_oracle(cuda, mode, dtype, c):
 def test_resize_two_and_four_channels(cuda, c):
     src = rand_image((31, 45, c), np.uint8, seed=c)
     for mode in (OL.RESIZE_CATMULL_ROM, OL.RESIZE_BILINEAR, OL.RESIZE_SPLINE36):
-        for dw, dh in ((90, 62), (20, 14)):
+        for dw, dh in ((90, 62), (20, 14), (180, 124), (91, 62)):
             assert np.array_equal(P.resize(src, (dw, dh), mode=mode), OL.resize(src, dw, dh, mode))
+
+
+@pytest.mark.parametrize("k", [2, 4])
+def test_staged_upscale_multi_tile(cuda, k):
+    """Exact K-times upscales over many 64 x 32 output tiles and ragged edges, every type."""
+    for dtype in DTYPES:
+        src = rand_image((37, 150, 2), dtype, seed=k)
+        for mode in (OL.RESIZE_CATMULL_ROM, OL.RESIZE_BILINEAR, OL.RESIZE_POINT, 12):
+            got = P.resize(src, (150 * k, 37 * k), mode=mode)
+            compare_resize(got, OL.resize(src, 150 * k, 37 * k, mode), mode == 12)
 
 
 def test_resize_large_downscale_and_limits(cuda):
