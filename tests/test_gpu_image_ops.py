@@ -172,3 +172,28 @@ def test_colour_in_place_and_bad_shapes(cuda):
     assert np.array_equal(buf, want)
     wrong = P.Image(np.zeros((9, 10, 3), np.uint8), mapped=True)
     assert L.ac_rgb2yuv(P.Image(rgb, mapped=True)._p, wrong._p) == -256
+
+
+def test_device_stage_entry_points(cuda):
+    """achip_rgb2yuv / achip_yuv2rgb / achip_resize / achip_resize_yuv2rgb on device tensors
+    equal the oracle's colour conversion, resize, and resize followed by the merge."""
+    import torch
+    from anime4kcpp_amd import hip
+    for dtype, tdt in ((np.uint8, torch.uint8), (np.float32, torch.float32)):
+        for c in (3, 4):
+            rgb = rand_image((70, 130, c), dtype, seed=20 + c)
+            y, uv = hip.rgb2yuv(torch.from_numpy(rgb).cuda())
+            want = oracle_split(rgb)
+            assert same(y.cpu().numpy(), want[..., 0]) and same(uv.cpu().numpy(), want[..., 1:])
+            back = hip.yuv2rgb(y, uv).cpu().numpy()
+            assert same(back, oracle_merge(want))
+            for mode in (OL.RESIZE_CATMULL_ROM, OL.RESIZE_BILINEAR):
+                small = np.ascontiguousarray(want[:35, :65, 1:])          # chroma at half size
+                ys = np.ascontiguousarray(want[:70, :130, 0])
+                got = hip.resize_yuv2rgb(torch.from_numpy(ys).cuda(), torch.from_numpy(small).cuda(), mode).cpu().numpy()
+                up = OL.resize(small, 130, 70, mode)
+                assert same(got, oracle_merge(np.concatenate([ys[..., None], up], axis=2)))
+                r = hip.resize(torch.from_numpy(small).cuda(), (97, 51), mode).cpu().numpy()
+                assert same(r, OL.resize(small, 97, 51, mode))
+    with pytest.raises(_native.AchipError):
+        hip.resize(torch.zeros((8, 4000), dtype=torch.uint8, device="cuda"), (40, 8), 12)
