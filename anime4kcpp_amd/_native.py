@@ -65,6 +65,11 @@ class ACProcessor(ctypes.Structure):
 
 # name -> (restype, argtypes); the C-ABI surface this package relies on
 SIGNATURES = {
+    "achip_net8_create": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
+    "achip_net_blocks": (c_int, [c_void_p]),
+    "achip_pass8_workspace_bytes": (ctypes.c_size_t, [c_int, c_int]),
+    "achip_pass8": (c_int, [c_void_p, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
+                            ctypes.c_size_t, c_void_p]),
     "achip_rgb2yuv": (c_int, [c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p]),
     "achip_yuv2rgb": (c_int, [c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_void_p]),
     "achip_resize": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p]),

@@ -350,8 +350,107 @@ void achip_net_destroy(ACHipNet* net)
     {
         DeviceGuard g(net->device);
         if (net->wts) (void)hipFree(net->wts);
+        if (net->a8wts) (void)hipFree(net->a8wts);
     }
     delete net;
+}
+
+// ---- ACNet<8> ----
+
+namespace {
+// FP32 MFMA A operands of one conv3x3 layer with cout output channels (8 or 4; rows of the
+// missing channels zero), kernels k[n * 72 + (3 ky + kx) * 8 + c]: as mfma32_operands
+void layer32_operands(const float* k, int cout, float* al)
+{
+    for (int t = 0; t < 3; t++)
+        for (int c = 0; c < 8; c++)
+            for (int lane = 0; lane < 64; lane++) {
+                const int i = lane % 16, g = lane / 16, n = i % 8, ky = g - i / 8;
+                al[(t * 8 + c) * 64 + lane] = (n < cout && ky >= 0 && ky <= 2) ? k[n * 72 + (ky * 3 + t) * 8 + c] : 0.0f;
+            }
+}
+} // namespace
+
+int achip_net8_create(int device, int blocks, const float* kernels, const float* biases, const float* alphas, ACHipNet** out)
+{
+    if (!out) return fail(ACHIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!kernels || !biases || !alphas) return fail(ACHIP_EINVAL, "NULL weights");
+    if (blocks < 1 || blocks > 64) return fail(ACHIP_EINVAL, "blocks %d out of range", blocks);
+    const int n = achip_device_count();
+    if (n <= 0) return fail(ACHIP_ENODEV, "no HIP device");
+    if (device < 0 || device >= n) device = 0;
+    std::vector<float> blob(achip::acnet8_blob_floats(blocks), 0.0f);
+    for (int p = 0; p < 9; p++)
+        for (int o = 0; o < 8; o++) blob[achip::kA8OffW0 + p * 8 + o] = kernels[o * 9 + p];
+    for (int o = 0; o < 8; o++) {
+        blob[achip::kA8OffB0 + o] = biases[o];
+        blob[achip::kA8OffAlpha0 + o] = alphas[o];
+    }
+    for (int o = 0; o < 4; o++) blob[achip::kA8OffFinalB + o] = biases[8 + 8 * blocks + o];
+    for (int l = 1; l <= blocks; l++)
+        for (int o = 0; o < 8; o++) {
+            blob[achip::kA8OffLayer + 16 * (l - 1) + o] = biases[8 * l + o];
+            blob[achip::kA8OffLayer + 16 * (l - 1) + 8 + o] = alphas[8 * l + o];
+        }
+    std::vector<float> wa(static_cast<size_t>(blocks + 1) * achip::kMfma32LayerFloats);
+    for (int l = 1; l <= blocks + 1; l++)
+        layer32_operands(kernels + 72 + 576 * (l - 1), l <= blocks ? 8 : 4, wa.data() + (l - 1) * achip::kMfma32LayerFloats);
+
+    DeviceGuard g(device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    const size_t offa = (blob.size() * sizeof(float) + 255) / 256 * 256;
+    char* d = nullptr;
+    hipError_t e = hipMalloc(&d, offa + wa.size() * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(weights)");
+    e = hipMemcpy(d, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + offa, wa.data(), wa.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "hipMemcpy(weights)");
+    }
+    auto* net = new ACHipNet{ device, nullptr, nullptr, nullptr, nullptr };
+    net->blocks = blocks;
+    net->a8wts = reinterpret_cast<float*>(d);
+    net->a8wa = reinterpret_cast<float*>(d + offa);
+    *out = net;
+    g_error.clear();
+    return ACHIP_OK;
+}
+
+int achip_net_blocks(const ACHipNet* net) { return net ? net->blocks : ACHIP_EINVAL; }
+
+size_t achip_pass8_workspace_bytes(int w, int h)
+{
+    if (w <= 0 || h <= 0) return 0;
+    return 2 * static_cast<size_t>(achip::acnet8_feature_pitch(w)) * static_cast<size_t>(h);
+}
+
+int achip_pass8(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h, void* dst, long dst_stride,
+                int dst_type, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    if (!net->blocks) return fail(ACHIP_EINVAL, "an ACNetLegacy net runs through achip_pass");
+    int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
+    if (r) return r;
+    const size_t need = achip_pass8_workspace_bytes(w, h);
+    if (!workspace || workspace_bytes < need)
+        return fail(ACHIP_EINVAL, "workspace of %zu bytes, a %dx%d pass needs %zu", workspace_bytes, w, h, need);
+    if (reinterpret_cast<uintptr_t>(workspace) % 256) return fail(ACHIP_EINVAL, "workspace not 256-byte aligned");
+    r = check_device_buffer(workspace, need, net->device, "workspace");
+    if (r) return r;
+    const char* ws = static_cast<const char*>(workspace);
+    const char* s0 = static_cast<const char*>(src);
+    const char* d0 = static_cast<const char*>(dst);
+    const size_t sbytes = static_cast<size_t>(h - 1) * src_stride + static_cast<size_t>(w) * (src_type & 0xff);
+    const size_t dbytes = static_cast<size_t>(2 * h - 1) * dst_stride + static_cast<size_t>(2 * w) * (dst_type & 0xff);
+    if ((ws < s0 + sbytes && s0 < ws + need) || (ws < d0 + dbytes && d0 < ws + need))
+        return fail(ACHIP_EINVAL, "the workspace overlaps src or dst");
+    DeviceGuard g(net->device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    const hipError_t e = achip::launch_acnet8(net->blocks, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
+                                              net->a8wts, net->a8wa, workspace, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "kernel launch");
 }
 
 int achip_net_device(const ACHipNet* net) { return net ? net->device : ACHIP_EINVAL; }
@@ -400,6 +499,7 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
         config = achip::auto_config(net, precision, src_type, dst_type, w, h);
     }
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
+    if (net && net->blocks) return fail(ACHIP_EINVAL, "an ACNet<8> net runs through achip_pass8");
     int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
     if (r) return r;
     DeviceGuard g(net->device);
@@ -576,6 +676,7 @@ int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
     if (!images || n <= 0) return fail(ACHIP_EINVAL, "empty batch");
     if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    if (net->blocks) return fail(ACHIP_EINVAL, "batched passes take an ACNetLegacy net");
     const size_t need = achip_batch_workspace_bytes(n);
     if (!workspace || workspace_bytes < need)
         return fail(ACHIP_EINVAL, "workspace of %zu bytes, the batch needs %zu", workspace_bytes, need);

@@ -9,10 +9,13 @@
 
 struct ACHipNet {
     int device;
-    float* wts;      // device FP32 weight blob, acnet_device.hpp layout
+    float* wts;      // device FP32 weight blob, acnet_device.hpp layout (ACNetLegacy)
     void* wa16;      // MFMA A operands, FP16
     void* wabf16;    // MFMA A operands, BF16
     float* wa32;     // FP32 MFMA A operands
+    int blocks = 0;          // ACNet<8>: its block count (the fields above are then unused)
+    float* a8wts = nullptr;  // ACNet<8> blob, acnet_device.hpp layout (one allocation with a8wa)
+    float* a8wa = nullptr;   // ACNet<8> A operands: blocks 1..B, then the final layer
 };
 
 namespace achip {

@@ -39,6 +39,14 @@ constexpr int kMfma32L0Floats = 3 * 64;
 constexpr int kMfma32LayerFloats = 24 * 64;
 constexpr int kMfma32BlobFloats = kMfma32L0Floats + 8 * kMfma32LayerFloats;   // 12480
 
+// ACNet<8> device blob (model arrays of core/include/AC/Core/Model/ACNet.hpp:63-118,
+// repacked): L0 weights w0[p][n] (72), bias (8), PReLU slopes (8); the final layer's 4
+// biases (padded to 8); then per block l = 1..B its 8 biases and 8 slopes.  The conv weights
+// of the blocks and of the final 8 -> 4 layer (output channels 4-7 zero) are FP32 MFMA
+// per-lane A operands, kMfma32LayerFloats per layer (layout of the legacy layers 1..8).
+constexpr int kA8OffW0 = 0, kA8OffB0 = 72, kA8OffAlpha0 = 80, kA8OffFinalB = 88, kA8OffLayer = 96;
+__host__ __device__ constexpr int acnet8_blob_floats(int blocks) { return kA8OffLayer + 16 * blocks; }
+
 // toFloat (core/internal/AC/Core/Internal/Util.hpp:49-61): unsigned -> v / max, the
 // correctly rounded quotient as the host's IEEE divide.  Computed as q0 = v * RN(1/max),
 // one FMA residual and one FMA correction: for every one of the 256 (u8) and 65536 (u16)

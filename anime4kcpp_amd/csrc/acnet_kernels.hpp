@@ -78,6 +78,13 @@ hipError_t launch_mfma32(int cfg, int stype, int dtype, const void* src, long ss
 hipError_t launch_mfma32_batch(int cfg, int stype, int dtype, const BatchDesc* desc, int n, int total, int* counter,
                                const float* wts, const float* wa32, int max_wg, hipStream_t stream);
 
+// ACNet<8> 2x pass, layer by layer through two feature maps in `workspace` (acnet8.hip):
+// blocks 4, 8 or 18; wts the acnet_device.hpp ACNet<8> blob, wa the A operands of the
+// blocks and the final layer.  Workspace: 2 * acnet8_feature_pitch(W) * H bytes.
+long acnet8_feature_pitch(int w);
+hipError_t launch_acnet8(int blocks, int stype, int dtype, const void* src, long ss, int W, int H, void* dst, long ds,
+                         const float* wts, const float* wa, void* workspace, hipStream_t stream);
+
 // ---- colour conversion and resize (acnet_color.hip) ----
 
 // One colour component of an image in device memory: component (x, y) is the element at

@@ -12,10 +12,9 @@
 // Layer 0 (one channel) is 3 MFMAs per item, B read from the staged input tile.
 #include "acnet_device.hpp"
 #include "acnet_kernels.hpp"
+#include "acnet_mfma32_item.hpp"
 
 namespace achip {
-
-typedef float f32x4m __attribute__((ext_vector_type(4)));
 
 // ES: element size of the source type (the staged input tile's bytes per pixel)
 template <int TW, int TH, int NT, int ES>
@@ -94,41 +93,6 @@ __device__ inline void fill_ring32(float4* __restrict__ pa, float4* __restrict__
     }
 }
 
-__device__ inline f32x4m mma32(float a, float b, f32x4m c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-
-// B operands of one item: the 8 channels of this lane's pixel in patch columns t = 0..2
-struct Item32 {
-    float4 lo[3], hi[3];
-};
-
-__device__ __forceinline__ Item32 load_item(const float4* __restrict__ pa, const float4* __restrict__ pb, int p)
-{
-    Item32 q;
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-        q.lo[t] = pa[p + t];
-        q.hi[t] = pb[p + t];
-    }
-    return q;
-}
-
-// the 24 MFMAs of one item on two accumulator chains, the first starting from the bias
-__device__ __forceinline__ f32x4m item24(const Item32& q, const float (&a)[24], f32x4m bias)
-{
-    f32x4m c0 = bias, c1 = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-    for (int t = 0; t < 3; t++) {
-        c0 = mma32(a[t * 8 + 0], q.lo[t].x, c0);
-        c1 = mma32(a[t * 8 + 1], q.lo[t].y, c1);
-        c0 = mma32(a[t * 8 + 2], q.lo[t].z, c0);
-        c1 = mma32(a[t * 8 + 3], q.lo[t].w, c1);
-        c0 = mma32(a[t * 8 + 4], q.hi[t].x, c0);
-        c1 = mma32(a[t * 8 + 5], q.hi[t].y, c1);
-        c0 = mma32(a[t * 8 + 6], q.hi[t].z, c0);
-        c1 = mma32(a[t * 8 + 7], q.hi[t].w, c1);
-    }
-    return c0 + c1;
-}
 
 template <typename TIN, typename TOUT, int TW, int TH, int NT>
 __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long sstride, int W, int H,

@@ -37,6 +37,8 @@ namespace ac::core::hip
     {
     public:
         HIPProcessorACNetLegacy(int device, const model::ACNetLegacy& model, int precision) noexcept;
+        // ACNet<8>: the same processor around an achip_net8_create net (FP32 only)
+        HIPProcessorACNetLegacy(int device, const model::ACNet<8>& model, int precision) noexcept;
         ~HIPProcessorACNetLegacy() noexcept override;
 
         bool ok() noexcept override;
@@ -58,6 +60,7 @@ namespace ac::core::hip
         {
             hipStream_t stream = nullptr;
             DeviceBuffer buf[8];
+            DeviceBuffer ws;          // ACNet<8> feature maps
             std::string error;
         };
         ThreadState& local();
@@ -66,10 +69,15 @@ namespace ac::core::hip
         bool run(const Image& src, Image& dst, int passes);
         bool pipeline(const Image& src, Image& dst, double factor);
         bool stream_ready(ThreadState& st);
+        // one 2x pass of the net on device buffers (ACNetLegacy: achip_pass; ACNet<8>:
+        // achip_pass8 with this thread's workspace)
+        bool pass(ThreadState& st, const void* src, long sp, int stype, int w, int h, void* dst, long dp, int dtype);
+        void finishInit();
 
     private:
         ACHipNet* net = nullptr;
         int precision;
+        int blocks8 = 0;
         std::string devName{ "no HIP device" };
         std::string initError;
         std::shared_mutex mtx;
@@ -96,11 +104,47 @@ ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device
         net = nullptr;
         return;
     }
+    finishInit();
+    if (achip_config_count(precision) <= 0)
+        initError = "HIP processor: precision " + std::to_string(precision) + " is not available in this build";
+}
+
+ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device, const model::ACNet<8>& model, const int precision) noexcept
+    : precision(precision), blocks8(model.blocks())
+{
+    if (precision != 0)
+    {
+        initError = "HIP processor: ACNet<8> runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's)";
+        return;
+    }
+    const int rc = achip_net8_create(device, model.blocks(), model.kernel(0), model.bias(0), model.alpha(0), &net);
+    if (rc != ACHIP_OK)
+    {
+        initError = std::string("HIP processor: ") + achip_last_error();
+        net = nullptr;
+        return;
+    }
+    finishInit();
+}
+
+void ac::core::hip::HIPProcessorACNetLegacy::finishInit()
+{
     idx = achip_net_device(net);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, idx) == hipSuccess) devName = prop.name;
-    if (achip_config_count(precision) <= 0)
-        initError = "HIP processor: precision " + std::to_string(precision) + " is not available in this build";
+}
+
+bool ac::core::hip::HIPProcessorACNetLegacy::pass(ThreadState& st, const void* src, const long sp, const int stype, const int w,
+                                                  const int h, void* dst, const long dp, const int dtype)
+{
+    int rc;
+    if (blocks8)
+    {
+        if (!ensure(st.ws, achip_pass8_workspace_bytes(w, h))) return false;
+        rc = achip_pass8(net, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream);
+    }
+    else rc = achip_pass(net, precision, src, sp, stype, w, h, dst, dp, dtype, st.stream);
+    return rc == ACHIP_OK ? true : fail(std::string(blocks8 ? "achip_pass8: " : "achip_pass: ") + achip_last_error());
 }
 
 ac::core::hip::HIPProcessorACNetLegacy::~HIPProcessorACNetLegacy() noexcept
@@ -113,6 +157,7 @@ ac::core::hip::HIPProcessorACNetLegacy::~HIPProcessorACNetLegacy() noexcept
             auto& st = *kv.second;
             if (st.stream) (void)hipStreamSynchronize(st.stream);
             for (auto& b : st.buf) if (b.ptr) (void)hipFree(b.ptr);
+            if (st.ws.ptr) (void)hipFree(st.ws.ptr);
             if (st.stream) (void)hipStreamDestroy(st.stream);
         }
     }
@@ -204,9 +249,7 @@ bool ac::core::hip::HIPProcessorACNetLegacy::run(const Image& src, Image& dst, c
         const size_t opitch = alignUp(static_cast<size_t>(2 * w) * (last ? des : es), 256);
         DeviceBuffer& ob = st.buf[1 + (i & 1)];
         if (!ensure(ob, opitch * 2 * h)) return false;
-        const int rc = achip_pass(net, precision, cur, static_cast<long>(curPitch), src.type(), w, h,
-                                  ob.ptr, static_cast<long>(opitch), ots, st.stream);
-        if (rc != ACHIP_OK) return fail(std::string("achip_pass: ") + achip_last_error());
+        if (!pass(st, cur, static_cast<long>(curPitch), src.type(), w, h, ob.ptr, static_cast<long>(opitch), ots)) return false;
         cur = ob.ptr;
         curPitch = opitch;
         w *= 2;
@@ -303,8 +346,7 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pipeline(const Image& src, Image& d
         const long op = pitch_of(2 * cw, ot & 0xff);
         DeviceBuffer& ob = st.buf[3 + (i & 1)];
         if (!ensure(ob, op * 2 * ch)) return false;
-        const int rc = achip_pass(net, precision, y, yPitch, T, static_cast<int>(cw), static_cast<int>(ch), ob.ptr, op, ot, s);
-        if (rc != ACHIP_OK) return fail(std::string("achip_pass: ") + achip_last_error());
+        if (!pass(st, y, yPitch, T, static_cast<int>(cw), static_cast<int>(ch), ob.ptr, op, ot)) return false;
         y = ob.ptr;
         yPitch = op;
         cw *= 2;
@@ -343,6 +385,11 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pipeline(const Image& src, Image& d
 }
 
 std::shared_ptr<ac::core::Processor> ac::core::detail::createHIP(const int device, const model::ACNetLegacy& model, const int precision)
+{
+    return std::make_shared<hip::HIPProcessorACNetLegacy>(device, model, precision);
+}
+
+std::shared_ptr<ac::core::Processor> ac::core::detail::createHIP8(const int device, const model::ACNet<8>& model, const int precision)
 {
     return std::make_shared<hip::HIPProcessorACNetLegacy>(device, model, precision);
 }

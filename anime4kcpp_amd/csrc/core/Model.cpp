@@ -1,4 +1,5 @@
-// Variant -> weight table binding of ACNetLegacy (core/src/Model.cpp:8-34).  The tables
+// Variant -> weight table binding of ACNetLegacy (core/src/Model.cpp:8-34) and ACNet<8>
+// (core/src/Model.cpp:36-120).  The tables
 // are generated at build time from anime4kcpp_amd/weights/*.f32 (parsed from
 // core/internal/AC/Core/Internal/Model/Param/ACNet.p, sha256 in weights/MANIFEST.json).
 #include "AC/Core/Model/ACNet.hpp"
@@ -15,6 +16,11 @@ namespace ac::core::model::param
     extern const float ACNetLegacy_HDN2_biases[72];
     extern const float ACNetLegacy_HDN3_kernels[4712];
     extern const float ACNetLegacy_HDN3_biases[72];
+#define AC_A8_TABLES(NAME) extern const float NAME##_kernels[]; extern const float NAME##_biases[]; extern const float NAME##_alphas[];
+    AC_A8_TABLES(ACNet_F8B4) AC_A8_TABLES(ACNet_F8B4_HDN) AC_A8_TABLES(ACNet_F8B4_Box) AC_A8_TABLES(ACNet_F8B4_Box_HDN)
+    AC_A8_TABLES(ACNet_F8B8) AC_A8_TABLES(ACNet_F8B8_HDN) AC_A8_TABLES(ACNet_F8B8_Box) AC_A8_TABLES(ACNet_F8B8_Box_HDN)
+    AC_A8_TABLES(ACNet_F8B18) AC_A8_TABLES(ACNet_F8B18_HDN) AC_A8_TABLES(ACNet_F8B18_Box) AC_A8_TABLES(ACNet_F8B18_Box_HDN)
+#undef AC_A8_TABLES
 }
 
 ac::core::model::ACNetLegacy::ACNetLegacy(const Variant v) noexcept : v(v), blockNum(8)
@@ -30,3 +36,29 @@ ac::core::model::ACNetLegacy::ACNetLegacy(const Variant v) noexcept : v(v), bloc
     default: kptr = ACNetLegacy_GAN_kernels; bptr = ACNetLegacy_GAN_biases; break;
     }
 }
+
+template<>
+ac::core::model::ACNet<8>::ACNet(const Variant v) noexcept : v(v)
+{
+    using namespace ac::core::model::param;
+#define AC_A8_BIND(B, NAME) blockNum = B; kptr = NAME##_kernels; bptr = NAME##_biases; aptr = NAME##_alphas; break;
+    switch (v)
+    {
+    case Variant::B4_NORMAL: AC_A8_BIND(4, ACNet_F8B4)
+    case Variant::B4_HDN: AC_A8_BIND(4, ACNet_F8B4_HDN)
+    case Variant::B4_BOX: AC_A8_BIND(4, ACNet_F8B4_Box)
+    case Variant::B4_BOX_HDN: AC_A8_BIND(4, ACNet_F8B4_Box_HDN)
+    case Variant::B8_HDN: AC_A8_BIND(8, ACNet_F8B8_HDN)
+    case Variant::B8_BOX: AC_A8_BIND(8, ACNet_F8B8_Box)
+    case Variant::B8_BOX_HDN: AC_A8_BIND(8, ACNet_F8B8_Box_HDN)
+    case Variant::B18_NORMAL: AC_A8_BIND(18, ACNet_F8B18)
+    case Variant::B18_HDN: AC_A8_BIND(18, ACNet_F8B18_HDN)
+    case Variant::B18_BOX: AC_A8_BIND(18, ACNet_F8B18_Box)
+    case Variant::B18_BOX_HDN: AC_A8_BIND(18, ACNet_F8B18_Box_HDN)
+    case Variant::B8_NORMAL:
+    default: AC_A8_BIND(8, ACNet_F8B8)
+    }
+#undef AC_A8_BIND
+}
+
+template class ac::core::model::ACNet<8>;

@@ -45,22 +45,26 @@ namespace ac::core::detail
     }
 
     // Processor.cpp:26-187 restricted to what this library implements: ACNetLegacy
-    // ("acnet-legacy-{gan,hdn0..3}"; "hdn" without a digit is HDN0, the first digit
-    // 0-3 after it selects the variant; no model string means GAN).  Other families
-    // report their name in `family`.
-    bool findModel(const char* model, model::ACNetLegacy::Variant& variant, std::string& family) noexcept
+    // ("acnet-legacy-{gan,hdn0..3}"; "hdn" without a digit is HDN0, the first digit 0-3 after
+    // it selects the variant; no model string means GAN) and ACNet<8> ("acnet" without
+    // "legacy": box / hdn select the family member, b4 / b8 / b18 the depth, B8 by default).
+    // Other families report their name in `family`.
+    int findModel(const char* model, model::ACNetLegacy::Variant& variant, model::ACNet<8>::Variant& variant8,
+                  std::string& family) noexcept
     {
         variant = model::ACNetLegacy::Variant::GAN;
-        if (!model) return true;
+        variant8 = model::ACNet<8>::Variant::B8_NORMAL;
+        if (!model) return 0;
         std::string s = model;
         for (char& ch : s) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
-        if (s.find("fsrcnnx") != std::string::npos) { family = "FSRCNNX"; return false; }
-        if (s.find("artcnn") != std::string::npos) { family = "ArtCNN"; return false; }
-        if (s.find("arnet") != std::string::npos) { family = "ARNet"; return false; }
-        if (s.find("acnet") != std::string::npos)
+        auto has = [&](const char* t) { return s.find(t) != std::string::npos; };
+        if (has("fsrcnnx")) { family = "FSRCNNX"; return -1; }
+        if (has("artcnn")) { family = "ArtCNN"; return -1; }
+        if (has("arnet")) { family = "ARNet"; return -1; }
+        if (!has("acnet")) return 0;
+        if (has("legacy"))
         {
-            if (s.find("legacy") == std::string::npos) { family = "ACNet"; return false; }
-            if (s.find("hdn") != std::string::npos)
+            if (has("hdn"))
             {
                 variant = model::ACNetLegacy::Variant::HDN0;
                 for (char ch : s)
@@ -73,8 +77,17 @@ namespace ac::core::detail
                     break;
                 }
             }
+            return 0;
         }
-        return true;
+        using V = model::ACNet<8>::Variant;
+        const bool box = has("box"), hdn = has("hdn");
+        const int member = box ? (hdn ? 3 : 2) : (hdn ? 1 : 0);   // normal, hdn, box, box_hdn
+        int depth = 1;                                            // b8 unless b4 / b18 (b8 tested first)
+        if (has("b4")) depth = 0;
+        else if (has("b8")) depth = 1;
+        else if (has("b18")) depth = 2;
+        variant8 = static_cast<V>(depth * 4 + member);
+        return 1;
     }
 
     // ceil(log2(v)) from the IEEE exponent and mantissa (core/internal/.../Util.hpp:80-89)
@@ -244,15 +257,18 @@ const char* ac::core::Processor::error() noexcept
 std::shared_ptr<ac::core::Processor> ac::core::Processor::create(const char* const type, const int device, const char* const model)
 {
     model::ACNetLegacy::Variant variant{};
+    model::ACNet<8>::Variant variant8{};
     std::string family;
     int t = detail::findProcessorType(type);
     if (t == -1) t = Processor::HIP;
     if (t != Processor::HIP)
         return std::make_shared<detail::UnavailableProcessor>(t, std::string("processor type '") + detail::typeNameOf(t) +
             "' is not part of this library, which provides the HIP processor only");
-    if (!detail::findModel(model, variant, family))
+    const int f = detail::findModel(model, variant, variant8, family);
+    if (f < 0)
         return std::make_shared<detail::UnavailableProcessor>(t, "model family " + family +
-            " is not part of this library, which provides ACNetLegacy only");
+            " is not part of this library, which provides ACNetLegacy and ACNet<8>");
+    if (f == 1) return detail::createHIP8(device, model::ACNet<8>{ variant8 }, detail::findPrecision(type));
     return detail::createHIP(device, model::ACNetLegacy{ variant }, detail::findPrecision(type));
 }
 
@@ -260,6 +276,12 @@ template<>
 AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::ACNetLegacy>(const int idx, const model::ACNetLegacy& model)
 {
     return detail::createHIP(idx, model, 0);
+}
+
+template<>
+AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::ACNet<8>>(const int idx, const model::ACNet<8>& model)
+{
+    return detail::createHIP8(idx, model, 0);
 }
 
 const char* ac::core::Processor::listInfo()

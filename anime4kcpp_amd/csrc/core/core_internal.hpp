@@ -11,7 +11,10 @@ namespace ac::core::detail
 {
     int findProcessorType(const char* type) noexcept;
     int findPrecision(const char* type) noexcept;
-    bool findModel(const char* model, model::ACNetLegacy::Variant& variant, std::string& family) noexcept;
+    // model string -> 0 ACNetLegacy (`variant`), 1 ACNet<8> (`variant8`), -1 another family
+    // (its name in `family`)
+    int findModel(const char* model, model::ACNetLegacy::Variant& variant, model::ACNet<8>::Variant& variant8,
+                  std::string& family) noexcept;
     int ceilLog2(double v) noexcept;
     const char* typeNameOf(int type) noexcept;
 
@@ -21,4 +24,6 @@ namespace ac::core::detail
 
     // HIP processor for ACNetLegacy; precision 0 FP32, 1 FP16-MFMA, 2 BF16-MFMA
     std::shared_ptr<Processor> createHIP(int device, const model::ACNetLegacy& model, int precision);
+    // HIP processor for ACNet<8> (FP32; precision must be 0)
+    std::shared_ptr<Processor> createHIP8(int device, const model::ACNet<8>& model, int precision);
 }

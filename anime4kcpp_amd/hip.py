@@ -25,6 +25,21 @@ def load_weights(variant):
     return np.ascontiguousarray(k), np.ascontiguousarray(b)
 
 
+# ACNet<8> variants (core/include/AC/Core/Model/ACNet.hpp:67-72): blocks x {normal, hdn, box, box_hdn}
+VARIANTS8 = tuple(f"b{b}{s}" for b in (4, 8, 18) for s in ("", "_hdn", "_box", "_box_hdn"))
+
+
+def load_weights8(variant):
+    """(kernels, biases, alphas, blocks) float32 of an ACNet<8> variant (ACNet.hpp:77-82)."""
+    v = variant.lower()
+    if v not in VARIANTS8:
+        raise ValueError(f"unknown ACNet<8> variant {variant!r}")
+    blocks = int(v[1:].split("_")[0])
+    arrs = [np.ascontiguousarray(np.fromfile(os.path.join(WEIGHTS_DIR, f"acnet8_{v}.{kind}.f32"), dtype="<f4"))
+            for kind in ("kernels", "biases", "alphas")]
+    return arrs[0], arrs[1], arrs[2], blocks
+
+
 def elem_type(dtype):
     import torch
     table = {torch.uint8: U8, torch.uint16: U16, torch.float16: F16, torch.float32: F32}
@@ -144,6 +159,59 @@ class Net:
         ms = (ctypes.c_float * max(n, 1))()
         check(L.achip_autotune(self._h, prec, st, dt, w, h, ctypes.byref(best), ms), "achip_autotune")
         return best.value, [ms[i] for i in range(n)]
+
+
+class Net8:
+    """One ACNet<8> weight set resident on one HIP device (achip_net8_create / achip_pass8)."""
+
+    def __init__(self, variant="b8_hdn", device=0):
+        k, b, a, blocks = load_weights8(variant)
+        h = ctypes.c_void_p()
+        check(_native.lib().achip_net8_create(device, blocks, k.ctypes.data, b.ctypes.data, a.ctypes.data,
+                                              ctypes.byref(h)), "achip_net8_create")
+        self._h = h
+        self.variant = variant
+        self.blocks = blocks
+        self.device = _native.lib().achip_net_device(h)
+        self._ws = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _native.lib().achip_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def workspace(self, w, h, dev):
+        import torch
+        need = _native.lib().achip_pass8_workspace_bytes(w, h)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=dev)
+        return self._ws
+
+    def upscale(self, src, dst=None, stream=None, out_dtype=None):
+        """One ACNet<8> 2x pass: src (H x W) -> dst (2H x 2W) on `stream` (default: torch's
+        current stream); the feature-map workspace is kept on the net between calls."""
+        import torch
+        if src.dim() != 2 or not src.is_cuda or src.stride(1) != 1:
+            raise ValueError("src must be a 2-D device tensor with contiguous rows")
+        h, w = src.shape
+        if dst is None:
+            dst = torch.empty((2 * h, 2 * w), dtype=out_dtype or src.dtype, device=src.device)
+        ws = self.workspace(w, h, src.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(src.device).cuda_stream
+        elif hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        check(_native.lib().achip_pass8(self._h, ctypes.c_void_p(src.data_ptr()), src.stride(0) * src.element_size(),
+                                        elem_type(src.dtype), w, h, ctypes.c_void_p(dst.data_ptr()),
+                                        dst.stride(0) * dst.element_size(), elem_type(dst.dtype),
+                                        ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(stream)), "achip_pass8")
+        return dst
 
 
 def configs(precision="fp32"):

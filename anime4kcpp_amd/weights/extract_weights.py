@@ -1,11 +1,14 @@
-"""Extract the ACNetLegacy weight tables into little-endian f32 blobs.
+"""Extract the ACNet weight tables into little-endian f32 blobs.
 
-Source: core/internal/AC/Core/Internal/Model/Param/ACNet.p (the five
-``ACNetLegacy_{GAN,HDN0..3}_NHWC_{kernels,biases}`` constexpr arrays, bound to
-the model variants in core/src/Model.cpp:8-34).  The text is parsed, never
-compiled or executed.  Output: ``acnet_legacy_<variant>.kernels.f32`` (4712
-floats) and ``.biases.f32`` (72 floats) beside this script, plus
-``MANIFEST.json`` with the source line ranges and sha256 of every blob.
+Source: core/internal/AC/Core/Internal/Model/Param/ACNet.p: the five
+``ACNetLegacy_{GAN,HDN0..3}_NHWC_{kernels,biases}`` constexpr arrays (bound to the
+model variants in core/src/Model.cpp:8-34) and the twelve
+``ACNet_F8B{4,8,18}{,_HDN,_Box,_Box_HDN}_NHWC_{kernels,biases,alphas}`` arrays of
+ACNet<8> (core/src/Model.cpp:36-119).  The text is parsed, never compiled or
+executed.  Output beside this script: ``acnet_legacy_<variant>.{kernels,biases}.f32``
+(4712 and 72 floats), ``acnet8_<variant>.{kernels,biases,alphas}.f32`` (lengths of
+core/include/AC/Core/Model/ACNet.hpp:77-82), and ``MANIFEST.json`` with the source
+line ranges and sha256 of every blob.
 
 Run once in a tree that has the reference checked out:
     python anime4kcpp_amd/weights/extract_weights.py /root/reference
@@ -21,7 +24,15 @@ import numpy as np
 VARIANTS = ("GAN", "HDN0", "HDN1", "HDN2", "HDN3")
 KERNEL_LEN = 8 * 9 + 8 * 8 * 9 * 8 + 8 * 4   # ACNet.hpp:34 with blockNum = 8
 BIAS_LEN = 8 + 8 * 8                          # ACNet.hpp:36
-_HEAD = re.compile(r"constexpr float (ACNetLegacy_(\w+?)_NHWC_(kernels|biases))\[\] = \{")
+_HEAD = re.compile(r"constexpr float ((ACNetLegacy|ACNet)_(\w+?)_NHWC_(kernels|biases|alphas))\[\] = \{")
+# ACNet<8> variants: blob name -> (table infix, blocks)
+ACNET8 = {f"b{b}{s.lower()}": (f"F8B{b}{s}", b) for b in (4, 8, 18) for s in ("", "_HDN", "_Box", "_Box_HDN")}
+
+
+def acnet8_lengths(blocks):
+    """ACNet<8> (F = 8): kernels F*9 + F*F*9*blocks + F*4*9, biases F + F*blocks + 4,
+    alphas F*(blocks + 1) (ACNet.hpp:77-82)."""
+    return {"kernels": 8 * 9 + 64 * 9 * blocks + 8 * 4 * 9, "biases": 8 + 8 * blocks + 4, "alphas": 8 * (blocks + 1)}
 _NUM = re.compile(r"[-+]?\d+\.\d+(?:[eE][-+]?\d+)?f")
 
 
@@ -44,7 +55,7 @@ def parse(path):
         # decimal literal -> nearest float32, as the C++ compiler does for an
         # 'f'-suffixed literal (7 significant digits never hit a double-rounding tie)
         arr = np.array([np.float32(v[:-1]) for v in vals], dtype="<f4")
-        tables[(m.group(2), m.group(3))] = (arr, start, i + 1)
+        tables[(m.group(2), m.group(3), m.group(4))] = (arr, start, i + 1)
         i += 1
     return tables
 
@@ -60,7 +71,7 @@ def main(ref_root):
     for v in VARIANTS:
         entry = {}
         for kind, n in (("kernels", KERNEL_LEN), ("biases", BIAS_LEN)):
-            arr, first, last = tables[(v, kind)]
+            arr, first, last = tables[("ACNetLegacy", v, kind)]
             if arr.size != n:
                 raise SystemExit(f"{v} {kind}: expected {n} values, parsed {arr.size}")
             blob = arr.tobytes()
@@ -70,6 +81,20 @@ def main(ref_root):
             entry[kind] = {"file": name, "count": n, "lines": f"{first}-{last}",
                            "sha256": hashlib.sha256(blob).hexdigest()}
         manifest["variants"][v.lower()] = entry
+    manifest["acnet8"] = {}
+    for name, (infix, blocks) in ACNET8.items():
+        entry = {"blocks": blocks}
+        for kind, n in acnet8_lengths(blocks).items():
+            arr, first, last = tables[("ACNet", infix, kind)]
+            if arr.size != n:
+                raise SystemExit(f"ACNet {infix} {kind}: expected {n} values, parsed {arr.size}")
+            blob = arr.tobytes()
+            fname = f"acnet8_{name}.{kind}.f32"
+            with open(os.path.join(here, fname), "wb") as f:
+                f.write(blob)
+            entry[kind] = {"file": fname, "count": n, "lines": f"{first}-{last}",
+                           "sha256": hashlib.sha256(blob).hexdigest()}
+        manifest["acnet8"][name] = entry
     with open(os.path.join(here, "MANIFEST.json"), "w") as f:
         json.dump(manifest, f, indent=1)
     print(json.dumps(manifest, indent=1))

@@ -347,6 +347,40 @@ def stages():
     return out
 
 
+def acnet8(frames_np, steps=10):
+    """ACNet<8> (the reference's default model family) 1080p u8 -> 4K 2x passes, layer by
+    layer on FP32 MFMA, for 4, 8 and 18 blocks.  FLOP per input pixel: 2 x (72 + 576 blocks
+    + 288) MAC."""
+    import torch
+    from anime4kcpp_amd import hip
+    dev = torch.device("cuda:0")
+    srcs = [torch.from_numpy(f).to(dev) for f in frames_np[:4]]
+    h, w = frames_np[0].shape
+    dsts = [torch.empty((2 * h, 2 * w), dtype=torch.uint8, device=dev) for _ in srcs]
+    out = {}
+    for variant in ("b4_hdn", "b8_hdn", "b18_hdn"):
+        net = hip.Net8(variant, 0)
+        for i in range(len(srcs)):
+            net.upscale(srcs[i], dsts[i])
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            for i in range(len(srcs)):
+                net.upscale(srcs[i], dsts[i])
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
+        flop = 2 * (72 + 576 * net.blocks + 288) * w * h
+        out[variant] = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
+                        "blocks": net.blocks, "achieved_tflops": round(flop / sec / 1e12, 2),
+                        "frac_of_fp32_peak": round(flop / sec / 1e12 / PEAK_FP32_TFLOPS, 4),
+                        "hbm_bytes_per_frame": (2 * 32 * (net.blocks + 1) + 1 + 4) * w * h}
+        net.close()
+    out["workload"] = "1920x1080 u8 -> 3840x2160 u8, ACNet<8> HDN weights, layer by layer (8-channel f32 maps in HBM)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,6 +394,7 @@ def main():
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K f32 x4 precision measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-stages", action="store_true", help="skip the colour / resize / PCIe stage measurement")
+    ap.add_argument("--no-acnet8", action="store_true", help="skip the ACNet<8> measurement")
     ap.add_argument("--serving-loads", default="fp16:300,1200;fp32:60,240",
                     help="offered loads (requests/s) per precision")
     ap.add_argument("--serving-seconds", type=float, default=3.0)
@@ -466,6 +501,8 @@ def main():
         result["precision_4k_x4"] = precision_4k_x4(nets, weights, precs)
     if not args.no_stages:
         result["stages"] = stages()
+    if not args.no_acnet8:
+        result["acnet8"] = acnet8(frames_np)
     if not args.no_serving:
         result["serving"] = {}
         for spec in args.serving_loads.split(";"):

@@ -114,6 +114,27 @@ ACHIP_API int achip_autotune(ACHipNet* net, int precision, int src_type, int dst
 /* Calling thread's last error message ("" if none). */
 ACHIP_API const char* achip_last_error(void);
 
+/* ---- ACNet<8> ----
+ * The reference's default model family (core/include/AC/Core/Model/ACNet.hpp:63-118; its CPU
+ * pass core/src/processor/cpu/CPUProcessor.cpp:1417-1436): conv3x3 1 -> 8 + PReLU, `blocks`
+ * x conv3x3 8 -> 8 + PReLU, conv3x3 8 -> 4, pixel shuffle to 2x, plus the source pixel.
+ * kernels F*9 + F*F*9*blocks + F*4*9, biases F + F*blocks + 4, alphas F*(blocks+1) floats
+ * (F = 8), in the reference's layout.  FP32 throughout.  Such a net runs through
+ * achip_pass8 only (achip_pass, achip_pass_batch and the server take ACNetLegacy nets). */
+ACHIP_API int achip_net8_create(int device, int blocks, const float* kernels, const float* biases,
+                                const float* alphas, ACHipNet** out);
+/* block count of an ACNet<8> net, 0 for an ACNetLegacy net */
+ACHIP_API int achip_net_blocks(const ACHipNet* net);
+/* device workspace (two 8-channel f32 feature maps) one w x h pass needs */
+ACHIP_API size_t achip_pass8_workspace_bytes(int w, int h);
+/* one ACNet<8> 2x pass, device pointers as achip_pass; workspace: 256-byte aligned device
+ * memory of at least achip_pass8_workspace_bytes(w, h), not overlapping src or dst.  The
+ * residual reads the source in its own element type (the reference reads it as the output
+ * type, which only agrees when both types are the same). */
+ACHIP_API int achip_pass8(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h,
+                          void* dst, long dst_stride, int dst_type, void* workspace, size_t workspace_bytes,
+                          void* stream);
+
 /* ---- Device-memory colour and resize stages ----
  * The steps of Processor::process around the 2x passes (core/src/processor/Processor.cpp:
  * 207-213, 242-274) on device pointers of the current HIP device: the colour conversions of

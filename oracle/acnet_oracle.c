@@ -138,9 +138,9 @@ static inline void conv_cin1_generic(const float* r, const float* k, const float
 
 /* 8->8: OpImplFMA::conv<8,8,9> (AVX.hpp:32-59,126-146): per output, one 8-lane
  * accumulator over channels, fma over taps 0..8, hsum, added to the bias. */
-static inline void conv8_fma(const float* const* rp, const float* k, const float* b, float* out)
+static inline void convn_fma(const float* const* rp, const float* k, const float* b, float* out, int cout)
 {
-    for (int n = 0; n < CH; n++) {
+    for (int n = 0; n < cout; n++) {
         const float* kn = k + n * 72;
         float s[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
         for (int p = 0; p < 9; p++)
@@ -151,9 +151,9 @@ static inline void conv8_fma(const float* const* rp, const float* k, const float
 
 /* 8->8: OpImplGeneric::conv (Generic.hpp:71-81): sum over channels (outer) of
  * the left-folded 9-tap block sums, + bias. */
-static inline void conv8_generic(const float* const* rp, const float* k, const float* b, float* out)
+static inline void convn_generic(const float* const* rp, const float* k, const float* b, float* out, int cout)
 {
-    for (int n = 0; n < CH; n++) {
+    for (int n = 0; n < cout; n++) {
         const float* kn = k + n * 72;
         float sum = 0.0f;
         for (int c = 0; c < 8; c++) {
@@ -164,6 +164,9 @@ static inline void conv8_generic(const float* const* rp, const float* k, const f
         out[n] = sum + b[n];
     }
 }
+
+static inline void conv8_fma(const float* const* rp, const float* k, const float* b, float* out) { convn_fma(rp, k, b, out, CH); }
+static inline void conv8_generic(const float* const* rp, const float* k, const float* b, float* out) { convn_generic(rp, k, b, out, CH); }
 
 /* deconv dot<8>: FMA path (AVX.hpp:62-91) = 0 + hsum(products); Generic = left fold */
 static inline float dot8_fma(const float* v, const float* k)
@@ -276,6 +279,103 @@ int orc_acnet_pass(const void* src, long src_stride, int w, int h, int src_type,
                         float v = fma_order ? dot8_fma(s, kk) : dot8_generic(s, kk);
                         orc_from_float(v, db + (long)(2 * i + dy) * dst_stride + (long)(2 * j + dx) * des, dst_type);
                     }
+            }
+        }
+    }
+    free(t1);
+    free(t2);
+    return 0;
+}
+
+/* ---- ACNet<8> (CPUProcessor.cpp:1417-1436) ---------------------------------------- */
+
+/* PReLU (Common.hpp:42-49): max(v, 0) + alpha * min(v, 0) */
+static inline float prelu_(float v, float a) { return (v > 0.0f ? v : 0.0f) + a * (v < 0.0f ? v : 0.0f); }
+
+int orc_acnet8_pass(const void* src, long src_stride, int w, int h, int type, void* dst, long dst_stride,
+                    const float* kernels, const float* biases, const float* alphas, int blocks, int order, int threads)
+{
+    if (!src || !dst || w <= 0 || h <= 0 || !kernels || !biases || !alphas || blocks < 1) return -22;
+    size_t plane = (size_t)w * h * CH;
+    float* t1 = (float*)malloc(plane * sizeof(float));
+    float* t2 = (float*)malloc(plane * sizeof(float));
+    if (!t1 || !t2) { free(t1); free(t2); return -12; }
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+    const int fma_order = (order == ORC_ORDER_FMA);
+    const unsigned char* sb = (const unsigned char*)src;
+    const int es = elem_size(type);
+
+    /* L0: conv3x3 1->8 + PReLU (Backend.hpp:58-75, Common.hpp:166-197) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+    for (int i = 0; i < h; i++) {
+        int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+        for (int j = 0; j < w; j++) {
+            int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+            int ys[3] = { i - tp, i, i + bp }, xs[3] = { j - lp, j, j + rp };
+            float r[9];
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++)
+                    r[a * 3 + c] = orc_to_float(sb + ys[a] * src_stride + (long)xs[c] * es, type);
+            float* out = t1 + ((size_t)i * w + j) * CH;
+            if (fma_order) conv_cin1_fma(r, kernels, biases, out); else conv_cin1_generic(r, kernels, biases, out);
+            for (int n = 0; n < CH; n++) out[n] = prelu_(out[n], alphas[n]);
+        }
+    }
+    float* tin = t1;
+    float* tout = t2;
+    /* blocks: conv3x3 8->8 + PReLU (Backend.hpp:95-98, Common.hpp:116-164) */
+    for (int l = 1; l <= blocks; l++) {
+        const float* k = kernels + 72 + 576 * (l - 1);
+        const float* b = biases + 8 * l;
+        const float* al = alphas + 8 * l;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++) {
+            int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+            for (int j = 0; j < w; j++) {
+                int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+                int ys[3] = { i - tp, i, i + bp }, xs[3] = { j - lp, j, j + rp };
+                const float* rpp[9];
+                for (int a = 0; a < 3; a++)
+                    for (int c = 0; c < 3; c++) rpp[a * 3 + c] = tin + ((size_t)ys[a] * w + xs[c]) * CH;
+                float* out = tout + ((size_t)i * w + j) * CH;
+                if (fma_order) convn_fma(rpp, k, b, out, CH); else convn_generic(rpp, k, b, out, CH);
+                for (int n = 0; n < CH; n++) out[n] = prelu_(out[n], al[n]);
+            }
+        }
+        float* tmp = tin; tin = tout; tout = tmp;
+    }
+    /* final: conv3x3 8->4, identity, + the source pixel, pixel shuffle 2x (Backend.hpp:116-133,
+     * Common.hpp:290-341); output channel n goes to (2x + n % 2, 2y + n / 2) */
+    {
+        const float* k = kernels + 72 + 576 * blocks;
+        const float* b = biases + 8 + 8 * blocks;
+        unsigned char* db = (unsigned char*)dst;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++) {
+            int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+            for (int j = 0; j < w; j++) {
+                int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+                int ys[3] = { i - tp, i, i + bp }, xs[3] = { j - lp, j, j + rp };
+                const float* rpp[9];
+                for (int a = 0; a < 3; a++)
+                    for (int c = 0; c < 3; c++) rpp[a * 3 + c] = tin + ((size_t)ys[a] * w + xs[c]) * CH;
+                float s[4];
+                if (fma_order) convn_fma(rpp, k, b, s, 4); else convn_generic(rpp, k, b, s, 4);
+                const float res = orc_to_float(sb + (long)i * src_stride + (long)j * es, type);
+                for (int n = 0; n < 4; n++) {
+                    const float v = s[n] * 1.0f + res;
+                    orc_from_float(v, db + (long)(2 * i + (n >> 1)) * dst_stride + (long)(2 * j + (n & 1)) * es, type);
+                }
             }
         }
     }

@@ -49,6 +49,13 @@ int orc_acnet_pass(const void* src, long src_stride, int w, int h, int src_type,
                    void* dst, long dst_stride, int dst_type,
                    const float* kernels, const float* biases, int order, int threads);
 
+/* ACNet<8> 2x pass (core/src/processor/cpu/CPUProcessor.cpp:1417-1436): conv3x3 1->8 +
+ * PReLU, `blocks` x conv3x3 8->8 + PReLU, conv3x3 8->4 + the source pixel, pixel shuffle;
+ * src and dst of one element type; weights in the reference layout
+ * (core/include/AC/Core/Model/ACNet.hpp:77-117).  order as orc_acnet_pass. */
+int orc_acnet8_pass(const void* src, long src_stride, int w, int h, int type, void* dst, long dst_stride,
+                    const float* kernels, const float* biases, const float* alphas, int blocks, int order, int threads);
+
 /* The same network in double precision.  src is the toFloat'ed input
  * (w*h floats, dense); dst receives the saturated 2w x 2h output in double. */
 int orc_acnet_pass_f64(const float* src, int w, int h, double* dst,

@@ -6,6 +6,10 @@ int acref_pass_generic(const void*, long, int, int, int, void*, long, int, const
 int acref_pass_fma(const void*, long, int, int, int, void*, long, int, const float*, const float*);
 int acref_pass_avx512(const void*, long, int, int, int, void*, long, int, const float*, const float*);
 
+int acref_pass8_generic(const void*, long, int, int, int, void*, long, const float*, const float*, const float*, int);
+int acref_pass8_fma(const void*, long, int, int, int, void*, long, const float*, const float*, const float*, int);
+int acref_pass8_avx512(const void*, long, int, int, int, void*, long, const float*, const float*, const float*, int);
+
 int acref_has_arch(int arch)
 {
     __builtin_cpu_init();
@@ -25,6 +29,18 @@ int acref_pass(int arch, const void* src, long sstride, int w, int h, int stype,
     case 1: return acref_pass_generic(src, sstride, w, h, stype, dst, dstride, dtype, kernels, biases);
     case 4: return acref_pass_fma(src, sstride, w, h, stype, dst, dstride, dtype, kernels, biases);
     case 5: return acref_pass_avx512(src, sstride, w, h, stype, dst, dstride, dtype, kernels, biases);
+    default: return -22;
+    }
+}
+
+int acref_pass8(int arch, const void* src, long sstride, int w, int h, int type, void* dst, long dstride,
+                const float* kernels, const float* biases, const float* alphas, int blocks)
+{
+    if (!acref_has_arch(arch)) return -95;
+    switch (arch) {
+    case 1: return acref_pass8_generic(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, blocks);
+    case 4: return acref_pass8_fma(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, blocks);
+    case 5: return acref_pass8_avx512(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, blocks);
     default: return -22;
     }
 }

@@ -9,6 +9,7 @@
 // The per-pixel replicate-clamped loops of Common.hpp:116-197,344-393 cannot be
 // included (Common.hpp pulls in Image.hpp, which needs a CMake-generated export
 // header), so the three loops below restate them and call the reference kernels.
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <vector>
@@ -82,6 +83,75 @@ void last(const float* in, int w, int h, void* dst, long dstride, const float* k
         }
     }
 }
+// ---- ACNet<8> (CPUProcessor.cpp:1417-1436) ----
+
+inline float prelu(float v, float a) { return std::max(v, 0.0f) + a * std::min(v, 0.0f); }   // Common.hpp:42-49
+
+template <typename IN>
+void a8_layer0(const void* src, long sstride, int w, int h, float* out, const float* k, const float* b, const float* al)
+{
+#pragma omp parallel for schedule(guided)
+    for (int i = 0; i < h; i++) {
+        int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+        auto row = [&](int y) { return reinterpret_cast<const IN*>(static_cast<const std::uint8_t*>(src) + y * sstride); };
+        for (int j = 0; j < w; j++) {
+            int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+            alignas(64) const float r[] = {
+                ac::core::toFloat(row(i - tp)[j - lp]), ac::core::toFloat(row(i - tp)[j]), ac::core::toFloat(row(i - tp)[j + rp]),
+                ac::core::toFloat(row(i)[j - lp]),      ac::core::toFloat(row(i)[j]),      ac::core::toFloat(row(i)[j + rp]),
+                ac::core::toFloat(row(i + bp)[j - lp]), ac::core::toFloat(row(i + bp)[j]), ac::core::toFloat(row(i + bp)[j + rp]),
+            };
+            float* o = out + (static_cast<long>(i) * w + j) * 8;
+            Op::conv_cin1<8, 9>(r, o, k, b);
+            for (int n = 0; n < 8; n++) o[n] = prelu(o[n], al[n]);
+        }
+    }
+}
+
+void a8_block(const float* in, int w, int h, float* out, const float* k, const float* b, const float* al)
+{
+#pragma omp parallel for schedule(guided)
+    for (int i = 0; i < h; i++) {
+        int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+        for (int j = 0; j < w; j++) {
+            int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+            auto px = [&](int y, int x) { return in + (static_cast<long>(y) * w + x) * 8; };
+            const float* rptr[] = { px(i - tp, j - lp), px(i - tp, j), px(i - tp, j + rp),
+                                    px(i, j - lp),      px(i, j),      px(i, j + rp),
+                                    px(i + bp, j - lp), px(i + bp, j), px(i + bp, j + rp) };
+            alignas(64) float sum[8]{};
+            Op::conv<8, 8, 9>(rptr, sum, k, b);
+            float* o = out + (static_cast<long>(i) * w + j) * 8;
+            for (int n = 0; n < 8; n++) o[n] = prelu(sum[n], al[n]);
+        }
+    }
+}
+
+// Common.hpp:290-341 with Identity and the source pixel as the nearest residual
+template <typename IN, typename OUT>
+void a8_last(const float* in, int w, int h, const void* src, long sstride, void* dst, long dstride, const float* k,
+             const float* b)
+{
+#pragma omp parallel for schedule(guided)
+    for (int i = 0; i < h; i++) {
+        int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+        for (int j = 0; j < w; j++) {
+            int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+            auto px = [&](int y, int x) { return in + (static_cast<long>(y) * w + x) * 8; };
+            const float* rptr[] = { px(i - tp, j - lp), px(i - tp, j), px(i - tp, j + rp),
+                                    px(i, j - lp),      px(i, j),      px(i, j + rp),
+                                    px(i + bp, j - lp), px(i + bp, j), px(i + bp, j + rp) };
+            alignas(64) float sum[4]{};
+            Op::conv<8, 4, 9>(rptr, sum, k, b);
+            const float res = ac::core::toFloat(reinterpret_cast<const IN*>(static_cast<const std::uint8_t*>(src) + i * sstride)[j]);
+            for (int n = 0; n < 4; n++) {
+                const float v = sum[n] * 1.0f + res;
+                auto o = reinterpret_cast<OUT*>(static_cast<std::uint8_t*>(dst) + (2 * i + (n >> 1)) * dstride) + 2 * j + (n & 1);
+                *o = ac::core::fromFloat<OUT>(v);
+            }
+        }
+    }
+}
 } // namespace
 
 #define CAT2(a, b) a##b
@@ -112,6 +182,36 @@ extern "C" int CAT(acref_pass_, BACKEND_SUFFIX)(const void* src, long sstride, i
     case 0x001: last<std::uint8_t>(ti, w, h, dst, dstride, k8, b8, k9); break;
     case 0x002: last<std::uint16_t>(ti, w, h, dst, dstride, k8, b8, k9); break;
     case 0x204: last<float>(ti, w, h, dst, dstride, k8, b8, k9); break;
+    default: return -22;
+    }
+    return 0;
+}
+
+// ACNet<8>: same element types as acref_pass; src and dst of one type (the residual is read
+// in that type, as Common.hpp:330 reads it in the output type)
+extern "C" int CAT(acref_pass8_, BACKEND_SUFFIX)(const void* src, long sstride, int w, int h, int type, void* dst,
+                                                 long dstride, const float* kernels, const float* biases,
+                                                 const float* alphas, int blocks)
+{
+    std::vector<float> t1(static_cast<size_t>(w) * h * 8), t2(static_cast<size_t>(w) * h * 8);
+    switch (type) {
+    case 0x001: a8_layer0<std::uint8_t>(src, sstride, w, h, t1.data(), kernels, biases, alphas); break;
+    case 0x002: a8_layer0<std::uint16_t>(src, sstride, w, h, t1.data(), kernels, biases, alphas); break;
+    case 0x204: a8_layer0<float>(src, sstride, w, h, t1.data(), kernels, biases, alphas); break;
+    default: return -22;
+    }
+    float* ti = t1.data();
+    float* to = t2.data();
+    for (int l = 1; l <= blocks; l++) {
+        a8_block(ti, w, h, to, kernels + 72 + 576 * (l - 1), biases + 8 * l, alphas + 8 * l);
+        std::swap(ti, to);
+    }
+    const float* kf = kernels + 72 + 576 * blocks;
+    const float* bf = biases + 8 + 8 * blocks;
+    switch (type) {
+    case 0x001: a8_last<std::uint8_t, std::uint8_t>(ti, w, h, src, sstride, dst, dstride, kf, bf); break;
+    case 0x002: a8_last<std::uint16_t, std::uint16_t>(ti, w, h, src, sstride, dst, dstride, kf, bf); break;
+    case 0x204: a8_last<float, float>(ti, w, h, src, sstride, dst, dstride, kf, bf); break;
     default: return -22;
     }
     return 0;

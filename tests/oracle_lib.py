@@ -46,6 +46,8 @@ def orc():
         o.orc_resize2x.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int]
         o.orc_resize.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        o.orc_acnet8_pass.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, P, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int]
         o.orc_filter_weight.argtypes = [ctypes.c_int, ctypes.c_float]
         o.orc_filter_weight.restype = ctypes.c_float
         o.orc_filter_support.argtypes = [ctypes.c_int]
@@ -64,6 +66,7 @@ def ref():
         r = ctypes.CDLL(REF_SO)
         r.acref_pass.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int, P, P]
         r.acref_has_arch.argtypes = [ctypes.c_int]
+        r.acref_pass8.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, P, ctypes.c_int]
         _ref = r
     return _ref
 
@@ -122,4 +125,27 @@ def resize(src, dw, dh, mode):
     dst = np.zeros((dh, dw) + (() if c == 1 else (c,)), src.dtype)
     assert orc().orc_resize(src.ctypes.data, src.strides[0], w, h, c, TYPE_OF[src.dtype], dst.ctypes.data,
                             dst.strides[0], dw, dh, mode) == 0
+    return dst
+
+
+def oracle_pass8(src, kernels, biases, alphas, blocks, order=ORDER_FMA, threads=0):
+    """ACNet<8> 2x pass of a 2-D array (u8/u16/f32); output of the same type."""
+    src = np.ascontiguousarray(src)
+    h, w = src.shape
+    dst = np.zeros((2 * h, 2 * w), src.dtype)
+    k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases, alphas))
+    assert orc().orc_acnet8_pass(src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
+                                 dst.strides[0], k.ctypes.data, b.ctypes.data, a.ctypes.data, blocks, order, threads) == 0
+    return dst
+
+
+def ref_pass8(arch, src, kernels, biases, alphas, blocks):
+    """The same pass through the reference's own per-ISA kernels (oracle/_ref)."""
+    src = np.ascontiguousarray(src)
+    h, w = src.shape
+    dst = np.zeros((2 * h, 2 * w), src.dtype)
+    k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases, alphas))
+    rc = ref().acref_pass8(arch, src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
+                           dst.strides[0], k.ctypes.data, b.ctypes.data, a.ctypes.data, blocks)
+    assert rc == 0, rc
     return dst
