@@ -65,6 +65,10 @@ class ACProcessor(ctypes.Structure):
 
 # name -> (restype, argtypes); the C-ABI surface this package relies on
 SIGNATURES = {
+    "achip_net_artcnn_create": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
+    "achip_net_workspace_bytes": (ctypes.c_size_t, [c_void_p, c_int, c_int]),
+    "achip_pass_net": (c_int, [c_void_p, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
+                               ctypes.c_size_t, c_void_p]),
     "achip_net8_create": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
     "achip_net_blocks": (c_int, [c_void_p]),
     "achip_pass8_workspace_bytes": (ctypes.c_size_t, [c_int, c_int]),

@@ -351,6 +351,7 @@ void achip_net_destroy(ACHipNet* net)
         DeviceGuard g(net->device);
         if (net->wts) (void)hipFree(net->wts);
         if (net->a8wts) (void)hipFree(net->a8wts);
+        if (net->artk) (void)hipFree(net->artk);
     }
     delete net;
 }
@@ -418,7 +419,103 @@ int achip_net8_create(int device, int blocks, const float* kernels, const float*
     return ACHIP_OK;
 }
 
-int achip_net_blocks(const ACHipNet* net) { return net ? net->blocks : ACHIP_EINVAL; }
+int achip_net_blocks(const ACHipNet* net) { return net ? (net->artf ? net->artblocks : net->blocks) : ACHIP_EINVAL; }
+
+// ---- ArtCNN<F> ----
+
+int achip_net_artcnn_create(int device, int channels, int blocks, const float* kernels, const float* biases, ACHipNet** out)
+{
+    if (!out) return fail(ACHIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!kernels || !biases) return fail(ACHIP_EINVAL, "NULL weights");
+    if (channels != 16 && channels != 32) return fail(ACHIP_EINVAL, "ArtCNN has 16 or 32 channels, not %d", channels);
+    if (blocks < 1 || blocks > 64) return fail(ACHIP_EINVAL, "blocks %d out of range", blocks);
+    const int n = achip_device_count();
+    if (n <= 0) return fail(ACHIP_ENODEV, "no HIP device");
+    if (device < 0 || device >= n) device = 0;
+    const int F = channels, KQ = F / 4, MT = F / 16;
+    const size_t klen = static_cast<size_t>(F) * 9 + static_cast<size_t>(F) * F * 9 * (blocks + 1) + F * 4 * 9;   // ArtCNN.hpp:28
+    const size_t blen = static_cast<size_t>(F) + static_cast<size_t>(F) * (blocks + 1) + 4;
+    const size_t wl = 9 * 64 * static_cast<size_t>(KQ * MT);
+    // A operands of the body, residual and tail layers: [tap][lane][q * MT + h] holds the
+    // weight of output channel 16 h + lane % 16 and input channel KQ * (lane / 16) + q
+    std::vector<float> wa(wl * (blocks + 2), 0.0f);
+    for (int l = 1; l <= blocks + 2; l++) {
+        const float* k = kernels + F * 9 + static_cast<size_t>(F) * F * 9 * (l - 1);
+        const int cout = l == blocks + 2 ? 4 : F;
+        float* al = wa.data() + wl * (l - 1);
+        for (int p = 0; p < 9; p++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int q = 0; q < KQ; q++)
+                    for (int h = 0; h < MT; h++) {
+                        const int o = 16 * h + lane % 16, c = KQ * (lane / 16) + q;
+                        al[(static_cast<size_t>(p) * 64 + lane) * (KQ * MT) + q * MT + h] =
+                            o < cout ? k[static_cast<size_t>(o) * 9 * F + p * F + c] : 0.0f;
+                    }
+    }
+    DeviceGuard g(device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    const size_t offb = (klen * sizeof(float) + 255) / 256 * 256;
+    const size_t offa = offb + (blen * sizeof(float) + 255) / 256 * 256;
+    char* d = nullptr;
+    hipError_t e = hipMalloc(&d, offa + wa.size() * sizeof(float));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(weights)");
+    e = hipMemcpy(d, kernels, klen * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + offb, biases, blen * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + offa, wa.data(), wa.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "hipMemcpy(weights)");
+    }
+    auto* net = new ACHipNet{ device, nullptr, nullptr, nullptr, nullptr };
+    net->artf = F;
+    net->artblocks = blocks;
+    net->artk = reinterpret_cast<float*>(d);
+    net->artb = reinterpret_cast<float*>(d + offb);
+    net->artwa = reinterpret_cast<float*>(d + offa);
+    *out = net;
+    g_error.clear();
+    return ACHIP_OK;
+}
+
+size_t achip_net_workspace_bytes(const ACHipNet* net, int w, int h)
+{
+    if (!net || w <= 0 || h <= 0) return 0;
+    if (net->artf) return 3 * static_cast<size_t>(achip::artcnn_feature_pitch(net->artf, w)) * static_cast<size_t>(h);
+    if (net->blocks) return achip_pass8_workspace_bytes(w, h);
+    return 0;
+}
+
+int achip_pass_net(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h, void* dst, long dst_stride,
+                   int dst_type, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    if (!net->artf && !net->blocks)
+        return achip_pass(net, ACHIP_FP32, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, stream);
+    if (net->blocks)
+        return achip_pass8(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, workspace, workspace_bytes, stream);
+    int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
+    if (r) return r;
+    const size_t need = achip_net_workspace_bytes(net, w, h);
+    if (!workspace || workspace_bytes < need)
+        return fail(ACHIP_EINVAL, "workspace of %zu bytes, a %dx%d pass needs %zu", workspace_bytes, w, h, need);
+    if (reinterpret_cast<uintptr_t>(workspace) % 256) return fail(ACHIP_EINVAL, "workspace not 256-byte aligned");
+    r = check_device_buffer(workspace, need, net->device, "workspace");
+    if (r) return r;
+    const char* ws = static_cast<const char*>(workspace);
+    const char* s0 = static_cast<const char*>(src);
+    const char* d0 = static_cast<const char*>(dst);
+    const size_t sbytes = static_cast<size_t>(h - 1) * src_stride + static_cast<size_t>(w) * (src_type & 0xff);
+    const size_t dbytes = static_cast<size_t>(2 * h - 1) * dst_stride + static_cast<size_t>(2 * w) * (dst_type & 0xff);
+    if ((ws < s0 + sbytes && s0 < ws + need) || (ws < d0 + dbytes && d0 < ws + need))
+        return fail(ACHIP_EINVAL, "the workspace overlaps src or dst");
+    DeviceGuard g(net->device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    const hipError_t e = achip::launch_artcnn(net->artf, net->artblocks, src_type, dst_type, src, src_stride, w, h, dst,
+                                              dst_stride, net->artk, net->artb, net->artwa, workspace,
+                                              static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "kernel launch");
+}
 
 size_t achip_pass8_workspace_bytes(int w, int h)
 {
@@ -430,7 +527,7 @@ int achip_pass8(ACHipNet* net, const void* src, long src_stride, int src_type, i
                 int dst_type, void* workspace, size_t workspace_bytes, void* stream)
 {
     if (!net) return fail(ACHIP_EINVAL, "net is NULL");
-    if (!net->blocks) return fail(ACHIP_EINVAL, "an ACNetLegacy net runs through achip_pass");
+    if (!net->blocks) return fail(ACHIP_EINVAL, "achip_pass8 takes an ACNet<8> net");
     int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
     if (r) return r;
     const size_t need = achip_pass8_workspace_bytes(w, h);
@@ -500,6 +597,7 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
     }
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
     if (net && net->blocks) return fail(ACHIP_EINVAL, "an ACNet<8> net runs through achip_pass8");
+    if (net && net->artf) return fail(ACHIP_EINVAL, "an ArtCNN net runs through achip_pass_net");
     int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
     if (r) return r;
     DeviceGuard g(net->device);
@@ -676,7 +774,7 @@ int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
     if (!images || n <= 0) return fail(ACHIP_EINVAL, "empty batch");
     if (!net) return fail(ACHIP_EINVAL, "net is NULL");
-    if (net->blocks) return fail(ACHIP_EINVAL, "batched passes take an ACNetLegacy net");
+    if (net->blocks || net->artf) return fail(ACHIP_EINVAL, "batched passes take an ACNetLegacy net");
     const size_t need = achip_batch_workspace_bytes(n);
     if (!workspace || workspace_bytes < need)
         return fail(ACHIP_EINVAL, "workspace of %zu bytes, the batch needs %zu", workspace_bytes, need);

@@ -16,6 +16,11 @@ struct ACHipNet {
     int blocks = 0;          // ACNet<8>: its block count (the fields above are then unused)
     float* a8wts = nullptr;  // ACNet<8> blob, acnet_device.hpp layout (one allocation with a8wa)
     float* a8wa = nullptr;   // ACNet<8> A operands: blocks 1..B, then the final layer
+    int artf = 0;            // ArtCNN<F>: F (16 or 32), with artblocks body blocks
+    int artblocks = 0;
+    float* artk = nullptr;   // ArtCNN kernels and biases (reference layout) and A operands, one allocation
+    float* artb = nullptr;
+    float* artwa = nullptr;
 };
 
 namespace achip {

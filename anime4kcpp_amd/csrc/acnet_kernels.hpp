@@ -85,6 +85,15 @@ long acnet8_feature_pitch(int w);
 hipError_t launch_acnet8(int blocks, int stype, int dtype, const void* src, long ss, int W, int H, void* dst, long ds,
                          const float* wts, const float* wa, void* workspace, hipStream_t stream);
 
+// ArtCNN<F> 2x pass, F = 16 or 32 (artcnn.hip): kernels/biases in the reference layout
+// (device copies), wa the per-lane A operands of the B body layers, the residual layer and
+// the F -> 4 tail, 9 * 64 * (F/4) * (F/16) floats each ([tap][lane][K step][channel tile]).
+// Workspace: 3 * artcnn_feature_pitch(F, W) * H bytes (head features and two maps).
+long artcnn_feature_pitch(int f, int w);
+hipError_t launch_artcnn(int f, int blocks, int stype, int dtype, const void* src, long ss, int W, int H, void* dst,
+                         long ds, const float* kernels, const float* biases, const float* wa, void* workspace,
+                         hipStream_t stream);
+
 // ---- colour conversion and resize (acnet_color.hip) ----
 
 // One colour component of an image in device memory: component (x, y) is the element at

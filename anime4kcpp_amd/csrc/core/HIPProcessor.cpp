@@ -37,8 +37,11 @@ namespace ac::core::hip
     {
     public:
         HIPProcessorACNetLegacy(int device, const model::ACNetLegacy& model, int precision) noexcept;
-        // ACNet<8>: the same processor around an achip_net8_create net (FP32 only)
+        // ACNet<8> and ArtCNN: the same processor around an achip_net8_create or
+        // achip_net_artcnn_create net (FP32 only)
         HIPProcessorACNetLegacy(int device, const model::ACNet<8>& model, int precision) noexcept;
+        template<int F>
+        HIPProcessorACNetLegacy(int device, const model::ArtCNN<F>& model, int precision) noexcept;
         ~HIPProcessorACNetLegacy() noexcept override;
 
         bool ok() noexcept override;
@@ -60,7 +63,7 @@ namespace ac::core::hip
         {
             hipStream_t stream = nullptr;
             DeviceBuffer buf[8];
-            DeviceBuffer ws;          // ACNet<8> feature maps
+            DeviceBuffer ws;          // ACNet<8> / ArtCNN feature maps
             std::string error;
         };
         ThreadState& local();
@@ -69,15 +72,15 @@ namespace ac::core::hip
         bool run(const Image& src, Image& dst, int passes);
         bool pipeline(const Image& src, Image& dst, double factor);
         bool stream_ready(ThreadState& st);
-        // one 2x pass of the net on device buffers (ACNetLegacy: achip_pass; ACNet<8>:
-        // achip_pass8 with this thread's workspace)
+        // one 2x pass of the net on device buffers (ACNetLegacy: achip_pass; ACNet<8> and
+        // ArtCNN: achip_pass_net with this thread's workspace)
         bool pass(ThreadState& st, const void* src, long sp, int stype, int w, int h, void* dst, long dp, int dtype);
         void finishInit();
 
     private:
         ACHipNet* net = nullptr;
         int precision;
-        int blocks8 = 0;
+        bool layered = false;     // ACNet<8> or ArtCNN: feature maps in a workspace
         std::string devName{ "no HIP device" };
         std::string initError;
         std::shared_mutex mtx;
@@ -110,7 +113,7 @@ ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device
 }
 
 ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device, const model::ACNet<8>& model, const int precision) noexcept
-    : precision(precision), blocks8(model.blocks())
+    : precision(precision), layered(true)
 {
     if (precision != 0)
     {
@@ -118,6 +121,25 @@ ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device
         return;
     }
     const int rc = achip_net8_create(device, model.blocks(), model.kernel(0), model.bias(0), model.alpha(0), &net);
+    if (rc != ACHIP_OK)
+    {
+        initError = std::string("HIP processor: ") + achip_last_error();
+        net = nullptr;
+        return;
+    }
+    finishInit();
+}
+
+template<int F>
+ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device, const model::ArtCNN<F>& model, const int precision) noexcept
+    : precision(precision), layered(true)
+{
+    if (precision != 0)
+    {
+        initError = "HIP processor: ArtCNN runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's)";
+        return;
+    }
+    const int rc = achip_net_artcnn_create(device, F, model.blocks(), model.kernel(0), model.bias(0), &net);
     if (rc != ACHIP_OK)
     {
         initError = std::string("HIP processor: ") + achip_last_error();
@@ -138,13 +160,13 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pass(ThreadState& st, const void* s
                                                   const int h, void* dst, const long dp, const int dtype)
 {
     int rc;
-    if (blocks8)
+    if (layered)
     {
-        if (!ensure(st.ws, achip_pass8_workspace_bytes(w, h))) return false;
-        rc = achip_pass8(net, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream);
+        if (!ensure(st.ws, achip_net_workspace_bytes(net, w, h))) return false;
+        rc = achip_pass_net(net, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream);
     }
     else rc = achip_pass(net, precision, src, sp, stype, w, h, dst, dp, dtype, st.stream);
-    return rc == ACHIP_OK ? true : fail(std::string(blocks8 ? "achip_pass8: " : "achip_pass: ") + achip_last_error());
+    return rc == ACHIP_OK ? true : fail(std::string(layered ? "achip_pass_net: " : "achip_pass: ") + achip_last_error());
 }
 
 ac::core::hip::HIPProcessorACNetLegacy::~HIPProcessorACNetLegacy() noexcept
@@ -393,6 +415,14 @@ std::shared_ptr<ac::core::Processor> ac::core::detail::createHIP8(const int devi
 {
     return std::make_shared<hip::HIPProcessorACNetLegacy>(device, model, precision);
 }
+
+template<int F>
+std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPArt(const int device, const model::ArtCNN<F>& model, const int precision)
+{
+    return std::make_shared<hip::HIPProcessorACNetLegacy>(device, model, precision);
+}
+template std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPArt<16>(int, const model::ArtCNN<16>&, int);
+template std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPArt<32>(int, const model::ArtCNN<32>&, int);
 
 template<>
 AC_CORE_EXPORT const char* ac::core::Processor::info<ac::core::Processor::HIP>()

@@ -1,8 +1,9 @@
-// Variant -> weight table binding of ACNetLegacy (core/src/Model.cpp:8-34) and ACNet<8>
-// (core/src/Model.cpp:36-120).  The tables
+// Variant -> weight table binding of ACNetLegacy (core/src/Model.cpp:8-34), ACNet<8>
+// (core/src/Model.cpp:36-120) and ArtCNN<16>/<32> (core/src/Model.cpp:232-281).  The tables
 // are generated at build time from anime4kcpp_amd/weights/*.f32 (parsed from
 // core/internal/AC/Core/Internal/Model/Param/ACNet.p, sha256 in weights/MANIFEST.json).
 #include "AC/Core/Model/ACNet.hpp"
+#include "AC/Core/Model/ArtCNN.hpp"
 
 namespace ac::core::model::param
 {
@@ -21,6 +22,10 @@ namespace ac::core::model::param
     AC_A8_TABLES(ACNet_F8B8) AC_A8_TABLES(ACNet_F8B8_HDN) AC_A8_TABLES(ACNet_F8B8_Box) AC_A8_TABLES(ACNet_F8B8_Box_HDN)
     AC_A8_TABLES(ACNet_F8B18) AC_A8_TABLES(ACNet_F8B18_HDN) AC_A8_TABLES(ACNet_F8B18_Box) AC_A8_TABLES(ACNet_F8B18_Box_HDN)
 #undef AC_A8_TABLES
+#define AC_ART_TABLES(NAME) extern const float NAME##_kernels[]; extern const float NAME##_biases[];
+    AC_ART_TABLES(ArtCNN_C4F16) AC_ART_TABLES(ArtCNN_C4F16_DN) AC_ART_TABLES(ArtCNN_C4F16_DS)
+    AC_ART_TABLES(ArtCNN_C4F32) AC_ART_TABLES(ArtCNN_C4F32_DN) AC_ART_TABLES(ArtCNN_C4F32_DS)
+#undef AC_ART_TABLES
 }
 
 ac::core::model::ACNetLegacy::ACNetLegacy(const Variant v) noexcept : v(v), blockNum(8)
@@ -62,3 +67,31 @@ ac::core::model::ACNet<8>::ACNet(const Variant v) noexcept : v(v)
 }
 
 template class ac::core::model::ACNet<8>;
+
+template<int F>
+ac::core::model::ArtCNN<F>::ArtCNN(const Variant v) noexcept : v(v), blockNum(4)
+{
+    static_assert(F == 16 || F == 32, "ArtCNN has 16 or 32 channels");
+    using namespace ac::core::model::param;
+#define AC_ART_BIND(NAME) kptr = NAME##_kernels; bptr = NAME##_biases; break;
+    if constexpr (F == 16)
+        switch (v)
+        {
+        case Variant::C4_DN: AC_ART_BIND(ArtCNN_C4F16_DN)
+        case Variant::C4_DS: AC_ART_BIND(ArtCNN_C4F16_DS)
+        case Variant::C4_NORMAL:
+        default: AC_ART_BIND(ArtCNN_C4F16)
+        }
+    else
+        switch (v)
+        {
+        case Variant::C4_DN: AC_ART_BIND(ArtCNN_C4F32_DN)
+        case Variant::C4_DS: AC_ART_BIND(ArtCNN_C4F32_DS)
+        case Variant::C4_NORMAL:
+        default: AC_ART_BIND(ArtCNN_C4F32)
+        }
+#undef AC_ART_BIND
+}
+
+template class ac::core::model::ArtCNN<16>;
+template class ac::core::model::ArtCNN<32>;

@@ -46,22 +46,28 @@ namespace ac::core::detail
 
     // Processor.cpp:26-187 restricted to what this library implements: ACNetLegacy
     // ("acnet-legacy-{gan,hdn0..3}"; "hdn" without a digit is HDN0, the first digit 0-3 after
-    // it selects the variant; no model string means GAN) and ACNet<8> ("acnet" without
-    // "legacy": box / hdn select the family member, b4 / b8 / b18 the depth, B8 by default).
-    // Other families report their name in `family`.
-    int findModel(const char* model, model::ACNetLegacy::Variant& variant, model::ACNet<8>::Variant& variant8,
-                  std::string& family) noexcept
+    // it selects the variant; no model string means GAN), ACNet<8> ("acnet" without
+    // "legacy": box / hdn select the family member, b4 / b8 / b18 the depth, B8 by default)
+    // and ArtCNN ("artcnn": f32 selects 32 channels, else 16; dn, else ds, the weight set;
+    // Processor.cpp:59-77).  Other families report their name.
+    ModelChoice findModel(const char* model) noexcept
     {
-        variant = model::ACNetLegacy::Variant::GAN;
-        variant8 = model::ACNet<8>::Variant::B8_NORMAL;
-        if (!model) return 0;
+        ModelChoice m;
+        if (!model) return m;
         std::string s = model;
         for (char& ch : s) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
         auto has = [&](const char* t) { return s.find(t) != std::string::npos; };
-        if (has("fsrcnnx")) { family = "FSRCNNX"; return -1; }
-        if (has("artcnn")) { family = "ArtCNN"; return -1; }
-        if (has("arnet")) { family = "ARNet"; return -1; }
-        if (!has("acnet")) return 0;
+        if (has("fsrcnnx")) { m.family = -1; m.other = "FSRCNNX"; return m; }
+        if (has("artcnn"))
+        {
+            using V = model::ArtCNN<16>::Variant;
+            m.family = has("f32") ? 3 : 2;
+            m.art = has("dn") ? V::C4_DN : (has("ds") ? V::C4_DS : V::C4_NORMAL);
+            return m;
+        }
+        if (has("arnet")) { m.family = -1; m.other = "ARNet"; return m; }
+        if (!has("acnet")) return m;
+        auto& variant = m.legacy;
         if (has("legacy"))
         {
             if (has("hdn"))
@@ -77,7 +83,7 @@ namespace ac::core::detail
                     break;
                 }
             }
-            return 0;
+            return m;
         }
         using V = model::ACNet<8>::Variant;
         const bool box = has("box"), hdn = has("hdn");
@@ -86,8 +92,9 @@ namespace ac::core::detail
         if (has("b4")) depth = 0;
         else if (has("b8")) depth = 1;
         else if (has("b18")) depth = 2;
-        variant8 = static_cast<V>(depth * 4 + member);
-        return 1;
+        m.family = 1;
+        m.acnet8 = static_cast<V>(depth * 4 + member);
+        return m;
     }
 
     // ceil(log2(v)) from the IEEE exponent and mantissa (core/internal/.../Util.hpp:80-89)
@@ -256,20 +263,24 @@ const char* ac::core::Processor::error() noexcept
 // Processor.cpp:286-317: "auto" tries the GPU first.  Here it is HIP and only HIP.
 std::shared_ptr<ac::core::Processor> ac::core::Processor::create(const char* const type, const int device, const char* const model)
 {
-    model::ACNetLegacy::Variant variant{};
-    model::ACNet<8>::Variant variant8{};
-    std::string family;
     int t = detail::findProcessorType(type);
     if (t == -1) t = Processor::HIP;
     if (t != Processor::HIP)
         return std::make_shared<detail::UnavailableProcessor>(t, std::string("processor type '") + detail::typeNameOf(t) +
             "' is not part of this library, which provides the HIP processor only");
-    const int f = detail::findModel(model, variant, variant8, family);
-    if (f < 0)
-        return std::make_shared<detail::UnavailableProcessor>(t, "model family " + family +
-            " is not part of this library, which provides ACNetLegacy and ACNet<8>");
-    if (f == 1) return detail::createHIP8(device, model::ACNet<8>{ variant8 }, detail::findPrecision(type));
-    return detail::createHIP(device, model::ACNetLegacy{ variant }, detail::findPrecision(type));
+    const detail::ModelChoice m = detail::findModel(model);
+    const int precision = detail::findPrecision(type);
+    switch (m.family)
+    {
+    case 0: return detail::createHIP(device, model::ACNetLegacy{ m.legacy }, precision);
+    case 1: return detail::createHIP8(device, model::ACNet<8>{ m.acnet8 }, precision);
+    case 2: return detail::createHIPArt<16>(device, model::ArtCNN<16>{ m.art }, precision);
+    case 3:
+        return detail::createHIPArt<32>(device, model::ArtCNN<32>{ static_cast<model::ArtCNN<32>::Variant>(m.art) }, precision);
+    default:
+        return std::make_shared<detail::UnavailableProcessor>(t, "model family " + m.other +
+            " is not part of this library, which provides ACNetLegacy, ACNet<8> and ArtCNN");
+    }
 }
 
 template<>
@@ -282,6 +293,18 @@ template<>
 AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::ACNet<8>>(const int idx, const model::ACNet<8>& model)
 {
     return detail::createHIP8(idx, model, 0);
+}
+
+template<>
+AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::ArtCNN<16>>(const int idx, const model::ArtCNN<16>& model)
+{
+    return detail::createHIPArt<16>(idx, model, 0);
+}
+
+template<>
+AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::ArtCNN<32>>(const int idx, const model::ArtCNN<32>& model)
+{
+    return detail::createHIPArt<32>(idx, model, 0);
 }
 
 const char* ac::core::Processor::listInfo()

@@ -11,10 +11,18 @@ namespace ac::core::detail
 {
     int findProcessorType(const char* type) noexcept;
     int findPrecision(const char* type) noexcept;
-    // model string -> 0 ACNetLegacy (`variant`), 1 ACNet<8> (`variant8`), -1 another family
-    // (its name in `family`)
-    int findModel(const char* model, model::ACNetLegacy::Variant& variant, model::ACNet<8>::Variant& variant8,
-                  std::string& family) noexcept;
+    // What a model string names: family 0 ACNetLegacy (`legacy`), 1 ACNet<8> (`acnet8`),
+    // 2 ArtCNN<16> and 3 ArtCNN<32> (`art`), -1 a family this library does not provide (its
+    // name in `other`).
+    struct ModelChoice
+    {
+        int family = 0;
+        model::ACNetLegacy::Variant legacy = model::ACNetLegacy::Variant::GAN;
+        model::ACNet<8>::Variant acnet8 = model::ACNet<8>::Variant::B8_NORMAL;
+        model::ArtCNN<16>::Variant art = model::ArtCNN<16>::Variant::C4_NORMAL;
+        std::string other;
+    };
+    ModelChoice findModel(const char* model) noexcept;
     int ceilLog2(double v) noexcept;
     const char* typeNameOf(int type) noexcept;
 
@@ -26,4 +34,7 @@ namespace ac::core::detail
     std::shared_ptr<Processor> createHIP(int device, const model::ACNetLegacy& model, int precision);
     // HIP processor for ACNet<8> (FP32; precision must be 0)
     std::shared_ptr<Processor> createHIP8(int device, const model::ACNet<8>& model, int precision);
+    // HIP processor for ArtCNN<16> / ArtCNN<32> (FP32; precision must be 0)
+    template<int F>
+    std::shared_ptr<Processor> createHIPArt(int device, const model::ArtCNN<F>& model, int precision);
 }

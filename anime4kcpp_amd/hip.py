@@ -40,6 +40,20 @@ def load_weights8(variant):
     return arrs[0], arrs[1], arrs[2], blocks
 
 
+# ArtCNN variants (core/include/AC/Core/Model/ArtCNN.hpp:20-23): channels x {normal, dn, ds}, 4 blocks
+VARIANTS_ART = tuple(f"f{f}{s}" for f in (16, 32) for s in ("", "_dn", "_ds"))
+
+
+def load_weights_artcnn(variant):
+    """(kernels, biases, channels, blocks) float32 of an ArtCNN variant."""
+    v = variant.lower()
+    if v not in VARIANTS_ART:
+        raise ValueError(f"unknown ArtCNN variant {variant!r}")
+    k = np.ascontiguousarray(np.fromfile(os.path.join(WEIGHTS_DIR, f"artcnn_{v}.kernels.f32"), dtype="<f4"))
+    b = np.ascontiguousarray(np.fromfile(os.path.join(WEIGHTS_DIR, f"artcnn_{v}.biases.f32"), dtype="<f4"))
+    return k, b, int(v[1:3]), 4
+
+
 def elem_type(dtype):
     import torch
     table = {torch.uint8: U8, torch.uint16: U16, torch.float16: F16, torch.float32: F32}
@@ -211,6 +225,51 @@ class Net8:
                                         elem_type(src.dtype), w, h, ctypes.c_void_p(dst.data_ptr()),
                                         dst.stride(0) * dst.element_size(), elem_type(dst.dtype),
                                         ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(stream)), "achip_pass8")
+        return dst
+
+
+class NetArt(Net8):
+    """One ArtCNN<16|32> weight set resident on one HIP device (achip_net_artcnn_create /
+    achip_pass_net); same calls as Net8."""
+
+    def __init__(self, variant="f16", device=0):
+        k, b, f, blocks = load_weights_artcnn(variant)
+        h = ctypes.c_void_p()
+        check(_native.lib().achip_net_artcnn_create(device, f, blocks, k.ctypes.data, b.ctypes.data, ctypes.byref(h)),
+              "achip_net_artcnn_create")
+        self._h = h
+        self.variant = variant
+        self.channels = f
+        self.blocks = blocks
+        self.device = _native.lib().achip_net_device(h)
+        self._ws = None
+
+    def workspace(self, w, h, dev):
+        import torch
+        need = _native.lib().achip_net_workspace_bytes(self._h, w, h)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=dev)
+        return self._ws
+
+    def upscale(self, src, dst=None, stream=None, out_dtype=None):
+        """One ArtCNN 2x pass: src (H x W) -> dst (2H x 2W) on `stream` (default: torch's
+        current stream); the feature-map workspace is kept on the net between calls."""
+        import torch
+        if src.dim() != 2 or not src.is_cuda or src.stride(1) != 1:
+            raise ValueError("src must be a 2-D device tensor with contiguous rows")
+        h, w = src.shape
+        if dst is None:
+            dst = torch.empty((2 * h, 2 * w), dtype=out_dtype or src.dtype, device=src.device)
+        ws = self.workspace(w, h, src.device)
+        if stream is None:
+            stream = torch.cuda.current_stream(src.device).cuda_stream
+        elif hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        check(_native.lib().achip_pass_net(self._h, ctypes.c_void_p(src.data_ptr()), src.stride(0) * src.element_size(),
+                                           elem_type(src.dtype), w, h, ctypes.c_void_p(dst.data_ptr()),
+                                           dst.stride(0) * dst.element_size(), elem_type(dst.dtype),
+                                           ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(stream)),
+              "achip_pass_net")
         return dst
 
 

@@ -381,6 +381,40 @@ def acnet8(frames_np, steps=10):
     return out
 
 
+def artcnn(frames_np, steps=5):
+    """ArtCNN 1080p u8 -> 4K 2x passes, layer by layer on FP32 MFMA, for 16 and 32 channels.
+    FLOP per input pixel: 2 x (9 F + 9 F^2 (blocks + 1) + 36 F) MAC."""
+    import torch
+    from anime4kcpp_amd import hip
+    dev = torch.device("cuda:0")
+    srcs = [torch.from_numpy(f).to(dev) for f in frames_np[:4]]
+    h, w = frames_np[0].shape
+    dsts = [torch.empty((2 * h, 2 * w), dtype=torch.uint8, device=dev) for _ in srcs]
+    out = {}
+    for variant in ("f16", "f32"):
+        net = hip.NetArt(variant, 0)
+        for i in range(len(srcs)):
+            net.upscale(srcs[i], dsts[i])
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            for i in range(len(srcs)):
+                net.upscale(srcs[i], dsts[i])
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
+        f, b = net.channels, net.blocks
+        flop = 2 * (9 * f + 9 * f * f * (b + 1) + 36 * f) * w * h
+        out[variant] = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
+                        "channels": f, "blocks": b, "achieved_tflops": round(flop / sec / 1e12, 2),
+                        "frac_of_fp32_peak": round(flop / sec / 1e12 / PEAK_FP32_TFLOPS, 4),
+                        "hbm_bytes_per_frame": (4 * f * (2 * (b + 2) + 1) + 1 + 4) * w * h}
+        net.close()
+    out["workload"] = "1920x1080 u8 -> 3840x2160 u8, ArtCNN C4 weights, layer by layer (F-channel f32 maps in HBM)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -395,6 +429,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-stages", action="store_true", help="skip the colour / resize / PCIe stage measurement")
     ap.add_argument("--no-acnet8", action="store_true", help="skip the ACNet<8> measurement")
+    ap.add_argument("--no-artcnn", action="store_true", help="skip the ArtCNN measurement")
     ap.add_argument("--serving-loads", default="fp16:300,1200;fp32:60,240",
                     help="offered loads (requests/s) per precision")
     ap.add_argument("--serving-seconds", type=float, default=3.0)
@@ -403,7 +438,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(every image is independent; see the sharding note in DESIGN.md)", file=sys.stderr)
+              "(every image is independent: one process per GPU, each taking its own frames)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -503,6 +538,8 @@ def main():
         result["stages"] = stages()
     if not args.no_acnet8:
         result["acnet8"] = acnet8(frames_np)
+    if not args.no_artcnn:
+        result["artcnn"] = artcnn(frames_np)
     if not args.no_serving:
         result["serving"] = {}
         for spec in args.serving_loads.split(";"):

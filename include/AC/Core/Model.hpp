@@ -2,5 +2,6 @@
 #define AC_CORE_MODEL_HPP
 
 #include "AC/Core/Model/ACNet.hpp"
+#include "AC/Core/Model/ArtCNN.hpp"
 
 #endif

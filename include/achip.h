@@ -135,6 +135,22 @@ ACHIP_API int achip_pass8(ACHipNet* net, const void* src, long src_stride, int s
                           void* dst, long dst_stride, int dst_type, void* workspace, size_t workspace_bytes,
                           void* stream);
 
+/* ---- ArtCNN<F> ----
+ * core/include/AC/Core/Model/ArtCNN.hpp (F = 16 or 32; CPU pass CPUProcessor.cpp:1519-1602):
+ * conv3x3 1 -> F (identity) = feat, `blocks` x conv3x3 F -> F + ReLU, conv3x3 F -> F + feat,
+ * conv3x3 F -> 4 and pixel shuffle to 2x.  kernels F*9 + F*F*9*(blocks+1) + F*4*9, biases
+ * F + F*(blocks+1) + 4 floats in the reference's layout.  FP32; runs through achip_pass_net. */
+ACHIP_API int achip_net_artcnn_create(int device, int channels, int blocks, const float* kernels,
+                                      const float* biases, ACHipNet** out);
+
+/* ---- any net ----
+ * Device workspace one w x h pass of `net` needs (0 for ACNetLegacy), and that pass: an
+ * ACNetLegacy net runs achip_pass in FP32, the others their layer-by-layer forms. */
+ACHIP_API size_t achip_net_workspace_bytes(const ACHipNet* net, int w, int h);
+ACHIP_API int achip_pass_net(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h,
+                             void* dst, long dst_stride, int dst_type, void* workspace, size_t workspace_bytes,
+                             void* stream);
+
 /* ---- Device-memory colour and resize stages ----
  * The steps of Processor::process around the 2x passes (core/src/processor/Processor.cpp:
  * 207-213, 242-274) on device pointers of the current HIP device: the colour conversions of

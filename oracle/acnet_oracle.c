@@ -384,6 +384,141 @@ int orc_acnet8_pass(const void* src, long src_stride, int w, int h, int type, vo
     return 0;
 }
 
+/* ---- ArtCNN<F> (CPUProcessor.cpp:1519-1602) --------------------------------------- */
+
+/* conv_cin1<cout, 9> in the FMA order (AVX.hpp:93-124) and the Generic order */
+static void cin1_fma_n(const float* r, const float* k, const float* b, float* out, int cout)
+{
+    for (int n = 0; n < cout; n++) {
+        const float* kp = k + n * 9;
+        float s[8];
+        for (int p = 0; p < 8; p++) s[p] = fmaf(r[p], kp[p], 0.0f);
+        float sum = hsum8(s);
+        float t = r[8] * kp[8];
+        sum = sum + t;
+        out[n] = sum + b[n];
+    }
+}
+static void cin1_generic_n(const float* r, const float* k, const float* b, float* out, int cout)
+{
+    for (int n = 0; n < cout; n++) {
+        const float* kp = k + n * 9;
+        float sum = r[0] * kp[0];
+        for (int p = 1; p < 9; p++) { float t = r[p] * kp[p]; sum = sum + t; }
+        out[n] = sum + b[n];
+    }
+}
+/* conv<cin, cout, 9>: FMA order (AVX.hpp:32-59): an 8-lane accumulator, lane l taking
+ * channels l, l + 8, ... of every tap in tap order, then hsum added to the bias; Generic
+ * order (Generic.hpp:71-81): per channel the left-folded 9-tap sum, channels summed in order */
+static void conv_fma_cin(const float* const* rp, const float* k, const float* b, float* out, int cin, int cout)
+{
+    for (int n = 0; n < cout; n++) {
+        const float* kn = k + (long)n * cin * 9;
+        float s[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        for (int p = 0; p < 9; p++)
+            for (int idx = 0; idx < cin / 8; idx++)
+                for (int l = 0; l < 8; l++) s[l] = fmaf(rp[p][idx * 8 + l], kn[cin * p + idx * 8 + l], s[l]);
+        out[n] = b[n] + hsum8(s);
+    }
+}
+static void conv_generic_cin(const float* const* rp, const float* k, const float* b, float* out, int cin, int cout)
+{
+    for (int n = 0; n < cout; n++) {
+        const float* kn = k + (long)n * cin * 9;
+        float sum = 0.0f;
+        for (int c = 0; c < cin; c++) {
+            float bs = rp[0][c] * kn[c];
+            for (int p = 1; p < 9; p++) { float t = rp[p][c] * kn[cin * p + c]; bs = bs + t; }
+            sum = sum + bs;
+        }
+        out[n] = sum + b[n];
+    }
+}
+
+/* one conv3x3 layer over an F-channel map with the clamped taps of Common.hpp:120-143 */
+#define ORC_TAPS(MAP, F)                                                                   \
+    int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0, lp = j > 0 ? 1 : 0, rpp_ = j < w - 1 ? 1 : 0; \
+    int ys[3] = { i - tp, i, i + bp }, xs[3] = { j - lp, j, j + rpp_ };                    \
+    const float* rp9[9];                                                                   \
+    for (int a = 0; a < 3; a++)                                                            \
+        for (int c = 0; c < 3; c++) rp9[a * 3 + c] = (MAP) + ((size_t)ys[a] * w + xs[c]) * (F);
+
+int orc_artcnn_pass(const void* src, long src_stride, int w, int h, int type, void* dst, long dst_stride,
+                    const float* kernels, const float* biases, int f, int blocks, int order, int threads)
+{
+    if (!src || !dst || w <= 0 || h <= 0 || !kernels || !biases || (f != 16 && f != 32) || blocks < 1) return -22;
+    const size_t plane = (size_t)w * h * f;
+    float* feat = (float*)malloc(plane * sizeof(float));
+    float* t1 = (float*)malloc(plane * sizeof(float));
+    float* t2 = (float*)malloc(plane * sizeof(float));
+    if (!feat || !t1 || !t2) { free(feat); free(t1); free(t2); return -12; }
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+    const int fma_order = (order == ORC_ORDER_FMA);
+    const unsigned char* sb = (const unsigned char*)src;
+    const int es = elem_size(type);
+    /* head: conv3x3 1 -> F, identity (Backend.hpp:135-156) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+            int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0, lp = j > 0 ? 1 : 0, rq = j < w - 1 ? 1 : 0;
+            int ys[3] = { i - tp, i, i + bp }, xs[3] = { j - lp, j, j + rq };
+            float r[9];
+            for (int a = 0; a < 3; a++)
+                for (int c = 0; c < 3; c++) r[a * 3 + c] = orc_to_float(sb + ys[a] * src_stride + (long)xs[c] * es, type);
+            float* out = feat + ((size_t)i * w + j) * f;
+            if (fma_order) cin1_fma_n(r, kernels, biases, out, f); else cin1_generic_n(r, kernels, biases, out, f);
+        }
+    /* body: `blocks` x conv3x3 F -> F + ReLU; then F -> F identity + feat (Backend.hpp:157-160) */
+    const float* in = feat;
+    float* out = t1;
+    for (int l = 1; l <= blocks + 1; l++) {
+        const float* k = kernels + f * 9 + (size_t)f * f * 9 * (l - 1);
+        const float* b = biases + f * l;
+        const int add = l == blocks + 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++) {
+                ORC_TAPS(in, f)
+                float* o = out + ((size_t)i * w + j) * f;
+                if (fma_order) conv_fma_cin(rp9, k, b, o, f, f); else conv_generic_cin(rp9, k, b, o, f, f);
+                const float* fe = feat + ((size_t)i * w + j) * f;
+                for (int n = 0; n < f; n++) o[n] = add ? o[n] * 1.0f + fe[n] : (o[n] > 0.0f ? o[n] : 0.0f);
+            }
+        in = out;
+        out = out == t1 ? t2 : t1;
+    }
+    /* tail: conv3x3 F -> 4, identity, pixel shuffle (Backend.hpp:161-178, Common.hpp:290-341) */
+    {
+        const float* k = kernels + f * 9 + (size_t)f * f * 9 * (blocks + 1);
+        const float* b = biases + f * (blocks + 2);
+        unsigned char* db = (unsigned char*)dst;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++) {
+                ORC_TAPS(in, f)
+                float s4[4];
+                if (fma_order) conv_fma_cin(rp9, k, b, s4, f, 4); else conv_generic_cin(rp9, k, b, s4, f, 4);
+                for (int n = 0; n < 4; n++)
+                    orc_from_float(s4[n], db + (long)(2 * i + (n >> 1)) * dst_stride + (long)(2 * j + (n & 1)) * es, type);
+            }
+    }
+    free(feat);
+    free(t1);
+    free(t2);
+    return 0;
+}
+
 /* ---- FP64 restatement (error reference only) --------------------------------------- */
 
 int orc_acnet_pass_f64(const float* src, int w, int h, double* dst,

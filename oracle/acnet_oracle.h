@@ -56,6 +56,13 @@ int orc_acnet_pass(const void* src, long src_stride, int w, int h, int src_type,
 int orc_acnet8_pass(const void* src, long src_stride, int w, int h, int type, void* dst, long dst_stride,
                     const float* kernels, const float* biases, const float* alphas, int blocks, int order, int threads);
 
+/* ArtCNN<F> 2x pass, F = 16 or 32 (core/src/processor/cpu/CPUProcessor.cpp:1519-1602): head
+ * conv3x3 1->F, `blocks` x conv3x3 F->F + ReLU, conv3x3 F->F + head features, conv3x3 F->4 +
+ * pixel shuffle; src and dst of one element type.  order as orc_acnet_pass (FMA = the AVX/FMA
+ * kernels' order; AVX512 sums 16 lanes and may differ in the last bit). */
+int orc_artcnn_pass(const void* src, long src_stride, int w, int h, int type, void* dst, long dst_stride,
+                    const float* kernels, const float* biases, int f, int blocks, int order, int threads);
+
 /* The same network in double precision.  src is the toFloat'ed input
  * (w*h floats, dense); dst receives the saturated 2w x 2h output in double. */
 int orc_acnet_pass_f64(const float* src, int w, int h, double* dst,

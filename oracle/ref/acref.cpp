@@ -10,6 +10,10 @@ int acref_pass8_generic(const void*, long, int, int, int, void*, long, const flo
 int acref_pass8_fma(const void*, long, int, int, int, void*, long, const float*, const float*, const float*, int);
 int acref_pass8_avx512(const void*, long, int, int, int, void*, long, const float*, const float*, const float*, int);
 
+int acref_artcnn_generic(const void*, long, int, int, int, void*, long, const float*, const float*, int, int);
+int acref_artcnn_fma(const void*, long, int, int, int, void*, long, const float*, const float*, int, int);
+int acref_artcnn_avx512(const void*, long, int, int, int, void*, long, const float*, const float*, int, int);
+
 int acref_has_arch(int arch)
 {
     __builtin_cpu_init();
@@ -41,6 +45,18 @@ int acref_pass8(int arch, const void* src, long sstride, int w, int h, int type,
     case 1: return acref_pass8_generic(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, blocks);
     case 4: return acref_pass8_fma(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, blocks);
     case 5: return acref_pass8_avx512(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, blocks);
+    default: return -22;
+    }
+}
+
+int acref_artcnn(int arch, const void* src, long sstride, int w, int h, int type, void* dst, long dstride,
+                 const float* kernels, const float* biases, int f, int blocks)
+{
+    if (!acref_has_arch(arch)) return -95;
+    switch (arch) {
+    case 1: return acref_artcnn_generic(src, sstride, w, h, type, dst, dstride, kernels, biases, f, blocks);
+    case 4: return acref_artcnn_fma(src, sstride, w, h, type, dst, dstride, kernels, biases, f, blocks);
+    case 5: return acref_artcnn_avx512(src, sstride, w, h, type, dst, dstride, kernels, biases, f, blocks);
     default: return -22;
     }
 }

@@ -152,6 +152,94 @@ void a8_last(const float* in, int w, int h, const void* src, long sstride, void*
         }
     }
 }
+// ---- ArtCNN<F> (CPUProcessor.cpp:1519-1602) ----
+
+template <int F>
+struct ArtLayers {
+    static const float** taps(const float* in, int w, int i, int j, int h, const float** rptr)
+    {
+        int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0, lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+        auto px = [&](int y, int x) { return in + (static_cast<long>(y) * w + x) * F; };
+        const float* t[] = { px(i - tp, j - lp), px(i - tp, j), px(i - tp, j + rp), px(i, j - lp), px(i, j), px(i, j + rp),
+                             px(i + bp, j - lp), px(i + bp, j), px(i + bp, j + rp) };
+        for (int p = 0; p < 9; p++) rptr[p] = t[p];
+        return rptr;
+    }
+    template <typename IN>
+    static void head(const void* src, long sstride, int w, int h, float* out, const float* k, const float* b)
+    {
+#pragma omp parallel for schedule(guided)
+        for (int i = 0; i < h; i++) {
+            int tp = i > 0 ? 1 : 0, bp = i < h - 1 ? 1 : 0;
+            auto row = [&](int y) { return reinterpret_cast<const IN*>(static_cast<const std::uint8_t*>(src) + y * sstride); };
+            for (int j = 0; j < w; j++) {
+                int lp = j > 0 ? 1 : 0, rp = j < w - 1 ? 1 : 0;
+                alignas(64) const float r[] = {
+                    ac::core::toFloat(row(i - tp)[j - lp]), ac::core::toFloat(row(i - tp)[j]), ac::core::toFloat(row(i - tp)[j + rp]),
+                    ac::core::toFloat(row(i)[j - lp]),      ac::core::toFloat(row(i)[j]),      ac::core::toFloat(row(i)[j + rp]),
+                    ac::core::toFloat(row(i + bp)[j - lp]), ac::core::toFloat(row(i + bp)[j]), ac::core::toFloat(row(i + bp)[j + rp]),
+                };
+                Op::conv_cin1<F, 9>(r, out + (static_cast<long>(i) * w + j) * F, k, b);
+            }
+        }
+    }
+    static void body(const float* in, int w, int h, float* out, const float* k, const float* b, const float* feat)
+    {
+#pragma omp parallel for schedule(guided)
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++) {
+                const float* rptr[9];
+                alignas(64) float sum[F]{};
+                Op::conv<F, F, 9>(taps(in, w, i, j, h, rptr), sum, k, b);
+                float* o = out + (static_cast<long>(i) * w + j) * F;
+                const float* fe = feat ? feat + (static_cast<long>(i) * w + j) * F : nullptr;
+                for (int n = 0; n < F; n++) o[n] = fe ? sum[n] * 1.0f + fe[n] : std::max(sum[n], 0.0f);
+            }
+    }
+    template <typename OUT>
+    static void tail(const float* in, int w, int h, void* dst, long dstride, const float* k, const float* b)
+    {
+#pragma omp parallel for schedule(guided)
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++) {
+                const float* rptr[9];
+                alignas(64) float sum[4]{};
+                Op::conv<F, 4, 9>(taps(in, w, i, j, h, rptr), sum, k, b);
+                for (int n = 0; n < 4; n++) {
+                    auto o = reinterpret_cast<OUT*>(static_cast<std::uint8_t*>(dst) + (2 * i + (n >> 1)) * dstride) + 2 * j + (n & 1);
+                    *o = ac::core::fromFloat<OUT>(sum[n]);
+                }
+            }
+    }
+    static int run(const void* src, long sstride, int w, int h, int type, void* dst, long dstride, const float* kernels,
+                   const float* biases, int blocks)
+    {
+        std::vector<float> feat(static_cast<size_t>(w) * h * F), t1(feat.size()), t2(feat.size());
+        switch (type) {
+        case 0x001: head<std::uint8_t>(src, sstride, w, h, feat.data(), kernels, biases); break;
+        case 0x002: head<std::uint16_t>(src, sstride, w, h, feat.data(), kernels, biases); break;
+        case 0x204: head<float>(src, sstride, w, h, feat.data(), kernels, biases); break;
+        default: return -22;
+        }
+        const float* in = feat.data();
+        float* out = t1.data();
+        for (int l = 1; l <= blocks + 1; l++) {
+            body(in, w, h, out, kernels + F * 9 + static_cast<size_t>(F) * F * 9 * (l - 1), biases + F * l,
+                 l == blocks + 1 ? feat.data() : nullptr);
+            in = out;
+            out = out == t1.data() ? t2.data() : t1.data();
+        }
+        const float* kt = kernels + F * 9 + static_cast<size_t>(F) * F * 9 * (blocks + 1);
+        const float* bt = biases + F * (blocks + 2);
+        switch (type) {
+        case 0x001: tail<std::uint8_t>(in, w, h, dst, dstride, kt, bt); break;
+        case 0x002: tail<std::uint16_t>(in, w, h, dst, dstride, kt, bt); break;
+        case 0x204: tail<float>(in, w, h, dst, dstride, kt, bt); break;
+        default: return -22;
+        }
+        return 0;
+    }
+};
 } // namespace
 
 #define CAT2(a, b) a##b
@@ -215,4 +303,13 @@ extern "C" int CAT(acref_pass8_, BACKEND_SUFFIX)(const void* src, long sstride, 
     default: return -22;
     }
     return 0;
+}
+
+extern "C" int CAT(acref_artcnn_, BACKEND_SUFFIX)(const void* src, long sstride, int w, int h, int type, void* dst,
+                                                  long dstride, const float* kernels, const float* biases, int f,
+                                                  int blocks)
+{
+    if (f == 16) return ArtLayers<16>::run(src, sstride, w, h, type, dst, dstride, kernels, biases, blocks);
+    if (f == 32) return ArtLayers<32>::run(src, sstride, w, h, type, dst, dstride, kernels, biases, blocks);
+    return -22;
 }

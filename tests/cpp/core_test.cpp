@@ -90,13 +90,30 @@ TEST_CASE(model_acnet_legacy)
     CHECK(std::fabs(M{ M::Variant::GAN }.kernel(0)[0] - 0.0441616f) < 1e-9f);
 }
 
+TEST_CASE(artcnn_layer_algebra)
+{
+    // ArtCNN.hpp:28-55: head F*9, blockNum + 1 F->F layers, tail F*4*9
+    using A16 = ac::core::model::ArtCNN<16>;
+    using A32 = ac::core::model::ArtCNN<32>;
+    A16 m{ A16::Variant::C4_DS };
+    CHECK(m.blocks() == 4 && m.kernels() == 7 && m.biases() == 7);
+    CHECK(m.kernelLength() == 144 + 2304 * 5 + 576 && m.biasLength() == 16 + 80 + 4);
+    CHECK(m.kernelOffset(1) == 144 && m.kernelOffset(6) == 144 + 2304 * 5 && m.kernelLength(6) == 576);
+    CHECK(m.biasOffset(6) == 96 && m.biasLength(6) == 4 && m.biasLength(5) == 16);
+    A32 n{ A32::Variant::C4_DN };
+    CHECK(n.kernelLength() == 288 + 9216 * 5 + 1152 && n.biasLength() == 32 + 160 + 4);
+    CHECK(A16{ A16::Variant::C4_NORMAL }.kernel(0) != m.kernel(0) && A16{ A16::Variant::C4_DN }.kernel(0) != m.kernel(0));
+}
+
 TEST_CASE(processor_factory_without_gpu)
 {
     auto cpu = ac::core::Processor::create("cpu", 0, "acnet-legacy-hdn0");
     CHECK(static_cast<bool>(cpu) && !cpu->ok() && cpu->type() == ac::core::Processor::CPU);
     CHECK(std::strstr(cpu->error(), "HIP processor only") != nullptr);
-    auto fam = ac::core::Processor::create("hip", 0, "artcnn-c4f16");
-    CHECK(!fam->ok() && std::strstr(fam->error(), "ArtCNN") != nullptr);
+    auto fam = ac::core::Processor::create("hip", 0, "fsrcnnx-f8");
+    CHECK(!fam->ok() && std::strstr(fam->error(), "FSRCNNX") != nullptr);
+    auto art = ac::core::Processor::create("hip", 0, "artcnn-c4f16");   // a provided family: fails on the device only
+    CHECK(!art->ok() && std::strstr(art->error(), "no HIP device") != nullptr);
     auto p = ac::core::Processor::create("HIP", 0, "acnet-legacy-hdn2");
     CHECK(static_cast<bool>(p) && p->type() == ac::core::Processor::HIP && std::strcmp(p->typeName(), "HIP") == 0);
     auto a = ac::core::Processor::create("auto", -1, nullptr);

@@ -48,6 +48,8 @@ def orc():
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int]
         o.orc_acnet8_pass.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, P, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int]
+        o.orc_artcnn_pass.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int]
         o.orc_filter_weight.argtypes = [ctypes.c_int, ctypes.c_float]
         o.orc_filter_weight.restype = ctypes.c_float
         o.orc_filter_support.argtypes = [ctypes.c_int]
@@ -66,6 +68,8 @@ def ref():
         r = ctypes.CDLL(REF_SO)
         r.acref_pass.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int, P, P]
         r.acref_has_arch.argtypes = [ctypes.c_int]
+        r.acref_artcnn.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P,
+                                   ctypes.c_int, ctypes.c_int]
         r.acref_pass8.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, P, ctypes.c_int]
         _ref = r
     return _ref
@@ -147,5 +151,27 @@ def ref_pass8(arch, src, kernels, biases, alphas, blocks):
     k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases, alphas))
     rc = ref().acref_pass8(arch, src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
                            dst.strides[0], k.ctypes.data, b.ctypes.data, a.ctypes.data, blocks)
+    assert rc == 0, rc
+    return dst
+
+
+def oracle_artcnn(src, kernels, biases, f, blocks=4, order=ORDER_FMA, threads=0):
+    """ArtCNN<f> 2x pass of a 2-D array (u8/u16/f32); output of the same type."""
+    src = np.ascontiguousarray(src)
+    h, w = src.shape
+    dst = np.zeros((2 * h, 2 * w), src.dtype)
+    k, b = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases))
+    assert orc().orc_artcnn_pass(src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
+                                 dst.strides[0], k.ctypes.data, b.ctypes.data, f, blocks, order, threads) == 0
+    return dst
+
+
+def ref_artcnn(arch, src, kernels, biases, f, blocks=4):
+    src = np.ascontiguousarray(src)
+    h, w = src.shape
+    dst = np.zeros((2 * h, 2 * w), src.dtype)
+    k, b = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases))
+    rc = ref().acref_artcnn(arch, src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
+                            dst.strides[0], k.ctypes.data, b.ctypes.data, f, blocks)
     assert rc == 0, rc
     return dst
