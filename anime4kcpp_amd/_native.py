@@ -66,6 +66,7 @@ class ACProcessor(ctypes.Structure):
 # name -> (restype, argtypes); the C-ABI surface this package relies on
 SIGNATURES = {
     "achip_net_artcnn_create": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
+    "achip_net_fsrcnnx_create": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
     "achip_net_workspace_bytes": (ctypes.c_size_t, [c_void_p, c_int, c_int]),
     "achip_pass_net": (c_int, [c_void_p, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
                                ctypes.c_size_t, c_void_p]),

@@ -351,26 +351,13 @@ void achip_net_destroy(ACHipNet* net)
         DeviceGuard g(net->device);
         if (net->wts) (void)hipFree(net->wts);
         if (net->a8wts) (void)hipFree(net->a8wts);
-        if (net->artk) (void)hipFree(net->artk);
+        if (net->lk) (void)hipFree(net->lk);
     }
     delete net;
 }
 
 // ---- ACNet<8> ----
 
-namespace {
-// FP32 MFMA A operands of one conv3x3 layer with cout output channels (8 or 4; rows of the
-// missing channels zero), kernels k[n * 72 + (3 ky + kx) * 8 + c]: as mfma32_operands
-void layer32_operands(const float* k, int cout, float* al)
-{
-    for (int t = 0; t < 3; t++)
-        for (int c = 0; c < 8; c++)
-            for (int lane = 0; lane < 64; lane++) {
-                const int i = lane % 16, g = lane / 16, n = i % 8, ky = g - i / 8;
-                al[(t * 8 + c) * 64 + lane] = (n < cout && ky >= 0 && ky <= 2) ? k[n * 72 + (ky * 3 + t) * 8 + c] : 0.0f;
-            }
-}
-} // namespace
 
 int achip_net8_create(int device, int blocks, const float* kernels, const float* biases, const float* alphas, ACHipNet** out)
 {
@@ -396,7 +383,7 @@ int achip_net8_create(int device, int blocks, const float* kernels, const float*
         }
     std::vector<float> wa(static_cast<size_t>(blocks + 1) * achip::kMfma32LayerFloats);
     for (int l = 1; l <= blocks + 1; l++)
-        layer32_operands(kernels + 72 + 576 * (l - 1), l <= blocks ? 8 : 4, wa.data() + (l - 1) * achip::kMfma32LayerFloats);
+        achip::pack_conv8_operands(kernels + 72 + 576 * (l - 1), l <= blocks ? 8 : 4, wa.data() + (l - 1) * achip::kMfma32LayerFloats);
 
     DeviceGuard g(device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
@@ -419,7 +406,60 @@ int achip_net8_create(int device, int blocks, const float* kernels, const float*
     return ACHIP_OK;
 }
 
-int achip_net_blocks(const ACHipNet* net) { return net ? (net->artf ? net->artblocks : net->blocks) : ACHIP_EINVAL; }
+int achip_net_blocks(const ACHipNet* net) { return net ? (net->lfamily ? net->lblocks : net->blocks) : ACHIP_EINVAL; }
+
+// ---- ArtCNN<F> ----
+
+namespace {
+// Uploads a layered net (ArtCNN, FSRCNNX): kernels, biases, slopes (may be empty), the
+// conv3x3 A operands and the conv1x1 ones (may be empty), each 256-byte aligned, in one
+// allocation.
+int upload_layered(int device, int family, int f, int blocks, const float* kernels, size_t klen, const float* biases,
+                   size_t blen, const float* alphas, size_t alen, const std::vector<float>& wa,
+                   const std::vector<float>& w1, ACHipNet** out)
+{
+    auto up = [](size_t floats) { return (floats * sizeof(float) + 255) / 256 * 256; };
+    const size_t offb = up(klen), offa = offb + up(blen), offwa = offa + up(alen), offw1 = offwa + up(wa.size());
+    DeviceGuard g(device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    char* d = nullptr;
+    hipError_t e = hipMalloc(&d, offw1 + up(w1.size()));
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(weights)");
+    e = hipMemcpy(d, kernels, klen * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + offb, biases, blen * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && alen) e = hipMemcpy(d + offa, alphas, alen * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + offwa, wa.data(), wa.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !w1.empty()) e = hipMemcpy(d + offw1, w1.data(), w1.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "hipMemcpy(weights)");
+    }
+    auto* net = new ACHipNet{ device, nullptr, nullptr, nullptr, nullptr };
+    net->lfamily = family;
+    net->lf = f;
+    net->lblocks = blocks;
+    net->lk = reinterpret_cast<float*>(d);
+    net->lb = reinterpret_cast<float*>(d + offb);
+    net->la = alen ? reinterpret_cast<float*>(d + offa) : nullptr;
+    net->lwa = reinterpret_cast<float*>(d + offwa);
+    net->lw1 = w1.empty() ? nullptr : reinterpret_cast<float*>(d + offw1);
+    *out = net;
+    g_error.clear();
+    return ACHIP_OK;
+}
+
+// conv3x3 A operands of layers F -> F ... F -> 4 laid end to end (cout of each given)
+std::vector<float> pack_layers(int f, const float* k0, size_t kstride, const std::vector<int>& couts)
+{
+    const size_t wl = achip::conv_operand_floats(f);
+    std::vector<float> wa(wl * couts.size(), 0.0f);
+    for (size_t i = 0; i < couts.size(); i++) {
+        if (f == 8) achip::pack_conv8_operands(k0 + kstride * i, couts[i], wa.data() + wl * i);
+        else achip::pack_convf_operands(k0 + kstride * i, f, couts[i], wa.data() + wl * i);
+    }
+    return wa;
+}
+} // namespace
 
 // ---- ArtCNN<F> ----
 
@@ -433,55 +473,52 @@ int achip_net_artcnn_create(int device, int channels, int blocks, const float* k
     const int n = achip_device_count();
     if (n <= 0) return fail(ACHIP_ENODEV, "no HIP device");
     if (device < 0 || device >= n) device = 0;
-    const int F = channels, KQ = F / 4, MT = F / 16;
+    const int F = channels;
     const size_t klen = static_cast<size_t>(F) * 9 + static_cast<size_t>(F) * F * 9 * (blocks + 1) + F * 4 * 9;   // ArtCNN.hpp:28
     const size_t blen = static_cast<size_t>(F) + static_cast<size_t>(F) * (blocks + 1) + 4;
-    const size_t wl = 9 * 64 * static_cast<size_t>(KQ * MT);
-    // A operands of the body, residual and tail layers: [tap][lane][q * MT + h] holds the
-    // weight of output channel 16 h + lane % 16 and input channel KQ * (lane / 16) + q
-    std::vector<float> wa(wl * (blocks + 2), 0.0f);
-    for (int l = 1; l <= blocks + 2; l++) {
-        const float* k = kernels + F * 9 + static_cast<size_t>(F) * F * 9 * (l - 1);
-        const int cout = l == blocks + 2 ? 4 : F;
-        float* al = wa.data() + wl * (l - 1);
-        for (int p = 0; p < 9; p++)
-            for (int lane = 0; lane < 64; lane++)
-                for (int q = 0; q < KQ; q++)
-                    for (int h = 0; h < MT; h++) {
-                        const int o = 16 * h + lane % 16, c = KQ * (lane / 16) + q;
-                        al[(static_cast<size_t>(p) * 64 + lane) * (KQ * MT) + q * MT + h] =
-                            o < cout ? k[static_cast<size_t>(o) * 9 * F + p * F + c] : 0.0f;
-                    }
-    }
-    DeviceGuard g(device);
-    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
-    const size_t offb = (klen * sizeof(float) + 255) / 256 * 256;
-    const size_t offa = offb + (blen * sizeof(float) + 255) / 256 * 256;
-    char* d = nullptr;
-    hipError_t e = hipMalloc(&d, offa + wa.size() * sizeof(float));
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(weights)");
-    e = hipMemcpy(d, kernels, klen * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d + offb, biases, blen * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d + offa, wa.data(), wa.size() * sizeof(float), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void)hipFree(d);
-        return hip_fail(e, "hipMemcpy(weights)");
-    }
-    auto* net = new ACHipNet{ device, nullptr, nullptr, nullptr, nullptr };
-    net->artf = F;
-    net->artblocks = blocks;
-    net->artk = reinterpret_cast<float*>(d);
-    net->artb = reinterpret_cast<float*>(d + offb);
-    net->artwa = reinterpret_cast<float*>(d + offa);
-    *out = net;
-    g_error.clear();
-    return ACHIP_OK;
+    // the B body layers and the residual layer (F -> F), then the tail (F -> 4)
+    std::vector<int> couts(blocks + 1, F);
+    couts.push_back(4);
+    const std::vector<float> wa = pack_layers(F, kernels + F * 9, static_cast<size_t>(F) * F * 9, couts);
+    return upload_layered(device, kNetArtCNN, F, blocks, kernels, klen, biases, blen, nullptr, 0, wa, {}, out);
+}
+
+// ---- FSRCNNX<F> ----
+
+int achip_net_fsrcnnx_create(int device, int channels, int blocks, const float* kernels, const float* biases,
+                             const float* alphas, ACHipNet** out)
+{
+    if (!out) return fail(ACHIP_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!kernels || !biases || !alphas) return fail(ACHIP_EINVAL, "NULL weights");
+    if (channels != 8 && channels != 16) return fail(ACHIP_EINVAL, "FSRCNNX has 8 or 16 channels, not %d", channels);
+    if (blocks < 2 || blocks > 64) return fail(ACHIP_EINVAL, "blocks %d out of range", blocks);
+    const int n = achip_device_count();
+    if (n <= 0) return fail(ACHIP_ENODEV, "no HIP device");
+    if (device < 0 || device >= n) device = 0;
+    const int F = channels;
+    const size_t k3 = static_cast<size_t>(F) * F * 9;
+    const size_t klen = static_cast<size_t>(F) * 25 + k3 * blocks + F * F + F * 4 * 9;   // FSRCNNX.hpp:29
+    const size_t blen = static_cast<size_t>(F) + static_cast<size_t>(F) * (blocks + 1) + 4;
+    const size_t alen = static_cast<size_t>(F) * (blocks + 1);
+    // the B conv3x3 layers (F -> F), then the tail (F -> 4), which follows the conv1x1
+    std::vector<float> wa = pack_layers(F, kernels + F * 25, k3, std::vector<int>(blocks, F));
+    const float* k1 = kernels + F * 25 + k3 * blocks;
+    const std::vector<float> wt = pack_layers(F, k1 + F * F, 0, { 4 });
+    wa.insert(wa.end(), wt.begin(), wt.end());
+    std::vector<float> w1(achip::conv_1x1_floats(F));
+    if (F == 8) achip::pack_conv8_1x1(k1, w1.data());
+    else achip::pack_convf_1x1(k1, F, w1.data());
+    return upload_layered(device, kNetFSRCNNX, F, blocks, kernels, klen, biases, blen, alphas, alen, wa, w1, out);
 }
 
 size_t achip_net_workspace_bytes(const ACHipNet* net, int w, int h)
 {
     if (!net || w <= 0 || h <= 0) return 0;
-    if (net->artf) return 3 * static_cast<size_t>(achip::artcnn_feature_pitch(net->artf, w)) * static_cast<size_t>(h);
+    if (net->lfamily) {
+        const long fp = net->lfamily == kNetArtCNN ? achip::artcnn_feature_pitch(net->lf, w) : achip::fsrcnnx_feature_pitch(net->lf, w);
+        return 3 * static_cast<size_t>(fp) * static_cast<size_t>(h);
+    }
     if (net->blocks) return achip_pass8_workspace_bytes(w, h);
     return 0;
 }
@@ -490,7 +527,7 @@ int achip_pass_net(ACHipNet* net, const void* src, long src_stride, int src_type
                    int dst_type, void* workspace, size_t workspace_bytes, void* stream)
 {
     if (!net) return fail(ACHIP_EINVAL, "net is NULL");
-    if (!net->artf && !net->blocks)
+    if (!net->lfamily && !net->blocks)
         return achip_pass(net, ACHIP_FP32, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, stream);
     if (net->blocks)
         return achip_pass8(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, workspace, workspace_bytes, stream);
@@ -511,9 +548,13 @@ int achip_pass_net(ACHipNet* net, const void* src, long src_stride, int src_type
         return fail(ACHIP_EINVAL, "the workspace overlaps src or dst");
     DeviceGuard g(net->device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
-    const hipError_t e = achip::launch_artcnn(net->artf, net->artblocks, src_type, dst_type, src, src_stride, w, h, dst,
-                                              dst_stride, net->artk, net->artb, net->artwa, workspace,
-                                              static_cast<hipStream_t>(stream));
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const hipError_t e =
+        net->lfamily == kNetArtCNN
+            ? achip::launch_artcnn(net->lf, net->lblocks, src_type, dst_type, src, src_stride, w, h, dst, dst_stride, net->lk,
+                                   net->lb, net->lwa, workspace, st)
+            : achip::launch_fsrcnnx(net->lf, net->lblocks, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
+                                    net->lk, net->lb, net->la, net->lwa, net->lw1, workspace, st);
     return e == hipSuccess ? ACHIP_OK : hip_fail(e, "kernel launch");
 }
 
@@ -597,7 +638,7 @@ int achip_pass_config(ACHipNet* net, int precision, int config,
     }
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
     if (net && net->blocks) return fail(ACHIP_EINVAL, "an ACNet<8> net runs through achip_pass8");
-    if (net && net->artf) return fail(ACHIP_EINVAL, "an ArtCNN net runs through achip_pass_net");
+    if (net && net->lfamily) return fail(ACHIP_EINVAL, "an ArtCNN or FSRCNNX net runs through achip_pass_net");
     int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
     if (r) return r;
     DeviceGuard g(net->device);
@@ -774,7 +815,7 @@ int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int
     if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
     if (!images || n <= 0) return fail(ACHIP_EINVAL, "empty batch");
     if (!net) return fail(ACHIP_EINVAL, "net is NULL");
-    if (net->blocks || net->artf) return fail(ACHIP_EINVAL, "batched passes take an ACNetLegacy net");
+    if (net->blocks || net->lfamily) return fail(ACHIP_EINVAL, "batched passes take an ACNetLegacy net");
     const size_t need = achip_batch_workspace_bytes(n);
     if (!workspace || workspace_bytes < need)
         return fail(ACHIP_EINVAL, "workspace of %zu bytes, the batch needs %zu", workspace_bytes, need);

@@ -16,12 +16,20 @@ struct ACHipNet {
     int blocks = 0;          // ACNet<8>: its block count (the fields above are then unused)
     float* a8wts = nullptr;  // ACNet<8> blob, acnet_device.hpp layout (one allocation with a8wa)
     float* a8wa = nullptr;   // ACNet<8> A operands: blocks 1..B, then the final layer
-    int artf = 0;            // ArtCNN<F>: F (16 or 32), with artblocks body blocks
-    int artblocks = 0;
-    float* artk = nullptr;   // ArtCNN kernels and biases (reference layout) and A operands, one allocation
-    float* artb = nullptr;
-    float* artwa = nullptr;
+    // ArtCNN<F> and FSRCNNX<F>: family, F channels, B blocks; the reference-layout kernels,
+    // biases and slopes and the per-lane A operands of the conv3x3 layers and the conv1x1,
+    // one allocation starting at lk
+    int lfamily = 0;         // kNetArtCNN or kNetFSRCNNX; 0 for the nets above
+    int lf = 0;
+    int lblocks = 0;
+    float* lk = nullptr;
+    float* lb = nullptr;
+    float* la = nullptr;
+    float* lwa = nullptr;
+    float* lw1 = nullptr;
 };
+
+constexpr int kNetArtCNN = 1, kNetFSRCNNX = 2;
 
 namespace achip {
 

@@ -638,7 +638,7 @@ int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipSer
 {
     if (!net || !out) return ACHIP_EINVAL;
     *out = nullptr;
-    if (net->blocks || net->artf) return ACHIP_EINVAL;   // the serving loop runs ACNetLegacy nets
+    if (net->blocks || net->lfamily) return ACHIP_EINVAL;   // the serving loop runs ACNetLegacy nets
     ACHipServerConfig c = config ? *config : ACHipServerConfig{};
     if (c.precision != ACHIP_FP32 && c.precision != ACHIP_FP16 && c.precision != ACHIP_BF16) return ACHIP_EINVAL;
     if (c.hbm_budget == 0) c.hbm_budget = static_cast<size_t>(16) << 30;

@@ -94,6 +94,28 @@ hipError_t launch_artcnn(int f, int blocks, int stype, int dtype, const void* sr
                          long ds, const float* kernels, const float* biases, const float* wa, void* workspace,
                          hipStream_t stream);
 
+// FSRCNNX<F> 2x pass, F = 8 or 16 (fsrcnnx.hip): kernels/biases/alphas in the reference
+// layout (device copies); wa the per-lane A operands of the B conv3x3 layers and the F -> 4
+// tail (pack_conv8_operands for F = 8, pack_convf_operands for F = 16, each layer
+// conv_operand_floats(F) floats), w1 those of the conv1x1 (pack_conv8_1x1 / pack_convf_1x1).
+// Workspace: 3 * fsrcnnx_feature_pitch(F, W) * H bytes (head features and two maps).
+long fsrcnnx_feature_pitch(int f, int w);
+hipError_t launch_fsrcnnx(int f, int blocks, int stype, int dtype, const void* src, long ss, int W, int H, void* dst,
+                          long ds, const float* kernels, const float* biases, const float* alphas, const float* wa,
+                          const float* w1, void* workspace, hipStream_t stream);
+
+// Host-side packing of the per-lane A operands the layer kernels (conv_layers.hpp) read.
+// F = 8 (2-row items), conv3x3 8 -> cout (8 or 4): [t * 8 + c][lane] for patch column t,
+// channel c; conv1x1 8 -> 8: [r][lane], block-diagonal over the item's two output rows.
+// F = 16 or 32 (channel tiles), conv3x3 F -> cout (F or 4): [tap][lane][q * F/16 + h];
+// conv1x1 F -> F: [4 h + r][lane][h2].  k in the reference layout k[n][tap][c].
+void pack_conv8_operands(const float* k, int cout, float* out);
+void pack_conv8_1x1(const float* k1, float* out);
+void pack_convf_operands(const float* k, int f, int cout, float* out);
+void pack_convf_1x1(const float* k1, int f, float* out);
+inline size_t conv_operand_floats(int f) { return f == 8 ? 24 * 64 : 9 * 64 * static_cast<size_t>(f / 4) * (f / 16); }
+inline size_t conv_1x1_floats(int f) { return f == 8 ? 4 * 64 : static_cast<size_t>(f / 4) * 64 * (f / 16); }
+
 // ---- colour conversion and resize (acnet_color.hip) ----
 
 // One colour component of an image in device memory: component (x, y) is the element at

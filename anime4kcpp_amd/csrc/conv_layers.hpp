@@ -1,0 +1,333 @@
+// The FP32 MFMA conv3x3 layer kernels of the layer-by-layer networks (ACNet<8>, ArtCNN<F>,
+// FSRCNNX<F>): one launch per layer, reading an F-channel f32 map from HBM (channel quads
+// interleaved per pixel: pixel x at float4 index F/4 * x + q, rows of `fp` bytes), and
+// writing the next map or, for the last layer, the 2x output image.  The layer's epilogue is
+// chosen at compile time (LayerMode); every form keeps the reference's separately rounded
+// steps after the conv sum (Common.hpp:116-164, 227-288, 290-341):
+//   kRelu          max(v, 0)                                        ArtCNN body
+//   kPrelu         max(v, 0) + a * min(v, 0)                        ACNet<8>, FSRCNNX body
+//   kAddFeat       v * 1 + feat                                     ArtCNN's last F -> F
+//   kPreluConv1x1  PReLU, then conv1x1 F -> F + bias, * 1 + feat, PReLU
+//                                                                   FSRCNNX's last block
+//   kShuffle       conv F -> 4, pixel shuffle to 2x                 ArtCNN, FSRCNNX tail
+//   kShuffleSrc    the same, + the source pixel                     ACNet<8> final
+// Two MFMA mappings (v_mfma_f32_16x16x4_f32):
+//   conv8_layer  F = 8: the 2-row item of acnet_mfma32_item.hpp, 16 pixels x 2 output rows
+//                per item, all 16 MFMA rows useful;
+//   convf_layer  F = 16 or 32: M = 16 output channels per tile (F/16 tiles), N = 16 pixels of
+//                one row, K = 9 taps x F; lane group g supplies input channels g*F/4 ..
+//                g*F/4 + F/4 - 1 from channel-quad LDS planes; 4 items per wave batch so a
+//                tap's A operands are loaded once for 4.
+// In both, a lane's D registers hold 4 consecutive channels of one pixel, which is exactly
+// the B operand a conv1x1 MFMA needs (K step r of lane group g = the r-th of them), so the
+// fused conv1x1 costs F/4 x F/16 more MFMAs and no data movement.
+#pragma once
+
+#include "acnet_device.hpp"
+#include "acnet_mfma32_item.hpp"
+
+namespace achip {
+
+#pragma clang fp contract(off)
+
+enum LayerMode : int { kRelu = 0, kPrelu = 1, kAddFeat = 2, kPreluConv1x1 = 3, kShuffle = 4, kShuffleSrc = 5 };
+
+// Arguments of one layer launch; a mode leaves the fields it does not read null.
+struct LayerArgs {
+    const float* wa;        // this layer's per-lane A operands
+    const float* bias;      // F biases (4 for the shuffle modes)
+    const float* alpha;     // F PReLU slopes
+    const float* w1;        // conv1x1 A operands
+    const float* b1;        // conv1x1 biases (F)
+    const float* alpha1;    // F PReLU slopes after the skip connection
+    const float4* feat;     // skip-connection map, same pitch as the input map
+    const void* src;        // source image (kShuffleSrc)
+    long ss;
+    void* dst;              // output image (shuffle modes)
+    long ds;
+};
+
+// PReLU (Common.hpp:42-49): max(v, 0) + alpha * min(v, 0), each step rounded
+__device__ inline float prelu(float v, float alpha)
+{
+    return __fadd_rn(fmaxf(v, 0.0f), __fmul_rn(alpha, fminf(v, 0.0f)));
+}
+
+__device__ inline float add_skip(float v, float f) { return __fadd_rn(__fmul_rn(v, 1.0f), f); }
+
+// ---- F = 8: the 2-row item ----
+// Tile: 64 x 16 output pixels, 256 threads (4 waves); the input region (66 x 18 pixels with
+// the one-pixel halo) staged in LDS as two float4 planes (channels 0-3, 4-7) with a row pitch
+// of 80 pixels (a multiple of 16: the 16-byte item reads run at full rate).  Items: 4
+// segments of 16 pixels x 8 row pairs = 32 per tile, 8 per wave.
+constexpr int kC8TW = 64, kC8TH = 16, kC8PW = 80, kC8PH = kC8TH + 2;
+
+template <int MODE, typename TIN, typename TOUT>
+__global__ __launch_bounds__(256) void conv8_layer(const float4* __restrict__ fin, long fp, int W, int H,
+                                                   float4* __restrict__ fout, const LayerArgs la)
+{
+    static_assert(MODE == kPrelu || MODE == kPreluConv1x1 || MODE == kShuffle || MODE == kShuffleSrc,
+                  "conv8_layer epilogues");
+    constexpr bool SHUFFLE = MODE == kShuffle || MODE == kShuffleSrc;
+    __shared__ float4 pa[kC8PW * kC8PH];
+    __shared__ float4 pb[kC8PW * kC8PH];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int x0 = tx * kC8TW, y0 = ty * kC8TH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
+
+    // stage rows y0-1 .. y0+16, columns x0-1 .. x0+64 (replicated at the image edges)
+    for (int e = tid; e < kC8PH * (kC8TW + 2); e += 256) {
+        const int r = e / (kC8TW + 2), c = e - r * (kC8TW + 2);
+        const int sy = clampi(y0 - 1 + r, 0, H - 1), sx = clampi(x0 - 1 + c, 0, W - 1);
+        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + 2 * sx;
+        pa[r * kC8PW + c] = px[0];
+        pb[r * kC8PW + c] = px[1];
+    }
+    // this layer's A operands, bias and slopes; lane (g, j) owns channels 4 half .. + 3
+    float a[24];
+#pragma unroll
+    for (int s = 0; s < 24; s++) a[s] = la.wa[s * 64 + lane];
+    f32x4m bias, bias1 = { 0.f, 0.f, 0.f, 0.f };
+    float alpha[4] = { 0.f, 0.f, 0.f, 0.f }, alpha1[4] = { 0.f, 0.f, 0.f, 0.f }, w1[4] = { 0.f, 0.f, 0.f, 0.f };
+    if constexpr (SHUFFLE) {
+        bias = half ? (f32x4m){ 0.f, 0.f, 0.f, 0.f } : (f32x4m){ la.bias[0], la.bias[1], la.bias[2], la.bias[3] };
+    } else {
+        const float* bl = la.bias + 4 * half;
+        bias = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
+#pragma unroll
+        for (int r = 0; r < 4; r++) alpha[r] = la.alpha[4 * half + r];
+    }
+    if constexpr (MODE == kPreluConv1x1) {
+        const float* bl = la.b1 + 4 * half;
+        bias1 = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            alpha1[r] = la.alpha1[4 * half + r];
+            w1[r] = la.w1[r * 64 + lane];
+        }
+    }
+    __syncthreads();
+
+    const int gx1 = min(x0 + kC8TW, W), gy1 = min(y0 + kC8TH, H);
+    const int nseg = (gx1 - x0 + 15) >> 4;
+    const int nitems = nseg * ((gy1 - y0 + 1) >> 1);
+    const int lane_rd = g * kC8PW + j;
+    for (int item = wave; item < nitems; item += 4) {
+        const int pair = item / nseg, seg = item - pair * nseg;
+        // patch row g of output rows (y0 + 2 pair, +1) is staged row 2 pair + g; column j + t
+        const Item32 q = load_item(pa, pb, 2 * pair * kC8PW + 16 * seg + lane_rd);
+        const f32x4m acc = item24(q, a, bias);
+        const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
+        if constexpr (MODE == kPrelu) {
+            if (x < gx1 && y < gy1) {
+                float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(fout) + static_cast<long>(y) * fp) + 2 * x + half;
+                *px = make_float4(prelu(acc[0], alpha[0]), prelu(acc[1], alpha[1]), prelu(acc[2], alpha[2]),
+                                  prelu(acc[3], alpha[3]));
+            }
+        } else if constexpr (MODE == kPreluConv1x1) {
+            // conv1x1 over (output row, channel): K step r of lane group g is channel
+            // 4 half + r of row ro -- this lane's own D register r
+            f32x4m d = bias1;
+#pragma unroll
+            for (int r = 0; r < 4; r++) d = mma32(w1[r], prelu(acc[r], alpha[r]), d);
+            if (x < gx1 && y < gy1) {
+                const long off = static_cast<long>(y) * fp;
+                const float4 f = *(reinterpret_cast<const float4*>(reinterpret_cast<const char*>(la.feat) + off) + 2 * x + half);
+                float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(fout) + off) + 2 * x + half;
+                *px = make_float4(prelu(add_skip(d[0], f.x), alpha1[0]), prelu(add_skip(d[1], f.y), alpha1[1]),
+                                  prelu(add_skip(d[2], f.z), alpha1[2]), prelu(add_skip(d[3], f.w), alpha1[3]));
+            }
+        } else {
+            // lanes with half = 0 hold the four sub-pixels n = 0..3 (output (2x + n % 2,
+            // 2y + n / 2)); half = 1 lanes take row 2y + 1 from them
+            const float n2 = __shfl_xor(acc[2], 16), n3 = __shfl_xor(acc[3], 16);
+            if (x < gx1 && y < gy1) {
+                float s0 = half ? n2 : acc[0], s1 = half ? n3 : acc[1];
+                if constexpr (MODE == kShuffleSrc) {
+                    const float res = to_float<TIN>(row_ptr<TIN>(la.src, la.ss, y)[x]);
+                    s0 = add_skip(s0, res);
+                    s1 = add_skip(s1, res);
+                }
+                store_pair<TOUT>(row_ptr<TOUT>(la.dst, la.ds, 2 * y + half) + 2 * x, s0, s1);
+            }
+        }
+    }
+}
+
+// ---- F = 16 or 32: channel tiles ----
+template <int F>
+struct ConvTile {
+    static constexpr int TW = F == 16 ? 64 : 32, TH = 16, NT = 512, NW = NT / 64;
+    static constexpr int PW = (TW + 2 + 15) / 16 * 16, PH = TH + 2;    // plane pitch: 16-pixel multiple
+    static constexpr int NQ = F / 4;                                    // channel-quad planes
+    static constexpr int KQ = F / 4;                                    // K steps per tap (channels per lane group)
+    static constexpr int MT = F / 16;                                   // 16-channel output tiles
+    static constexpr int NI = 4;                                        // items per wave batch
+    static constexpr int SEGS = TW / 16, ITEMS = SEGS * TH;
+    static constexpr int BATCHES = (ITEMS / NW + NI - 1) / NI;
+    static_assert(ITEMS % (NW * NI) == 0, "whole batches");
+};
+
+template <int F, int MODE, typename TOUT>
+__global__ __launch_bounds__(512) void convf_layer(const float4* __restrict__ fin, long fp, int W, int H,
+                                                   float4* __restrict__ fout, const LayerArgs la)
+{
+    static_assert(MODE != kShuffleSrc, "convf_layer epilogues");
+    using C = ConvTile<F>;
+    __shared__ float4 planes[C::NQ][C::PH * C::PW];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int x0 = tx * C::TW, y0 = ty * C::TH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, j = lane & 15;
+
+    // stage rows y0-1 .. y0+TH, columns x0-1 .. x0+TW (replicated at the image edges)
+    for (int e = tid; e < C::PH * (C::TW + 2); e += C::NT) {
+        const int r = e / (C::TW + 2), c = e - r * (C::TW + 2);
+        const int sy = clampi(y0 - 1 + r, 0, H - 1), sx = clampi(x0 - 1 + c, 0, W - 1);
+        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + C::NQ * sx;
+#pragma unroll
+        for (int q = 0; q < C::NQ; q++) planes[q][r * C::PW + c] = px[q];
+    }
+    // lane (g, j) of output tile h owns channels 16 h + 4 g .. + 3
+    f32x4m b0[C::MT];
+#pragma unroll
+    for (int h = 0; h < C::MT; h++) {
+        if constexpr (MODE == kShuffle)   // output channels 0..3 only: rows 0-3 of the first tile
+            b0[h] = (h == 0 && g == 0) ? (f32x4m){ la.bias[0], la.bias[1], la.bias[2], la.bias[3] } : (f32x4m){ 0.f, 0.f, 0.f, 0.f };
+        else {
+            const float* bl = la.bias + 16 * h + 4 * g;
+            b0[h] = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
+        }
+    }
+    // slopes and conv1x1 operands of this lane's channels, in registers for the whole tile
+    constexpr bool PRELU = MODE == kPrelu || MODE == kPreluConv1x1;
+    constexpr int NA = PRELU ? C::MT : 1, N1 = MODE == kPreluConv1x1 ? C::MT : 1;
+    float al[NA][4], al1[N1][4], w1[4 * N1][N1];
+    f32x4m b1[N1];
+    if constexpr (PRELU) {
+#pragma unroll
+        for (int h = 0; h < C::MT; h++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) al[h][r] = la.alpha[16 * h + 4 * g + r];
+    }
+    if constexpr (MODE == kPreluConv1x1) {
+#pragma unroll
+        for (int h = 0; h < C::MT; h++) {
+            const float* bl = la.b1 + 16 * h + 4 * g;
+            b1[h] = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                al1[h][r] = la.alpha1[16 * h + 4 * g + r];
+#pragma unroll
+                for (int h2 = 0; h2 < C::MT; h2++) w1[4 * h + r][h2] = la.w1[((4 * h + r) * 64 + lane) * C::MT + h2];
+            }
+        }
+    }
+    __syncthreads();
+
+    const int gx1 = min(x0 + C::TW, W), gy1 = min(y0 + C::TH, H);
+    for (int batch = 0; batch < C::BATCHES; batch++) {
+        f32x4m acc[C::NI][C::MT];
+        int base[C::NI];
+#pragma unroll
+        for (int i = 0; i < C::NI; i++) {
+            const int item = wave + C::NW * (batch * C::NI + i);
+            const int row = item / C::SEGS, seg = item - row * C::SEGS;
+            // item pixels (x0 + 16 seg + j, y0 + row): staged at (16 seg + j + 1, row + 1)
+            base[i] = row * C::PW + 16 * seg + j;
+#pragma unroll
+            for (int h = 0; h < C::MT; h++) acc[i][h] = b0[h];
+        }
+#pragma unroll 1
+        for (int p = 0; p < 9; p++) {
+            const int dy = p / 3, dx = p - dy * 3;
+            float a[C::KQ * C::MT];
+            const float4* ap = reinterpret_cast<const float4*>(la.wa + (static_cast<long>(p) * 64 + lane) * (C::KQ * C::MT));
+#pragma unroll
+            for (int v = 0; v < C::KQ * C::MT / 4; v++) {
+                const float4 f = ap[v];
+                a[4 * v] = f.x; a[4 * v + 1] = f.y; a[4 * v + 2] = f.z; a[4 * v + 3] = f.w;
+            }
+#pragma unroll
+            for (int i = 0; i < C::NI; i++) {
+                const int off = base[i] + dy * C::PW + dx;
+                float bv[C::KQ];
+#pragma unroll
+                for (int v = 0; v < C::KQ / 4; v++) {
+                    const float4 f = planes[(C::KQ / 4) * g + v][off];
+                    bv[4 * v] = f.x; bv[4 * v + 1] = f.y; bv[4 * v + 2] = f.z; bv[4 * v + 3] = f.w;
+                }
+#pragma unroll
+                for (int q = 0; q < C::KQ; q++)
+#pragma unroll
+                    for (int h = 0; h < C::MT; h++) acc[i][h] = mma32(a[q * C::MT + h], bv[q], acc[i][h]);
+            }
+        }
+        if constexpr (MODE == kPreluConv1x1) {
+            // PReLU, then conv1x1: K step 4 h + r of lane group g is channel 16 h + 4 g + r,
+            // this lane's D register r of tile h; A operands [4 h + r][lane][h2]
+#pragma unroll
+            for (int i = 0; i < C::NI; i++) {
+                f32x4m d[C::MT];
+#pragma unroll
+                for (int h2 = 0; h2 < C::MT; h2++) d[h2] = b1[h2];
+#pragma unroll
+                for (int h = 0; h < C::MT; h++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float bv = prelu(acc[i][h][r], al[h][r]);
+#pragma unroll
+                        for (int h2 = 0; h2 < C::MT; h2++) d[h2] = mma32(w1[4 * h + r][h2], bv, d[h2]);
+                    }
+#pragma unroll
+                for (int h2 = 0; h2 < C::MT; h2++) acc[i][h2] = d[h2];
+            }
+        }
+        // epilogue: lane (g, j) holds output channels 16 h + 4 g .. + 3 of pixel j
+#pragma unroll
+        for (int i = 0; i < C::NI; i++) {
+            const int item = wave + C::NW * (batch * C::NI + i);
+            const int row = item / C::SEGS, seg = item - row * C::SEGS;
+            const int x = x0 + 16 * seg + j, y = y0 + row;
+            if (x >= gx1 || y >= gy1) continue;
+            if constexpr (MODE == kShuffle) {
+                if (g == 0) {   // the 4 sub-pixels n: output (2x + n % 2, 2y + n / 2)
+                    const f32x4m v = acc[i][0];
+                    store_pair<TOUT>(row_ptr<TOUT>(la.dst, la.ds, 2 * y) + 2 * x, v[0], v[1]);
+                    store_pair<TOUT>(row_ptr<TOUT>(la.dst, la.ds, 2 * y + 1) + 2 * x, v[2], v[3]);
+                }
+            } else {
+                const long off = static_cast<long>(y) * fp;
+                float4* po = reinterpret_cast<float4*>(reinterpret_cast<char*>(fout) + off) + C::NQ * x;
+#pragma unroll
+                for (int h = 0; h < C::MT; h++) {
+                    const f32x4m v = acc[i][h];
+                    float4 o;
+                    if constexpr (MODE == kRelu) {
+                        o = make_float4(fmaxf(v[0], 0.f), fmaxf(v[1], 0.f), fmaxf(v[2], 0.f), fmaxf(v[3], 0.f));
+                    } else if constexpr (MODE == kPrelu) {
+                        o = make_float4(prelu(v[0], al[h][0]), prelu(v[1], al[h][1]), prelu(v[2], al[h][2]), prelu(v[3], al[h][3]));
+                    } else {   // kAddFeat, kPreluConv1x1: * 1 + feat (then PReLU)
+                        const float4 r = *(reinterpret_cast<const float4*>(reinterpret_cast<const char*>(la.feat) + off) + C::NQ * x + 4 * h + g);
+                        o = make_float4(add_skip(v[0], r.x), add_skip(v[1], r.y), add_skip(v[2], r.z), add_skip(v[3], r.w));
+                        if constexpr (MODE == kPreluConv1x1)
+                            o = make_float4(prelu(o.x, al1[h][0]), prelu(o.y, al1[h][1]), prelu(o.z, al1[h][2]), prelu(o.w, al1[h][3]));
+                    }
+                    po[4 * h + g] = o;
+                }
+            }
+        }
+    }
+}
+
+// Grids of the two mappings for a W x H map.
+inline dim3 conv8_grid(int W, int H) { return dim3((W + kC8TW - 1) / kC8TW, (H + kC8TH - 1) / kC8TH); }
+template <int F>
+inline dim3 convf_grid(int W, int H) { return dim3((W + ConvTile<F>::TW - 1) / ConvTile<F>::TW, (H + ConvTile<F>::TH - 1) / ConvTile<F>::TH); }
+
+} // namespace achip

@@ -37,11 +37,13 @@ namespace ac::core::hip
     {
     public:
         HIPProcessorACNetLegacy(int device, const model::ACNetLegacy& model, int precision) noexcept;
-        // ACNet<8> and ArtCNN: the same processor around an achip_net8_create or
-        // achip_net_artcnn_create net (FP32 only)
+        // ACNet<8>, ArtCNN and FSRCNNX: the same processor around an achip_net8_create,
+        // achip_net_artcnn_create or achip_net_fsrcnnx_create net (FP32 only)
         HIPProcessorACNetLegacy(int device, const model::ACNet<8>& model, int precision) noexcept;
         template<int F>
         HIPProcessorACNetLegacy(int device, const model::ArtCNN<F>& model, int precision) noexcept;
+        template<int F>
+        HIPProcessorACNetLegacy(int device, const model::FSRCNNX<F>& model, int precision) noexcept;
         ~HIPProcessorACNetLegacy() noexcept override;
 
         bool ok() noexcept override;
@@ -140,6 +142,25 @@ ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device
         return;
     }
     const int rc = achip_net_artcnn_create(device, F, model.blocks(), model.kernel(0), model.bias(0), &net);
+    if (rc != ACHIP_OK)
+    {
+        initError = std::string("HIP processor: ") + achip_last_error();
+        net = nullptr;
+        return;
+    }
+    finishInit();
+}
+
+template<int F>
+ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device, const model::FSRCNNX<F>& model, const int precision) noexcept
+    : precision(precision), layered(true)
+{
+    if (precision != 0)
+    {
+        initError = "HIP processor: FSRCNNX runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's)";
+        return;
+    }
+    const int rc = achip_net_fsrcnnx_create(device, F, model.blocks(), model.kernel(0), model.bias(0), model.alpha(0), &net);
     if (rc != ACHIP_OK)
     {
         initError = std::string("HIP processor: ") + achip_last_error();
@@ -423,6 +444,14 @@ std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPArt(const int de
 }
 template std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPArt<16>(int, const model::ArtCNN<16>&, int);
 template std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPArt<32>(int, const model::ArtCNN<32>&, int);
+
+template<int F>
+std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPFsr(const int device, const model::FSRCNNX<F>& model, const int precision)
+{
+    return std::make_shared<hip::HIPProcessorACNetLegacy>(device, model, precision);
+}
+template std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPFsr<8>(int, const model::FSRCNNX<8>&, int);
+template std::shared_ptr<ac::core::Processor> ac::core::detail::createHIPFsr<16>(int, const model::FSRCNNX<16>&, int);
 
 template<>
 AC_CORE_EXPORT const char* ac::core::Processor::info<ac::core::Processor::HIP>()

@@ -1,9 +1,11 @@
 // Variant -> weight table binding of ACNetLegacy (core/src/Model.cpp:8-34), ACNet<8>
-// (core/src/Model.cpp:36-120) and ArtCNN<16>/<32> (core/src/Model.cpp:232-281).  The tables
+// (core/src/Model.cpp:36-120), ArtCNN<16>/<32> (core/src/Model.cpp:232-281) and
+// FSRCNNX<8>/<16> (core/src/Model.cpp:283-326).  The tables
 // are generated at build time from anime4kcpp_amd/weights/*.f32 (parsed from
 // core/internal/AC/Core/Internal/Model/Param/ACNet.p, sha256 in weights/MANIFEST.json).
 #include "AC/Core/Model/ACNet.hpp"
 #include "AC/Core/Model/ArtCNN.hpp"
+#include "AC/Core/Model/FSRCNNX.hpp"
 
 namespace ac::core::model::param
 {
@@ -26,6 +28,9 @@ namespace ac::core::model::param
     AC_ART_TABLES(ArtCNN_C4F16) AC_ART_TABLES(ArtCNN_C4F16_DN) AC_ART_TABLES(ArtCNN_C4F16_DS)
     AC_ART_TABLES(ArtCNN_C4F32) AC_ART_TABLES(ArtCNN_C4F32_DN) AC_ART_TABLES(ArtCNN_C4F32_DS)
 #undef AC_ART_TABLES
+#define AC_FSR_TABLES(NAME) extern const float NAME##_kernels[]; extern const float NAME##_biases[]; extern const float NAME##_alphas[];
+    AC_FSR_TABLES(FSRCNNX_F8) AC_FSR_TABLES(FSRCNNX_F8_DistortPlus) AC_FSR_TABLES(FSRCNNX_F16) AC_FSR_TABLES(FSRCNNX_F16_DistortPlus)
+#undef AC_FSR_TABLES
 }
 
 ac::core::model::ACNetLegacy::ACNetLegacy(const Variant v) noexcept : v(v), blockNum(8)
@@ -95,3 +100,25 @@ ac::core::model::ArtCNN<F>::ArtCNN(const Variant v) noexcept : v(v), blockNum(4)
 
 template class ac::core::model::ArtCNN<16>;
 template class ac::core::model::ArtCNN<32>;
+
+template<int F>
+ac::core::model::FSRCNNX<F>::FSRCNNX(const Variant v) noexcept : v(v), blockNum(4)
+{
+    static_assert(F == 8 || F == 16, "FSRCNNX has 8 or 16 channels");
+    using namespace ac::core::model::param;
+#define AC_FSR_BIND(NAME) kptr = NAME##_kernels; bptr = NAME##_biases; aptr = NAME##_alphas;
+    if constexpr (F == 8)
+    {
+        if (v == Variant::B4_DISTORT_PLUS) { AC_FSR_BIND(FSRCNNX_F8_DistortPlus) }
+        else { AC_FSR_BIND(FSRCNNX_F8) }
+    }
+    else
+    {
+        if (v == Variant::B4_DISTORT_PLUS) { AC_FSR_BIND(FSRCNNX_F16_DistortPlus) }
+        else { AC_FSR_BIND(FSRCNNX_F16) }
+    }
+#undef AC_FSR_BIND
+}
+
+template class ac::core::model::FSRCNNX<8>;
+template class ac::core::model::FSRCNNX<16>;

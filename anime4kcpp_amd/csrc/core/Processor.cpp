@@ -48,8 +48,10 @@ namespace ac::core::detail
     // ("acnet-legacy-{gan,hdn0..3}"; "hdn" without a digit is HDN0, the first digit 0-3 after
     // it selects the variant; no model string means GAN), ACNet<8> ("acnet" without
     // "legacy": box / hdn select the family member, b4 / b8 / b18 the depth, B8 by default)
-    // and ArtCNN ("artcnn": f32 selects 32 channels, else 16; dn, else ds, the weight set;
-    // Processor.cpp:59-77).  Other families report their name.
+    // ArtCNN ("artcnn": f32 selects 32 channels, else 16; dn, else ds, the weight set;
+    // Processor.cpp:59-77) and FSRCNNX ("fsrcnnx": f16 selects 16 channels, else 8;
+    // "distort" or "dp" the distort-plus weights; Processor.cpp:40-58).  ARNet reports its
+    // name.
     ModelChoice findModel(const char* model) noexcept
     {
         ModelChoice m;
@@ -57,7 +59,13 @@ namespace ac::core::detail
         std::string s = model;
         for (char& ch : s) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
         auto has = [&](const char* t) { return s.find(t) != std::string::npos; };
-        if (has("fsrcnnx")) { m.family = -1; m.other = "FSRCNNX"; return m; }
+        if (has("fsrcnnx"))
+        {
+            using V = model::FSRCNNX<8>::Variant;
+            m.family = has("f16") ? 5 : 4;
+            m.fsr = has("distort") || has("dp") ? V::B4_DISTORT_PLUS : V::B4_NORMAL;
+            return m;
+        }
         if (has("artcnn"))
         {
             using V = model::ArtCNN<16>::Variant;
@@ -277,9 +285,12 @@ std::shared_ptr<ac::core::Processor> ac::core::Processor::create(const char* con
     case 2: return detail::createHIPArt<16>(device, model::ArtCNN<16>{ m.art }, precision);
     case 3:
         return detail::createHIPArt<32>(device, model::ArtCNN<32>{ static_cast<model::ArtCNN<32>::Variant>(m.art) }, precision);
+    case 4: return detail::createHIPFsr<8>(device, model::FSRCNNX<8>{ m.fsr }, precision);
+    case 5:
+        return detail::createHIPFsr<16>(device, model::FSRCNNX<16>{ static_cast<model::FSRCNNX<16>::Variant>(m.fsr) }, precision);
     default:
         return std::make_shared<detail::UnavailableProcessor>(t, "model family " + m.other +
-            " is not part of this library, which provides ACNetLegacy, ACNet<8> and ArtCNN");
+            " is not part of this library, which provides ACNetLegacy, ACNet<8>, ArtCNN and FSRCNNX");
     }
 }
 
@@ -305,6 +316,18 @@ template<>
 AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::ArtCNN<32>>(const int idx, const model::ArtCNN<32>& model)
 {
     return detail::createHIPArt<32>(idx, model, 0);
+}
+
+template<>
+AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::FSRCNNX<8>>(const int idx, const model::FSRCNNX<8>& model)
+{
+    return detail::createHIPFsr<8>(idx, model, 0);
+}
+
+template<>
+AC_CORE_EXPORT std::shared_ptr<ac::core::Processor> ac::core::Processor::create<ac::core::Processor::HIP, ac::core::model::FSRCNNX<16>>(const int idx, const model::FSRCNNX<16>& model)
+{
+    return detail::createHIPFsr<16>(idx, model, 0);
 }
 
 const char* ac::core::Processor::listInfo()

@@ -54,6 +54,21 @@ def load_weights_artcnn(variant):
     return k, b, int(v[1:3]), 4
 
 
+# FSRCNNX variants (core/include/AC/Core/Model/FSRCNNX.hpp:21-24): channels x {normal,
+# distort-plus}, 4 blocks
+VARIANTS_FSR = ("f8", "f8_dp", "f16", "f16_dp")
+
+
+def load_weights_fsrcnnx(variant):
+    """(kernels, biases, alphas, channels, blocks) float32 of an FSRCNNX variant."""
+    v = variant.lower()
+    if v not in VARIANTS_FSR:
+        raise ValueError(f"unknown FSRCNNX variant {variant!r}")
+    k, b, a = (np.ascontiguousarray(np.fromfile(os.path.join(WEIGHTS_DIR, f"fsrcnnx_{v}.{kind}.f32"), dtype="<f4"))
+               for kind in ("kernels", "biases", "alphas"))
+    return k, b, a, int(v[1:].split("_")[0]), 4
+
+
 def elem_type(dtype):
     import torch
     table = {torch.uint8: U8, torch.uint16: U16, torch.float16: F16, torch.float32: F32}
@@ -271,6 +286,23 @@ class NetArt(Net8):
                                            ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(stream)),
               "achip_pass_net")
         return dst
+
+
+class NetFsr(NetArt):
+    """One FSRCNNX<8|16> weight set resident on one HIP device (achip_net_fsrcnnx_create /
+    achip_pass_net); same calls as Net8."""
+
+    def __init__(self, variant="f8", device=0):
+        k, b, a, f, blocks = load_weights_fsrcnnx(variant)
+        h = ctypes.c_void_p()
+        check(_native.lib().achip_net_fsrcnnx_create(device, f, blocks, k.ctypes.data, b.ctypes.data, a.ctypes.data,
+                                                     ctypes.byref(h)), "achip_net_fsrcnnx_create")
+        self._h = h
+        self.variant = variant
+        self.channels = f
+        self.blocks = blocks
+        self.device = _native.lib().achip_net_device(h)
+        self._ws = None
 
 
 def configs(precision="fp32"):

@@ -381,9 +381,11 @@ def acnet8(frames_np, steps=10):
     return out
 
 
-def artcnn(frames_np, steps=5):
-    """ArtCNN 1080p u8 -> 4K 2x passes, layer by layer on FP32 MFMA, for 16 and 32 channels.
-    FLOP per input pixel: 2 x (9 F + 9 F^2 (blocks + 1) + 36 F) MAC."""
+def layered(frames_np, family, steps=5):
+    """ArtCNN (16 and 32 channels) or FSRCNNX (8 and 16 channels) 1080p u8 -> 4K 2x passes,
+    layer by layer on FP32 MFMA.  FLOP per input pixel: ArtCNN 2 x (9 F + 9 F^2 (B + 1) +
+    36 F) MAC; FSRCNNX 2 x (25 F + 9 F^2 B + F^2 + 36 F) MAC (B = 4 blocks).  HBM bytes per
+    input pixel: the u8 input and 4 outputs, plus 4F per map read or written."""
     import torch
     from anime4kcpp_amd import hip
     dev = torch.device("cuda:0")
@@ -391,8 +393,9 @@ def artcnn(frames_np, steps=5):
     h, w = frames_np[0].shape
     dsts = [torch.empty((2 * h, 2 * w), dtype=torch.uint8, device=dev) for _ in srcs]
     out = {}
-    for variant in ("f16", "f32"):
-        net = hip.NetArt(variant, 0)
+    cls, variants = (hip.NetArt, ("f16", "f32")) if family == "artcnn" else (hip.NetFsr, ("f8", "f16"))
+    for variant in variants:
+        net = cls(variant, 0)
         for i in range(len(srcs)):
             net.upscale(srcs[i], dsts[i])
         torch.cuda.synchronize()
@@ -405,13 +408,18 @@ def artcnn(frames_np, steps=5):
         torch.cuda.synchronize()
         sec = e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
         f, b = net.channels, net.blocks
-        flop = 2 * (9 * f + 9 * f * f * (b + 1) + 36 * f) * w * h
+        if family == "artcnn":
+            mac, maps = 9 * f + 9 * f * f * (b + 1) + 36 * f, 2 * (b + 2) + 1
+        else:
+            mac, maps = 25 * f + 9 * f * f * b + f * f + 36 * f, 2 * b + 3
+        flop = 2 * mac * w * h
         out[variant] = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
                         "channels": f, "blocks": b, "achieved_tflops": round(flop / sec / 1e12, 2),
                         "frac_of_fp32_peak": round(flop / sec / 1e12 / PEAK_FP32_TFLOPS, 4),
-                        "hbm_bytes_per_frame": (4 * f * (2 * (b + 2) + 1) + 1 + 4) * w * h}
+                        "hbm_bytes_per_frame": (4 * f * maps + 1 + 4) * w * h}
         net.close()
-    out["workload"] = "1920x1080 u8 -> 3840x2160 u8, ArtCNN C4 weights, layer by layer (F-channel f32 maps in HBM)"
+    name = "ArtCNN C4" if family == "artcnn" else "FSRCNNX B4"
+    out["workload"] = f"1920x1080 u8 -> 3840x2160 u8, {name} weights, layer by layer (F-channel f32 maps in HBM)"
     return out
 
 
@@ -430,6 +438,7 @@ def main():
     ap.add_argument("--no-stages", action="store_true", help="skip the colour / resize / PCIe stage measurement")
     ap.add_argument("--no-acnet8", action="store_true", help="skip the ACNet<8> measurement")
     ap.add_argument("--no-artcnn", action="store_true", help="skip the ArtCNN measurement")
+    ap.add_argument("--no-fsrcnnx", action="store_true", help="skip the FSRCNNX measurement")
     ap.add_argument("--serving-loads", default="fp16:300,1200;fp32:60,240",
                     help="offered loads (requests/s) per precision")
     ap.add_argument("--serving-seconds", type=float, default=3.0)
@@ -539,7 +548,9 @@ def main():
     if not args.no_acnet8:
         result["acnet8"] = acnet8(frames_np)
     if not args.no_artcnn:
-        result["artcnn"] = artcnn(frames_np)
+        result["artcnn"] = layered(frames_np, "artcnn")
+    if not args.no_fsrcnnx:
+        result["fsrcnnx"] = layered(frames_np, "fsrcnnx")
     if not args.no_serving:
         result["serving"] = {}
         for spec in args.serving_loads.split(";"):

@@ -143,6 +143,15 @@ ACHIP_API int achip_pass8(ACHipNet* net, const void* src, long src_stride, int s
 ACHIP_API int achip_net_artcnn_create(int device, int channels, int blocks, const float* kernels,
                                       const float* biases, ACHipNet** out);
 
+/* ---- FSRCNNX<F> ----
+ * core/include/AC/Core/Model/FSRCNNX.hpp (F = 8 or 16; CPU pass CPUProcessor.cpp:1629-1659):
+ * conv5x5 1 -> F (identity) = feat, blocks - 1 x conv3x3 F -> F + PReLU, conv3x3 F -> F +
+ * PReLU then conv1x1 F -> F + feat + PReLU, conv3x3 F -> 4 and pixel shuffle to 2x.  kernels
+ * F*25 + F*F*9*blocks + F*F + F*4*9, biases F + F*(blocks+1) + 4, alphas F*(blocks+1) floats
+ * in the reference's layout.  FP32; runs through achip_pass_net. */
+ACHIP_API int achip_net_fsrcnnx_create(int device, int channels, int blocks, const float* kernels,
+                                       const float* biases, const float* alphas, ACHipNet** out);
+
 /* ---- any net ----
  * Device workspace one w x h pass of `net` needs (0 for ACNetLegacy), and that pass: an
  * ACNetLegacy net runs achip_pass in FP32, the others their layer-by-layer forms. */

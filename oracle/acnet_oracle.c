@@ -386,50 +386,55 @@ int orc_acnet8_pass(const void* src, long src_stride, int w, int h, int type, vo
 
 /* ---- ArtCNN<F> (CPUProcessor.cpp:1519-1602) --------------------------------------- */
 
-/* conv_cin1<cout, 9> in the FMA order (AVX.hpp:93-124) and the Generic order */
-static void cin1_fma_n(const float* r, const float* k, const float* b, float* out, int cout)
+/* conv_cin1<cout, cpos> (cpos = 9 or 25 taps of one channel).  FMA order (AVX.hpp:93-124):
+ * an 8-lane accumulator over the first cpos - cpos % 8 taps, its hsum, then the remaining
+ * taps added one by one, then the bias; Generic order (Generic.hpp:60-69): a left fold of
+ * the tap products, then the bias. */
+static void cin1_fma_n(const float* r, const float* k, const float* b, float* out, int cout, int cpos)
 {
+    const int count = cpos / 8, remain = cpos % 8;
     for (int n = 0; n < cout; n++) {
-        const float* kp = k + n * 9;
-        float s[8];
-        for (int p = 0; p < 8; p++) s[p] = fmaf(r[p], kp[p], 0.0f);
-        float sum = hsum8(s);
-        float t = r[8] * kp[8];
-        sum = sum + t;
-        out[n] = sum + b[n];
-    }
-}
-static void cin1_generic_n(const float* r, const float* k, const float* b, float* out, int cout)
-{
-    for (int n = 0; n < cout; n++) {
-        const float* kp = k + n * 9;
-        float sum = r[0] * kp[0];
-        for (int p = 1; p < 9; p++) { float t = r[p] * kp[p]; sum = sum + t; }
-        out[n] = sum + b[n];
-    }
-}
-/* conv<cin, cout, 9>: FMA order (AVX.hpp:32-59): an 8-lane accumulator, lane l taking
- * channels l, l + 8, ... of every tap in tap order, then hsum added to the bias; Generic
- * order (Generic.hpp:71-81): per channel the left-folded 9-tap sum, channels summed in order */
-static void conv_fma_cin(const float* const* rp, const float* k, const float* b, float* out, int cin, int cout)
-{
-    for (int n = 0; n < cout; n++) {
-        const float* kn = k + (long)n * cin * 9;
+        const float* kp = k + n * cpos;
         float s[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
-        for (int p = 0; p < 9; p++)
+        for (int idx = 0; idx < count; idx++)
+            for (int l = 0; l < 8; l++) s[l] = fmaf(r[idx * 8 + l], kp[idx * 8 + l], s[l]);
+        float sum = hsum8(s);
+        for (int i = 0; i < remain; i++) { float t = r[count * 8 + i] * kp[count * 8 + i]; sum = sum + t; }
+        out[n] = sum + b[n];
+    }
+}
+static void cin1_generic_n(const float* r, const float* k, const float* b, float* out, int cout, int cpos)
+{
+    for (int n = 0; n < cout; n++) {
+        const float* kp = k + n * cpos;
+        float sum = r[0] * kp[0];
+        for (int p = 1; p < cpos; p++) { float t = r[p] * kp[p]; sum = sum + t; }
+        out[n] = sum + b[n];
+    }
+}
+/* conv<cin, cout, cpos> (cin a multiple of 8; cpos = 9 taps or 1): FMA order (AVX.hpp:32-59,
+ * 126-146): an 8-lane accumulator, lane l taking channels l, l + 8, ... of every tap in tap
+ * order, its hsum added to the bias; Generic order (Generic.hpp:71-81): per channel the
+ * left-folded tap sum, channels summed in order, then the bias */
+static void conv_fma_cin(const float* const* rp, const float* k, const float* b, float* out, int cin, int cout, int cpos)
+{
+    for (int n = 0; n < cout; n++) {
+        const float* kn = k + (long)n * cin * cpos;
+        float s[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+        for (int p = 0; p < cpos; p++)
             for (int idx = 0; idx < cin / 8; idx++)
                 for (int l = 0; l < 8; l++) s[l] = fmaf(rp[p][idx * 8 + l], kn[cin * p + idx * 8 + l], s[l]);
         out[n] = b[n] + hsum8(s);
     }
 }
-static void conv_generic_cin(const float* const* rp, const float* k, const float* b, float* out, int cin, int cout)
+static void conv_generic_cin(const float* const* rp, const float* k, const float* b, float* out, int cin, int cout, int cpos)
 {
     for (int n = 0; n < cout; n++) {
-        const float* kn = k + (long)n * cin * 9;
+        const float* kn = k + (long)n * cin * cpos;
         float sum = 0.0f;
         for (int c = 0; c < cin; c++) {
             float bs = rp[0][c] * kn[c];
-            for (int p = 1; p < 9; p++) { float t = rp[p][c] * kn[cin * p + c]; bs = bs + t; }
+            for (int p = 1; p < cpos; p++) { float t = rp[p][c] * kn[cin * p + c]; bs = bs + t; }
             sum = sum + bs;
         }
         out[n] = sum + b[n];
@@ -473,7 +478,7 @@ int orc_artcnn_pass(const void* src, long src_stride, int w, int h, int type, vo
             for (int a = 0; a < 3; a++)
                 for (int c = 0; c < 3; c++) r[a * 3 + c] = orc_to_float(sb + ys[a] * src_stride + (long)xs[c] * es, type);
             float* out = feat + ((size_t)i * w + j) * f;
-            if (fma_order) cin1_fma_n(r, kernels, biases, out, f); else cin1_generic_n(r, kernels, biases, out, f);
+            if (fma_order) cin1_fma_n(r, kernels, biases, out, f, 9); else cin1_generic_n(r, kernels, biases, out, f, 9);
         }
     /* body: `blocks` x conv3x3 F -> F + ReLU; then F -> F identity + feat (Backend.hpp:157-160) */
     const float* in = feat;
@@ -489,7 +494,7 @@ int orc_artcnn_pass(const void* src, long src_stride, int w, int h, int type, vo
             for (int j = 0; j < w; j++) {
                 ORC_TAPS(in, f)
                 float* o = out + ((size_t)i * w + j) * f;
-                if (fma_order) conv_fma_cin(rp9, k, b, o, f, f); else conv_generic_cin(rp9, k, b, o, f, f);
+                if (fma_order) conv_fma_cin(rp9, k, b, o, f, f, 9); else conv_generic_cin(rp9, k, b, o, f, f, 9);
                 const float* fe = feat + ((size_t)i * w + j) * f;
                 for (int n = 0; n < f; n++) o[n] = add ? o[n] * 1.0f + fe[n] : (o[n] > 0.0f ? o[n] : 0.0f);
             }
@@ -508,7 +513,100 @@ int orc_artcnn_pass(const void* src, long src_stride, int w, int h, int type, vo
             for (int j = 0; j < w; j++) {
                 ORC_TAPS(in, f)
                 float s4[4];
-                if (fma_order) conv_fma_cin(rp9, k, b, s4, f, 4); else conv_generic_cin(rp9, k, b, s4, f, 4);
+                if (fma_order) conv_fma_cin(rp9, k, b, s4, f, 4, 9); else conv_generic_cin(rp9, k, b, s4, f, 4, 9);
+                for (int n = 0; n < 4; n++)
+                    orc_from_float(s4[n], db + (long)(2 * i + (n >> 1)) * dst_stride + (long)(2 * j + (n & 1)) * es, type);
+            }
+    }
+    free(feat);
+    free(t1);
+    free(t2);
+    return 0;
+}
+
+/* ---- FSRCNNX<F> (CPUProcessor.cpp:1629-1659 and the F = 16 twin) -------------------- */
+
+int orc_fsrcnnx_pass(const void* src, long src_stride, int w, int h, int type, void* dst, long dst_stride,
+                     const float* kernels, const float* biases, const float* alphas, int f, int blocks, int order,
+                     int threads)
+{
+    if (!src || !dst || w <= 0 || h <= 0 || !kernels || !biases || !alphas || (f != 8 && f != 16) || blocks < 2)
+        return -22;
+    const size_t plane = (size_t)w * h * f;
+    float* feat = (float*)malloc(plane * sizeof(float));
+    float* t1 = (float*)malloc(plane * sizeof(float));
+    float* t2 = (float*)malloc(plane * sizeof(float));
+    if (!feat || !t1 || !t2) { free(feat); free(t1); free(t2); return -12; }
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+    const int fma_order = (order == ORC_ORDER_FMA);
+    const unsigned char* sb = (const unsigned char*)src;
+    const int es = elem_size(type);
+    /* head: conv5x5 1 -> F, identity, taps clamped to the image (Common.hpp:199-225) */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+            float r[25];
+            for (int a = 0; a < 5; a++)
+                for (int c = 0; c < 5; c++)
+                    r[a * 5 + c] = orc_to_float(sb + clampi(i + a - 2, 0, h - 1) * src_stride + (long)clampi(j + c - 2, 0, w - 1) * es, type);
+            float* out = feat + ((size_t)i * w + j) * f;
+            if (fma_order) cin1_fma_n(r, kernels, biases, out, f, 25); else cin1_generic_n(r, kernels, biases, out, f, 25);
+        }
+    /* body: layers 1 .. blocks - 1 conv3x3 F -> F + PReLU; layer `blocks` conv3x3 + PReLU
+     * fused with layer blocks + 1, conv1x1 F -> F, * 1 + feat, PReLU (Backend.hpp:243-254,
+     * Common.hpp:227-288) */
+    const float* in = feat;
+    float* out = t1;
+    for (int l = 1; l <= blocks; l++) {
+        const float* k = kernels + f * 25 + (size_t)f * f * 9 * (l - 1);
+        const float* b = biases + f * l;
+        const float* al = alphas + f * (l - 1);
+        const int fused = l == blocks;
+        const float* k1 = kernels + f * 25 + (size_t)f * f * 9 * blocks;
+        const float* b1 = biases + f * (blocks + 1);
+        const float* a1 = alphas + f * blocks;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++) {
+                ORC_TAPS(in, f)
+                float buf[16];
+                if (fma_order) conv_fma_cin(rp9, k, b, buf, f, f, 9); else conv_generic_cin(rp9, k, b, buf, f, f, 9);
+                for (int n = 0; n < f; n++) buf[n] = prelu_(buf[n], al[n]);
+                float* o = out + ((size_t)i * w + j) * f;
+                if (!fused) {
+                    for (int n = 0; n < f; n++) o[n] = buf[n];
+                    continue;
+                }
+                const float* bp1[1] = { buf };
+                float sum[16];
+                if (fma_order) conv_fma_cin(bp1, k1, b1, sum, f, f, 1); else conv_generic_cin(bp1, k1, b1, sum, f, f, 1);
+                const float* fe = feat + ((size_t)i * w + j) * f;
+                for (int n = 0; n < f; n++) o[n] = prelu_(sum[n] * 1.0f + fe[n], a1[n]);
+            }
+        in = out;
+        out = out == t1 ? t2 : t1;
+    }
+    /* tail: conv3x3 F -> 4, identity, pixel shuffle (Common.hpp:290-341) */
+    {
+        const float* k = kernels + f * 25 + (size_t)f * f * 9 * blocks + f * f;
+        const float* b = biases + f * (blocks + 2);
+        unsigned char* db = (unsigned char*)dst;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++) {
+                ORC_TAPS(in, f)
+                float s4[4];
+                if (fma_order) conv_fma_cin(rp9, k, b, s4, f, 4, 9); else conv_generic_cin(rp9, k, b, s4, f, 4, 9);
                 for (int n = 0; n < 4; n++)
                     orc_from_float(s4[n], db + (long)(2 * i + (n >> 1)) * dst_stride + (long)(2 * j + (n & 1)) * es, type);
             }

@@ -63,6 +63,13 @@ int orc_acnet8_pass(const void* src, long src_stride, int w, int h, int type, vo
 int orc_artcnn_pass(const void* src, long src_stride, int w, int h, int type, void* dst, long dst_stride,
                     const float* kernels, const float* biases, int f, int blocks, int order, int threads);
 
+/* FSRCNNX<F> 2x pass, F = 8 or 16 (core/src/processor/cpu/CPUProcessor.cpp:1629-1659):
+ * conv5x5 1->F, `blocks` - 1 x conv3x3 F->F + PReLU, conv3x3 F->F + PReLU then conv1x1 F->F
+ * + head features + PReLU, conv3x3 F->4 + pixel shuffle; src and dst of one element type. */
+int orc_fsrcnnx_pass(const void* src, long src_stride, int w, int h, int type, void* dst, long dst_stride,
+                     const float* kernels, const float* biases, const float* alphas, int f, int blocks, int order,
+                     int threads);
+
 /* The same network in double precision.  src is the toFloat'ed input
  * (w*h floats, dense); dst receives the saturated 2w x 2h output in double. */
 int orc_acnet_pass_f64(const float* src, int w, int h, double* dst,

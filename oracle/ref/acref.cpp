@@ -14,6 +14,10 @@ int acref_artcnn_generic(const void*, long, int, int, int, void*, long, const fl
 int acref_artcnn_fma(const void*, long, int, int, int, void*, long, const float*, const float*, int, int);
 int acref_artcnn_avx512(const void*, long, int, int, int, void*, long, const float*, const float*, int, int);
 
+int acref_fsrcnnx_generic(const void*, long, int, int, int, void*, long, const float*, const float*, const float*, int, int);
+int acref_fsrcnnx_fma(const void*, long, int, int, int, void*, long, const float*, const float*, const float*, int, int);
+int acref_fsrcnnx_avx512(const void*, long, int, int, int, void*, long, const float*, const float*, const float*, int, int);
+
 int acref_has_arch(int arch)
 {
     __builtin_cpu_init();
@@ -57,6 +61,18 @@ int acref_artcnn(int arch, const void* src, long sstride, int w, int h, int type
     case 1: return acref_artcnn_generic(src, sstride, w, h, type, dst, dstride, kernels, biases, f, blocks);
     case 4: return acref_artcnn_fma(src, sstride, w, h, type, dst, dstride, kernels, biases, f, blocks);
     case 5: return acref_artcnn_avx512(src, sstride, w, h, type, dst, dstride, kernels, biases, f, blocks);
+    default: return -22;
+    }
+}
+
+int acref_fsrcnnx(int arch, const void* src, long sstride, int w, int h, int type, void* dst, long dstride,
+                  const float* kernels, const float* biases, const float* alphas, int f, int blocks)
+{
+    if (!acref_has_arch(arch)) return -95;
+    switch (arch) {
+    case 1: return acref_fsrcnnx_generic(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, f, blocks);
+    case 4: return acref_fsrcnnx_fma(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, f, blocks);
+    case 5: return acref_fsrcnnx_avx512(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, f, blocks);
     default: return -22;
     }
 }

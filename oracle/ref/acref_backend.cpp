@@ -240,6 +240,79 @@ struct ArtLayers {
         return 0;
     }
 };
+// ---- FSRCNNX<F> (CPUProcessor.cpp:1629-1659) ----
+
+template <int F>
+struct FsrLayers {
+    static float prelu(float v, float a) { return std::max(v, 0.0f) + a * std::min(v, 0.0f); }
+    template <typename IN>
+    static void head(const void* src, long sstride, int w, int h, float* out, const float* k, const float* b)
+    {
+#pragma omp parallel for schedule(guided)
+        for (int i = 0; i < h; i++) {
+            auto row = [&](int y) { return reinterpret_cast<const IN*>(static_cast<const std::uint8_t*>(src) + y * sstride); };
+            for (int j = 0; j < w; j++) {
+                alignas(64) float r[25];
+                for (int a = 0; a < 5; a++)
+                    for (int c = 0; c < 5; c++)
+                        r[a * 5 + c] = ac::core::toFloat(row(std::clamp(i + a - 2, 0, h - 1))[std::clamp(j + c - 2, 0, w - 1)]);
+                Op::conv_cin1<F, 25>(r, out + (static_cast<long>(i) * w + j) * F, k, b);
+            }
+        }
+    }
+    static void body(const float* in, int w, int h, float* out, const float* k, const float* b, const float* al,
+                     const float* k1, const float* b1, const float* a1, const float* feat)
+    {
+#pragma omp parallel for schedule(guided)
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++) {
+                const float* rptr[9];
+                alignas(64) float buf[F]{};
+                Op::conv<F, F, 9>(ArtLayers<F>::taps(in, w, i, j, h, rptr), buf, k, b);
+                for (int n = 0; n < F; n++) buf[n] = prelu(buf[n], al[n]);
+                float* o = out + (static_cast<long>(i) * w + j) * F;
+                if (!feat) {
+                    for (int n = 0; n < F; n++) o[n] = buf[n];
+                    continue;
+                }
+                rptr[0] = buf;
+                alignas(64) float sum[F]{};
+                Op::conv<F, F, 1>(rptr, sum, k1, b1);
+                const float* fe = feat + (static_cast<long>(i) * w + j) * F;
+                for (int n = 0; n < F; n++) o[n] = prelu(sum[n] * 1.0f + fe[n], a1[n]);
+            }
+    }
+    static int run(const void* src, long sstride, int w, int h, int type, void* dst, long dstride, const float* kernels,
+                   const float* biases, const float* alphas, int blocks)
+    {
+        std::vector<float> feat(static_cast<size_t>(w) * h * F), t1(feat.size()), t2(feat.size());
+        switch (type) {
+        case 0x001: head<std::uint8_t>(src, sstride, w, h, feat.data(), kernels, biases); break;
+        case 0x002: head<std::uint16_t>(src, sstride, w, h, feat.data(), kernels, biases); break;
+        case 0x204: head<float>(src, sstride, w, h, feat.data(), kernels, biases); break;
+        default: return -22;
+        }
+        const float* in = feat.data();
+        float* out = t1.data();
+        const float* k1 = kernels + F * 25 + static_cast<size_t>(F) * F * 9 * blocks;
+        for (int l = 1; l <= blocks; l++) {
+            body(in, w, h, out, kernels + F * 25 + static_cast<size_t>(F) * F * 9 * (l - 1), biases + F * l,
+                 alphas + F * (l - 1), k1, biases + F * (blocks + 1), alphas + F * blocks,
+                 l == blocks ? feat.data() : nullptr);
+            in = out;
+            out = out == t1.data() ? t2.data() : t1.data();
+        }
+        const float* kt = k1 + F * F;
+        const float* bt = biases + F * (blocks + 2);
+        switch (type) {
+        case 0x001: ArtLayers<F>::template tail<std::uint8_t>(in, w, h, dst, dstride, kt, bt); break;
+        case 0x002: ArtLayers<F>::template tail<std::uint16_t>(in, w, h, dst, dstride, kt, bt); break;
+        case 0x204: ArtLayers<F>::template tail<float>(in, w, h, dst, dstride, kt, bt); break;
+        default: return -22;
+        }
+        return 0;
+    }
+};
 } // namespace
 
 #define CAT2(a, b) a##b
@@ -311,5 +384,14 @@ extern "C" int CAT(acref_artcnn_, BACKEND_SUFFIX)(const void* src, long sstride,
 {
     if (f == 16) return ArtLayers<16>::run(src, sstride, w, h, type, dst, dstride, kernels, biases, blocks);
     if (f == 32) return ArtLayers<32>::run(src, sstride, w, h, type, dst, dstride, kernels, biases, blocks);
+    return -22;
+}
+
+extern "C" int CAT(acref_fsrcnnx_, BACKEND_SUFFIX)(const void* src, long sstride, int w, int h, int type, void* dst,
+                                                   long dstride, const float* kernels, const float* biases,
+                                                   const float* alphas, int f, int blocks)
+{
+    if (f == 8) return FsrLayers<8>::run(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, blocks);
+    if (f == 16) return FsrLayers<16>::run(src, sstride, w, h, type, dst, dstride, kernels, biases, alphas, blocks);
     return -22;
 }

@@ -105,13 +105,32 @@ TEST_CASE(artcnn_layer_algebra)
     CHECK(A16{ A16::Variant::C4_NORMAL }.kernel(0) != m.kernel(0) && A16{ A16::Variant::C4_DN }.kernel(0) != m.kernel(0));
 }
 
+TEST_CASE(fsrcnnx_layer_algebra)
+{
+    // FSRCNNX.hpp:29-78: head F*25, blockNum conv3x3 F*F*9, conv1x1 F*F, tail F*4*9; slopes
+    // for layers 1..blockNum+1 (alpha(0) is alpha(1))
+    using X8 = ac::core::model::FSRCNNX<8>;
+    using X16 = ac::core::model::FSRCNNX<16>;
+    X8 m{ X8::Variant::B4_NORMAL };
+    CHECK(m.blocks() == 4 && m.kernels() == 7 && m.biases() == 7 && m.alphas() == 5);
+    CHECK(m.kernelLength() == 200 + 576 * 4 + 64 + 288 && m.biasLength() == 8 + 40 + 4 && m.alphaLength() == 40);
+    CHECK(m.kernelOffset(1) == 200 && m.kernelOffset(5) == 200 + 576 * 4 && m.kernelOffset(6) == 200 + 576 * 4 + 64);
+    CHECK(m.kernelLength(5) == 64 && m.kernelLength(6) == 288 && m.biasLength(6) == 4);
+    CHECK(m.alphaOffset(0) == 0 && m.alphaOffset(1) == 0 && m.alphaOffset(5) == 32 && m.alpha(0) == m.alpha(1));
+    X16 n{ X16::Variant::B4_DISTORT_PLUS };
+    CHECK(n.kernelLength() == 400 + 2304 * 4 + 256 + 576 && n.biasLength() == 16 + 80 + 4 && n.alphaLength() == 80);
+    CHECK(X16{ X16::Variant::B4_NORMAL }.kernel(0) != n.kernel(0));
+}
+
 TEST_CASE(processor_factory_without_gpu)
 {
     auto cpu = ac::core::Processor::create("cpu", 0, "acnet-legacy-hdn0");
     CHECK(static_cast<bool>(cpu) && !cpu->ok() && cpu->type() == ac::core::Processor::CPU);
     CHECK(std::strstr(cpu->error(), "HIP processor only") != nullptr);
-    auto fam = ac::core::Processor::create("hip", 0, "fsrcnnx-f8");
-    CHECK(!fam->ok() && std::strstr(fam->error(), "FSRCNNX") != nullptr);
+    auto fam = ac::core::Processor::create("hip", 0, "arnet-b8");
+    CHECK(!fam->ok() && std::strstr(fam->error(), "ARNet") != nullptr);
+    auto fsr = ac::core::Processor::create("hip", 0, "fsrcnnx-f16-dp");   // a provided family: fails on the device only
+    CHECK(!fsr->ok() && std::strstr(fsr->error(), "no HIP device") != nullptr);
     auto art = ac::core::Processor::create("hip", 0, "artcnn-c4f16");   // a provided family: fails on the device only
     CHECK(!art->ok() && std::strstr(art->error(), "no HIP device") != nullptr);
     auto p = ac::core::Processor::create("HIP", 0, "acnet-legacy-hdn2");

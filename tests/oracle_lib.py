@@ -50,6 +50,8 @@ def orc():
                                       ctypes.c_int, ctypes.c_int]
         o.orc_artcnn_pass.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        o.orc_fsrcnnx_pass.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, P, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int]
         o.orc_filter_weight.argtypes = [ctypes.c_int, ctypes.c_float]
         o.orc_filter_weight.restype = ctypes.c_float
         o.orc_filter_support.argtypes = [ctypes.c_int]
@@ -70,6 +72,8 @@ def ref():
         r.acref_has_arch.argtypes = [ctypes.c_int]
         r.acref_artcnn.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P,
                                    ctypes.c_int, ctypes.c_int]
+        r.acref_fsrcnnx.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, P,
+                                    ctypes.c_int, ctypes.c_int]
         r.acref_pass8.argtypes = [ctypes.c_int, P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, P, ctypes.c_int]
         _ref = r
     return _ref
@@ -173,5 +177,28 @@ def ref_artcnn(arch, src, kernels, biases, f, blocks=4):
     k, b = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases))
     rc = ref().acref_artcnn(arch, src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
                             dst.strides[0], k.ctypes.data, b.ctypes.data, f, blocks)
+    assert rc == 0, rc
+    return dst
+
+
+def oracle_fsrcnnx(src, kernels, biases, alphas, f, blocks=4, order=ORDER_FMA, threads=0):
+    """FSRCNNX<f> 2x pass of a 2-D array (u8/u16/f32); output of the same type."""
+    src = np.ascontiguousarray(src)
+    h, w = src.shape
+    dst = np.zeros((2 * h, 2 * w), src.dtype)
+    k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases, alphas))
+    assert orc().orc_fsrcnnx_pass(src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
+                                  dst.strides[0], k.ctypes.data, b.ctypes.data, a.ctypes.data, f, blocks, order,
+                                  threads) == 0
+    return dst
+
+
+def ref_fsrcnnx(arch, src, kernels, biases, alphas, f, blocks=4):
+    src = np.ascontiguousarray(src)
+    h, w = src.shape
+    dst = np.zeros((2 * h, 2 * w), src.dtype)
+    k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases, alphas))
+    rc = ref().acref_fsrcnnx(arch, src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
+                             dst.strides[0], k.ctypes.data, b.ctypes.data, a.ctypes.data, f, blocks)
     assert rc == 0, rc
     return dst
