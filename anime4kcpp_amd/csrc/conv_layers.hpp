@@ -24,6 +24,7 @@
 #pragma once
 
 #include "acnet_device.hpp"
+#include "acnet_kernels.hpp"
 #include "acnet_mfma32_item.hpp"
 
 namespace achip {
@@ -158,25 +159,29 @@ __global__ __launch_bounds__(256) void conv8_layer(const float4* __restrict__ fi
 }
 
 // ---- F = 16 or 32: channel tiles ----
-template <int F>
+// TW x 16 output pixels per workgroup of NT threads; NI items per wave batch.  The defaults
+// are the fastest of a sweep over 1080p maps on one MI355X: 32 x 16 (F = 16, 55 KB of LDS)
+// and 16 x 16 (F = 32, 74 KB) tiles let two workgroups share a CU, so one stages its tile
+// while the other computes.
+template <int F, int TW_ = (F == 16 ? 32 : 16), int NT_ = 512, int NI_ = (F == 16 ? 4 : 2), int PF_ = 1>
 struct ConvTile {
-    static constexpr int TW = F == 16 ? 64 : 32, TH = 16, NT = 512, NW = NT / 64;
+    static constexpr bool PF = PF_;                                     // load tap p + 1's A operands during tap p
+    static constexpr int TW = TW_, TH = 16, NT = NT_, NW = NT / 64;
     static constexpr int PW = (TW + 2 + 15) / 16 * 16, PH = TH + 2;    // plane pitch: 16-pixel multiple
     static constexpr int NQ = F / 4;                                    // channel-quad planes
     static constexpr int KQ = F / 4;                                    // K steps per tap (channels per lane group)
     static constexpr int MT = F / 16;                                   // 16-channel output tiles
-    static constexpr int NI = 4;                                        // items per wave batch
+    static constexpr int NI = NI_;                                      // items per wave batch
     static constexpr int SEGS = TW / 16, ITEMS = SEGS * TH;
     static constexpr int BATCHES = (ITEMS / NW + NI - 1) / NI;
     static_assert(ITEMS % (NW * NI) == 0, "whole batches");
 };
 
-template <int F, int MODE, typename TOUT>
-__global__ __launch_bounds__(512) void convf_layer(const float4* __restrict__ fin, long fp, int W, int H,
-                                                   float4* __restrict__ fout, const LayerArgs la)
+template <int F, int MODE, typename TOUT, class C = ConvTile<F>>
+__global__ __launch_bounds__(C::NT) void convf_layer(const float4* __restrict__ fin, long fp, int W, int H,
+                                                     float4* __restrict__ fout, const LayerArgs la)
 {
     static_assert(MODE != kShuffleSrc, "convf_layer epilogues");
-    using C = ConvTile<F>;
     __shared__ float4 planes[C::NQ][C::PH * C::PW];
     const int tiles_x = gridDim.x;
     const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
@@ -243,15 +248,32 @@ __global__ __launch_bounds__(512) void convf_layer(const float4* __restrict__ fi
 #pragma unroll
             for (int h = 0; h < C::MT; h++) acc[i][h] = b0[h];
         }
+        float4 an[C::KQ * C::MT / 4];
+        const float4* const ap0 = reinterpret_cast<const float4*>(la.wa + static_cast<long>(lane) * (C::KQ * C::MT));
+        constexpr long kTapF4 = 64L * C::KQ * C::MT / 4;   // float4s per tap
+        if constexpr (C::PF) {
+#pragma unroll
+            for (int v = 0; v < C::KQ * C::MT / 4; v++) an[v] = ap0[v];
+        }
 #pragma unroll 1
         for (int p = 0; p < 9; p++) {
             const int dy = p / 3, dx = p - dy * 3;
             float a[C::KQ * C::MT];
-            const float4* ap = reinterpret_cast<const float4*>(la.wa + (static_cast<long>(p) * 64 + lane) * (C::KQ * C::MT));
+            if constexpr (C::PF) {
 #pragma unroll
-            for (int v = 0; v < C::KQ * C::MT / 4; v++) {
-                const float4 f = ap[v];
-                a[4 * v] = f.x; a[4 * v + 1] = f.y; a[4 * v + 2] = f.z; a[4 * v + 3] = f.w;
+                for (int v = 0; v < C::KQ * C::MT / 4; v++) {
+                    a[4 * v] = an[v].x; a[4 * v + 1] = an[v].y; a[4 * v + 2] = an[v].z; a[4 * v + 3] = an[v].w;
+                }
+                if (p < 8) {
+#pragma unroll
+                    for (int v = 0; v < C::KQ * C::MT / 4; v++) an[v] = ap0[(p + 1) * kTapF4 + v];
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < C::KQ * C::MT / 4; v++) {
+                    const float4 f = ap0[p * kTapF4 + v];
+                    a[4 * v] = f.x; a[4 * v + 1] = f.y; a[4 * v + 2] = f.z; a[4 * v + 3] = f.w;
+                }
             }
 #pragma unroll
             for (int i = 0; i < C::NI; i++) {
@@ -327,7 +349,7 @@ __global__ __launch_bounds__(512) void convf_layer(const float4* __restrict__ fi
 
 // Grids of the two mappings for a W x H map.
 inline dim3 conv8_grid(int W, int H) { return dim3((W + kC8TW - 1) / kC8TW, (H + kC8TH - 1) / kC8TH); }
-template <int F>
-inline dim3 convf_grid(int W, int H) { return dim3((W + ConvTile<F>::TW - 1) / ConvTile<F>::TW, (H + ConvTile<F>::TH - 1) / ConvTile<F>::TH); }
+template <int F, class C = ConvTile<F>>
+inline dim3 convf_grid(int W, int H) { return dim3((W + C::TW - 1) / C::TW, (H + C::TH - 1) / C::TH); }
 
 } // namespace achip
