@@ -122,17 +122,21 @@ TEST_CASE(fsrcnnx_layer_algebra)
     CHECK(X16{ X16::Variant::B4_NORMAL }.kernel(0) != n.kernel(0));
 }
 
-TEST_CASE(processor_factory_without_gpu)
+// gpu: a device is present, so the families this library provides must come up ok
+static void processor_factory(const bool gpu)
 {
     auto cpu = ac::core::Processor::create("cpu", 0, "acnet-legacy-hdn0");
     CHECK(static_cast<bool>(cpu) && !cpu->ok() && cpu->type() == ac::core::Processor::CPU);
     CHECK(std::strstr(cpu->error(), "HIP processor only") != nullptr);
     auto fam = ac::core::Processor::create("hip", 0, "arnet-b8");
     CHECK(!fam->ok() && std::strstr(fam->error(), "ARNet") != nullptr);
-    auto fsr = ac::core::Processor::create("hip", 0, "fsrcnnx-f16-dp");   // a provided family: fails on the device only
-    CHECK(!fsr->ok() && std::strstr(fsr->error(), "no HIP device") != nullptr);
-    auto art = ac::core::Processor::create("hip", 0, "artcnn-c4f16");   // a provided family: fails on the device only
-    CHECK(!art->ok() && std::strstr(art->error(), "no HIP device") != nullptr);
+    // provided families: without a device they fail on the device only
+    for (const char* model : { "fsrcnnx-f16-dp", "artcnn-c4f16", "acnet-b4" })
+    {
+        auto p = ac::core::Processor::create("hip", 0, model);
+        if (gpu) CHECK(p->ok());
+        else CHECK(!p->ok() && std::strstr(p->error(), "no HIP device") != nullptr);
+    }
     auto p = ac::core::Processor::create("HIP", 0, "acnet-legacy-hdn2");
     CHECK(static_cast<bool>(p) && p->type() == ac::core::Processor::HIP && std::strcmp(p->typeName(), "HIP") == 0);
     auto a = ac::core::Processor::create("auto", -1, nullptr);
@@ -169,6 +173,17 @@ TEST_CASE(processor_gpu)
     // a constant image stays constant (up to the 2x2 deconv pattern) under replicate padding
     const std::uint8_t* d = d4.data();
     CHECK(d[0] == d[2 * d4.stride() + 2] && d[1] == d[2 * d4.stride() + 3]);
+    // the layer-by-layer families through the same factory, from a model object
+    auto art = ac::core::Processor::create<ac::core::Processor::HIP>(0, ac::core::model::ArtCNN<32>{ ac::core::model::ArtCNN<32>::Variant::C4_DN });
+    auto fsr = ac::core::Processor::create<ac::core::Processor::HIP>(0, ac::core::model::FSRCNNX<8>{ ac::core::model::FSRCNNX<8>::Variant::B4_NORMAL });
+    for (auto* q : { art.get(), fsr.get() })
+    {
+        auto o = q->process(src, 2.0);
+        CHECK(q->ok() && o.width() == 64 && o.height() == 64 && o.type() == src.type());
+        // constant in, constant out away from the borders (replicate padding)
+        const std::uint8_t* e = o.data();
+        CHECK(e[20 * o.stride() + 20] == e[30 * o.stride() + 32]);   // same 2x2 sub-pixel phase
+    }
 }
 
 int main(int argc, char** argv)
@@ -179,7 +194,9 @@ int main(int argc, char** argv)
     image_external_and_views();
     image_from_to_clone();
     model_acnet_legacy();
-    processor_factory_without_gpu();
+    artcnn_layer_algebra();
+    fsrcnnx_layer_algebra();
+    processor_factory(gpu);
     if (gpu) processor_gpu();
     else processor_no_device();
     std::printf("%d checks, %d failures\n", checks, failures);
