@@ -107,7 +107,8 @@ hipError_t launch_fsrcnnx(int f, int blocks, int stype, int dtype, const void* s
 // Host-side packing of the per-lane A operands the layer kernels (conv_layers.hpp) read.
 // F = 8 (2-row items), conv3x3 8 -> cout (8 or 4): [t * 8 + c][lane] for patch column t,
 // channel c; conv1x1 8 -> 8: [r][lane], block-diagonal over the item's two output rows.
-// F = 16 or 32 (channel tiles), conv3x3 F -> cout (F or 4): [tap][lane][q * F/16 + h];
+// F = 16 or 32 (channel tiles), conv3x3 F -> F: [tap][lane][q * F/16 + h]; F -> 4 (the tail,
+// one output tile): [tap][lane][q];
 // conv1x1 F -> F: [4 h + r][lane][h2].  k in the reference layout k[n][tap][c].
 void pack_conv8_operands(const float* k, int cout, float* out);
 void pack_conv8_1x1(const float* k1, float* out);

@@ -209,6 +209,8 @@ __global__ __launch_bounds__(C::NT) void convf_layer(const float4* __restrict__ 
             b0[h] = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
         }
     }
+    // the F -> 4 tail needs only the first output tile (channels 0-3)
+    constexpr int MTU = MODE == kShuffle ? 1 : C::MT;
     // slopes and conv1x1 operands of this lane's channels, in registers for the whole tile
     constexpr bool PRELU = MODE == kPrelu || MODE == kPreluConv1x1;
     constexpr int NA = PRELU ? C::MT : 1, N1 = MODE == kPreluConv1x1 ? C::MT : 1;
@@ -248,47 +250,53 @@ __global__ __launch_bounds__(C::NT) void convf_layer(const float4* __restrict__ 
 #pragma unroll
             for (int h = 0; h < C::MT; h++) acc[i][h] = b0[h];
         }
-        float4 an[C::KQ * C::MT / 4];
-        const float4* const ap0 = reinterpret_cast<const float4*>(la.wa + static_cast<long>(lane) * (C::KQ * C::MT));
-        constexpr long kTapF4 = 64L * C::KQ * C::MT / 4;   // float4s per tap
+        // A operands [tap][lane][q * MTU + h]: KQ * MTU floats per tap and lane
+        constexpr int AW = C::KQ * MTU;
+        float4 an[AW / 4];
+        const float4* const ap0 = reinterpret_cast<const float4*>(la.wa + static_cast<long>(lane) * AW);
+        constexpr long kTapF4 = 64L * AW / 4;   // float4s per tap
         if constexpr (C::PF) {
 #pragma unroll
-            for (int v = 0; v < C::KQ * C::MT / 4; v++) an[v] = ap0[v];
+            for (int v = 0; v < AW / 4; v++) an[v] = ap0[v];
         }
 #pragma unroll 1
         for (int p = 0; p < 9; p++) {
             const int dy = p / 3, dx = p - dy * 3;
-            float a[C::KQ * C::MT];
+            float a[AW];
             if constexpr (C::PF) {
 #pragma unroll
-                for (int v = 0; v < C::KQ * C::MT / 4; v++) {
+                for (int v = 0; v < AW / 4; v++) {
                     a[4 * v] = an[v].x; a[4 * v + 1] = an[v].y; a[4 * v + 2] = an[v].z; a[4 * v + 3] = an[v].w;
                 }
                 if (p < 8) {
 #pragma unroll
-                    for (int v = 0; v < C::KQ * C::MT / 4; v++) an[v] = ap0[(p + 1) * kTapF4 + v];
+                    for (int v = 0; v < AW / 4; v++) an[v] = ap0[(p + 1) * kTapF4 + v];
                 }
             } else {
 #pragma unroll
-                for (int v = 0; v < C::KQ * C::MT / 4; v++) {
+                for (int v = 0; v < AW / 4; v++) {
                     const float4 f = ap0[p * kTapF4 + v];
                     a[4 * v] = f.x; a[4 * v + 1] = f.y; a[4 * v + 2] = f.z; a[4 * v + 3] = f.w;
                 }
             }
+            // B operands of every item first, then the MFMAs with the K step outermost, so
+            // that consecutive MFMAs update different accumulators (NI x MTU chains)
+            float bv[C::NI][C::KQ];
 #pragma unroll
             for (int i = 0; i < C::NI; i++) {
                 const int off = base[i] + dy * C::PW + dx;
-                float bv[C::KQ];
 #pragma unroll
                 for (int v = 0; v < C::KQ / 4; v++) {
                     const float4 f = planes[(C::KQ / 4) * g + v][off];
-                    bv[4 * v] = f.x; bv[4 * v + 1] = f.y; bv[4 * v + 2] = f.z; bv[4 * v + 3] = f.w;
+                    bv[i][4 * v] = f.x; bv[i][4 * v + 1] = f.y; bv[i][4 * v + 2] = f.z; bv[i][4 * v + 3] = f.w;
                 }
-#pragma unroll
-                for (int q = 0; q < C::KQ; q++)
-#pragma unroll
-                    for (int h = 0; h < C::MT; h++) acc[i][h] = mma32(a[q * C::MT + h], bv[q], acc[i][h]);
             }
+#pragma unroll
+            for (int q = 0; q < C::KQ; q++)
+#pragma unroll
+                for (int i = 0; i < C::NI; i++)
+#pragma unroll
+                    for (int h = 0; h < MTU; h++) acc[i][h] = mma32(a[q * MTU + h], bv[i][q], acc[i][h]);
         }
         if constexpr (MODE == kPreluConv1x1) {
             // PReLU, then conv1x1: K step 4 h + r of lane group g is channel 16 h + 4 g + r,

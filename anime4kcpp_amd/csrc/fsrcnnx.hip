@@ -179,8 +179,9 @@ void pack_conv8_1x1(const float* k1, float* out)
 
 void pack_convf_operands(const float* k, int f, int cout, float* out)
 {
-    // output channel 16 h + lane % 16, input channel (f / 4) (lane / 16) + q
-    const int kq = f / 4, mt = f / 16;
+    // output channel 16 h + lane % 16, input channel (f / 4) (lane / 16) + q; the F -> 4 tail
+    // has one output tile
+    const int kq = f / 4, mt = cout <= 16 ? 1 : f / 16;
     for (int p = 0; p < 9; p++)
         for (int lane = 0; lane < 64; lane++)
             for (int q = 0; q < kq; q++)
