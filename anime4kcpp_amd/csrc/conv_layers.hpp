@@ -57,16 +57,22 @@ __device__ inline float prelu(float v, float alpha)
 __device__ inline float add_skip(float v, float f) { return __fadd_rn(__fmul_rn(v, 1.0f), f); }
 
 // ---- F = 8: the 2-row item ----
-// Tile: 64 x 16 output pixels, 256 threads (4 waves); the input region (66 x 18 pixels with
-// the one-pixel halo) staged in LDS as two float4 planes (channels 0-3, 4-7) with a row pitch
-// of 80 pixels (a multiple of 16: the 16-byte item reads run at full rate).  Items: 4
-// segments of 16 pixels x 8 row pairs = 32 per tile, 8 per wave.
-constexpr int kC8TW = 64, kC8TH = 16, kC8PW = 80, kC8PH = kC8TH + 2;
+// Tile: TW x TH output pixels, NT threads; the input region (with the one-pixel halo) staged
+// in LDS as two float4 planes (channels 0-3, 4-7) with a row pitch that is a multiple of 16
+// pixels (the 16-byte item reads run at full rate).  Items: 16-pixel segments x row pairs,
+// dealt to the waves round-robin.  Default 64 x 16, 256 threads: 46 KB, three workgroups
+// per CU.
+template <int TW_ = 64, int TH_ = 16, int NT_ = 256>
+struct Conv8Tile {
+    static constexpr int TW = TW_, TH = TH_, NT = NT_, NW = NT / 64;
+    static constexpr int PW = (TW + 2 + 15) / 16 * 16, PH = TH + 2;
+};
 
-template <int MODE, typename TIN, typename TOUT>
-__global__ __launch_bounds__(256) void conv8_layer(const float4* __restrict__ fin, long fp, int W, int H,
-                                                   float4* __restrict__ fout, const LayerArgs la)
+template <int MODE, typename TIN, typename TOUT, class C = Conv8Tile<>>
+__global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ fin, long fp, int W, int H,
+                                                     float4* __restrict__ fout, const LayerArgs la)
 {
+    constexpr int kC8TW = C::TW, kC8TH = C::TH, kC8PW = C::PW, kC8PH = C::PH;
     static_assert(MODE == kPrelu || MODE == kPreluConv1x1 || MODE == kShuffle || MODE == kShuffleSrc,
                   "conv8_layer epilogues");
     constexpr bool SHUFFLE = MODE == kShuffle || MODE == kShuffleSrc;
@@ -80,7 +86,7 @@ __global__ __launch_bounds__(256) void conv8_layer(const float4* __restrict__ fi
     const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
 
     // stage rows y0-1 .. y0+16, columns x0-1 .. x0+64 (replicated at the image edges)
-    for (int e = tid; e < kC8PH * (kC8TW + 2); e += 256) {
+    for (int e = tid; e < kC8PH * (kC8TW + 2); e += C::NT) {
         const int r = e / (kC8TW + 2), c = e - r * (kC8TW + 2);
         const int sy = clampi(y0 - 1 + r, 0, H - 1), sx = clampi(x0 - 1 + c, 0, W - 1);
         const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + 2 * sx;
@@ -116,7 +122,7 @@ __global__ __launch_bounds__(256) void conv8_layer(const float4* __restrict__ fi
     const int nseg = (gx1 - x0 + 15) >> 4;
     const int nitems = nseg * ((gy1 - y0 + 1) >> 1);
     const int lane_rd = g * kC8PW + j;
-    for (int item = wave; item < nitems; item += 4) {
+    for (int item = wave; item < nitems; item += C::NW) {
         const int pair = item / nseg, seg = item - pair * nseg;
         // patch row g of output rows (y0 + 2 pair, +1) is staged row 2 pair + g; column j + t
         const Item32 q = load_item(pa, pb, 2 * pair * kC8PW + 16 * seg + lane_rd);
@@ -356,7 +362,8 @@ __global__ __launch_bounds__(C::NT) void convf_layer(const float4* __restrict__ 
 }
 
 // Grids of the two mappings for a W x H map.
-inline dim3 conv8_grid(int W, int H) { return dim3((W + kC8TW - 1) / kC8TW, (H + kC8TH - 1) / kC8TH); }
+template <class C = Conv8Tile<>>
+inline dim3 conv8_grid(int W, int H) { return dim3((W + C::TW - 1) / C::TW, (H + C::TH - 1) / C::TH); }
 template <int F, class C = ConvTile<F>>
 inline dim3 convf_grid(int W, int H) { return dim3((W + C::TW - 1) / C::TW, (H + C::TH - 1) / C::TH); }
 
