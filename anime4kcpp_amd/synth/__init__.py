@@ -27,16 +27,22 @@ def _L():
 
 
 def flat_shaded(seed, w, h, channels=1, dtype=np.float32):
-    """H x W (x C) image in [0,1] (float32) or its u8 quantisation round(255 v)."""
+    """H x W (x C) image in [0,1] (float32), its u8 quantisation round(255 v) or its u16
+    quantisation round(65535 v)."""
     shape = (h, w) if channels == 1 else (h, w, channels)
+    dtype = np.dtype(dtype)
     if dtype == np.uint8:
         out = np.empty(shape, np.uint8)
         rc = _L().ac_synth_flat_shaded_u8(seed, w, h, channels, out.ctypes.data)
-    else:
+    elif dtype in (np.float32, np.uint16):
         out = np.empty(shape, np.float32)
         rc = _L().ac_synth_flat_shaded(seed, w, h, channels, out.ctypes.data)
+    else:
+        raise ValueError(f"flat_shaded: unsupported dtype {dtype}")
     if rc != 0:
         raise ValueError(f"ac_synth_flat_shaded({seed}, {w}, {h}, {channels}) -> {rc}")
+    if dtype == np.uint16:
+        out = np.rint(out.astype(np.float64) * 65535.0).astype(np.uint16)
     return out
 
 
