@@ -59,44 +59,134 @@ __device__ inline T& comp_at_mut(const PlaneView& p, int x, int y)
     return row_ptr<T>(p.ptr, p.pitch, y)[static_cast<long>(x) * p.step + p.off];
 }
 
-// One pixel per thread, 64 x 4 per workgroup.
+// One pixel's conversion on float components (in and out in the component order of
+// ColourPlanes), shared by the kernels below.
+template <int KIND>
+__device__ inline void colour_px(const float (&in)[4], float (&out)[4])
+{
+    if constexpr (KIND == kRgb2Yuv || KIND == kRgba2Yuva) {
+        float r = in[0], g = in[1], b = in[2];
+        if constexpr (KIND == kRgba2Yuva) {
+            r = __fmul_rn(r, in[3]);
+            g = __fmul_rn(g, in[3]);
+            b = __fmul_rn(b, in[3]);
+            out[3] = in[3];
+        }
+        rgb_to_yuv(r, g, b, out[0], out[1], out[2]);
+    } else {
+        const float u = __fsub_rn(in[1], 0.5f), v = __fsub_rn(in[2], 0.5f);
+        yuv_to_rgb(in[0], u, v, out[0], out[1], out[2]);
+        if constexpr (KIND == kYuva2Rgba) {
+            unpremultiply(in[3], out[0], out[1], out[2]);
+            out[3] = in[3];
+        }
+    }
+}
+
+template <int KIND>
+constexpr int colour_components() { return KIND == kRgba2Yuva || KIND == kYuva2Rgba ? 4 : 3; }
+
+// One pixel per thread, 64 x 4 per workgroup; any element type and plane layout.
 template <typename T, int KIND>
 __global__ __launch_bounds__(256) void colour_kernel(ColourPlanes pl, int W, int H)
 {
+    constexpr int NC = colour_components<KIND>();
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int yy = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= W || yy >= H) return;
-    if constexpr (KIND == kRgb2Yuv || KIND == kRgba2Yuva) {
-        float r = to_float<T>(comp_at<T>(pl.in[0], x, yy));
-        float g = to_float<T>(comp_at<T>(pl.in[1], x, yy));
-        float b = to_float<T>(comp_at<T>(pl.in[2], x, yy));
-        float a = 1.0f;
-        if constexpr (KIND == kRgba2Yuva) {
-            a = to_float<T>(comp_at<T>(pl.in[3], x, yy));
-            r = __fmul_rn(r, a);
-            g = __fmul_rn(g, a);
-            b = __fmul_rn(b, a);
+    float in[4] = { 0.f, 0.f, 0.f, 1.f }, out[4] = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+    for (int k = 0; k < NC; k++) in[k] = to_float<T>(comp_at<T>(pl.in[k], x, yy));
+    colour_px<KIND>(in, out);
+#pragma unroll
+    for (int k = 0; k < NC; k++) comp_at_mut<T>(pl.out[k], x, yy) = from_float<T>(out[k]);
+}
+
+// u8 packed pixels <-> Y + interleaved chroma, four pixels per thread: the packed side is
+// NC dwords, the Y side one dword, the chroma side NC - 1 dwords (every base and pitch a
+// multiple of 4 bytes).  SPLIT: packed -> planes.  Pixels past the last multiple of four in
+// a row are done one by one by the row's last thread.  Same per-pixel steps as colour_kernel.
+template <int KIND, bool SPLIT>
+__global__ __launch_bounds__(256) void colour_u8x4_kernel(const uint8_t* __restrict__ pk, long ps, uint8_t* __restrict__ yp,
+                                                          long yps, uint8_t* __restrict__ cp, long cps, int W, int H)
+{
+    constexpr int NC = colour_components<KIND>();
+    const int q = blockIdx.x * 256 + threadIdx.x;   // pixel quad
+    const int x = 4 * q;
+    if (x >= W) return;
+    for (int yy = blockIdx.y; yy < H; yy += gridDim.y) {
+        uint8_t* prow = const_cast<uint8_t*>(pk) + static_cast<long>(yy) * ps;
+        uint8_t* yrow = yp + static_cast<long>(yy) * yps;
+        uint8_t* crow = cp + static_cast<long>(yy) * cps;
+        const int n = min(4, W - x);
+        uint8_t a[4][4], b[4][4];   // [pixel][component] of the source and the result
+        if (n == 4) {
+            if constexpr (SPLIT) {
+                uint32_t w[NC];
+#pragma unroll
+                for (int k = 0; k < NC; k++) w[k] = reinterpret_cast<const uint32_t*>(prow + x * NC)[k];
+#pragma unroll
+                for (int i = 0; i < 4 * NC; i++) a[i / NC][i % NC] = static_cast<uint8_t>(w[i / 4] >> (8 * (i % 4)));
+            } else {
+                const uint32_t wy = *reinterpret_cast<const uint32_t*>(yrow + x);
+                uint32_t wc[NC - 1];
+#pragma unroll
+                for (int k = 0; k < NC - 1; k++) wc[k] = reinterpret_cast<const uint32_t*>(crow + x * (NC - 1))[k];
+#pragma unroll
+                for (int p = 0; p < 4; p++) a[p][0] = static_cast<uint8_t>(wy >> (8 * p));
+#pragma unroll
+                for (int i = 0; i < 4 * (NC - 1); i++)
+                    a[i / (NC - 1)][1 + i % (NC - 1)] = static_cast<uint8_t>(wc[i / 4] >> (8 * (i % 4)));
+            }
+        } else {
+            for (int p = 0; p < n; p++)
+#pragma unroll
+                for (int k = 0; k < NC; k++)
+                    a[p][k] = SPLIT ? prow[(x + p) * NC + k] : (k == 0 ? yrow[x + p] : crow[(x + p) * (NC - 1) + k - 1]);
         }
-        float y, u, v;
-        rgb_to_yuv(r, g, b, y, u, v);
-        comp_at_mut<T>(pl.out[0], x, yy) = from_float<T>(y);
-        comp_at_mut<T>(pl.out[1], x, yy) = from_float<T>(u);
-        comp_at_mut<T>(pl.out[2], x, yy) = from_float<T>(v);
-        if constexpr (KIND == kRgba2Yuva) comp_at_mut<T>(pl.out[3], x, yy) = from_float<T>(a);
-    } else {
-        const float y = to_float<T>(comp_at<T>(pl.in[0], x, yy));
-        const float u = __fsub_rn(to_float<T>(comp_at<T>(pl.in[1], x, yy)), 0.5f);
-        const float v = __fsub_rn(to_float<T>(comp_at<T>(pl.in[2], x, yy)), 0.5f);
-        float r, g, b;
-        yuv_to_rgb(y, u, v, r, g, b);
-        if constexpr (KIND == kYuva2Rgba) {
-            const float a = to_float<T>(comp_at<T>(pl.in[3], x, yy));
-            unpremultiply(a, r, g, b);
-            comp_at_mut<T>(pl.out[3], x, yy) = from_float<T>(a);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            float in[4] = { 0.f, 0.f, 0.f, 1.f }, out[4] = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+            for (int k = 0; k < NC; k++) in[k] = to_float<uint8_t>(a[p][k]);
+            colour_px<KIND>(in, out);
+#pragma unroll
+            for (int k = 0; k < NC; k++) b[p][k] = from_float<uint8_t>(out[k]);
         }
-        comp_at_mut<T>(pl.out[0], x, yy) = from_float<T>(r);
-        comp_at_mut<T>(pl.out[1], x, yy) = from_float<T>(g);
-        comp_at_mut<T>(pl.out[2], x, yy) = from_float<T>(b);
+        if (n == 4) {
+            if constexpr (SPLIT) {
+                uint32_t wy = 0, wc[NC - 1];
+#pragma unroll
+                for (int p = 0; p < 4; p++) wy |= static_cast<uint32_t>(b[p][0]) << (8 * p);
+#pragma unroll
+                for (int k = 0; k < NC - 1; k++) wc[k] = 0;
+#pragma unroll
+                for (int i = 0; i < 4 * (NC - 1); i++)
+                    wc[i / 4] |= static_cast<uint32_t>(b[i / (NC - 1)][1 + i % (NC - 1)]) << (8 * (i % 4));
+                *reinterpret_cast<uint32_t*>(yrow + x) = wy;
+#pragma unroll
+                for (int k = 0; k < NC - 1; k++) reinterpret_cast<uint32_t*>(crow + x * (NC - 1))[k] = wc[k];
+            } else {
+                uint32_t w[NC];
+#pragma unroll
+                for (int k = 0; k < NC; k++) w[k] = 0;
+#pragma unroll
+                for (int i = 0; i < 4 * NC; i++) w[i / 4] |= static_cast<uint32_t>(b[i / NC][i % NC]) << (8 * (i % 4));
+#pragma unroll
+                for (int k = 0; k < NC; k++) reinterpret_cast<uint32_t*>(prow + x * NC)[k] = w[k];
+            }
+        } else {
+            for (int p = 0; p < n; p++)
+#pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if constexpr (SPLIT) {
+                        if (k == 0) yrow[x + p] = b[p][0];
+                        else crow[(x + p) * (NC - 1) + k - 1] = b[p][k];
+                    } else {
+                        prow[(x + p) * NC + k] = b[p][k];
+                    }
+                }
+        }
     }
 }
 
@@ -538,10 +628,25 @@ static ColourPlanes split_planes(int channels, const void* rgb, long rs, void* y
     return pl;
 }
 
+static bool aligned4(const void* p, long pitch) { return (reinterpret_cast<uintptr_t>(p) % 4) == 0 && pitch % 4 == 0; }
+
+template <int KIND, bool SPLIT>
+static hipError_t colour_u8x4(const void* pk, long ps, void* y, long ys, void* c, long cs, int W, int H, hipStream_t stream)
+{
+    const dim3 grid((W + 1023) / 1024, H < 32768 ? H : 32768), block(256);
+    hipLaunchKernelGGL((colour_u8x4_kernel<KIND, SPLIT>), grid, block, 0, stream, static_cast<const uint8_t*>(pk), ps,
+                       static_cast<uint8_t*>(y), ys, static_cast<uint8_t*>(c), cs, W, H);
+    return hipGetLastError();
+}
+
 hipError_t launch_color_split(int type, int channels, const void* src, long ss, int W, int H,
                               void* y, long ys, void* uv, long uvs, hipStream_t stream)
 {
     if (channels != 3 && channels != 4) return hipErrorInvalidValue;
+    if (W <= 0 || H <= 0) return hipSuccess;
+    if (type == U8 && aligned4(src, ss) && aligned4(y, ys) && aligned4(uv, uvs))
+        return channels == 4 ? colour_u8x4<kRgba2Yuva, true>(src, ss, y, ys, uv, uvs, W, H, stream)
+                             : colour_u8x4<kRgb2Yuv, true>(src, ss, y, ys, uv, uvs, W, H, stream);
     return launch_colour(type, channels == 4 ? kRgba2Yuva : kRgb2Yuv, split_planes(channels, src, ss, y, ys, uv, uvs), W, H,
                          stream);
 }
@@ -550,6 +655,10 @@ hipError_t launch_color_merge(int type, int channels, const void* y, long ys, co
                               void* dst, long ds, hipStream_t stream)
 {
     if (channels != 3 && channels != 4) return hipErrorInvalidValue;
+    if (W <= 0 || H <= 0) return hipSuccess;
+    if (type == U8 && aligned4(dst, ds) && aligned4(y, ys) && aligned4(uv, uvs))
+        return channels == 4 ? colour_u8x4<kYuva2Rgba, false>(dst, ds, const_cast<void*>(y), ys, const_cast<void*>(uv), uvs, W, H, stream)
+                             : colour_u8x4<kYuv2Rgb, false>(dst, ds, const_cast<void*>(y), ys, const_cast<void*>(uv), uvs, W, H, stream);
     ColourPlanes pl{};
     pl.in[0] = { const_cast<void*>(y), ys, 1, 0 };
     for (int k = 1; k < channels; k++) pl.in[k] = { const_cast<void*>(uv), uvs, channels - 1, k - 1 };

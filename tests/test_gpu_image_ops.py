@@ -197,3 +197,25 @@ def test_device_stage_entry_points(cuda):
                 assert same(r, OL.resize(small, 97, 51, mode))
     with pytest.raises(_native.AchipError):
         hip.resize(torch.zeros((8, 4000), dtype=torch.uint8, device="cuda"), (40, 8), 12)
+
+
+def test_device_colour_four_pixel_path(cuda):
+    """The u8 split/merge with every base and pitch a multiple of 4 bytes runs four pixels per
+    thread with dword loads and stores; rows whose width is not a multiple of four finish
+    pixel by pixel.  Both must equal the oracle bit for bit."""
+    import torch
+    from anime4kcpp_amd import hip
+    for c in (3, 4):
+        for w in (1, 2, 4, 5, 131, 132, 133, 135):
+            h = 9
+            wp = (w + 3) // 4 * 4 + 4                      # padded pitch: a multiple of 4 pixels
+            full = rand_image((h, wp, c), np.uint8, seed=40 + w + c)
+            rgb_t = torch.from_numpy(full).cuda()[:, :w]
+            y_t = torch.zeros((h, wp), dtype=torch.uint8, device="cuda")[:, :w]
+            uv_t = torch.zeros((h, wp, c - 1), dtype=torch.uint8, device="cuda")[:, :w]
+            hip.rgb2yuv(rgb_t, y_t, uv_t)
+            want = oracle_split(np.ascontiguousarray(full[:, :w]))
+            assert same(y_t.cpu().numpy(), want[..., 0]) and same(uv_t.cpu().numpy(), want[..., 1:]), (c, w)
+            out_t = torch.zeros((h, wp, c), dtype=torch.uint8, device="cuda")[:, :w]
+            hip.yuv2rgb(y_t, uv_t, out_t)
+            assert same(out_t.cpu().numpy(), oracle_merge(want)), (c, w)
