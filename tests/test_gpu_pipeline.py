@@ -20,8 +20,11 @@ def ceil_log2(f):
     return 1 if f <= 2.0 else math.ceil(math.log2(f))
 
 
-def oracle_pipeline(src, factor, k, b):
-    """Processor.cpp:199-276 with dst empty, composed from oracle steps."""
+def oracle_pipeline(src, factor, k, b, pass_fn=None):
+    """Processor.cpp:199-276 with dst empty, composed from oracle steps; pass_fn (default: the
+    ACNetLegacy pass with k, b) is one 2x pass of a 2-D array."""
+    if pass_fn is None:
+        pass_fn = lambda a: OL.oracle_pass(a, k, b)   # noqa: E731
     h, w = src.shape[:2]
     c = 1 if src.ndim == 2 else src.shape[2]
     t = OL.TYPE_OF[src.dtype]
@@ -36,7 +39,7 @@ def oracle_pipeline(src, factor, k, b):
         y = src
     out = y
     for _ in range(power):
-        out = OL.oracle_pass(out, k, b)
+        out = pass_fn(out)
     ph, pw = out.shape
     fw, fh = int(pw * fxy), int(ph * fxy)
     if fxy != 1.0:
