@@ -1,13 +1,13 @@
 // ACNet<8> 2x pass (core/src/processor/cpu/CPUProcessor.cpp:1417-1436) on gfx950, layer by
 // layer, FP32 throughout:
 //   L0     conv3x3 1 -> 8 + PReLU            (Backend.hpp:58-75)        VALU, one pixel per thread
-//   L1..LB conv3x3 8 -> 8 + PReLU            (Backend.hpp:95-98)        conv8_layer<kPrelu>
+//   L1..LB conv3x3 8 -> 8 + PReLU            (Backend.hpp:95-98)        conv8_pair: two per launch
 //   final  conv3x3 8 -> 4, pixel shuffle 2x, + the source pixel  (Backend.hpp:116-133,
 //          Common.hpp:290-341)                                          conv8_layer<kShuffleSrc>
-// Features between layers are 8-channel f32 maps in HBM (32 bytes per pixel, rows padded to
-// 256 bytes).  With a halo of one pixel per layer there is no recompute, and at 8 TB/s the
-// 64 bytes a pixel moves per layer cost less than the layer's MFMA time; the depth (4, 8
-// or 18 blocks) is a loop on the host.  Borders replicate the edge pixel at every layer,
+// Features between launches are 8-channel f32 maps in HBM (32 bytes per pixel, rows padded
+// to 256 bytes); the blocks run two per launch (conv8_pair), so each map round trip covers
+// two layers at the cost of a one-pixel recompute ring; the depth (4, 8 or 18 blocks) is a
+// loop on the host.  Borders replicate the edge pixel at every layer,
 // as the reference's clamped taps (Common.hpp:120-160) do.
 #include <utility>
 
@@ -54,15 +54,24 @@ static hipError_t acnet8_pass_t(int blocks, const void* src, long ss, int W, int
                                 const float* wts, const float* wa, float4* f0, float4* f1, long fp, hipStream_t stream)
 {
     hipLaunchKernelGGL((acnet8_l0<TIN>), dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, stream, src, ss, W, H, f0, fp, wts);
-    const dim3 grid = conv8_grid(W, H), block(256);
-    float4* in = f0;
-    float4* out = f1;
-    for (int l = 1; l <= blocks; l++) {
+    const dim3 grid = conv8_grid(W, H), pgrid = conv8_pair_grid(W, H), block(256);
+    auto block_args = [&](int l) {
         LayerArgs la{};
         la.wa = wa + static_cast<long>(l - 1) * kMfma32LayerFloats;
         la.bias = wts + kA8OffLayer + (l - 1) * 16;   // 8 biases, then 8 slopes
         la.alpha = la.bias + 8;
-        hipLaunchKernelGGL((conv8_layer<kPrelu, float, float>), grid, block, 0, stream, in, fp, W, H, out, la);
+        return la;
+    };
+    float4* in = f0;
+    float4* out = f1;
+    int l = 1;
+    for (; l + 1 <= blocks; l += 2) {   // blocks l and l + 1 in one launch
+        hipLaunchKernelGGL((conv8_pair<kPrelu, float, float>), pgrid, block, 0, stream, in, fp, W, H, out, block_args(l),
+                           block_args(l + 1));
+        std::swap(in, out);
+    }
+    if (l == blocks) {                  // an odd block count leaves one
+        hipLaunchKernelGGL((conv8_layer<kPrelu, float, float>), grid, block, 0, stream, in, fp, W, H, out, block_args(l));
         std::swap(in, out);
     }
     LayerArgs la{};

@@ -68,68 +68,44 @@ struct Conv8Tile {
     static constexpr int PW = (TW + 2 + 15) / 16 * 16, PH = TH + 2;
 };
 
-template <int MODE, typename TIN, typename TOUT, class C = Conv8Tile<>>
-__global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ fin, long fp, int W, int H,
-                                                     float4* __restrict__ fout, const LayerArgs la)
-{
-    constexpr int kC8TW = C::TW, kC8TH = C::TH, kC8PW = C::PW, kC8PH = C::PH;
+// The epilogue of an 8-channel item: this lane's bias, slopes and conv1x1 operands (lane
+// (g, j) owns channels 4 half .. + 3 of output row ro of pixel j), and what it does with the
+// item's D registers.  `ok`: the lane's pixel (x, y) is an output of this tile.
+template <int MODE, typename TIN, typename TOUT>
+struct Epi8 {
     static_assert(MODE == kPrelu || MODE == kPreluConv1x1 || MODE == kShuffle || MODE == kShuffleSrc,
-                  "conv8_layer epilogues");
-    constexpr bool SHUFFLE = MODE == kShuffle || MODE == kShuffleSrc;
-    __shared__ float4 pa[kC8PW * kC8PH];
-    __shared__ float4 pb[kC8PW * kC8PH];
-    const int tiles_x = gridDim.x;
-    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
-    const int ty = t / tiles_x, tx = t - ty * tiles_x;
-    const int x0 = tx * kC8TW, y0 = ty * kC8TH;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
-
-    // stage rows y0-1 .. y0+16, columns x0-1 .. x0+64 (replicated at the image edges)
-    for (int e = tid; e < kC8PH * (kC8TW + 2); e += C::NT) {
-        const int r = e / (kC8TW + 2), c = e - r * (kC8TW + 2);
-        const int sy = clampi(y0 - 1 + r, 0, H - 1), sx = clampi(x0 - 1 + c, 0, W - 1);
-        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + 2 * sx;
-        pa[r * kC8PW + c] = px[0];
-        pb[r * kC8PW + c] = px[1];
-    }
-    // this layer's A operands, bias and slopes; lane (g, j) owns channels 4 half .. + 3
-    float a[24];
-#pragma unroll
-    for (int s = 0; s < 24; s++) a[s] = la.wa[s * 64 + lane];
+                  "8-channel epilogues");
+    static constexpr bool SHUFFLE = MODE == kShuffle || MODE == kShuffleSrc;
     f32x4m bias, bias1 = { 0.f, 0.f, 0.f, 0.f };
     float alpha[4] = { 0.f, 0.f, 0.f, 0.f }, alpha1[4] = { 0.f, 0.f, 0.f, 0.f }, w1[4] = { 0.f, 0.f, 0.f, 0.f };
-    if constexpr (SHUFFLE) {
-        bias = half ? (f32x4m){ 0.f, 0.f, 0.f, 0.f } : (f32x4m){ la.bias[0], la.bias[1], la.bias[2], la.bias[3] };
-    } else {
-        const float* bl = la.bias + 4 * half;
-        bias = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
+
+    __device__ void load(const LayerArgs& la, int lane, int half)
+    {
+        if constexpr (SHUFFLE) {
+            bias = half ? (f32x4m){ 0.f, 0.f, 0.f, 0.f } : (f32x4m){ la.bias[0], la.bias[1], la.bias[2], la.bias[3] };
+        } else {
+            const float* bl = la.bias + 4 * half;
+            bias = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
 #pragma unroll
-        for (int r = 0; r < 4; r++) alpha[r] = la.alpha[4 * half + r];
-    }
-    if constexpr (MODE == kPreluConv1x1) {
-        const float* bl = la.b1 + 4 * half;
-        bias1 = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
+            for (int r = 0; r < 4; r++) alpha[r] = la.alpha[4 * half + r];
+        }
+        if constexpr (MODE == kPreluConv1x1) {
+            const float* bl = la.b1 + 4 * half;
+            bias1 = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            alpha1[r] = la.alpha1[4 * half + r];
-            w1[r] = la.w1[r * 64 + lane];
+            for (int r = 0; r < 4; r++) {
+                alpha1[r] = la.alpha1[4 * half + r];
+                w1[r] = la.w1[r * 64 + lane];
+            }
         }
     }
-    __syncthreads();
 
-    const int gx1 = min(x0 + kC8TW, W), gy1 = min(y0 + kC8TH, H);
-    const int nseg = (gx1 - x0 + 15) >> 4;
-    const int nitems = nseg * ((gy1 - y0 + 1) >> 1);
-    const int lane_rd = g * kC8PW + j;
-    for (int item = wave; item < nitems; item += C::NW) {
-        const int pair = item / nseg, seg = item - pair * nseg;
-        // patch row g of output rows (y0 + 2 pair, +1) is staged row 2 pair + g; column j + t
-        const Item32 q = load_item(pa, pb, 2 * pair * kC8PW + 16 * seg + lane_rd);
-        const f32x4m acc = item24(q, a, bias);
-        const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
+    // every lane of the wave calls this (the conv1x1 MFMAs and the shuffle take all lanes)
+    __device__ void apply(const f32x4m& acc, int x, int y, bool ok, int half, long fp, float4* __restrict__ fout,
+                          const LayerArgs& la) const
+    {
         if constexpr (MODE == kPrelu) {
-            if (x < gx1 && y < gy1) {
+            if (ok) {
                 float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(fout) + static_cast<long>(y) * fp) + 2 * x + half;
                 *px = make_float4(prelu(acc[0], alpha[0]), prelu(acc[1], alpha[1]), prelu(acc[2], alpha[2]),
                                   prelu(acc[3], alpha[3]));
@@ -140,7 +116,7 @@ __global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ 
             f32x4m d = bias1;
 #pragma unroll
             for (int r = 0; r < 4; r++) d = mma32(w1[r], prelu(acc[r], alpha[r]), d);
-            if (x < gx1 && y < gy1) {
+            if (ok) {
                 const long off = static_cast<long>(y) * fp;
                 const float4 f = *(reinterpret_cast<const float4*>(reinterpret_cast<const char*>(la.feat) + off) + 2 * x + half);
                 float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(fout) + off) + 2 * x + half;
@@ -151,7 +127,7 @@ __global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ 
             // lanes with half = 0 hold the four sub-pixels n = 0..3 (output (2x + n % 2,
             // 2y + n / 2)); half = 1 lanes take row 2y + 1 from them
             const float n2 = __shfl_xor(acc[2], 16), n3 = __shfl_xor(acc[3], 16);
-            if (x < gx1 && y < gy1) {
+            if (ok) {
                 float s0 = half ? n2 : acc[0], s1 = half ? n3 : acc[1];
                 if constexpr (MODE == kShuffleSrc) {
                     const float res = to_float<TIN>(row_ptr<TIN>(la.src, la.ss, y)[x]);
@@ -162,7 +138,139 @@ __global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ 
             }
         }
     }
+};
+
+template <int MODE, typename TIN, typename TOUT, class C = Conv8Tile<>>
+__global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ fin, long fp, int W, int H,
+                                                     float4* __restrict__ fout, const LayerArgs la)
+{
+    constexpr int kC8TW = C::TW, kC8TH = C::TH, kC8PW = C::PW, kC8PH = C::PH;
+    __shared__ float4 pa[kC8PW * kC8PH];
+    __shared__ float4 pb[kC8PW * kC8PH];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int x0 = tx * kC8TW, y0 = ty * kC8TH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
+
+    // stage rows y0-1 .. y0+TH, columns x0-1 .. x0+TW (replicated at the image edges)
+    for (int e = tid; e < kC8PH * (kC8TW + 2); e += C::NT) {
+        const int r = e / (kC8TW + 2), c = e - r * (kC8TW + 2);
+        const int sy = clampi(y0 - 1 + r, 0, H - 1), sx = clampi(x0 - 1 + c, 0, W - 1);
+        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + 2 * sx;
+        pa[r * kC8PW + c] = px[0];
+        pb[r * kC8PW + c] = px[1];
+    }
+    float a[24];
+#pragma unroll
+    for (int s = 0; s < 24; s++) a[s] = la.wa[s * 64 + lane];
+    Epi8<MODE, TIN, TOUT> epi;
+    epi.load(la, lane, half);
+    __syncthreads();
+
+    const int gx1 = min(x0 + kC8TW, W), gy1 = min(y0 + kC8TH, H);
+    const int nseg = (gx1 - x0 + 15) >> 4;
+    const int nitems = nseg * ((gy1 - y0 + 1) >> 1);
+    const int lane_rd = g * kC8PW + j;
+    for (int item = wave; item < nitems; item += C::NW) {
+        const int pair = item / nseg, seg = item - pair * nseg;
+        // patch row g of output rows (y0 + 2 pair, +1) is staged row 2 pair + g; column j + t
+        const Item32 q = load_item(pa, pb, 2 * pair * kC8PW + 16 * seg + lane_rd);
+        const f32x4m acc = item24(q, a, epi.bias);
+        const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
+        epi.apply(acc, x, y, x < gx1 && y < gy1, half, fp, fout, la);
+    }
 }
+
+// Two consecutive 8-channel layers in one launch: layer A (conv3x3 + PReLU, arguments la0)
+// and layer B (conv3x3 with the MODE epilogue, arguments la), so the map between them stays
+// in LDS.  Output tile 62 x 16: layer A's region grows by one pixel per side to 64 x 18,
+// exactly 4 segments x 9 row pairs; the staged input is 66 x 20 (80-pixel pitch, 51 KB,
+// three workgroups per CU).  After layer A, tiles at the image edge overwrite A's pixels
+// outside the image with the nearest pixel inside, so layer B sees the replicate padding
+// every layer of the reference applies (Common.hpp:120-143).
+constexpr int kP8TW = 62, kP8TH = 16, kP8PW = 80, kP8PH = kP8TH + 4;
+
+template <int MODE, typename TIN, typename TOUT>
+__global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
+                                                  float4* __restrict__ fout, const LayerArgs la0, const LayerArgs la)
+{
+    __shared__ float4 pa[kP8PW * kP8PH];
+    __shared__ float4 pb[kP8PW * kP8PH];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int x0 = tx * kP8TW, y0 = ty * kP8TH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
+
+    // plane pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r); stage 66 x 20, clamped
+    for (int e = tid; e < kP8PH * (kP8TW + 4); e += 256) {
+        const int r = e / (kP8TW + 4), c = e - r * (kP8TW + 4);
+        const int sy = clampi(y0 - 2 + r, 0, H - 1), sx = clampi(x0 - 2 + c, 0, W - 1);
+        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + 2 * sx;
+        pa[r * kP8PW + c] = px[0];
+        pb[r * kP8PW + c] = px[1];
+    }
+    float a[24];
+#pragma unroll
+    for (int s = 0; s < 24; s++) a[s] = la0.wa[s * 64 + lane];
+    Epi8<kPrelu, float, float> epa;
+    epa.load(la0, lane, half);
+    __syncthreads();
+
+    // layer A over plane pixels (1..64, 1..18): 36 items, 9 per wave; item (pair, seg) reads
+    // patch rows 2 pair + g, columns 16 seg + j + t
+    const int lane_rd = g * kP8PW + j;
+    float4 res[9];
+#pragma unroll
+    for (int ii = 0; ii < 9; ii++) {
+        const int item = wave + 4 * ii, pair = item >> 2, seg = item & 3;
+        const f32x4m acc = item24(load_item(pa, pb, 2 * pair * kP8PW + 16 * seg + lane_rd), a, epa.bias);
+        res[ii] = make_float4(prelu(acc[0], epa.alpha[0]), prelu(acc[1], epa.alpha[1]), prelu(acc[2], epa.alpha[2]),
+                              prelu(acc[3], epa.alpha[3]));
+    }
+    // layer B's operands while the other waves finish
+#pragma unroll
+    for (int s = 0; s < 24; s++) a[s] = la.wa[s * 64 + lane];
+    Epi8<MODE, TIN, TOUT> epi;
+    epi.load(la, lane, half);
+    __syncthreads();
+    {
+        float4* const pw = half ? pb : pa;
+#pragma unroll
+        for (int ii = 0; ii < 9; ii++) {
+            const int item = wave + 4 * ii, pair = item >> 2, seg = item & 3;
+            pw[(1 + 2 * pair + ro) * kP8PW + 1 + 16 * seg + j] = res[ii];
+        }
+    }
+    __syncthreads();
+    // replicate ring: A's pixels outside the image take the nearest pixel inside
+    if (x0 - 1 < 0 || y0 - 1 < 0 || x0 + kP8TW >= W || y0 + kP8TH >= H) {
+        for (int e = tid; e < 18 * 64; e += 256) {
+            const int r = 1 + e / 64, c = 1 + (e & 63);
+            const int gy = y0 - 2 + r, gx = x0 - 2 + c;
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) continue;
+            const int s = (clampi(gy, 0, H - 1) - (y0 - 2)) * kP8PW + clampi(gx, 0, W - 1) - (x0 - 2);
+            pa[r * kP8PW + c] = pa[s];
+            pb[r * kP8PW + c] = pb[s];
+        }
+        __syncthreads();
+    }
+    // layer B over the 62 x 16 tile: 32 items, 8 per wave; patch rows 2 pair + 1 + g,
+    // columns 16 seg + j + t + 1 (columns past A's region feed only lanes past the tile)
+    const int gx1 = min(x0 + kP8TW, W), gy1 = min(y0 + kP8TH, H);
+#pragma unroll 2
+    for (int item = wave; item < 32; item += 4) {
+        const int pair = item >> 2, seg = item & 3;
+        const f32x4m acc = item24(load_item(pa, pb, (2 * pair + 1) * kP8PW + 16 * seg + 1 + lane_rd), a, epi.bias);
+        const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
+        epi.apply(acc, x, y, x < gx1 && y < gy1, half, fp, fout, la);
+    }
+}
+
+inline dim3 conv8_pair_grid(int W, int H) { return dim3((W + kP8TW - 1) / kP8TW, (H + kP8TH - 1) / kP8TH); }
 
 // ---- F = 16 or 32: channel tiles ----
 // TW x 16 output pixels per workgroup of NT threads; NI items per wave batch.  The defaults

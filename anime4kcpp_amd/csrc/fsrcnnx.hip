@@ -85,24 +85,41 @@ static hipError_t fsrcnnx_pass_t(int blocks, const void* src, long ss, int W, in
     hipLaunchKernelGGL((fsrcnnx_head<TIN, F>), dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, stream, src, ss, W, H, feat,
                        fp, kernels, biases);
     const long wl = static_cast<long>(conv_operand_floats(F));
-    const float4* in = feat;
-    float4* out = m0;
-    for (int l = 1; l <= blocks; l++) {
+    auto body_args = [&](int l) {
         LayerArgs la{};
         la.wa = wa + (l - 1) * wl;
         la.bias = biases + F * l;
         la.alpha = alphas + F * (l - 1);
-        if (l < blocks) {
-            layer<F, kPrelu, float>(W, H, in, fp, out, la, stream);
-        } else {   // layer `blocks` fused with the conv1x1 (layer blocks + 1) and the skip
+        if (l == blocks) {   // layer `blocks` fused with the conv1x1 (layer blocks + 1) and the skip
             la.w1 = w1;
             la.b1 = biases + F * (blocks + 1);
             la.alpha1 = alphas + F * blocks;
             la.feat = feat;
-            layer<F, kPreluConv1x1, float>(W, H, in, fp, out, la, stream);
         }
+        return la;
+    };
+    const float4* in = feat;
+    float4* out = m0;
+    auto advance = [&]() {
         in = out;
         out = out == m0 ? m1 : m0;
+    };
+    int l = 1;
+    if constexpr (F == 8) {   // two layers per launch where the first of them is a plain PReLU block
+        for (; l + 1 <= blocks; l += 2) {
+            if (l + 1 < blocks)
+                hipLaunchKernelGGL((conv8_pair<kPrelu, float, float>), conv8_pair_grid(W, H), dim3(256), 0, stream, in, fp, W,
+                                   H, out, body_args(l), body_args(l + 1));
+            else
+                hipLaunchKernelGGL((conv8_pair<kPreluConv1x1, float, float>), conv8_pair_grid(W, H), dim3(256), 0, stream, in,
+                                   fp, W, H, out, body_args(l), body_args(l + 1));
+            advance();
+        }
+    }
+    for (; l <= blocks; l++) {
+        if (l < blocks) layer<F, kPrelu, float>(W, H, in, fp, out, body_args(l), stream);
+        else layer<F, kPreluConv1x1, float>(W, H, in, fp, out, body_args(l), stream);
+        advance();
     }
     LayerArgs lt{};
     lt.wa = wa + blocks * wl;

@@ -52,7 +52,8 @@ def test_1080p_against_oracle(nets8):
 
 
 @pytest.mark.parametrize("model,variant", [("acnet-b4-hdn", "b4_hdn"), ("acnet", "b8"), ("ACNet-B18-Box-HDN", "b18_box_hdn"),
-                                           ("acnet-box", "b8_box")])
+                                           ("acnet-box", "b8_box"),
+                                           ("acnet-f8b8-hdn", "b8_hdn")])   # the reference CLI's and filters' default
 def test_processor_model_strings(cuda, model, variant):
     """Processor::create parses the reference's ACNet<8> names (Processor.cpp:130-186) and
     process(src, factor) chains passes on the device."""
