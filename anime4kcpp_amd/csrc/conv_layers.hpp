@@ -1,7 +1,7 @@
 // The FP32 MFMA conv3x3 layer kernels of the layer-by-layer networks (ACNet<8>, ArtCNN<F>,
-// FSRCNNX<F>): one launch per layer, reading an F-channel f32 map from HBM (channel quads
-// interleaved per pixel: pixel x at float4 index F/4 * x + q, rows of `fp` bytes), and
-// writing the next map or, for the last layer, the 2x output image.  The layer's epilogue is
+// FSRCNNX<F>): one launch per layer (two for conv8_pair), reading an F-channel f32 map from
+// HBM (channel quads interleaved per pixel: pixel x at float4 index F/4 * x + q, rows of `fp`
+// bytes), and writing the next map or, for the last layer, the 2x output image.  The layer's epilogue is
 // chosen at compile time (LayerMode); every form keeps the reference's separately rounded
 // steps after the conv sum (Common.hpp:116-164, 227-288, 290-341):
 //   kRelu          max(v, 0)                                        ArtCNN body
@@ -12,12 +12,12 @@
 //   kShuffle       conv F -> 4, pixel shuffle to 2x                 ArtCNN, FSRCNNX tail
 //   kShuffleSrc    the same, + the source pixel                     ACNet<8> final
 // Two MFMA mappings (v_mfma_f32_16x16x4_f32):
-//   conv8_layer  F = 8: the 2-row item of acnet_mfma32_item.hpp, 16 pixels x 2 output rows
-//                per item, all 16 MFMA rows useful;
+//   conv8_layer, conv8_pair  F = 8: the 2-row item of acnet_mfma32_item.hpp, 16 pixels x 2
+//                output rows per item, all 16 MFMA rows useful;
 //   convf_layer  F = 16 or 32: M = 16 output channels per tile (F/16 tiles), N = 16 pixels of
 //                one row, K = 9 taps x F; lane group g supplies input channels g*F/4 ..
-//                g*F/4 + F/4 - 1 from channel-quad LDS planes; 4 items per wave batch so a
-//                tap's A operands are loaded once for 4.
+//                g*F/4 + F/4 - 1 from channel-quad LDS planes; NI items per wave batch share
+//                each tap's A operands.
 // In both, a lane's D registers hold 4 consecutive channels of one pixel, which is exactly
 // the B operand a conv1x1 MFMA needs (K step r of lane group g = the r-th of them), so the
 // fused conv1x1 costs F/4 x F/16 more MFMAs and no data movement.
