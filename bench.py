@@ -348,9 +348,11 @@ def stages():
 
 
 def acnet8(frames_np, steps=10):
-    """ACNet<8> (the reference's default model family) 1080p u8 -> 4K 2x passes, layer by
-    layer on FP32 MFMA, for 4, 8 and 18 blocks.  FLOP per input pixel: 2 x (72 + 576 blocks
-    + 288) MAC."""
+    """ACNet<8> (the reference's default model family) 1080p u8 -> 4K 2x passes on FP32 MFMA,
+    blocks two per launch, for 4, 8 and 18 blocks.  FLOP per input pixel: 2 x (72 + 576 blocks
+    + 288) MAC.  HBM bytes per input pixel: the u8 source read twice (L0, the final residual),
+    4 output bytes, and a 32-byte map written by L0, read and written once per launch of
+    blocks, and read by the final layer."""
     import torch
     from anime4kcpp_amd import hip
     dev = torch.device("cuda:0")
@@ -375,7 +377,7 @@ def acnet8(frames_np, steps=10):
         out[variant] = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
                         "blocks": net.blocks, "achieved_tflops": round(flop / sec / 1e12, 2),
                         "frac_of_fp32_peak": round(flop / sec / 1e12 / PEAK_FP32_TFLOPS, 4),
-                        "hbm_bytes_per_frame": (2 * 32 * (net.blocks + 1) + 1 + 4) * w * h}
+                        "hbm_bytes_per_frame": (2 + 4 + 32 + 64 * ((net.blocks + 1) // 2) + 32) * w * h}
         net.close()
     out["workload"] = "1920x1080 u8 -> 3840x2160 u8, ACNet<8> HDN weights, layer by layer (8-channel f32 maps in HBM)"
     return out
@@ -383,9 +385,10 @@ def acnet8(frames_np, steps=10):
 
 def layered(frames_np, family, steps=5):
     """ArtCNN (16 and 32 channels) or FSRCNNX (8 and 16 channels) 1080p u8 -> 4K 2x passes,
-    layer by layer on FP32 MFMA.  FLOP per input pixel: ArtCNN 2 x (9 F + 9 F^2 (B + 1) +
-    36 F) MAC; FSRCNNX 2 x (25 F + 9 F^2 B + F^2 + 36 F) MAC (B = 4 blocks).  HBM bytes per
-    input pixel: the u8 input and 4 outputs, plus 4F per map read or written."""
+    layer by layer on FP32 MFMA (FSRCNNX F8: blocks two per launch).  FLOP per input pixel:
+    ArtCNN 2 x (9 F + 9 F^2 (B + 1) + 36 F) MAC; FSRCNNX 2 x (25 F + 9 F^2 B + F^2 + 36 F) MAC
+    (B = 4 blocks).  HBM bytes per input pixel: the u8 input and 4 outputs, plus 4F per map
+    read or written."""
     import torch
     from anime4kcpp_amd import hip
     dev = torch.device("cuda:0")
@@ -411,7 +414,9 @@ def layered(frames_np, family, steps=5):
         if family == "artcnn":
             mac, maps = 9 * f + 9 * f * f * (b + 1) + 36 * f, 2 * (b + 2) + 1
         else:
-            mac, maps = 25 * f + 9 * f * f * b + f * f + 36 * f, 2 * b + 3
+            # head write; per launch of blocks a read and a write (the last block also reads
+            # the head's map); the tail's read -- F8 runs the blocks two per launch
+            mac, maps = 25 * f + 9 * f * f * b + f * f + 36 * f, (b + 3 if f == 8 else 2 * b + 3)
         flop = 2 * mac * w * h
         out[variant] = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
                         "channels": f, "blocks": b, "achieved_tflops": round(flop / sec / 1e12, 2),
