@@ -60,3 +60,27 @@ def test_f16_images(cuda, family, variant, cls):
     assert got.dtype == want.dtype == np.float16
     d = np.abs(got.astype(np.float64) - want.astype(np.float64))
     assert (d == 0).mean() >= 0.99 and d.max() <= 2.0 ** -11 * 2, (float((d == 0).mean()), float(d.max()))
+
+
+@pytest.mark.parametrize("model", ["acnet-b4-hdn", "artcnn-c4f16-ds", "fsrcnnx"])
+def test_threads_share_one_processor(cuda, model):
+    """One layer-by-layer processor, eight threads at once: each thread's stream, buffers and
+    feature-map workspace are its own (sizes differ per thread, so workspaces grow apart)."""
+    import threading
+    proc = P.Processor("hip", 0, model)
+    assert proc.ok(), proc.error()
+    run = pass_fn(*FAMILIES[model])
+    srcs = [synth.flat_shaded(200 + i, 40 + 7 * i, 30 + 3 * i, dtype=np.uint8) for i in range(8)]
+    outs = [None] * 8
+
+    def work(i):
+        for _ in range(3):
+            outs[i] = proc.process(srcs[i], 2.0)
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(8)]
+    [t.start() for t in ts]
+    [t.join() for t in ts]
+    for i in range(8):
+        rc, out = outs[i]
+        assert rc == 0, proc.error()
+        check_int(out, run(srcs[i]))
