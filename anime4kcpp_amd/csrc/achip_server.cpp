@@ -151,6 +151,7 @@ struct Req {
     std::string msg;
     // one-off buffers: in (RGB), luma[0..passes], uv (RGB), chroma in c0 / out c1 (YUV), out (RGB)
     Plane in, luma[4], uv, c0[2], c1[2], out;
+    Plane ws;                         // layer-by-layer nets: feature maps of the largest pass
     int pass = 0, next_row = 0;
     hipEvent_t ev_up = nullptr;       // upload done
     hipEvent_t ev_done = nullptr;     // everything done (own event, or a step's / a frame's)
@@ -163,6 +164,7 @@ struct Slot {                          // one video stream
     int w = 0, h = 0;
     hipStream_t stream = nullptr;
     Plane y0, c0[2], y1, c1[2];
+    Plane ws;                          // layer-by-layer nets: feature maps of a frame's pass
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::deque<std::shared_ptr<Req>> inflight;
@@ -188,6 +190,7 @@ struct ACHipServer {
     ACHipNet* net = nullptr;
     ACHipServerConfig cfg{};
     int config = 0, tw = 64, th = 64;
+    bool layered = false;             // ACNet<8> / ArtCNN / FSRCNNX: whole passes, not tile rows
 
     std::mutex mtx;
     std::condition_variable cv_work, cv_done;
@@ -226,20 +229,24 @@ struct ACHipServer {
     int start_frame(const std::shared_ptr<Req>& q);
     int start_oneoff(const std::shared_ptr<Req>& q);
     bool build_step();
+    bool build_layered_step(Step& step);
+    hipError_t launch_finish(const Req& q);
     void poll();
     void complete(const std::shared_ptr<Req>& q, int err, const std::string& msg);
     void free_req(Req& q);
     int alloc(Plane& p, int w, int h, int ch, int es = 1);
+    int alloc_ws(Plane& p, int w, int h);
+    hipError_t pass_launch(int precision_cfg, int T, const Plane& a, const Plane& b, const Plane& ws, hipStream_t st) const;
     bool busy_work() const;
 };
 
 namespace {
 
 // device bytes a request needs (frames of an existing stream need none)
-size_t request_bytes(const ACHipRequest& r, const ACHipFrameFormat& fmt, int type, int passes)
+size_t request_bytes(const ACHipRequest& r, const ACHipFrameFormat& fmt, int type, int passes, size_t ws)
 {
     const size_t W = r.width, H = r.height, f = static_cast<size_t>(1) << passes, es = type & 0xff;
-    size_t b = 0;
+    size_t b = ws ? round_block(ws) : 0;
     for (int k = 0; k <= passes; k++) b += round_block((W << k) * (H << k) * es);
     if (r.kind == ACHIP_REQ_RGB) {
         b += round_block(3 * W * H) + round_block(2 * W * H) + round_block(3 * f * W * f * H);
@@ -273,9 +280,34 @@ int ACHipServer::alloc(Plane& p, int w, int h, int ch, int es)
     return pool.get(p.bytes(), &p.ptr) == hipSuccess ? ACHIP_OK : ACHIP_ENOMEM;
 }
 
+int ACHipServer::alloc_ws(Plane& p, int w, int h)
+{
+    const size_t bytes = achip_net_workspace_bytes(net, w, h);
+    p.w = p.h = 0;
+    p.stride = 0;
+    return pool.get(bytes, &p.ptr) == hipSuccess ? ACHIP_OK : ACHIP_ENOMEM;
+}
+
+// One 2x pass of the server's net from plane a into plane b: the fused ACNetLegacy kernel
+// (configuration pcfg), or the layer launches of a layer-by-layer net with its feature maps
+// in ws.  Kernels only, so frames can capture it into their graphs.
+hipError_t ACHipServer::pass_launch(int pcfg, int T, const Plane& a, const Plane& b, const Plane& ws, hipStream_t st) const
+{
+    if (net->blocks)
+        return achip::launch_acnet8(net->blocks, T, T, a.ptr, a.stride, a.w, a.h, b.ptr, b.stride, net->a8wts, net->a8wa, ws.ptr, st);
+    if (net->lfamily == kNetArtCNN)
+        return achip::launch_artcnn(net->lf, net->lblocks, T, T, a.ptr, a.stride, a.w, a.h, b.ptr, b.stride, net->lk, net->lb,
+                                    net->lwa, ws.ptr, st);
+    if (net->lfamily == kNetFSRCNNX)
+        return achip::launch_fsrcnnx(net->lf, net->lblocks, T, T, a.ptr, a.stride, a.w, a.h, b.ptr, b.stride, net->lk, net->lb,
+                                     net->la, net->lwa, net->lw1, ws.ptr, st);
+    return achip::launch_pass(net, cfg.precision, pcfg, T, T, a.ptr, a.stride, a.w, a.h, b.ptr, b.stride, st);
+}
+
 void ACHipServer::free_req(Req& q)
 {
-    for (Plane* p : { &q.in, &q.luma[0], &q.luma[1], &q.luma[2], &q.luma[3], &q.uv, &q.c0[0], &q.c0[1], &q.c1[0], &q.c1[1], &q.out }) {
+    for (Plane* p : { &q.in, &q.luma[0], &q.luma[1], &q.luma[2], &q.luma[3], &q.uv, &q.c0[0], &q.c0[1], &q.c1[0], &q.c1[1], &q.out,
+                      &q.ws }) {
         pool.put(p->ptr);
         p->ptr = nullptr;
     }
@@ -357,6 +389,7 @@ int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
             e = alloc(s.c0[i], c.w, c.h, c.ch, es);
             if (!e) e = alloc(s.c1[i], 2 * c.w, 2 * c.h, c.ch, es);
         }
+        if (!e && layered) e = alloc_ws(s.ws, r.width, r.height);
         if (e) return e;
         s.w = r.width;
         s.h = r.height;
@@ -364,9 +397,7 @@ int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
         if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ACHIP_EIO;
         hipError_t he = hipSuccess;
         if (f.shift) he = achip::launch_shift(T, 1, f.shift, s.y0.ptr, s.y0.stride, s.y0.ptr, s.y0.stride, s.w, s.h, s.stream);
-        if (he == hipSuccess)
-            he = achip::launch_pass(net, cfg.precision, pcfg, T, T, s.y0.ptr, s.y0.stride, s.w, s.h, s.y1.ptr, s.y1.stride,
-                                    s.stream);
+        if (he == hipSuccess) he = pass_launch(pcfg, T, s.y0, s.y1, s.ws, s.stream);
         if (he == hipSuccess && f.shift)
             he = achip::launch_shift(T, 0, f.shift, s.y1.ptr, s.y1.stride, s.y1.ptr, s.y1.stride, s.y1.w, s.y1.h, s.stream);
         for (int i = 0; i < c.n && he == hipSuccess; i++)
@@ -415,6 +446,7 @@ int ACHipServer::start_oneoff(const std::shared_ptr<Req>& q)
         e = alloc(q->c0[i], c.w, c.h, c.ch, es);
         if (!e) e = alloc(q->c1[i], f * c.w, f * c.h, c.ch, es);
     }
+    if (!e && layered) e = alloc_ws(q->ws, W << (P - 1), H << (P - 1));   // the largest pass
     if (e) return e;
     if (r.kind == ACHIP_REQ_RGB) e = copy_plane(q->in.ptr, q->in.stride, r.src[0], r.src_stride[0], q->in.row(), H, upload);
     else e = copy_plane(q->luma[0].ptr, q->luma[0].stride, r.src[0], r.src_stride[0], q->luma[0].row(), H, upload);
@@ -439,6 +471,8 @@ bool ACHipServer::build_step()
     int inflight = 0;
     for (const Step& s : steps) inflight += s.busy ? 1 : 0;
     if (inflight >= kRing - 2) return false;
+
+    if (layered) return build_layered_step(step);
 
     // descriptors of u8 passes first, then u16 passes: one persistent launch per element
     // type, each with its own tile counter (bytes 0 and 64 of the header) and tile numbering
@@ -489,18 +523,68 @@ bool ACHipServer::build_step()
                                 reinterpret_cast<int*>(ws + 64), cfg.max_workgroups, batch);
     for (const std::shared_ptr<Req>& q : finishing) {
         if (e != hipSuccess) break;
-        const int P = q->passes, T = q->type;
-        const Plane& y = q->luma[P];
-        if (q->r.kind == ACHIP_REQ_RGB) {
-            e = achip::launch_resize_merge(kU8, 3, achip::kResizeCatmullRom, y.ptr, y.stride, q->uv.ptr, q->uv.stride, q->uv.w,
-                                           q->uv.h, y.w, y.h, q->out.ptr, q->out.stride, batch);
-            continue;
+        e = launch_finish(*q);
+    }
+    if (e == hipSuccess) e = hipEventRecord(step.ev, batch);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(batch);   // nothing of this step may still use the buffers
+        for (const std::shared_ptr<Req>& q : members) complete(q, ACHIP_EIO, "step launch failed");
+        active.erase(std::remove_if(active.begin(), active.end(), [](const std::shared_ptr<Req>& q) { return q->st == St::Done; }),
+                     active.end());
+        return false;
+    }
+    step.busy = true;
+    step.finishing = std::move(finishing);
+    step_next = (step_next + 1) % kRing;
+    std::lock_guard<std::mutex> lk(mtx);
+    nsteps++;
+    return true;
+}
+
+// The kernels after a one-off's last pass: chroma resize fused with the merge (RGB), or
+// the LSB shift and the chroma resizes (luma / YUV).
+hipError_t ACHipServer::launch_finish(const Req& q)
+{
+    const int P = q.passes, T = q.type;
+    const Plane& y = q.luma[P];
+    if (q.r.kind == ACHIP_REQ_RGB)
+        return achip::launch_resize_merge(kU8, 3, achip::kResizeCatmullRom, y.ptr, y.stride, q.uv.ptr, q.uv.stride, q.uv.w,
+                                          q.uv.h, y.w, y.h, q.out.ptr, q.out.stride, batch);
+    hipError_t e = hipSuccess;
+    if (q.fmt.shift) e = achip::launch_shift(T, 0, q.fmt.shift, y.ptr, y.stride, y.ptr, y.stride, y.w, y.h, batch);
+    const Chroma c = chroma_of(q.fmt, q.r.width, q.r.height);
+    for (int i = 0; i < c.n && e == hipSuccess; i++)
+        e = achip::launch_resize(T, c.ch, chroma_mode(q.fmt), q.c0[i].ptr, q.c0[i].stride, q.c0[i].w, q.c0[i].h, q.c1[i].ptr,
+                                 q.c1[i].stride, q.c1[i].w, q.c1[i].h, batch);
+    return e;
+}
+
+// A step of a layer-by-layer net: the admitted one-offs' next passes, each whole (its layer
+// launches in turn on the batch stream), in admission order until the step's pixel budget
+// (step_tiles x the tile area) is spent; then the finishing kernels of the requests whose
+// last pass this step ran.
+bool ACHipServer::build_layered_step(Step& step)
+{
+    long budget = static_cast<long>(cfg.step_tiles) * tw * th;
+    std::vector<std::shared_ptr<Req>> finishing, members;
+    hipError_t e = hipSuccess;
+    for (const std::shared_ptr<Req>& q : active) {
+        if (budget <= 0 || e != hipSuccess) break;
+        if (q->st != St::Passing) continue;
+        members.push_back(q);
+        const Plane& a = q->luma[q->pass];
+        e = pass_launch(0, q->type, a, q->luma[q->pass + 1], q->ws, batch);
+        budget -= static_cast<long>(a.w) * a.h;
+        if (++q->pass == q->passes) {
+            q->st = St::Finishing;
+            finishing.push_back(q);
         }
-        if (q->fmt.shift) e = achip::launch_shift(T, 0, q->fmt.shift, y.ptr, y.stride, y.ptr, y.stride, y.w, y.h, batch);
-        const Chroma c = chroma_of(q->fmt, q->r.width, q->r.height);
-        for (int i = 0; i < c.n && e == hipSuccess; i++)
-            e = achip::launch_resize(T, c.ch, chroma_mode(q->fmt), q->c0[i].ptr, q->c0[i].stride, q->c0[i].w, q->c0[i].h,
-                                     q->c1[i].ptr, q->c1[i].stride, q->c1[i].w, q->c1[i].h, batch);
+    }
+    if (members.empty()) return false;
+    for (const std::shared_ptr<Req>& q : finishing) {
+        if (e != hipSuccess) break;
+        e = launch_finish(*q);
     }
     if (e == hipSuccess) e = hipEventRecord(step.ev, batch);
     if (e != hipSuccess) {
@@ -638,15 +722,17 @@ int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipSer
 {
     if (!net || !out) return ACHIP_EINVAL;
     *out = nullptr;
-    if (net->blocks || net->lfamily) return ACHIP_EINVAL;   // the serving loop runs ACNetLegacy nets
     ACHipServerConfig c = config ? *config : ACHipServerConfig{};
     if (c.precision != ACHIP_FP32 && c.precision != ACHIP_FP16 && c.precision != ACHIP_BF16) return ACHIP_EINVAL;
+    const bool layered = net->blocks || net->lfamily;
+    if (layered && c.precision != ACHIP_FP32) return ACHIP_EINVAL;   // those families run in FP32 only
     if (c.hbm_budget == 0) c.hbm_budget = static_cast<size_t>(16) << 30;
     if (c.step_tiles <= 0) c.step_tiles = c.precision == ACHIP_FP32 ? 2048 : 6144;
     if (c.max_workgroups < 0) return ACHIP_EINVAL;
     auto s = std::make_unique<ACHipServer>();
     s->net = net;
     s->cfg = c;
+    s->layered = layered;
     s->config = 0;
     if (achip_config_info(c.precision, s->config, &s->tw, &s->th, nullptr, nullptr, nullptr) != ACHIP_OK) return ACHIP_EINVAL;
     int dev = 0;
@@ -690,7 +776,7 @@ void achip_server_destroy(ACHipServer* s)
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
         if (sl.graph) (void)hipGraphDestroy(sl.graph);
-        for (Plane* p : { &sl.y0, &sl.c0[0], &sl.c0[1], &sl.y1, &sl.c1[0], &sl.c1[1] }) s->pool.put(p->ptr);
+        for (Plane* p : { &sl.y0, &sl.c0[0], &sl.c0[1], &sl.y1, &sl.c1[0], &sl.c1[1], &sl.ws }) s->pool.put(p->ptr);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     for (auto& kv : s->tickets) s->free_req(*kv.second);
@@ -755,7 +841,10 @@ int achip_server_submit(ACHipServer* s, const ACHipRequest* r, unsigned long lon
     q->fmt = f;
     q->type = type;
     q->passes = passes;
-    q->need = r->stream >= 0 ? request_bytes(*r, f, type, 1) : request_bytes(*r, f, type, passes);
+    const bool layered = s->net->blocks || s->net->lfamily;
+    const size_t ws = layered ? achip_net_workspace_bytes(s->net, r->width << (passes - 1), r->height << (passes - 1)) : 0;
+    q->need = r->stream >= 0 ? request_bytes(*r, f, type, 1, layered ? achip_net_workspace_bytes(s->net, r->width, r->height) : 0)
+                             : request_bytes(*r, f, type, passes, ws);
     if (q->need > s->cfg.hbm_budget) return ACHIP_ENOMEM;
     std::lock_guard<std::mutex> lk(s->mtx);
     if (s->stop) return ACHIP_EIO;

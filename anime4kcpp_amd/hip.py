@@ -215,6 +215,10 @@ class Net8:
         except Exception:
             pass
 
+    @property
+    def handle(self):
+        return self._h
+
     def workspace(self, w, h, dev):
         import torch
         need = _native.lib().achip_pass8_workspace_bytes(w, h)

@@ -257,6 +257,11 @@ typedef struct ACHipServerStats {
     long steps, graph_replays;
 } ACHipServerStats;
 
+/* A server runs one net.  ACNetLegacy nets run in any precision, their one-off passes
+ * batched into persistent tile-queue launches; ACNet<8>, ArtCNN and FSRCNNX nets run in
+ * ACHIP_FP32 only (ACHIP_EINVAL otherwise), each step running the admitted one-offs' next
+ * passes whole, with a feature-map workspace per request (and per video stream) counted
+ * against the HBM budget. */
 ACHIP_API int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipServer** out);
 /* Waits for all admitted work, then frees everything. */
 ACHIP_API void achip_server_destroy(ACHipServer* server);

@@ -172,3 +172,32 @@ def test_invalid_requests(net):
         g = yuv(4, 96, 48)
         with pytest.raises(hip._native.AchipError):
             srv.submit(hip.REQ_YUV420, g, 2, 1)                     # a stream keeps its geometry
+
+
+@pytest.mark.parametrize("model,cls,variant", [("acnet-b4-hdn", hip.Net8, "b4_hdn"), ("artcnn-c4f16-ds", hip.NetArt, "f16_ds"),
+                                               ("fsrcnnx", hip.NetFsr, "f8")])
+def test_layered_nets_serve(cuda, model, cls, variant):
+    """ACNet<8>, ArtCNN and FSRCNNX nets in the serving loop (FP32): the mixed load of one-offs
+    and frames gives every request the same result as alone, and RGB one-offs equal
+    Processor::process with the same model."""
+    net = cls(variant, 0)
+    reqs = make_requests()
+    with hip.Server(net, "fp32") as alone_srv:
+        alone = run(alone_srv, reqs, together=False)
+    with hip.Server(net, "fp32", step_tiles=16) as mix_srv:
+        mixed = run(mix_srv, reqs, together=True)
+        st = mix_srv.stats()
+    assert st["failed"] == 0 and st["completed_frames"] == 8 and st["completed_oneoffs"] == 6
+    assert st["graph_replays"] == 8 and st["steps"] >= 3      # whole passes: up to 2 per one-off
+    for i, (a, m) in enumerate(zip(alone, mixed)):
+        for pa, pm in zip(a, m):
+            assert np.array_equal(pa, pm), f"request {i} differs between alone and mixed"
+    proc = P.Processor("hip", 0, model)
+    assert proc.ok(), proc.error()
+    for i, (k, p, f, s) in enumerate(reqs):
+        if k == hip.REQ_RGB:
+            rc, want = proc.process(p[0], f)
+            assert rc == 0 and np.array_equal(mixed[i][0], want), f"request {i}"
+    with pytest.raises(Exception):
+        hip.Server(net, "fp16")                       # these families run in FP32 only
+    net.close()
