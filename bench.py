@@ -561,6 +561,11 @@ def main():
         for spec in args.serving_loads.split(";"):
             p, loads = spec.split(":")
             result["serving"][p] = serving(nets["hdn0"], p, [float(x) for x in loads.split(",")], args.serving_seconds)
+        if not args.no_acnet8:   # the reference's default model (acnet-f8b8-hdn), FP32
+            from anime4kcpp_amd import hip as _hip
+            n8 = _hip.Net8("b8_hdn", 0)
+            result["serving"]["acnet8_b8_hdn_fp32"] = serving(n8, "fp32", [60.0, 240.0], args.serving_seconds)
+            n8.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(frames_np, weights)
         result["cpu_baseline"]["gpu_over_cpu"] = round(value / result["cpu_baseline"]["value"], 1)
