@@ -56,6 +56,37 @@ __device__ inline float prelu(float v, float alpha)
 
 __device__ inline float add_skip(float v, float f) { return __fadd_rn(__fmul_rn(v, 1.0f), f); }
 
+// Stage ROWS x CW pixels of an f32 map (NQ channel quads per pixel, rows of fp bytes) whose
+// top-left is image pixel (cx0, ry0), clamped to the image, into LDS: quad q of staged pixel
+// (c, r) goes to planes[q][r * PW + c].  Each thread issues all of its loads before its first
+// LDS store, so a workgroup waits for one load latency rather than one per pass over the tile.
+template <int NT, int CW, int ROWS, int NQ, int PW>
+__device__ __forceinline__ void stage_map(const float4* __restrict__ fin, long fp, int W, int H, int cx0, int ry0,
+                                          float4* const* planes)
+{
+    constexpr int N = ROWS * CW, K = (N + NT - 1) / NT;
+    float4 v[K][NQ];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        // threads past the last pixel load it again (no branch around the loads) and skip the store
+        const int e = min(static_cast<int>(threadIdx.x) + k * NT, N - 1);
+        const int r = e / CW, c = e - r * CW;
+        const int sy = clampi(ry0 + r, 0, H - 1), sx = clampi(cx0 + c, 0, W - 1);
+        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + NQ * sx;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) v[k][q] = px[q];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int e = threadIdx.x + k * NT;
+        if (k < K - 1 || e < N) {
+            const int r = e / CW, c = e - r * CW;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) planes[q][r * PW + c] = v[k][q];
+        }
+    }
+}
+
 // ---- F = 8: the 2-row item ----
 // Tile: TW x TH output pixels, NT threads; the input region (with the one-pixel halo) staged
 // in LDS as two float4 planes (channels 0-3, 4-7) with a row pitch that is a multiple of 16
@@ -155,12 +186,9 @@ __global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ 
     const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
 
     // stage rows y0-1 .. y0+TH, columns x0-1 .. x0+TW (replicated at the image edges)
-    for (int e = tid; e < kC8PH * (kC8TW + 2); e += C::NT) {
-        const int r = e / (kC8TW + 2), c = e - r * (kC8TW + 2);
-        const int sy = clampi(y0 - 1 + r, 0, H - 1), sx = clampi(x0 - 1 + c, 0, W - 1);
-        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + 2 * sx;
-        pa[r * kC8PW + c] = px[0];
-        pb[r * kC8PW + c] = px[1];
+    {
+        float4* const pl[2] = { pa, pb };
+        stage_map<C::NT, kC8TW + 2, kC8PH, 2, kC8PW>(fin, fp, W, H, x0 - 1, y0 - 1, pl);
     }
     float a[24];
 #pragma unroll
@@ -206,12 +234,9 @@ __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin
     const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
 
     // plane pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r); stage 66 x 20, clamped
-    for (int e = tid; e < kP8PH * (kP8TW + 4); e += 256) {
-        const int r = e / (kP8TW + 4), c = e - r * (kP8TW + 4);
-        const int sy = clampi(y0 - 2 + r, 0, H - 1), sx = clampi(x0 - 2 + c, 0, W - 1);
-        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + 2 * sx;
-        pa[r * kP8PW + c] = px[0];
-        pb[r * kP8PW + c] = px[1];
+    {
+        float4* const pl[2] = { pa, pb };
+        stage_map<256, kP8TW + 4, kP8PH, 2, kP8PW>(fin, fp, W, H, x0 - 2, y0 - 2, pl);
     }
     float a[24];
 #pragma unroll
@@ -305,12 +330,11 @@ __global__ __launch_bounds__(C::NT) void convf_layer(const float4* __restrict__ 
     const int g = lane >> 4, j = lane & 15;
 
     // stage rows y0-1 .. y0+TH, columns x0-1 .. x0+TW (replicated at the image edges)
-    for (int e = tid; e < C::PH * (C::TW + 2); e += C::NT) {
-        const int r = e / (C::TW + 2), c = e - r * (C::TW + 2);
-        const int sy = clampi(y0 - 1 + r, 0, H - 1), sx = clampi(x0 - 1 + c, 0, W - 1);
-        const float4* px = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(fin) + static_cast<long>(sy) * fp) + C::NQ * sx;
+    {
+        float4* pl[C::NQ];
 #pragma unroll
-        for (int q = 0; q < C::NQ; q++) planes[q][r * C::PW + c] = px[q];
+        for (int q = 0; q < C::NQ; q++) pl[q] = planes[q];
+        stage_map<C::NT, C::TW + 2, C::PH, C::NQ, C::PW>(fin, fp, W, H, x0 - 1, y0 - 1, pl);
     }
     // lane (g, j) of output tile h owns channels 16 h + 4 g .. + 3
     f32x4m b0[C::MT];
