@@ -347,7 +347,7 @@ def stages():
     return out
 
 
-def acnet8(frames_np, steps=10):
+def acnet8(frames_np, steps=25):
     """ACNet<8> (the reference's default model family) 1080p u8 -> 4K 2x passes on FP32 MFMA,
     blocks two per launch, for 4, 8 and 18 blocks.  FLOP per input pixel: 2 x (72 + 576 blocks
     + 288) MAC.  HBM bytes per input pixel: the u8 source read twice (L0, the final residual),
@@ -362,8 +362,9 @@ def acnet8(frames_np, steps=10):
     out = {}
     for variant in ("b4_hdn", "b8_hdn", "b18_hdn"):
         net = hip.Net8(variant, 0)
-        for i in range(len(srcs)):
-            net.upscale(srcs[i], dsts[i])
+        for _ in range(3):   # warm-up: clocks, caches, first-launch costs
+            for i in range(len(srcs)):
+                net.upscale(srcs[i], dsts[i])
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -383,7 +384,7 @@ def acnet8(frames_np, steps=10):
     return out
 
 
-def layered(frames_np, family, steps=5):
+def layered(frames_np, family, steps=10):
     """ArtCNN (16 and 32 channels) or FSRCNNX (8 and 16 channels) 1080p u8 -> 4K 2x passes,
     layer by layer on FP32 MFMA (FSRCNNX F8: blocks two per launch).  FLOP per input pixel:
     ArtCNN 2 x (9 F + 9 F^2 (B + 1) + 36 F) MAC; FSRCNNX 2 x (25 F + 9 F^2 B + F^2 + 36 F) MAC
@@ -399,8 +400,9 @@ def layered(frames_np, family, steps=5):
     cls, variants = (hip.NetArt, ("f16", "f32")) if family == "artcnn" else (hip.NetFsr, ("f8", "f16"))
     for variant in variants:
         net = cls(variant, 0)
-        for i in range(len(srcs)):
-            net.upscale(srcs[i], dsts[i])
+        for _ in range(3):   # warm-up: clocks, caches, first-launch costs
+            for i in range(len(srcs)):
+                net.upscale(srcs[i], dsts[i])
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
