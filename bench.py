@@ -58,12 +58,32 @@ def cpu_baseline(frames_u8, weights, min_seconds=10.0):
         if el >= min_seconds or n >= 200:
             break
     mpix = n * 4 * W_IN * H_IN / 1e6 / el
-    return {"value": round(mpix, 2), "unit": "Mpix/s", "cores": threads,
-            "kind": "reference" if use_ref else "port",
-            "sample": f"{n} passes of one 1920x1080 u8 frame (HDN0) in {el:.1f} s, "
-                      + (f"reference per-ISA kernels (arch {arch}) built from its sources"
-                         if use_ref else "oracle restatement (FMA order)")
-                      + f", OpenMP {threads} threads"}
+    out = {"value": round(mpix, 2), "unit": "Mpix/s", "cores": threads,
+           "kind": "reference" if use_ref else "port",
+           "sample": f"{n} passes of one 1920x1080 u8 frame (HDN0) in {el:.1f} s, "
+                     + (f"reference per-ISA kernels (arch {arch}) built from its sources"
+                        if use_ref else "oracle restatement (FMA order)")
+                     + f", OpenMP {threads} threads"}
+    # the same passes on one thread (OpenMP's thread count of this thread's next parallel regions)
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+    except OSError:
+        return out
+    gomp.omp_set_num_threads(1)
+    try:
+        n1, t1 = 0, time.perf_counter()
+        while True:
+            run()
+            n1 += 1
+            el1 = time.perf_counter() - t1
+            if el1 >= min_seconds / 3 or n1 >= 50:
+                break
+    finally:
+        gomp.omp_set_num_threads(threads)
+    out["one_thread"] = {"value": round(n1 * 4 * W_IN * H_IN / 1e6 / el1, 2), "unit": "Mpix/s",
+                         "sample": f"{n1} pass(es) of the same frame in {el1:.1f} s on one thread"}
+    return out
 
 
 def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, stream):
