@@ -97,6 +97,11 @@ namespace ac::core::hip
     };
 
     inline size_t alignUp(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+    const char* precisionName(const int precision) noexcept
+    {
+        return precision == 1 ? "FP16" : (precision == 2 ? "BF16" : "FP32");
+    }
 }
 
 ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device, const model::ACNetLegacy& model, const int precision) noexcept
@@ -119,7 +124,7 @@ ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device
 {
     if (precision != 0)
     {
-        initError = "HIP processor: ACNet<8> runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's)";
+        initError = std::string("HIP processor: ACNet<8> runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's); the type string asked for ") + precisionName(precision);
         return;
     }
     const int rc = achip_net8_create(device, model.blocks(), model.kernel(0), model.bias(0), model.alpha(0), &net);
@@ -138,7 +143,7 @@ ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device
 {
     if (precision != 0)
     {
-        initError = "HIP processor: ArtCNN runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's)";
+        initError = std::string("HIP processor: ArtCNN runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's); the type string asked for ") + precisionName(precision);
         return;
     }
     const int rc = achip_net_artcnn_create(device, F, model.blocks(), model.kernel(0), model.bias(0), &net);
@@ -157,7 +162,7 @@ ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device
 {
     if (precision != 0)
     {
-        initError = "HIP processor: FSRCNNX runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's)";
+        initError = std::string("HIP processor: FSRCNNX runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's); the type string asked for ") + precisionName(precision);
         return;
     }
     const int rc = achip_net_fsrcnnx_create(device, F, model.blocks(), model.kernel(0), model.bias(0), model.alpha(0), &net);

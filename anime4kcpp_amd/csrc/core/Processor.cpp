@@ -33,14 +33,19 @@ namespace ac::core::detail
     }
 
     // Arithmetic of the HIP path's 8->8 layers from the type string: "hip" (FP32),
-    // "hip-fp16" / "hip:fp16", "hip-bf16" / "hip:bf16".
+    // "hip-fp16" / "hip:fp16" / "hip-f16" (FP16), "hip-bf16" / "hip:bf16" (BF16).  The
+    // suffix after the first '-' or ':' is matched whole, so "bf16" (which contains
+    // "f16") can never select FP16; any other suffix means FP32.
     int findPrecision(const char* type) noexcept
     {
         if (!type) return 0;
         std::string s = type;
         for (char& ch : s) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
-        if (s.find("fp16") != std::string::npos || s.find("f16") != std::string::npos) return 1;
-        if (s.find("bf16") != std::string::npos) return 2;
+        const auto cut = s.find_first_of("-:");
+        if (cut == std::string::npos) return 0;
+        const std::string suffix = s.substr(cut + 1);
+        if (suffix == "bf16") return 2;
+        if (suffix == "fp16" || suffix == "f16") return 1;
         return 0;
     }
 

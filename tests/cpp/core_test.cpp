@@ -137,6 +137,25 @@ static void processor_factory(const bool gpu)
         if (gpu) CHECK(p->ok());
         else CHECK(!p->ok() && std::strstr(p->error(), "no HIP device") != nullptr);
     }
+    // precision suffixes, parsed whole: "bf16" contains "f16" and must still mean BF16.
+    // The FP32-only families reject 16-bit types before touching a device and name the
+    // precision they were asked for, so the parse is visible without a GPU.
+    const struct { const char* type; const char* want; } suffixes[] = {
+        { "hip-bf16", "BF16" }, { "hip:bf16", "BF16" }, { "HIP-BF16", "BF16" },
+        { "hip-fp16", "FP16" }, { "hip:fp16", "FP16" }, { "hip-f16", "FP16" } };
+    for (const auto& c : suffixes)
+    {
+        auto q = ac::core::Processor::create(c.type, 0, "acnet-b8");
+        CHECK(!q->ok() && std::strstr(q->error(), "FP32 only") != nullptr);
+        const char* tail = std::strstr(q->error(), "asked for ");
+        CHECK(tail != nullptr && std::strcmp(tail + 10, c.want) == 0);
+    }
+    for (const char* fp32 : { "hip", "hip-", "hip-fp32", "hip-bf16x" })
+    {
+        auto q = ac::core::Processor::create(fp32, 0, "acnet-b8");
+        if (gpu) CHECK(q->ok());
+        else CHECK(!q->ok() && std::strstr(q->error(), "no HIP device") != nullptr);
+    }
     auto p = ac::core::Processor::create("HIP", 0, "acnet-legacy-hdn2");
     CHECK(static_cast<bool>(p) && p->type() == ac::core::Processor::HIP && std::strcmp(p->typeName(), "HIP") == 0);
     auto a = ac::core::Processor::create("auto", -1, nullptr);

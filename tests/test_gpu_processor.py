@@ -116,3 +116,25 @@ def test_threads_share_one_processor(proc, weights):
         rc, out = outs[i]
         assert rc == 0
         check_int(out, oracle_pass(srcs[i], *weights["hdn0"]))
+
+
+@pytest.mark.parametrize("type_str,precision", [("hip-fp16", "fp16"), ("hip:fp16", "fp16"),
+                                                ("hip-bf16", "bf16"), ("hip:bf16", "bf16"), ("hip", "fp32")])
+def test_type_string_selects_precision(cuda, type_str, precision):
+    """A processor built from a 16-bit type string runs that precision's kernel: its
+    output equals achip_pass at the same precision bit for bit (and the BF16 and FP16
+    forms differ from each other on this content)."""
+    import torch
+    from anime4kcpp_amd import hip
+    src = synth.flat_shaded(26, 150, 97, dtype=np.uint8)
+    p = P.Processor(type_str, 0, "acnet-legacy-hdn1")
+    assert p.ok(), p.error()
+    rc, out = p.process(src, 2.0)
+    assert rc == 0, p.error()
+    net = hip.Net("hdn1", 0)
+    want = {}
+    for prec in ("fp32", "fp16", "bf16"):
+        want[prec] = net.upscale(torch.from_numpy(src).to(cuda), precision=prec).cpu().numpy()
+    net.close()
+    assert not np.array_equal(want["fp16"], want["bf16"])
+    assert np.array_equal(out, want[precision])
