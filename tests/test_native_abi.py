@@ -80,3 +80,63 @@ def test_c_api_image_semantics():
     L.ac_image_free(ctypes.byref(v))
     L.ac_image_free(ctypes.byref(c))
     assert not v and not c
+
+
+def test_c_api_handle_lifecycle():
+    """binding/c/src/Binding.cpp:33-105,161-189 semantics: ref shares pixels, unref zeroes
+    the struct, free nulls the caller's pointer, every call on a struct without a handle
+    (or a NULL struct) is AC_EINVAL / NULL, and processor ref copies device/type/model."""
+    import numpy as np
+    L = _native.lib()
+    a = np.arange(6 * 9, dtype=np.uint8).reshape(6, 9)
+    src = L.ac_image_alloc()
+    s = src.contents
+    s.width, s.height, s.channels, s.element_type, s.stride = 9, 6, 1, 0x001, 9
+    s.ptr = a.ctypes.data
+    assert L.ac_image_map(src) == 0 and s.ptr == a.ctypes.data
+    r = L.ac_image_alloc()
+    assert L.ac_image_ref(src, r) == 0
+    assert (r.contents.ptr, r.contents.width, r.contents.height, r.contents.stride) == (a.ctypes.data, 9, 6, 9)
+    # a second ref into an existing handle replaces what it held
+    c = L.ac_image_alloc()
+    assert L.ac_image_clone(src, c) == 0 and c.contents.ptr != a.ctypes.data
+    assert L.ac_image_ref(src, c) == 0 and c.contents.ptr == a.ctypes.data
+    # create on a zeroed struct gives an empty image with a handle
+    e = L.ac_image_alloc()
+    assert L.ac_image_create(e) == 0 and not e.contents.ptr and e.contents.hptr
+    # unref zeroes every field; calls on a struct without a handle are rejected
+    L.ac_image_unref(r)
+    rc = r.contents
+    assert (rc.width, rc.height, rc.channels, rc.stride, rc.element_type) == (0, 0, 0, 0, 0)
+    assert not rc.ptr and not rc.hptr
+    out = np.zeros(54, np.uint8)
+    assert L.ac_image_to(r, out.ctypes.data, 9) == -22
+    assert L.ac_image_view(r, c, 0, 0, 1, 1) == -22
+    assert L.ac_image_to(src, None, 9) == -22
+    assert L.ac_image_to(src, out.ctypes.data, 9) == 0 and np.array_equal(out.reshape(6, 9), a)
+    assert L.ac_image_create(None) == -22 and L.ac_image_map(None) == -22 and L.ac_image_from(None, None) == -22
+    assert L.ac_rgb2yuv(None, r) == -22 and L.ac_resize(src, r, 2.0, 2.0, 0) == -22
+    L.ac_image_unref(None)
+    L.ac_image_free(None)
+    for p in (src, r, c, e):
+        L.ac_image_free(ctypes.byref(p))
+        assert not p
+
+    # processors: queries without a created processor
+    p = L.ac_processor_alloc()
+    assert L.ac_processor_ok(p) == -22 and L.ac_processor_type(p) == -22
+    assert L.ac_processor_error(p) is None and L.ac_processor_name(p) is None and L.ac_processor_type_name(p) is None
+    assert L.ac_processor_create(None) == -22 and L.ac_processor_ref(None, p) == -22
+    assert L.ac_processor_process(p, None, None, 2.0) == -22
+    t, m = b"cpu", b"acnet-legacy-hdn0"
+    p.contents.device, p.contents.type, p.contents.model = 3, t, m
+    assert L.ac_processor_create(p) == -256 and L.ac_processor_type(p) == 0
+    q = L.ac_processor_alloc()
+    assert L.ac_processor_ref(p, q) == 0
+    assert q.contents.device == 3 and q.contents.type == t and q.contents.model == m
+    assert L.ac_processor_type(q) == 0 and L.ac_processor_type_name(q) == b"CPU"
+    L.ac_processor_unref(q)
+    assert not q.contents.hptr and q.contents.device == 0 and not q.contents.type
+    for x in (p, q):
+        L.ac_processor_free(ctypes.byref(x))
+        assert not x
