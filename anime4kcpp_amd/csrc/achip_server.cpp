@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -146,7 +147,7 @@ struct Req {
     int passes = 0;
     size_t need = 0;                  // bytes reserved against the budget
     Clock::time_point t_submit, t_done;
-    St st = St::Pending;
+    std::atomic<St> st{ St::Pending };   // written by the scheduler, read by waiters
     int err = ACHIP_OK;
     std::string msg;
     // one-off buffers: in (RGB), luma[0..passes], uv (RGB), chroma in c0 / out c1 (YUV), out (RGB)
@@ -235,6 +236,8 @@ struct ACHipServer {
     void complete(const std::shared_ptr<Req>& q, int err, const std::string& msg);
     void free_req(Req& q);
     int alloc(Plane& p, int w, int h, int ch, int es = 1);
+    int setup_slot(Slot& s, const std::shared_ptr<Req>& q, const Chroma& c);
+    void reset_slot(Slot& s);
     int alloc_ws(Plane& p, int w, int h);
     hipError_t pass_launch(int precision_cfg, int T, const Plane& a, const Plane& b, const Plane& ws, hipStream_t st) const;
     bool busy_work() const;
@@ -374,38 +377,77 @@ void ACHipServer::admit_locked(std::vector<std::shared_ptr<Req>>& frames, std::v
     }
 }
 
-int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
+// Stream, buffers and the captured graph of a stream's first frame.  The slot's geometry
+// (s.w, s.h) is committed only once the graph is instantiated.
+int ACHipServer::setup_slot(Slot& s, const std::shared_ptr<Req>& q, const Chroma& c)
 {
     const ACHipRequest& r = q->r;
     const ACHipFrameFormat& f = q->fmt;
     const int T = q->type, es = T & 0xff;
-    const Chroma c = chroma_of(f, r.width, r.height);
+    if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) {
+        s.stream = nullptr;
+        return ACHIP_EIO;
+    }
+    int e = alloc(s.y0, r.width, r.height, 1, es);
+    if (!e) e = alloc(s.y1, 2 * r.width, 2 * r.height, 1, es);
+    for (int i = 0; i < c.n && !e; i++) {
+        e = alloc(s.c0[i], c.w, c.h, c.ch, es);
+        if (!e) e = alloc(s.c1[i], 2 * c.w, 2 * c.h, c.ch, es);
+    }
+    if (!e && layered) e = alloc_ws(s.ws, r.width, r.height);
+    if (e) return e;
+    const int w = r.width, h = r.height;
+    const int pcfg = achip::auto_config(net, cfg.precision, T, T, w, h);
+    if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ACHIP_EIO;
+    hipError_t he = hipSuccess;
+    if (f.shift) he = achip::launch_shift(T, 1, f.shift, s.y0.ptr, s.y0.stride, s.y0.ptr, s.y0.stride, w, h, s.stream);
+    if (he == hipSuccess) he = pass_launch(pcfg, T, s.y0, s.y1, s.ws, s.stream);
+    if (he == hipSuccess && f.shift)
+        he = achip::launch_shift(T, 0, f.shift, s.y1.ptr, s.y1.stride, s.y1.ptr, s.y1.stride, s.y1.w, s.y1.h, s.stream);
+    for (int i = 0; i < c.n && he == hipSuccess; i++)
+        he = achip::launch_resize(T, c.ch, chroma_mode(f), s.c0[i].ptr, s.c0[i].stride, s.c0[i].w, s.c0[i].h, s.c1[i].ptr,
+                                  s.c1[i].stride, s.c1[i].w, s.c1[i].h, s.stream);
+    const hipError_t ce = hipStreamEndCapture(s.stream, &s.graph);
+    if (ce != hipSuccess) s.graph = nullptr;
+    if (he != hipSuccess || ce != hipSuccess) return ACHIP_EIO;
+    if (hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0) != hipSuccess) {
+        s.exec = nullptr;
+        return ACHIP_EIO;
+    }
+    s.w = w;
+    s.h = h;
+    return ACHIP_OK;
+}
+
+// Everything a slot holds, returned; nothing of its own is in flight when this runs (its
+// first frame failed before any launch, or the server is closing).
+void ACHipServer::reset_slot(Slot& s)
+{
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    if (s.exec) (void)hipGraphExecDestroy(s.exec);
+    if (s.graph) (void)hipGraphDestroy(s.graph);
+    for (Plane* p : { &s.y0, &s.c0[0], &s.c0[1], &s.y1, &s.c1[0], &s.c1[1], &s.ws }) {
+        pool.put(p->ptr);
+        *p = Plane{};
+    }
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    s.stream = nullptr;
+    s.exec = nullptr;
+    s.graph = nullptr;
+    s.w = s.h = 0;
+}
+
+int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
+{
+    const ACHipRequest& r = q->r;
+    const Chroma c = chroma_of(q->fmt, r.width, r.height);
     Slot& s = slots[r.stream];
     if (!s.w) {    // first frame of this stream: buffers, stream and graph
-        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return ACHIP_EIO;
-        int e = alloc(s.y0, r.width, r.height, 1, es);
-        if (!e) e = alloc(s.y1, 2 * r.width, 2 * r.height, 1, es);
-        for (int i = 0; i < c.n && !e; i++) {
-            e = alloc(s.c0[i], c.w, c.h, c.ch, es);
-            if (!e) e = alloc(s.c1[i], 2 * c.w, 2 * c.h, c.ch, es);
+        const int e = setup_slot(s, q, c);
+        if (e) {
+            reset_slot(s);   // the next frame of this stream starts again from nothing
+            return e;
         }
-        if (!e && layered) e = alloc_ws(s.ws, r.width, r.height);
-        if (e) return e;
-        s.w = r.width;
-        s.h = r.height;
-        const int pcfg = achip::auto_config(net, cfg.precision, T, T, s.w, s.h);
-        if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return ACHIP_EIO;
-        hipError_t he = hipSuccess;
-        if (f.shift) he = achip::launch_shift(T, 1, f.shift, s.y0.ptr, s.y0.stride, s.y0.ptr, s.y0.stride, s.w, s.h, s.stream);
-        if (he == hipSuccess) he = pass_launch(pcfg, T, s.y0, s.y1, s.ws, s.stream);
-        if (he == hipSuccess && f.shift)
-            he = achip::launch_shift(T, 0, f.shift, s.y1.ptr, s.y1.stride, s.y1.ptr, s.y1.stride, s.y1.w, s.y1.h, s.stream);
-        for (int i = 0; i < c.n && he == hipSuccess; i++)
-            he = achip::launch_resize(T, c.ch, chroma_mode(f), s.c0[i].ptr, s.c0[i].stride, s.c0[i].w, s.c0[i].h, s.c1[i].ptr,
-                                      s.c1[i].stride, s.c1[i].w, s.c1[i].h, s.stream);
-        const hipError_t ce = hipStreamEndCapture(s.stream, &s.graph);
-        if (he != hipSuccess || ce != hipSuccess) return ACHIP_EIO;
-        if (hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0) != hipSuccess) return ACHIP_EIO;
     }
     int e = copy_plane(s.y0.ptr, s.y0.stride, r.src[0], r.src_stride[0], s.y0.row(), s.h, s.stream);
     for (int i = 0; i < c.n && !e; i++)
@@ -772,13 +814,7 @@ void achip_server_destroy(ACHipServer* s)
     (void)hipSetDevice(s->net->device);
     for (auto* st : { s->batch, s->upload, s->download })
         if (st) (void)hipStreamSynchronize(st);
-    for (Slot& sl : s->slots) {
-        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
-        if (sl.graph) (void)hipGraphDestroy(sl.graph);
-        for (Plane* p : { &sl.y0, &sl.c0[0], &sl.c0[1], &sl.y1, &sl.c1[0], &sl.c1[1], &sl.ws }) s->pool.put(p->ptr);
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
-    }
+    for (Slot& sl : s->slots) s->reset_slot(sl);
     for (auto& kv : s->tickets) s->free_req(*kv.second);
     s->tickets.clear();
     for (int i = 0; i < kRing; i++) {

@@ -77,6 +77,17 @@ def elem_type(dtype):
     return table[dtype]
 
 
+def _torch_stream(stream, dev):
+    """A torch stream object for `stream`: None (torch's current stream), a torch stream, or a
+    raw hipStream_t handle (int)."""
+    import torch
+    if stream is None:
+        return torch.cuda.current_stream(dev)
+    if hasattr(stream, "cuda_stream"):
+        return stream
+    return torch.cuda.ExternalStream(int(stream), device=dev)
+
+
 def device_count():
     return _native.lib().achip_device_count()
 
@@ -202,7 +213,7 @@ class Net8:
         self.variant = variant
         self.blocks = blocks
         self.device = _native.lib().achip_net_device(h)
-        self._ws = None
+        self._ws = {}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -219,27 +230,36 @@ class Net8:
     def handle(self):
         return self._h
 
-    def workspace(self, w, h, dev):
+    def _need(self, w, h):
+        return _native.lib().achip_pass8_workspace_bytes(w, h)
+
+    def workspace(self, w, h, dev, stream=None):
+        """The feature-map workspace for w x h passes launched on `stream` (a torch stream;
+        default: torch's current stream).  One per stream: passes on different streams never
+        share feature maps, and each workspace is allocated on its own stream, so torch's
+        allocator reuses a replaced one only after the work queued before it on that stream."""
         import torch
-        need = _native.lib().achip_pass8_workspace_bytes(w, h)
-        if self._ws is None or self._ws.numel() < need:
-            self._ws = torch.empty(need, dtype=torch.uint8, device=dev)
-        return self._ws
+        ts = stream if stream is not None else torch.cuda.current_stream(dev)
+        need = self._need(w, h)
+        ws = self._ws.get(ts.cuda_stream)
+        if ws is None or ws.numel() < need:
+            with torch.cuda.stream(ts):
+                ws = torch.empty(need, dtype=torch.uint8, device=dev)
+            self._ws[ts.cuda_stream] = ws
+        return ws
 
     def upscale(self, src, dst=None, stream=None, out_dtype=None):
         """One ACNet<8> 2x pass: src (H x W) -> dst (2H x 2W) on `stream` (default: torch's
-        current stream); the feature-map workspace is kept on the net between calls."""
+        current stream); the net keeps one feature-map workspace per stream between calls."""
         import torch
         if src.dim() != 2 or not src.is_cuda or src.stride(1) != 1:
             raise ValueError("src must be a 2-D device tensor with contiguous rows")
         h, w = src.shape
         if dst is None:
             dst = torch.empty((2 * h, 2 * w), dtype=out_dtype or src.dtype, device=src.device)
-        ws = self.workspace(w, h, src.device)
-        if stream is None:
-            stream = torch.cuda.current_stream(src.device).cuda_stream
-        elif hasattr(stream, "cuda_stream"):
-            stream = stream.cuda_stream
+        ts = _torch_stream(stream, src.device)
+        ws = self.workspace(w, h, src.device, ts)
+        stream = ts.cuda_stream
         check(_native.lib().achip_pass8(self._h, ctypes.c_void_p(src.data_ptr()), src.stride(0) * src.element_size(),
                                         elem_type(src.dtype), w, h, ctypes.c_void_p(dst.data_ptr()),
                                         dst.stride(0) * dst.element_size(), elem_type(dst.dtype),
@@ -261,29 +281,23 @@ class NetArt(Net8):
         self.channels = f
         self.blocks = blocks
         self.device = _native.lib().achip_net_device(h)
-        self._ws = None
+        self._ws = {}
 
-    def workspace(self, w, h, dev):
-        import torch
-        need = _native.lib().achip_net_workspace_bytes(self._h, w, h)
-        if self._ws is None or self._ws.numel() < need:
-            self._ws = torch.empty(need, dtype=torch.uint8, device=dev)
-        return self._ws
+    def _need(self, w, h):
+        return _native.lib().achip_net_workspace_bytes(self._h, w, h)
 
     def upscale(self, src, dst=None, stream=None, out_dtype=None):
         """One ArtCNN 2x pass: src (H x W) -> dst (2H x 2W) on `stream` (default: torch's
-        current stream); the feature-map workspace is kept on the net between calls."""
+        current stream); the net keeps one feature-map workspace per stream between calls."""
         import torch
         if src.dim() != 2 or not src.is_cuda or src.stride(1) != 1:
             raise ValueError("src must be a 2-D device tensor with contiguous rows")
         h, w = src.shape
         if dst is None:
             dst = torch.empty((2 * h, 2 * w), dtype=out_dtype or src.dtype, device=src.device)
-        ws = self.workspace(w, h, src.device)
-        if stream is None:
-            stream = torch.cuda.current_stream(src.device).cuda_stream
-        elif hasattr(stream, "cuda_stream"):
-            stream = stream.cuda_stream
+        ts = _torch_stream(stream, src.device)
+        ws = self.workspace(w, h, src.device, ts)
+        stream = ts.cuda_stream
         check(_native.lib().achip_pass_net(self._h, ctypes.c_void_p(src.data_ptr()), src.stride(0) * src.element_size(),
                                            elem_type(src.dtype), w, h, ctypes.c_void_p(dst.data_ptr()),
                                            dst.stride(0) * dst.element_size(), elem_type(dst.dtype),
@@ -306,7 +320,7 @@ class NetFsr(NetArt):
         self.channels = f
         self.blocks = blocks
         self.device = _native.lib().achip_net_device(h)
-        self._ws = None
+        self._ws = {}
 
 
 def configs(precision="fp32"):

@@ -37,33 +37,55 @@ FRAMES_PER_VARIANT = 2
 MEASURED_TRAFFIC = {"fp32": 11.24e6, "fp16": 11.08e6, "bf16": 11.08e6}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _median_reps(run, min_reps, min_seconds, max_reps):
+    """One warm-up call, then at least min_reps timed calls (and at least min_seconds of
+    them, up to max_reps); returns (median seconds per call, reps, total seconds)."""
+    import statistics
+    run()
+    times = []
+    while len(times) < min_reps or (sum(times) < min_seconds and len(times) < max_reps):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), len(times), sum(times)
+
+
 def cpu_baseline(frames_u8, weights, min_seconds=10.0):
-    """Bounded CPU sample: repeat 2x passes of 1080p frames on all host cores until
-    min_seconds of CPU work, report output Mpix/s."""
-    import numpy as np
+    """Bounded CPU sample: 2x passes of one 1080p u8 frame on all host cores, one warm-up then
+    the median of >= 5 timed passes (repeated up to about min_seconds); output Mpix/s.  Where
+    oracle/_ref is built, the passes run the reference's own per-ISA dot/conv kernels (OpImpl,
+    compiled from its sources) inside the oracle's restatement of its per-pixel loops;
+    otherwise the oracle's restatement alone."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as OL
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
-    use_ref = OL.ref_available() and OL.ref().acref_has_arch(5) or (OL.ref_available() and OL.ref().acref_has_arch(4))
-    arch = 5 if (OL.ref_available() and OL.ref().acref_has_arch(5)) else 4
+    have_ref = OL.ref_available()
+    arch = 5 if (have_ref and OL.ref().acref_has_arch(5)) else 4
+    use_ref = have_ref and OL.ref().acref_has_arch(arch)
     k, b = weights["hdn0"]
     src = frames_u8[0]
     run = (lambda: OL.ref_pass(arch, src, k, b)) if use_ref else (lambda: OL.oracle_pass(src, k, b))
-    run()   # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        run()
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= min_seconds or n >= 200:
-            break
-    mpix = n * 4 * W_IN * H_IN / 1e6 / el
-    out = {"value": round(mpix, 2), "unit": "Mpix/s", "cores": threads,
+    out_mpix = 4 * W_IN * H_IN / 1e6
+    med, n, el = _median_reps(run, 5, min_seconds, 200)
+    what = (f"reference dot/conv kernels (OpImpl, arch {arch}) built from its sources, per-pixel loops restated"
+            if use_ref else "oracle restatement (FMA order)")
+    out = {"value": round(out_mpix / med, 2), "unit": "Mpix/s", "cores": threads,
            "kind": "reference" if use_ref else "port",
-           "sample": f"{n} passes of one 1920x1080 u8 frame (HDN0) in {el:.1f} s, "
-                     + (f"reference per-ISA kernels (arch {arch}) built from its sources"
-                        if use_ref else "oracle restatement (FMA order)")
-                     + f", OpenMP {threads} threads"}
+           "cpu_model": cpu_model(),
+           "method": "1 warm-up, median of timed passes",
+           "sample": f"median of {n} passes of one 1920x1080 u8 frame (HDN0), {el:.1f} s in all; {what}; "
+                     f"OpenMP {threads} threads"}
     # the same passes on one thread (OpenMP's thread count of this thread's next parallel regions)
     try:
         import ctypes
@@ -72,17 +94,11 @@ def cpu_baseline(frames_u8, weights, min_seconds=10.0):
         return out
     gomp.omp_set_num_threads(1)
     try:
-        n1, t1 = 0, time.perf_counter()
-        while True:
-            run()
-            n1 += 1
-            el1 = time.perf_counter() - t1
-            if el1 >= min_seconds / 3 or n1 >= 50:
-                break
+        med1, n1, el1 = _median_reps(run, 5, min_seconds / 3, 50)
     finally:
         gomp.omp_set_num_threads(threads)
-    out["one_thread"] = {"value": round(n1 * 4 * W_IN * H_IN / 1e6 / el1, 2), "unit": "Mpix/s",
-                         "sample": f"{n1} pass(es) of the same frame in {el1:.1f} s on one thread"}
+    out["one_thread"] = {"value": round(out_mpix / med1, 2), "unit": "Mpix/s",
+                         "sample": f"median of {n1} passes of the same frame on one thread, {el1:.1f} s in all"}
     return out
 
 

@@ -4,10 +4,16 @@ core/internal/AC/Core/Internal/Processor/CPU/{Generic,X86/AVX}.hpp driven throug
 sequences of CPUProcessor.cpp, compiled from the reference sources by oracle/Makefile) in the
 Generic order (arch 1) and the FMA order (arch 4).
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py          # per-family fixtures
+    make -C oracle ref && python tests/golden/make_golden.py --c1     # the 256x256 workload
 
 One .npz per family (numpy, no pickles): for every case "<variant>/<w>x<h>/<dtype>" the
 arrays "<case>/in", "<case>/generic" and "<case>/fma".
+
+acnet_legacy_c1.npz holds the 256x256 luma workload of the plain 2x pass: one seeded f32
+input ("input/float32") and its u8 quantisation ("input/uint8"), and for every ACNetLegacy
+weight set the reference outputs "<variant>/<in>-<out>/{generic,fma}" for f32 -> u8 (the
+u8-quantised check of that workload) and u8 -> u8, plus f32 -> f32 for HDN0.
 """
 import os
 import sys
@@ -49,9 +55,32 @@ def cases(family):
             yield v, lambda arch, x, k=k, b=b, a=a, f=f, n=blocks: OL.ref_fsrcnnx(arch, x, k, b, a, f, n)
 
 
+C1_SIZE = 256
+C1_SEED = 256256
+
+
+def c1_arrays():
+    x = synth.flat_shaded(C1_SEED, C1_SIZE, C1_SIZE, dtype=np.float32)
+    x8 = synth.flat_shaded(C1_SEED, C1_SIZE, C1_SIZE, dtype=np.uint8)
+    arrays = {"input/float32": x, "input/uint8": x8}
+    for v in hip.VARIANTS:
+        k, b = hip.load_weights(v)
+        combos = [(x, np.uint8), (x8, np.uint8)] + ([(x, np.float32)] if v == "hdn0" else [])
+        for src, out in combos:
+            key = f"{v}/{src.dtype.name}-{np.dtype(out).name}"
+            arrays[key + "/generic"] = OL.ref_pass(1, src, k, b, out_dtype=out)
+            arrays[key + "/fma"] = OL.ref_pass(4, src, k, b, out_dtype=out)
+    return arrays
+
+
 def main():
     if not OL.ref_available():
         raise SystemExit("oracle/_ref is not built (make -C oracle ref needs the reference sources)")
+    if "--c1" in sys.argv:
+        path = os.path.join(HERE, "acnet_legacy_c1.npz")
+        np.savez_compressed(path, **c1_arrays())
+        print(f"{path}: {os.path.getsize(path)} bytes")
+        return
     for family, sizes in SIZES.items():
         arrays = {}
         for variant, run in cases(family):

@@ -66,6 +66,13 @@ def test_processor_model_strings(cuda, model, variant):
     check_int(out, OL.oracle_pass8(x, k, b, a, blocks))
     rc, out4 = proc.process(x, 4.0)
     assert rc == 0 and out4.shape == (112, 160)
+    # factor 4 is two passes with a u8 image between them: the chain equals pass 2 run on
+    # the device's own pass-1 output, which is held to the u8 bar of one pass
+    rc2, again = proc.process(out, 2.0)
+    assert rc2 == 0 and np.array_equal(out4, again)
+    check_int(out4, OL.oracle_pass8(out, k, b, a, blocks))
+    # against the oracle's own chain each side quantises its own pass-1 output, so pass 1's
+    # rare 1-LSB differences enter pass 2 on one side only
     check_int(out4, OL.oracle_pass8(OL.oracle_pass8(x, k, b, a, blocks), k, b, a, blocks), frac=0.995, maxlsb=3)
     assert not P.Processor("hip-fp16", 0, model).ok()     # FP32 only
 

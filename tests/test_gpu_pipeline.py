@@ -73,7 +73,7 @@ def test_rgb_u8(proc, weights, c, factor):
     assert rc == 0, proc.error()
     want = oracle_pipeline(src, factor, *weights["hdn1"])
     assert out.shape == want.shape
-    check_int(out, want, frac=0.999, maxlsb=3)
+    check_int(out, want, frac=0.999, maxlsb=2)
 
 
 @pytest.mark.parametrize("factor", [3.0, 1.5, 2.5])
@@ -84,7 +84,7 @@ def test_gray_non_power_of_two(proc, weights, factor):
     want = oracle_pipeline(src, factor, *weights["hdn1"])
     assert out.shape == want.shape == (int((57 << ceil_log2(factor)) * factor / (1 << ceil_log2(factor))),
                                        int((83 << ceil_log2(factor)) * factor / (1 << ceil_log2(factor))))
-    check_int(out, want, frac=0.999, maxlsb=3)
+    check_int(out, want, frac=0.999, maxlsb=2)
 
 
 def test_rgb_f32(proc, weights):
@@ -100,7 +100,7 @@ def test_rgb_preallocated(proc, weights):
     big = np.zeros((60, 100, 3), np.uint8)
     rc, _ = proc.process(src, 2.0, dst=big[:, 10:90])
     assert rc == 0, proc.error()
-    check_int(big[:, 10:90], oracle_pipeline(src, 2.0, *weights["hdn1"]), frac=0.999, maxlsb=3)
+    check_int(big[:, 10:90], oracle_pipeline(src, 2.0, *weights["hdn1"]), frac=0.999, maxlsb=2)
     assert big[:, :10].max() == 0 and big[:, 90:].max() == 0
 
 

@@ -111,7 +111,7 @@ def test_oneoffs_equal_processor(net, weights, prec):
             assert np.array_equal(out[0], want)
             if prec == "fp32":
                 k, b = weights[VARIANT]
-                check_int(out[0], oracle_pipeline(src, factor, k, b), frac=0.999, maxlsb=3)
+                check_int(out[0], oracle_pipeline(src, factor, k, b), frac=0.999, maxlsb=2)
 
 
 def test_frames_match_oracle(net, weights):
