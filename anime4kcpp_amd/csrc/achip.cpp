@@ -192,7 +192,7 @@ int validate(const ACHipNet* net, const void* src, long sstride, int stype, int 
     return ACHIP_OK;
 }
 
-constexpr size_t kBatchHeader = 256;   // counter, padded
+constexpr size_t kBatchHeader = achip::kBatchHeaderBytes;
 
 // autotuner cache: (device, precision, src type, dst type, log2 w, log2 h) -> config
 std::mutex g_tune_mtx;
@@ -266,6 +266,93 @@ int auto_config(const ACHipNet* net, int precision, int stype, int dtype, int w,
     std::lock_guard<std::mutex> lk(g_tune_mtx);
     auto it = g_tune.find(tune_key(net->device, precision, stype, dtype, w, h));
     return it != g_tune.end() ? it->second : 0;
+}
+
+int record_error(int code, const char* msg)
+{
+    return fail(code, "%s", msg);
+}
+
+// Sweep in address order: a span overlaps an earlier one iff it starts below that one's end.
+// A written span may overlap nothing; a read span may overlap other read spans only.
+static int check_disjoint_writes(BatchSpan* spans, int count)
+{
+    std::sort(spans, spans + count, [](const BatchSpan& a, const BatchSpan& b) { return a.lo < b.lo; });
+    uintptr_t end_any = 0, end_written = 0;
+    const BatchSpan* last_any = nullptr;
+    const BatchSpan* last_written = nullptr;
+    for (int i = 0; i < count; i++) {
+        const BatchSpan& s = spans[i];
+        const BatchSpan* hit = s.written ? (s.lo < end_any ? last_any : nullptr) : (s.lo < end_written ? last_written : nullptr);
+        if (hit) {
+            auto name = [](const BatchSpan& x) {
+                return x.image < 0 ? std::string("the workspace")
+                                   : "image " + std::to_string(x.image) + (x.written ? " dst" : " src");
+            };
+            return fail(ACHIP_EINVAL, "%s overlaps %s", name(*hit).c_str(), name(s).c_str());
+        }
+        if (s.hi > end_any) { end_any = s.hi; last_any = &s; }
+        if (s.written && s.hi > end_written) { end_written = s.hi; last_written = &s; }
+    }
+    return ACHIP_OK;
+}
+
+int batch_prepare(ACHipNet* net, int precision, int config, int src_type, int dst_type, const ACHipImage* images, int n,
+                  const void* workspace, size_t ws_bytes, char* host, BatchSpan* spans, int* total_tiles)
+{
+    if (precision != ACHIP_FP32 && precision != ACHIP_FP16 && precision != ACHIP_BF16)
+        return fail(ACHIP_EINVAL, "precision %d not available", precision);
+    if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
+    if (!images || n <= 0) return fail(ACHIP_EINVAL, "empty batch");
+    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    if (net->blocks || net->lfamily) return fail(ACHIP_EINVAL, "batched passes take an ACNetLegacy net");
+    const size_t need = achip_batch_workspace_bytes(n);
+    if (!workspace || ws_bytes < need)
+        return fail(ACHIP_EINVAL, "workspace of %zu bytes, the batch needs %zu", ws_bytes, need);
+    if (reinterpret_cast<uintptr_t>(workspace) % 256) return fail(ACHIP_EINVAL, "workspace not 256-byte aligned");
+    int tw = 0, th = 0;
+    achip_config_info(precision, config, &tw, &th, nullptr, nullptr, nullptr);
+    std::memset(host, 0, kBatchHeaderBytes);
+    auto* desc = reinterpret_cast<BatchDesc*>(host + kBatchHeaderBytes);
+    long total = 0;
+    int ns = 0;
+    for (int i = 0; i < n; i++) {
+        const ACHipImage& im = images[i];
+        const int r = validate(net, im.src, im.src_stride, src_type, im.w, im.h, im.dst, im.dst_stride, dst_type);
+        if (r) {
+            std::string msg = g_error;
+            return fail(r, "image %d: %s", i, msg.c_str());
+        }
+        const uintptr_t s0 = reinterpret_cast<uintptr_t>(im.src), d0 = reinterpret_cast<uintptr_t>(im.dst);
+        spans[ns++] = { s0, s0 + static_cast<size_t>(im.h - 1) * im.src_stride + static_cast<size_t>(im.w) * (src_type & 0xff), false, i };
+        spans[ns++] = { d0, d0 + static_cast<size_t>(2 * im.h - 1) * im.dst_stride + static_cast<size_t>(2 * im.w) * (dst_type & 0xff), true, i };
+        const int tx = (im.w + tw - 1) / tw, ty = (im.h + th - 1) / th;
+        desc[i] = BatchDesc{ im.src, im.dst, im.src_stride, im.dst_stride, im.w, im.h, tx, static_cast<int>(total), 0 };
+        total += static_cast<long>(tx) * ty;
+        if (total > (1L << 30)) return fail(ACHIP_EINVAL, "batch too large (%ld tiles)", total);
+    }
+    // Every launch writes the workspace and each dst while it reads each src: no written span
+    // may overlap any other span (an image's own src/dst pair was checked above).
+    spans[ns++] = { reinterpret_cast<uintptr_t>(workspace), reinterpret_cast<uintptr_t>(workspace) + need, true, -1 };
+    const int r = check_disjoint_writes(spans, ns);
+    if (r) return r;
+    *total_tiles = static_cast<int>(total);
+    return ACHIP_OK;
+}
+
+int batch_launch(ACHipNet* net, int precision, int config, int src_type, int dst_type, int n, int total, void* workspace,
+                 const char* host, hipStream_t st)
+{
+    DeviceGuard g(net->device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    // counter (= 0) and descriptors in one copy; from pageable memory the copy is staged
+    // before the call returns, so `host` may go out of scope
+    hipError_t e = hipMemcpyAsync(workspace, host, achip_batch_workspace_bytes(n), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(batch descriptors)");
+    auto* ddesc = reinterpret_cast<const BatchDesc*>(static_cast<char*>(workspace) + kBatchHeaderBytes);
+    e = launch_batch(net, precision, config, src_type, dst_type, ddesc, n, total, static_cast<int*>(workspace), 0, st);
+    if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
+    return ACHIP_OK;
 }
 
 } // namespace achip
@@ -449,7 +536,7 @@ int upload_layered(int device, int family, int f, int blocks, const float* kerne
 }
 
 // conv3x3 A operands of layers F -> F ... F -> 4 laid end to end (cout of each given)
-std::vector<float> pack_layers(int f, const float* k0, size_t kstride, const std::vector<int>& couts)
+static std::vector<float> pack_layers(int f, const float* k0, size_t kstride, const std::vector<int>& couts)
 {
     const size_t wl = achip::conv_operand_floats(f);
     std::vector<float> wa(wl * couts.size(), 0.0f);
@@ -810,51 +897,20 @@ size_t achip_batch_workspace_bytes(int n)
 int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int dst_type,
                      const ACHipImage* images, int n, void* workspace, size_t workspace_bytes, void* stream)
 {
-    if (precision != ACHIP_FP32 && precision != ACHIP_FP16 && precision != ACHIP_BF16)
-        return fail(ACHIP_EINVAL, "precision %d not available", precision);
-    if (config < 0 || config >= achip_config_count(precision)) return fail(ACHIP_EINVAL, "no config %d", config);
-    if (!images || n <= 0) return fail(ACHIP_EINVAL, "empty batch");
-    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
-    if (net->blocks || net->lfamily) return fail(ACHIP_EINVAL, "batched passes take an ACNetLegacy net");
-    const size_t need = achip_batch_workspace_bytes(n);
-    if (!workspace || workspace_bytes < need)
-        return fail(ACHIP_EINVAL, "workspace of %zu bytes, the batch needs %zu", workspace_bytes, need);
-    if (reinterpret_cast<uintptr_t>(workspace) % 256) return fail(ACHIP_EINVAL, "workspace not 256-byte aligned");
-    int r = check_device_buffer(workspace, need, net->device, "workspace");
-    if (r) return r;
-    int tw = 0, th = 0;
-    achip_config_info(precision, config, &tw, &th, nullptr, nullptr, nullptr);
-    std::vector<char> host(need, 0);
-    auto* desc = reinterpret_cast<achip::BatchDesc*>(host.data() + kBatchHeader);
-    long total = 0;
-    for (int i = 0; i < n; i++) {
-        const ACHipImage& im = images[i];
-        r = validate(net, im.src, im.src_stride, src_type, im.w, im.h, im.dst, im.dst_stride, dst_type);
-        if (r) {
-            std::string msg = g_error;
-            return fail(r, "image %d: %s", i, msg.c_str());
-        }
-        const char* ws = static_cast<const char*>(workspace);
-        const char* d0 = static_cast<const char*>(im.dst);
-        const size_t dbytes = static_cast<size_t>(2 * im.h - 1) * im.dst_stride + static_cast<size_t>(2 * im.w) * (dst_type & 0xff);
-        if (ws < d0 + dbytes && d0 < ws + need) return fail(ACHIP_EINVAL, "image %d: dst overlaps the workspace", i);
-        const int tx = (im.w + tw - 1) / tw, ty = (im.h + th - 1) / th;
-        desc[i] = achip::BatchDesc{ im.src, im.dst, im.src_stride, im.dst_stride, im.w, im.h, tx, static_cast<int>(total), 0 };
-        total += static_cast<long>(tx) * ty;
-        if (total > (1L << 30)) return fail(ACHIP_EINVAL, "batch too large (%ld tiles)", total);
+    if (n <= 0 || n > (1 << 24)) return fail(ACHIP_EINVAL, "batch of %d images", n);
+    if (workspace) {
+        const int r = check_device_buffer(workspace, std::min(workspace_bytes, achip_batch_workspace_bytes(n)),
+                                          net ? net->device : 0, "workspace");
+        if (r) return r;
     }
-    DeviceGuard g(net->device);
-    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
-    const hipStream_t st = static_cast<hipStream_t>(stream);
-    // counter (= 0) and descriptors in one copy; from pageable memory the copy is staged
-    // before the call returns, so `host` may go out of scope
-    hipError_t e = hipMemcpyAsync(workspace, host.data(), need, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(batch descriptors)");
-    auto* ddesc = reinterpret_cast<const achip::BatchDesc*>(static_cast<char*>(workspace) + kBatchHeader);
-    int* counter = static_cast<int*>(workspace);
-    e = achip::launch_batch(net, precision, config, src_type, dst_type, ddesc, n, static_cast<int>(total), counter, 0, st);
-    if (e != hipSuccess) return hip_fail(e, "batch kernel launch");
-    return ACHIP_OK;
+    std::vector<char> host(achip_batch_workspace_bytes(n));
+    std::vector<achip::BatchSpan> spans(2 * static_cast<size_t>(n) + 1);
+    int total = 0;
+    const int r = achip::batch_prepare(net, precision, config, src_type, dst_type, images, n, workspace, workspace_bytes,
+                                       host.data(), spans.data(), &total);
+    if (r) return r;
+    return achip::batch_launch(net, precision, config, src_type, dst_type, n, total, workspace, host.data(),
+                               static_cast<hipStream_t>(stream));
 }
 
 const char* achip_last_error(void) { return g_error.c_str(); }

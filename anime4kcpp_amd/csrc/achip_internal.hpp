@@ -45,4 +45,30 @@ hipError_t launch_batch(const ACHipNet* net, int precision, int config, int styp
 // The configuration ACHIP_AUTO resolves to for this shape (autotuner cache, else 0).
 int auto_config(const ACHipNet* net, int precision, int stype, int dtype, int w, int h);
 
+// A batch workspace: the tile-queue counter, padded to this many bytes, then one
+// BatchDesc per image.
+constexpr size_t kBatchHeaderBytes = 256;
+
+// A byte range [lo, hi) that a batched launch reads or writes; image -1 is the workspace.
+struct BatchSpan {
+    uintptr_t lo, hi;
+    bool written;
+    int image;
+};
+
+// Checks a batch (achip_pass_batch's arguments; the workspace only for size, alignment and
+// overlap) and writes its header (counter 0) and descriptors into host[0 ..
+// achip_batch_workspace_bytes(n)); spans has room for 2n + 1 entries.  No allocation on the
+// success path.  Returns ACHIP_OK or an ACHIP_* code (recorded for achip_last_error).
+int batch_prepare(ACHipNet* net, int precision, int config, int src_type, int dst_type, const ACHipImage* images, int n,
+                  const void* workspace, size_t ws_bytes, char* host, BatchSpan* spans, int* total_tiles);
+
+// The host-to-device copy of a prepared batch into `workspace` and the persistent launch, on
+// `stream`, with the net's device current.
+int batch_launch(ACHipNet* net, int precision, int config, int src_type, int dst_type, int n, int total, void* workspace,
+                 const char* host, hipStream_t stream);
+
+// Records msg for achip_last_error; returns code.
+int record_error(int code, const char* msg);
+
 } // namespace achip

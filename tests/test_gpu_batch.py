@@ -83,6 +83,33 @@ def test_batch_rejects_bad_workspace(net, cuda):
     torch.cuda.synchronize()
 
 
+def test_batch_rejects_overlaps(net, cuda):
+    """A batch whose written spans (each dst, the workspace) overlap any other image's src or
+    dst, or a src overlapping the workspace, is refused before anything is launched."""
+    L = _native.lib()
+    pool = torch.zeros(1 << 20, dtype=torch.uint8, device=cuda)
+    base = pool.data_ptr()
+    need = L.achip_batch_workspace_bytes(2)
+    ws = torch.zeros(need, dtype=torch.uint8, device=cuda)
+
+    def run(images, wsp=ws.data_ptr()):
+        arr = (_native.ACHipImage * len(images))(*[_native.ACHipImage(s, 8, d, 16, 8, 8) for s, d in images])
+        return L.achip_pass_batch(net.handle, 0, 0, 1, 1, arr, len(images), ctypes.c_void_p(wsp), need, None)
+
+    a_src, a_dst, b_src, b_dst = base, base + 4096, base + 8192, base + 12288
+    assert run([(a_src, a_dst), (b_src, b_dst)]) == 0
+    assert run([(a_src, a_dst), (a_src, b_dst)]) == 0                 # shared sources are fine
+    assert run([(a_src, a_dst), (b_src, a_dst + 64)]) == -22          # dst overlaps dst
+    assert b"image 0 dst overlaps image 1 dst" in L.achip_last_error()
+    assert run([(a_src, a_dst), (a_dst + 32, b_dst)]) == -22          # src inside another dst
+    assert run([(a_src, a_dst), (b_src, a_src)]) == -22               # dst over another src
+    wsbig = torch.zeros(need + 4096, dtype=torch.uint8, device=cuda)
+    w0 = (wsbig.data_ptr() + 255) // 256 * 256
+    assert run([(w0 + 256, a_dst), (b_src, b_dst)], wsp=w0) == -22   # src inside the workspace
+    assert b"workspace" in L.achip_last_error()
+    torch.cuda.synchronize()
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
 def test_autotune_choice_is_used(net, cuda, prec):
