@@ -100,6 +100,8 @@ SIGNATURES = {
     "achip_server_stats": (c_int, [c_void_p, ctypes.POINTER(ACHipServerStats), c_int]),
     "achip_autotune": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_float)]),
     "achip_pass_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p]),
+    "achip_noalloc_scratch_bytes": (ctypes.c_size_t, []),
+    "achip_pass_batch_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p]),
     # include/AC/Core/Image.h
     "ac_image_alloc": (ctypes.POINTER(ACImage), []),
     "ac_image_free": (None, [ctypes.POINTER(ctypes.POINTER(ACImage))]),

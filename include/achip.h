@@ -96,6 +96,20 @@ ACHIP_API int achip_pass_batch(ACHipNet* net, int precision, int config, int src
                                const ACHipImage* images, int n, void* workspace, size_t workspace_bytes,
                                void* stream);
 
+/* achip_pass_batch without a workspace argument and without any allocation, host or device.
+ * No counterpart in the reference: it is for callers that cannot allocate (code under HIP
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The tile-queue
+ * counter and the descriptors live in a fixed device array of the library
+ * (achip_noalloc_scratch_bytes() bytes on each device, held until the process ends), so a
+ * call takes at most ACHIP_NOALLOC_MAX_IMAGES images (more: ACHIP_EINVAL).  Calls on one
+ * device share that array: calls that overlap in time on one device race, so serialise them
+ * (one stream, or an event between them).  Same images, results and errors as
+ * achip_pass_batch otherwise. */
+#define ACHIP_NOALLOC_MAX_IMAGES 256
+ACHIP_API size_t achip_noalloc_scratch_bytes(void);
+ACHIP_API int achip_pass_batch_noalloc(ACHipNet* net, int precision, int config, int src_type, int dst_type,
+                                       const ACHipImage* images, int n, void* stream);
+
 /* Kernel configurations of a precision: tile width/height (input pixels), threads per
  * workgroup, pixels per thread run of the FP32 VALU kernel (0 for the MFMA kernels), LDS
  * bytes for f32 input.  FP32: the FP32-MFMA kernel first (configuration 0 is the default),
