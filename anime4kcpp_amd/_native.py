@@ -75,6 +75,8 @@ SIGNATURES = {
     "achip_pass8_workspace_bytes": (ctypes.c_size_t, [c_int, c_int]),
     "achip_pass8": (c_int, [c_void_p, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
                             ctypes.c_size_t, c_void_p]),
+    "achip_pass8_precision": (c_int, [c_void_p, c_int, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int,
+                                      c_void_p, ctypes.c_size_t, c_void_p]),
     "achip_rgb2yuv": (c_int, [c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_void_p]),
     "achip_yuv2rgb": (c_int, [c_int, c_int, c_void_p, c_long, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_void_p]),
     "achip_resize": (c_int, [c_int, c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p, c_long, c_int, c_int, c_void_p]),
@@ -100,6 +102,8 @@ SIGNATURES = {
     "achip_server_stats": (c_int, [c_void_p, ctypes.POINTER(ACHipServerStats), c_int]),
     "achip_autotune": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(ctypes.c_float)]),
     "achip_pass_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p]),
+    "achip_pass_layers": (c_int, [c_void_p, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
+                                  ctypes.c_size_t, c_void_p]),
     "achip_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "achip_pass_batch_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p]),
     # include/AC/Core/Image.h

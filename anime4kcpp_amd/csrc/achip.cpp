@@ -100,6 +100,21 @@ std::vector<uint16_t> mfma_operands(const float* kernels, bool bf16)
     return a;
 }
 
+// Per-lane 16-bit MFMA A operands of one conv3x3 8 -> cout layer (k in the reference layout
+// k[n][tap][c]), [t][lane] x 8 values, output rows n >= cout zero.
+void mfma16_conv8_operands(const float* k, int cout, bool bf16, uint16_t* out)
+{
+    for (int t = 0; t < 3; t++)
+        for (int lane = 0; lane < 64; lane++) {
+            const int i = lane % 16, g = lane / 16;
+            const int n = i % 8, ro = i / 8, ky = g - ro;
+            for (int c = 0; c < 8; c++) {
+                const float w = (n < cout && ky >= 0 && ky <= 2) ? k[n * 72 + (ky * 3 + t) * 8 + c] : 0.0f;
+                out[(t * 64 + lane) * 8 + c] = bf16 ? f32_to_bf16_bits(w) : f32_to_f16_bits(w);
+            }
+        }
+}
+
 // Per-lane FP32 MFMA A operands (acnet_device.hpp).
 std::vector<float> mfma32_operands(const float* kernels)
 {
@@ -472,14 +487,25 @@ int achip_net8_create(int device, int blocks, const float* kernels, const float*
     for (int l = 1; l <= blocks + 1; l++)
         achip::pack_conv8_operands(kernels + 72 + 576 * (l - 1), l <= blocks ? 8 : 4, wa.data() + (l - 1) * achip::kMfma32LayerFloats);
 
+    // the same layers as 16-bit operands: FP16 for all layers, then BF16
+    const size_t per16 = static_cast<size_t>(3) * 64 * 8;
+    std::vector<uint16_t> wa16(2 * static_cast<size_t>(blocks + 1) * per16);
+    for (int b16 = 0; b16 < 2; b16++)
+        for (int l = 1; l <= blocks + 1; l++)
+            mfma16_conv8_operands(kernels + 72 + 576 * (l - 1), l <= blocks ? 8 : 4, b16 != 0,
+                                  wa16.data() + (static_cast<size_t>(b16) * (blocks + 1) + (l - 1)) * per16);
+
     DeviceGuard g(device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
     const size_t offa = (blob.size() * sizeof(float) + 255) / 256 * 256;
+    const size_t off16 = offa + (wa.size() * sizeof(float) + 255) / 256 * 256;
+    const size_t half16 = static_cast<size_t>(blocks + 1) * per16 * sizeof(uint16_t);
     char* d = nullptr;
-    hipError_t e = hipMalloc(&d, offa + wa.size() * sizeof(float));
+    hipError_t e = hipMalloc(&d, off16 + wa16.size() * sizeof(uint16_t));
     if (e != hipSuccess) return hip_fail(e, "hipMalloc(weights)");
     e = hipMemcpy(d, blob.data(), blob.size() * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d + offa, wa.data(), wa.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + off16, wa16.data(), wa16.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         (void)hipFree(d);
         return hip_fail(e, "hipMemcpy(weights)");
@@ -488,6 +514,8 @@ int achip_net8_create(int device, int blocks, const float* kernels, const float*
     net->blocks = blocks;
     net->a8wts = reinterpret_cast<float*>(d);
     net->a8wa = reinterpret_cast<float*>(d + offa);
+    net->a8wa16 = d + off16;
+    net->a8wabf16 = d + off16 + half16;
     *out = net;
     g_error.clear();
     return ACHIP_OK;
@@ -654,6 +682,15 @@ size_t achip_pass8_workspace_bytes(int w, int h)
 int achip_pass8(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h, void* dst, long dst_stride,
                 int dst_type, void* workspace, size_t workspace_bytes, void* stream)
 {
+    return achip_pass8_precision(net, ACHIP_FP32, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, workspace,
+                                 workspace_bytes, stream);
+}
+
+int achip_pass8_precision(ACHipNet* net, int precision, const void* src, long src_stride, int src_type, int w, int h,
+                          void* dst, long dst_stride, int dst_type, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (precision != ACHIP_FP32 && precision != ACHIP_FP16 && precision != ACHIP_BF16)
+        return fail(ACHIP_EINVAL, "precision %d not available", precision);
     if (!net) return fail(ACHIP_EINVAL, "net is NULL");
     if (!net->blocks) return fail(ACHIP_EINVAL, "achip_pass8 takes an ACNet<8> net");
     int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
@@ -673,8 +710,41 @@ int achip_pass8(ACHipNet* net, const void* src, long src_stride, int src_type, i
         return fail(ACHIP_EINVAL, "the workspace overlaps src or dst");
     DeviceGuard g(net->device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
-    const hipError_t e = achip::launch_acnet8(net->blocks, src_type, dst_type, src, src_stride, w, h, dst, dst_stride,
-                                              net->a8wts, net->a8wa, workspace, static_cast<hipStream_t>(stream));
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const hipError_t e =
+        precision == ACHIP_FP32
+            ? achip::launch_acnet8(net->blocks, src_type, dst_type, src, src_stride, w, h, dst, dst_stride, net->a8wts,
+                                   net->a8wa, workspace, st)
+            : achip::launch_acnet8_16(precision == ACHIP_BF16, net->blocks, src_type, dst_type, src, src_stride, w, h, dst,
+                                      dst_stride, net->a8wts, precision == ACHIP_BF16 ? net->a8wabf16 : net->a8wa16,
+                                      workspace, st);
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "kernel launch");
+}
+
+int achip_pass_layers(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h, void* dst,
+                      long dst_stride, int dst_type, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    if (net->blocks || net->lfamily) return fail(ACHIP_EINVAL, "achip_pass_layers takes an ACNetLegacy net");
+    int r = validate(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type);
+    if (r) return r;
+    const size_t need = achip_pass8_workspace_bytes(w, h);
+    if (!workspace || workspace_bytes < need)
+        return fail(ACHIP_EINVAL, "workspace of %zu bytes, a %dx%d pass needs %zu", workspace_bytes, w, h, need);
+    if (reinterpret_cast<uintptr_t>(workspace) % 256) return fail(ACHIP_EINVAL, "workspace not 256-byte aligned");
+    r = check_device_buffer(workspace, need, net->device, "workspace");
+    if (r) return r;
+    const char* ws = static_cast<const char*>(workspace);
+    const char* s0 = static_cast<const char*>(src);
+    const char* d0 = static_cast<const char*>(dst);
+    const size_t sbytes = static_cast<size_t>(h - 1) * src_stride + static_cast<size_t>(w) * (src_type & 0xff);
+    const size_t dbytes = static_cast<size_t>(2 * h - 1) * dst_stride + static_cast<size_t>(2 * w) * (dst_type & 0xff);
+    if ((ws < s0 + sbytes && s0 < ws + need) || (ws < d0 + dbytes && d0 < ws + need))
+        return fail(ACHIP_EINVAL, "the workspace overlaps src or dst");
+    DeviceGuard g(net->device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    const hipError_t e = achip::launch_legacy_split(src_type, dst_type, src, src_stride, w, h, dst, dst_stride, net->wts,
+                                                    net->wa32, workspace, static_cast<hipStream_t>(stream));
     return e == hipSuccess ? ACHIP_OK : hip_fail(e, "kernel launch");
 }
 

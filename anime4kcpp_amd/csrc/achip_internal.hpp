@@ -16,6 +16,8 @@ struct ACHipNet {
     int blocks = 0;          // ACNet<8>: its block count (the fields above are then unused)
     float* a8wts = nullptr;  // ACNet<8> blob, acnet_device.hpp layout (one allocation with a8wa)
     float* a8wa = nullptr;   // ACNet<8> A operands: blocks 1..B, then the final layer
+    void* a8wa16 = nullptr;  // the same layers' 16-bit MFMA A operands, FP16 then BF16
+    void* a8wabf16 = nullptr;
     // ArtCNN<F> and FSRCNNX<F>: family, F channels, B blocks; the reference-layout kernels,
     // biases and slopes and the per-lane A operands of the conv3x3 layers and the conv1x1,
     // one allocation starting at lk

@@ -296,6 +296,9 @@ int ACHipServer::alloc_ws(Plane& p, int w, int h)
 // in ws.  Kernels only, so frames can capture it into their graphs.
 hipError_t ACHipServer::pass_launch(int pcfg, int T, const Plane& a, const Plane& b, const Plane& ws, hipStream_t st) const
 {
+    if (net->blocks && cfg.precision != ACHIP_FP32)
+        return achip::launch_acnet8_16(cfg.precision == ACHIP_BF16, net->blocks, T, T, a.ptr, a.stride, a.w, a.h, b.ptr, b.stride,
+                                       net->a8wts, cfg.precision == ACHIP_BF16 ? net->a8wabf16 : net->a8wa16, ws.ptr, st);
     if (net->blocks)
         return achip::launch_acnet8(net->blocks, T, T, a.ptr, a.stride, a.w, a.h, b.ptr, b.stride, net->a8wts, net->a8wa, ws.ptr, st);
     if (net->lfamily == kNetArtCNN)
@@ -767,7 +770,7 @@ int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipSer
     ACHipServerConfig c = config ? *config : ACHipServerConfig{};
     if (c.precision != ACHIP_FP32 && c.precision != ACHIP_FP16 && c.precision != ACHIP_BF16) return ACHIP_EINVAL;
     const bool layered = net->blocks || net->lfamily;
-    if (layered && c.precision != ACHIP_FP32) return ACHIP_EINVAL;   // those families run in FP32 only
+    if (net->lfamily && c.precision != ACHIP_FP32) return ACHIP_EINVAL;   // ArtCNN and FSRCNNX run in FP32 only
     if (c.hbm_budget == 0) c.hbm_budget = static_cast<size_t>(16) << 30;
     if (c.step_tiles <= 0) c.step_tiles = c.precision == ACHIP_FP32 ? 2048 : 6144;
     if (c.max_workgroups < 0) return ACHIP_EINVAL;

@@ -20,7 +20,9 @@ namespace achip {
 #pragma clang fp contract(off)
 
 // ---- L0: one pixel per thread, 64 x 4 pixels per workgroup ----
-template <typename TIN>
+// RELU: ACNetLegacy's layer 0 (ReLU; its blob has the same w[p][n] and bias layout at the
+// same offsets), else ACNet<8>'s (PReLU with the blob's slopes)
+template <typename TIN, bool RELU = false>
 __global__ __launch_bounds__(256) void acnet8_l0(const void* __restrict__ src, long ss, int W, int H,
                                                  float4* __restrict__ feat, long fp, const float* __restrict__ wts)
 {
@@ -40,7 +42,8 @@ __global__ __launch_bounds__(256) void acnet8_l0(const void* __restrict__ src, l
         float s = 0.0f;
 #pragma unroll
         for (int p = 0; p < 9; p++) s = __fadd_rn(s, __fmul_rn(r[p], wts[kA8OffW0 + p * 8 + n]));
-        o[n] = prelu(__fadd_rn(s, wts[kA8OffB0 + n]), wts[kA8OffAlpha0 + n]);
+        const float v = __fadd_rn(s, wts[kA8OffB0 + n]);
+        o[n] = RELU ? fmaxf(v, 0.0f) : prelu(v, wts[kA8OffAlpha0 + n]);
     }
     float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(feat) + static_cast<long>(y) * fp) + 2 * x;
     px[0] = make_float4(o[0], o[1], o[2], o[3]);
@@ -86,6 +89,68 @@ static hipError_t acnet8_pass_t(int blocks, const void* src, long ss, int W, int
 }
 
 long acnet8_feature_pitch(int w) { return (static_cast<long>(w) * 32 + 255) / 256 * 256; }
+
+// ---- ACNetLegacy, layer by layer (the layer-split form of the fused FP32 pass) ----
+// L0 (conv3x3 1 -> 8 + ReLU, VALU) into an 8-channel f32 map, then layers 1..8 two per launch
+// on conv8_pair (ReLU; the last pair ends in the deconv 2x2 8 -> 1 straight to the output):
+// five launches, each map written and read once through HBM, a one-pixel recompute ring per
+// pair instead of the fused kernel's eight-pixel halo.
+template <typename TIN, typename TOUT>
+static hipError_t legacy_split_t(const void* src, long ss, int W, int H, void* dst, long ds, const float* wts,
+                                 const float* wa32, float4* f0, float4* f1, long fp, hipStream_t stream)
+{
+    static_assert(kA8OffW0 == kOffW0 && kA8OffB0 == kOffB0, "layer 0 reads both blobs at the same offsets");
+    hipLaunchKernelGGL((acnet8_l0<TIN, true>), dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, stream, src, ss, W, H, f0, fp, wts);
+    const dim3 pgrid = conv8_pair_grid(W, H), block(256);
+    auto layer = [&](int l) {
+        LayerArgs la{};
+        la.wa = wa32 + kMfma32L0Floats + static_cast<long>(l - 1) * kMfma32LayerFloats;
+        la.bias = wts + offB(l);
+        return la;
+    };
+    float4* in = f0;
+    float4* out = f1;
+    for (int l = 1; l <= 5; l += 2) {
+        hipLaunchKernelGGL((conv8_pair<kRelu, float, float, kRelu>), pgrid, block, 0, stream, in, fp, W, H, out, layer(l),
+                           layer(l + 1));
+        std::swap(in, out);
+    }
+    LayerArgs last = layer(8);
+    last.k2 = wts + kOffK2;
+    last.dst = dst;
+    last.ds = ds;
+    hipLaunchKernelGGL((conv8_pair<kReluDeconv, float, TOUT, kRelu>), pgrid, block, 0, stream, in, fp, W, H, out, layer(7), last);
+    return hipGetLastError();
+}
+
+hipError_t launch_legacy_split(int stype, int dtype, const void* src, long ss, int W, int H, void* dst, long ds,
+                               const float* wts, const float* wa32, void* workspace, hipStream_t stream)
+{
+    const long fp = acnet8_feature_pitch(W);
+    float4* f0 = static_cast<float4*>(workspace);
+    float4* f1 = reinterpret_cast<float4*>(static_cast<char*>(workspace) + fp * H);
+#define CALL(TI, TO) legacy_split_t<TI, TO>(src, ss, W, H, dst, ds, wts, wa32, f0, f1, fp, stream)
+    switch (stype) {
+    case U8: switch (dtype) {
+        case U8: return CALL(uint8_t, uint8_t); case U16: return CALL(uint8_t, uint16_t);
+        case F16: return CALL(uint8_t, __half); case F32: return CALL(uint8_t, float);
+        default: return hipErrorInvalidValue; }
+    case U16: switch (dtype) {
+        case U8: return CALL(uint16_t, uint8_t); case U16: return CALL(uint16_t, uint16_t);
+        case F16: return CALL(uint16_t, __half); case F32: return CALL(uint16_t, float);
+        default: return hipErrorInvalidValue; }
+    case F16: switch (dtype) {
+        case U8: return CALL(__half, uint8_t); case U16: return CALL(__half, uint16_t);
+        case F16: return CALL(__half, __half); case F32: return CALL(__half, float);
+        default: return hipErrorInvalidValue; }
+    case F32: switch (dtype) {
+        case U8: return CALL(float, uint8_t); case U16: return CALL(float, uint16_t);
+        case F16: return CALL(float, __half); case F32: return CALL(float, float);
+        default: return hipErrorInvalidValue; }
+    default: return hipErrorInvalidValue;
+    }
+#undef CALL
+}
 
 hipError_t launch_acnet8(int blocks, int stype, int dtype, const void* src, long ss, int W, int H, void* dst, long ds,
                          const float* wts, const float* wa, void* workspace, hipStream_t stream)

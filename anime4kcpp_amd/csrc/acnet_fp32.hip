@@ -367,6 +367,7 @@ static hipError_t launch_types(int stype, int dtype, const void* src, long ss, i
 const Fp32Config kFp32Configs[] = {
     { 64, 32, 512, 4, Fp32Tile<64, 32, 512, 4>::lds_bytes },
     { 32, 32, 256, 4, Fp32Tile<32, 32, 256, 4>::lds_bytes },
+    { 16, 16, 256, 4, Fp32Tile<16, 16, 256, 4>::lds_bytes },   // the small end of the tile sweep
 };
 const int kFp32ConfigCount = sizeof(kFp32Configs) / sizeof(kFp32Configs[0]);
 
@@ -376,6 +377,7 @@ hipError_t launch_fp32_batch(int cfg, int stype, int dtype, const BatchDesc* des
     switch (cfg) {
     case 0: return launch_batch_types<64, 32, 512, 4>(stype, dtype, desc, n, total, counter, wts, max_wg, stream);
     case 1: return launch_batch_types<32, 32, 256, 4>(stype, dtype, desc, n, total, counter, wts, max_wg, stream);
+    case 2: return launch_batch_types<16, 16, 256, 4>(stype, dtype, desc, n, total, counter, wts, max_wg, stream);
     default: return hipErrorInvalidValue;
     }
 }
@@ -386,6 +388,7 @@ hipError_t launch_fp32(int cfg, int stype, int dtype, const void* src, long ss, 
     switch (cfg) {
     case 0: return launch_types<64, 32, 512, 4>(stype, dtype, src, ss, W, H, dst, ds, wts, stream);
     case 1: return launch_types<32, 32, 256, 4>(stype, dtype, src, ss, W, H, dst, ds, wts, stream);
+    case 2: return launch_types<16, 16, 256, 4>(stype, dtype, src, ss, W, H, dst, ds, wts, stream);
     default: return hipErrorInvalidValue;
     }
 }

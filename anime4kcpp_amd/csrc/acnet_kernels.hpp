@@ -85,6 +85,20 @@ long acnet8_feature_pitch(int w);
 hipError_t launch_acnet8(int blocks, int stype, int dtype, const void* src, long ss, int W, int H, void* dst, long ds,
                          const float* wts, const float* wa, void* workspace, hipStream_t stream);
 
+// ACNetLegacy's FP32 pass layer by layer (acnet8.hip): layer 0 into an f32 map, layers 1..8
+// two per launch, the last pair ending in the deconv; workspace as launch_acnet8 (two maps of
+// acnet8_feature_pitch(W) * H bytes).
+hipError_t launch_legacy_split(int stype, int dtype, const void* src, long ss, int W, int H, void* dst, long ds,
+                               const float* wts, const float* wa32, void* workspace, hipStream_t stream);
+
+// ACNet<8> 2x pass with the conv3x3 layers on 16-bit MFMA (acnet8_mfma16.hip): bf16 = 0 FP16,
+// 1 BF16 operands; wa16 that type's per-lane A operands of the blocks and the final layer
+// (3 x 64 x 8 values per layer, final layer rows 4-7 zero).  Workspace: two 16-bit maps of
+// acnet8_map16_pitch(W) * H bytes (fits in the FP32 path's workspace).
+long acnet8_map16_pitch(int w);
+hipError_t launch_acnet8_16(int bf16, int blocks, int stype, int dtype, const void* src, long ss, int W, int H, void* dst,
+                            long ds, const float* wts, const void* wa16, void* workspace, hipStream_t stream);
+
 // ArtCNN<F> 2x pass, F = 16 or 32 (artcnn.hip): kernels/biases in the reference layout
 // (device copies), wa the per-lane A operands of the B body layers, the residual layer and
 // the F -> 4 tail, 9 * 64 * (F/4) * (F/16) floats each ([tap][lane][K step][channel tile]).

@@ -11,6 +11,8 @@
 //                                                                   FSRCNNX's last block
 //   kShuffle       conv F -> 4, pixel shuffle to 2x                 ArtCNN, FSRCNNX tail
 //   kShuffleSrc    the same, + the source pixel                     ACNet<8> final
+//   kReluDeconv    ReLU, then the 2x2 stride-2 deconv 8 -> 1 to 2x  ACNetLegacy's last layer
+//                  (Common.hpp:344-393), layer by layer
 // Two MFMA mappings (v_mfma_f32_16x16x4_f32):
 //   conv8_layer, conv8_pair  F = 8: the 2-row item of acnet_mfma32_item.hpp, 16 pixels x 2
 //                output rows per item, all 16 MFMA rows useful;
@@ -31,7 +33,8 @@ namespace achip {
 
 #pragma clang fp contract(off)
 
-enum LayerMode : int { kRelu = 0, kPrelu = 1, kAddFeat = 2, kPreluConv1x1 = 3, kShuffle = 4, kShuffleSrc = 5 };
+enum LayerMode : int { kRelu = 0, kPrelu = 1, kAddFeat = 2, kPreluConv1x1 = 3, kShuffle = 4, kShuffleSrc = 5,
+                       kReluDeconv = 6 };
 
 // Arguments of one layer launch; a mode leaves the fields it does not read null.
 struct LayerArgs {
@@ -42,6 +45,7 @@ struct LayerArgs {
     const float* b1;        // conv1x1 biases (F)
     const float* alpha1;    // F PReLU slopes after the skip connection
     const float4* feat;     // skip-connection map, same pitch as the input map
+    const float* k2;        // deconv weights k2[c][q] (kReluDeconv)
     const void* src;        // source image (kShuffleSrc)
     long ss;
     void* dst;              // output image (shuffle modes)
@@ -104,15 +108,25 @@ struct Conv8Tile {
 // item's D registers.  `ok`: the lane's pixel (x, y) is an output of this tile.
 template <int MODE, typename TIN, typename TOUT>
 struct Epi8 {
-    static_assert(MODE == kPrelu || MODE == kPreluConv1x1 || MODE == kShuffle || MODE == kShuffleSrc,
+    static_assert(MODE == kRelu || MODE == kPrelu || MODE == kPreluConv1x1 || MODE == kShuffle || MODE == kShuffleSrc ||
+                      MODE == kReluDeconv,
                   "8-channel epilogues");
     static constexpr bool SHUFFLE = MODE == kShuffle || MODE == kShuffleSrc;
     f32x4m bias, bias1 = { 0.f, 0.f, 0.f, 0.f };
     float alpha[4] = { 0.f, 0.f, 0.f, 0.f }, alpha1[4] = { 0.f, 0.f, 0.f, 0.f }, w1[4] = { 0.f, 0.f, 0.f, 0.f };
+    float kd[4][4];   // kReluDeconv: this lane's 4 channels x 4 sub-pixels of k2
 
     __device__ void load(const LayerArgs& la, int lane, int half)
     {
-        if constexpr (SHUFFLE) {
+        if constexpr (MODE == kRelu || MODE == kReluDeconv) {
+            const float* bl = la.bias + 4 * half;
+            bias = (f32x4m){ bl[0], bl[1], bl[2], bl[3] };
+            if constexpr (MODE == kReluDeconv)
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) kd[c][q] = la.k2[(4 * half + c) * 4 + q];
+        } else if constexpr (SHUFFLE) {
             bias = half ? (f32x4m){ 0.f, 0.f, 0.f, 0.f } : (f32x4m){ la.bias[0], la.bias[1], la.bias[2], la.bias[3] };
         } else {
             const float* bl = la.bias + 4 * half;
@@ -135,7 +149,23 @@ struct Epi8 {
     __device__ void apply(const f32x4m& acc, int x, int y, bool ok, int half, long fp, float4* __restrict__ fout,
                           const LayerArgs& la) const
     {
-        if constexpr (MODE == kPrelu) {
+        if constexpr (MODE == kRelu) {
+            if (ok) {
+                float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(fout) + static_cast<long>(y) * fp) + 2 * x + half;
+                *px = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+            }
+        } else if constexpr (MODE == kReluDeconv) {
+            // this lane's 4 channels' share of the 4 sub-pixels, plus the partner lane's
+            // (lane ^ 16 holds the other 4 channels of the same pixel and row); half = 0
+            // lanes write output row 2y (q = 0, 1), half = 1 lanes row 2y + 1 (q = 2, 3)
+            const float v0 = fmaxf(acc[0], 0.f), v1 = fmaxf(acc[1], 0.f), v2 = fmaxf(acc[2], 0.f), v3 = fmaxf(acc[3], 0.f);
+            float sum[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) sum[q] = fmaf(v3, kd[3][q], fmaf(v2, kd[2][q], fmaf(v1, kd[1][q], v0 * kd[0][q])));
+#pragma unroll
+            for (int q = 0; q < 4; q++) sum[q] += __shfl_xor(sum[q], 16);
+            if (ok) store_pair<TOUT>(row_ptr<TOUT>(la.dst, la.ds, 2 * y + half) + 2 * x, half ? sum[2] : sum[0], half ? sum[3] : sum[1]);
+        } else if constexpr (MODE == kPrelu) {
             if (ok) {
                 float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(fout) + static_cast<long>(y) * fp) + 2 * x + half;
                 *px = make_float4(prelu(acc[0], alpha[0]), prelu(acc[1], alpha[1]), prelu(acc[2], alpha[2]),
@@ -211,7 +241,7 @@ __global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ 
     }
 }
 
-// Two consecutive 8-channel layers in one launch: layer A (conv3x3 + PReLU, arguments la0)
+// Two consecutive 8-channel layers in one launch: layer A (conv3x3 + PReLU or ReLU, arguments la0)
 // and layer B (conv3x3 with the MODE epilogue, arguments la), so the map between them stays
 // in LDS.  Output tile 62 x 16: layer A's region grows by one pixel per side to 64 x 18,
 // exactly 4 segments x 9 row pairs; the staged input is 66 x 20 (80-pixel pitch, 51 KB,
@@ -220,10 +250,11 @@ __global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ 
 // every layer of the reference applies (Common.hpp:120-143).
 constexpr int kP8TW = 62, kP8TH = 16, kP8PW = 80, kP8PH = kP8TH + 4;
 
-template <int MODE, typename TIN, typename TOUT>
+template <int MODE, typename TIN, typename TOUT, int MODE_A = kPrelu>
 __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
                                                   float4* __restrict__ fout, const LayerArgs la0, const LayerArgs la)
 {
+    static_assert(MODE_A == kPrelu || MODE_A == kRelu, "layer A: conv3x3 + PReLU or ReLU");
     __shared__ float4 pa[kP8PW * kP8PH];
     __shared__ float4 pb[kP8PW * kP8PH];
     const int tiles_x = gridDim.x;
@@ -241,7 +272,7 @@ __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin
     float a[24];
 #pragma unroll
     for (int s = 0; s < 24; s++) a[s] = la0.wa[s * 64 + lane];
-    Epi8<kPrelu, float, float> epa;
+    Epi8<MODE_A, float, float> epa;
     epa.load(la0, lane, half);
     __syncthreads();
 
@@ -253,8 +284,11 @@ __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin
     for (int ii = 0; ii < 9; ii++) {
         const int item = wave + 4 * ii, pair = item >> 2, seg = item & 3;
         const f32x4m acc = item24(load_item(pa, pb, 2 * pair * kP8PW + 16 * seg + lane_rd), a, epa.bias);
-        res[ii] = make_float4(prelu(acc[0], epa.alpha[0]), prelu(acc[1], epa.alpha[1]), prelu(acc[2], epa.alpha[2]),
-                              prelu(acc[3], epa.alpha[3]));
+        if constexpr (MODE_A == kPrelu)
+            res[ii] = make_float4(prelu(acc[0], epa.alpha[0]), prelu(acc[1], epa.alpha[1]), prelu(acc[2], epa.alpha[2]),
+                                  prelu(acc[3], epa.alpha[3]));
+        else
+            res[ii] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
     }
     // layer B's operands while the other waves finish
 #pragma unroll

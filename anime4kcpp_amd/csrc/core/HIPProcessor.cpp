@@ -83,6 +83,7 @@ namespace ac::core::hip
         ACHipNet* net = nullptr;
         int precision;
         bool layered = false;     // ACNet<8> or ArtCNN: feature maps in a workspace
+        bool acnet8 = false;      // ACNet<8>: achip_pass8_precision in this processor's arithmetic
         std::string devName{ "no HIP device" };
         std::string initError;
         std::shared_mutex mtx;
@@ -120,13 +121,8 @@ ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device
 }
 
 ac::core::hip::HIPProcessorACNetLegacy::HIPProcessorACNetLegacy(const int device, const model::ACNet<8>& model, const int precision) noexcept
-    : precision(precision), layered(true)
+    : precision(precision), layered(true), acnet8(true)
 {
-    if (precision != 0)
-    {
-        initError = std::string("HIP processor: ACNet<8> runs in FP32 only (the FP16/BF16 forms are ACNetLegacy's); the type string asked for ") + precisionName(precision);
-        return;
-    }
     const int rc = achip_net8_create(device, model.blocks(), model.kernel(0), model.bias(0), model.alpha(0), &net);
     if (rc != ACHIP_OK)
     {
@@ -189,7 +185,8 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pass(ThreadState& st, const void* s
     if (layered)
     {
         if (!ensure(st.ws, achip_net_workspace_bytes(net, w, h))) return false;
-        rc = achip_pass_net(net, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream);
+        rc = acnet8 ? achip_pass8_precision(net, precision, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream)
+                    : achip_pass_net(net, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream);
     }
     else rc = achip_pass(net, precision, src, sp, stype, w, h, dst, dp, dtype, st.stream);
     return rc == ACHIP_OK ? true : fail(std::string(layered ? "achip_pass_net: " : "achip_pass: ") + achip_last_error());

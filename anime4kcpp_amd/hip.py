@@ -204,8 +204,11 @@ class Net:
 class Net8:
     """One ACNet<8> weight set resident on one HIP device (achip_net8_create / achip_pass8)."""
 
-    def __init__(self, variant="b8_hdn", device=0):
-        k, b, a, blocks = load_weights8(variant)
+    def __init__(self, variant="b8_hdn", device=0, weights=None):
+        """weights: (kernels, biases, alphas, blocks) in the reference layout instead of a
+        shipped variant's."""
+        k, b, a, blocks = load_weights8(variant) if weights is None else weights
+        k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (k, b, a))
         h = ctypes.c_void_p()
         check(_native.lib().achip_net8_create(device, blocks, k.ctypes.data, b.ctypes.data, a.ctypes.data,
                                               ctypes.byref(h)), "achip_net8_create")
@@ -248,9 +251,10 @@ class Net8:
             self._ws[ts.cuda_stream] = ws
         return ws
 
-    def upscale(self, src, dst=None, stream=None, out_dtype=None):
+    def upscale(self, src, dst=None, stream=None, out_dtype=None, precision="fp32"):
         """One ACNet<8> 2x pass: src (H x W) -> dst (2H x 2W) on `stream` (default: torch's
-        current stream); the net keeps one feature-map workspace per stream between calls."""
+        current stream) in `precision` (fp32, or the fp16/bf16 MFMA forms); the net keeps one
+        feature-map workspace per stream between calls."""
         import torch
         if src.dim() != 2 or not src.is_cuda or src.stride(1) != 1:
             raise ValueError("src must be a 2-D device tensor with contiguous rows")
@@ -260,10 +264,12 @@ class Net8:
         ts = _torch_stream(stream, src.device)
         ws = self.workspace(w, h, src.device, ts)
         stream = ts.cuda_stream
-        check(_native.lib().achip_pass8(self._h, ctypes.c_void_p(src.data_ptr()), src.stride(0) * src.element_size(),
-                                        elem_type(src.dtype), w, h, ctypes.c_void_p(dst.data_ptr()),
-                                        dst.stride(0) * dst.element_size(), elem_type(dst.dtype),
-                                        ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(stream)), "achip_pass8")
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check(_native.lib().achip_pass8_precision(self._h, prec, ctypes.c_void_p(src.data_ptr()),
+                                                  src.stride(0) * src.element_size(), elem_type(src.dtype), w, h,
+                                                  ctypes.c_void_p(dst.data_ptr()), dst.stride(0) * dst.element_size(),
+                                                  elem_type(dst.dtype), ctypes.c_void_p(ws.data_ptr()), ws.numel(),
+                                                  ctypes.c_void_p(stream)), "achip_pass8_precision")
         return dst
 
 

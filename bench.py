@@ -125,6 +125,47 @@ def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, stream)
     return wall, e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
 
 
+def layer_split_pass(net, src, dst, ws, stream):
+    """One ACNetLegacy FP32 pass in the layer-split form (achip_pass_layers: layer 0, then
+    layers 1..8 two per launch through f32 maps in `ws`)."""
+    import ctypes
+    from anime4kcpp_amd import _native, hip
+    h, w = src.shape
+    rc = _native.lib().achip_pass_layers(net.handle, ctypes.c_void_p(src.data_ptr()), src.stride(0) * src.element_size(),
+                                         hip.elem_type(src.dtype), w, h, ctypes.c_void_p(dst.data_ptr()),
+                                         dst.stride(0) * dst.element_size(), hip.elem_type(dst.dtype),
+                                         ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(stream))
+    if rc != 0:
+        raise RuntimeError(f"achip_pass_layers: {_native.last_error()}")
+
+
+def time_layer_split(nets, srcs, dsts, order, steps, warmup, stream):
+    """The same frames in the layer-split FP32 form; returns (wall seconds, mean seconds per
+    frame on the stream)."""
+    import torch
+    from anime4kcpp_amd import _native
+    h, w = srcs[0].shape
+    ws = torch.empty(_native.lib().achip_pass8_workspace_bytes(w, h), dtype=torch.uint8, device=srcs[0].device)
+    st = stream.cuda_stream
+
+    def step():
+        for i, v in order:
+            layer_split_pass(nets[v], srcs[i], dsts[i], ws, st)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
+
+
 def error_vs_fp64(nets, weights, prec):
     """Max/mean abs error and PSNR of one 1920x1080 f32 frame (HDN0, f32 output) against
     the FP64 restatement of the network."""
@@ -174,13 +215,20 @@ def batched_uneven(nets, prec, seed=2024, n=64):
     best = min(sweep, key=sweep.get)
     sep = timed(lambda: [net.upscale(s, d, precision=prec, config=best) for s, d in zip(srcs, dsts)])
     tb = sweep[best]
+    names = {c: f"{hip.configs(prec)[c]['tw']}x{hip.configs(prec)[c]['th']}/{hip.configs(prec)[c]['nt']}thr"
+                + ("/mfma" if hip.configs(prec)[c]["p"] == 0 else "/valu") + "/halo-recompute" for c in sweep}
+    sweep_ms = {names[c]: round(t * 1e3, 3) for c, t in sweep.items()}
+    if prec == "fp32":
+        # the layer-split form (one image at a time: 5 launches, f32 maps through HBM)
+        lws = torch.empty(_native.lib().achip_pass8_workspace_bytes(2048, 2048), dtype=torch.uint8, device=dev)
+        st = torch.cuda.current_stream(dev).cuda_stream
+        sweep_ms["62x16/256thr/mfma/layer-split (per image)"] = round(
+            timed(lambda: [layer_split_pass(net, s, d, lws, st) for s, d in zip(srcs, dsts)]) * 1e3, 3)
     return {
         "images": n, "input_mpix": round(sum(s.numel() for s in srcs) / 1e6, 2), "output_mpix": round(out_mpix, 2),
         "one_launch": {"images_per_s": round(n / tb, 1), "mpix_per_s": round(out_mpix / tb, 1), "ms": round(tb * 1e3, 3)},
         "separate_launches": {"images_per_s": round(n / sep, 1), "mpix_per_s": round(out_mpix / sep, 1), "ms": round(sep * 1e3, 3)},
-        "autotune_sweep_ms": {f"{hip.configs(prec)[c]['tw']}x{hip.configs(prec)[c]['th']}/{hip.configs(prec)[c]['nt']}thr"
-                              + ("/mfma" if hip.configs(prec)[c]["p"] == 0 else "/valu"): round(t * 1e3, 3)
-                              for c, t in sweep.items()},
+        "autotune_sweep_ms": sweep_ms,
         "chosen_config": best,
     }
 
@@ -383,14 +431,19 @@ def stages():
     return out
 
 
-def acnet8(frames_np, steps=25):
-    """ACNet<8> (the reference's default model family) 1080p u8 -> 4K 2x passes on FP32 MFMA,
-    blocks two per launch, for 4, 8 and 18 blocks.  FLOP per input pixel: 2 x (72 + 576 blocks
-    + 288) MAC.  HBM bytes per input pixel: the u8 source read twice (L0, the final residual),
-    4 output bytes, and a 32-byte map written by L0, read and written once per launch of
-    blocks, and read by the final layer."""
+def acnet8(frames_np, weights=None, steps=25):
+    """ACNet<8> (the reference's default model family) 1080p u8 -> 4K 2x passes for 4, 8 and 18
+    blocks, in FP32 (FP32 MFMA, f32 maps) and in FP16 / BF16 (16-bit MFMA, 16-bit maps); blocks
+    two per launch.  FLOP per input pixel: 2 x (72 + 576 blocks + 288) MAC.  HBM bytes per
+    input pixel: the u8 source read twice (L0, the final residual), 4 output bytes, and an
+    8-channel map (32 bytes FP32, 16 bytes 16-bit) written by L0, read and written once per
+    launch of blocks, and read by the final layer.  Error of each form against the FP64
+    restatement on a 1080p f32 frame (B8 HDN)."""
+    import numpy as np
     import torch
-    from anime4kcpp_amd import hip
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    from anime4kcpp_amd import hip, synth
     dev = torch.device("cuda:0")
     srcs = [torch.from_numpy(f).to(dev) for f in frames_np[:4]]
     h, w = frames_np[0].shape
@@ -398,25 +451,42 @@ def acnet8(frames_np, steps=25):
     out = {}
     for variant in ("b4_hdn", "b8_hdn", "b18_hdn"):
         net = hip.Net8(variant, 0)
-        for _ in range(3):   # warm-up: clocks, caches, first-launch costs
-            for i in range(len(srcs)):
-                net.upscale(srcs[i], dsts[i])
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(steps):
-            for i in range(len(srcs)):
-                net.upscale(srcs[i], dsts[i])
-        e1.record()
-        torch.cuda.synchronize()
-        sec = e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
-        flop = 2 * (72 + 576 * net.blocks + 288) * w * h
-        out[variant] = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
-                        "blocks": net.blocks, "achieved_tflops": round(flop / sec / 1e12, 2),
-                        "frac_of_fp32_peak": round(flop / sec / 1e12 / PEAK_FP32_TFLOPS, 4),
-                        "hbm_bytes_per_frame": (2 + 4 + 32 + 64 * ((net.blocks + 1) // 2) + 32) * w * h}
+        out[variant] = {"blocks": net.blocks}
+        for prec in ("fp32", "fp16", "bf16"):
+            for _ in range(3):   # warm-up: clocks, caches, first-launch costs
+                for i in range(len(srcs)):
+                    net.upscale(srcs[i], dsts[i], precision=prec)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(steps):
+                for i in range(len(srcs)):
+                    net.upscale(srcs[i], dsts[i], precision=prec)
+            e1.record()
+            torch.cuda.synchronize()
+            sec = e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
+            flop = 2 * (72 + 576 * net.blocks + 288) * w * h
+            peak = PEAK_FP32_TFLOPS if prec == "fp32" else PEAK_MFMA16_TFLOPS
+            mb = 32 if prec == "fp32" else 16
+            entry = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
+                     "achieved_tflops": round(flop / sec / 1e12, 2), "peak_tflops": peak,
+                     "frac": round(flop / sec / 1e12 / peak, 4),
+                     "hbm_bytes_per_frame": (2 + 4 + mb + 2 * mb * ((net.blocks + 1) // 2) + mb) * w * h}
+            if variant == "b8_hdn":
+                k, b, a, blocks = hip.load_weights8(variant)
+                xf = synth.flat_shaded(7, w, h)
+                if "ref64" not in out:
+                    out["ref64"] = OL.oracle_pass8_f64(xf, k, b, a, blocks)
+                got = net.upscale(torch.from_numpy(xf).to(dev), precision=prec, out_dtype=torch.float32).cpu().numpy()
+                e = np.abs(got.astype(np.float64) - out["ref64"])
+                mse = float((e ** 2).mean())
+                entry["error_vs_fp64"] = {"max_abs": float(e.max()), "mean_abs": float(e.mean()),
+                                          "psnr_db": round(10 * np.log10(1.0 / mse), 2) if mse > 0 else None}
+            out[variant][prec] = entry
         net.close()
-    out["workload"] = "1920x1080 u8 -> 3840x2160 u8, ACNet<8> HDN weights, layer by layer (8-channel f32 maps in HBM)"
+    out.pop("ref64", None)
+    out["workload"] = ("1920x1080 u8 -> 3840x2160 u8, ACNet<8> HDN weights, layer by layer (8-channel maps in HBM: f32 "
+                       "for fp32, 16-bit for fp16/bf16); error on a 1920x1080 f32 frame (B8 HDN) against FP64")
     return out
 
 
@@ -563,6 +633,15 @@ def main():
             "flop_per_launch": FLOP_PER_INPUT_PX * W_IN * H_IN,
         },
     }
+    if "fp32" in precs:
+        # the layer-split alternative of the same FP32 pass (five launches, f32 maps through HBM)
+        wl, ml = time_layer_split(nets, srcs, dsts, order, args.steps, args.warmup, stream)
+        achl = FLOP_PER_INPUT_PX * W_IN * H_IN / ml / 1e12
+        result["fp32_layer_split"] = {
+            "value": round(out_px_per_step * args.steps / 1e6 / wl, 1), "unit": "Mpix/s",
+            "ms_per_step": round(wl * 1e3 / args.steps, 4), "mean_frame_us": round(ml * 1e6, 2),
+            "achieved_tflops": round(achl, 2), "frac": round(achl / PEAK_FP32_TFLOPS, 4),
+            "form": "layer 0, then layers 1..8 two per launch (conv8_pair), 8-channel f32 maps in HBM: 5 launches"}
     for prec in precs[1:]:
         w2, ml2 = timings[prec]
         v2 = out_px_per_step * args.steps / 1e6 / w2
@@ -599,10 +678,11 @@ def main():
         for spec in args.serving_loads.split(";"):
             p, loads = spec.split(":")
             result["serving"][p] = serving(nets["hdn0"], p, [float(x) for x in loads.split(",")], args.serving_seconds)
-        if not args.no_acnet8:   # the reference's default model (acnet-f8b8-hdn), FP32
+        if not args.no_acnet8:   # the reference's default model (acnet-f8b8-hdn), FP32 and FP16
             from anime4kcpp_amd import hip as _hip
             n8 = _hip.Net8("b8_hdn", 0)
             result["serving"]["acnet8_b8_hdn_fp32"] = serving(n8, "fp32", [60.0, 240.0], args.serving_seconds)
+            result["serving"]["acnet8_b8_hdn_fp16"] = serving(n8, "fp16", [300.0, 1200.0], args.serving_seconds)
             n8.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(frames_np, weights)

@@ -96,6 +96,15 @@ ACHIP_API int achip_pass_batch(ACHipNet* net, int precision, int config, int src
                                const ACHipImage* images, int n, void* workspace, size_t workspace_bytes,
                                void* stream);
 
+/* ACNetLegacy's FP32 2x pass layer by layer instead of fused: layer 0, then layers 1..8 two
+ * per launch through 8-channel f32 maps in `workspace` (achip_pass8_workspace_bytes(w, h)
+ * bytes, 256-byte aligned, device memory of the net's device, not overlapping src or dst):
+ * five launches, each map through HBM once, a one-pixel recompute ring per launch instead of
+ * the fused kernel's eight-pixel halo; results within the FP32 tolerance of the fused pass.
+ * The reference's CPU pass runs the layers this way (CPUProcessor.cpp:1371-1390). */
+ACHIP_API int achip_pass_layers(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h, void* dst,
+                                long dst_stride, int dst_type, void* workspace, size_t workspace_bytes, void* stream);
+
 /* achip_pass_batch without a workspace argument and without any allocation, host or device.
  * No counterpart in the reference: it is for callers that cannot allocate (code under HIP
  * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The tile-queue
@@ -148,6 +157,15 @@ ACHIP_API size_t achip_pass8_workspace_bytes(int w, int h);
 ACHIP_API int achip_pass8(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h,
                           void* dst, long dst_stride, int dst_type, void* workspace, size_t workspace_bytes,
                           void* stream);
+/* achip_pass8 in a given arithmetic: ACHIP_FP32 (the same as achip_pass8), or ACHIP_FP16 /
+ * ACHIP_BF16 -- the conv3x3 layers on 16-bit MFMA with FP32 accumulation, bias, PReLU and the
+ * final shuffle + residual in FP32, the feature maps between launches rounded to 16 bits; the
+ * tests bound its error against a double-precision restatement of the network.  Same
+ * workspace as achip_pass8.  The reference's ACNet<8> backends compute in FP32 on the CPU
+ * (CPUProcessor.cpp:1417-1436) and keep FP16 maps on CUDA (CUDAProcessor.cpp:463-464). */
+ACHIP_API int achip_pass8_precision(ACHipNet* net, int precision, const void* src, long src_stride, int src_type, int w,
+                                    int h, void* dst, long dst_stride, int dst_type, void* workspace,
+                                    size_t workspace_bytes, void* stream);
 
 /* ---- ArtCNN<F> ----
  * core/include/AC/Core/Model/ArtCNN.hpp (F = 16 or 32; CPU pass CPUProcessor.cpp:1519-1602):
@@ -272,9 +290,9 @@ typedef struct ACHipServerStats {
 } ACHipServerStats;
 
 /* A server runs one net.  ACNetLegacy nets run in any precision, their one-off passes
- * batched into persistent tile-queue launches; ACNet<8>, ArtCNN and FSRCNNX nets run in
- * ACHIP_FP32 only (ACHIP_EINVAL otherwise), each step running the admitted one-offs' next
- * passes whole, with a feature-map workspace per request (and per video stream) counted
+ * batched into persistent tile-queue launches; ACNet<8> nets in any precision and ArtCNN and
+ * FSRCNNX nets in ACHIP_FP32 only (ACHIP_EINVAL otherwise) run each step's admitted one-offs'
+ * next passes whole, with a feature-map workspace per request (and per video stream) counted
  * against the HBM budget. */
 ACHIP_API int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipServer** out);
 /* Waits for all admitted work, then frees everything. */

@@ -689,6 +689,74 @@ int orc_acnet_pass_f64(const float* src, int w, int h, double* dst,
     return 0;
 }
 
+/* ACNet<8> 2x pass in double precision (the same network as orc_acnet8_pass: replicate-padded
+ * conv3x3 layers, PReLU, the final conv 8 -> 4, pixel shuffle, + the source pixel, saturated to
+ * [0,1]); src is the toFloat'ed input.  Pinned against torch float64 conv2d / prelu /
+ * pixel_shuffle (tests/test_oracle_acnet8.py). */
+int orc_acnet8_pass_f64(const float* src, int w, int h, double* dst, const float* kernels, const float* biases,
+                        const float* alphas, int blocks, int threads)
+{
+    if (!src || !dst || w <= 0 || h <= 0 || !kernels || !biases || !alphas || blocks < 1) return -22;
+    size_t plane = (size_t)w * h * CH;
+    double* t1 = (double*)malloc(plane * sizeof(double));
+    double* t2 = (double*)malloc(plane * sizeof(double));
+    if (!t1 || !t2) { free(t1); free(t2); return -12; }
+#ifdef _OPENMP
+    int nt = threads > 0 ? threads : omp_get_max_threads();
+#else
+    (void)threads;
+#endif
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+    for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++)
+            for (int n = 0; n < CH; n++) {
+                double acc = biases[n];
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        int y = clampi(i + dy, 0, h - 1), x = clampi(j + dx, 0, w - 1);
+                        acc += (double)src[(size_t)y * w + x] * (double)kernels[n * 9 + (dy + 1) * 3 + (dx + 1)];
+                    }
+                t1[((size_t)i * w + j) * CH + n] = acc > 0.0 ? acc : (double)alphas[n] * acc;
+            }
+    double* tin = t1;
+    double* tout = t2;
+    for (int l = 1; l <= blocks + 1; l++) {
+        const int cout = l <= blocks ? CH : 4;
+        const float* k = kernels + 72 + 576 * (l - 1);
+        const float* b = biases + 8 * l;
+        const float* al = alphas + 8 * l;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(guided) num_threads(nt)
+#endif
+        for (int i = 0; i < h; i++)
+            for (int j = 0; j < w; j++)
+                for (int n = 0; n < cout; n++) {
+                    double acc = b[n];
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) {
+                            int y = clampi(i + dy, 0, h - 1), x = clampi(j + dx, 0, w - 1);
+                            const double* r = tin + ((size_t)y * w + x) * CH;
+                            const float* kk = k + n * 72 + ((dy + 1) * 3 + (dx + 1)) * 8;
+                            for (int c = 0; c < CH; c++) acc += r[c] * (double)kk[c];
+                        }
+                    if (l <= blocks) {
+                        tout[((size_t)i * w + j) * CH + n] = acc > 0.0 ? acc : (double)al[n] * acc;
+                    } else {
+                        /* sub-pixel n goes to (2j + n % 2, 2i + n / 2), plus the source pixel */
+                        double v = acc + (double)src[(size_t)i * w + j];
+                        v = v < 0.0 ? 0.0 : (v < 1.0 ? v : 1.0);
+                        dst[(size_t)(2 * i + (n >> 1)) * (2 * w) + (2 * j + (n & 1))] = v;
+                    }
+                }
+        double* tmp = tin; tin = tout; tout = tmp;
+    }
+    free(t1);
+    free(t2);
+    return 0;
+}
+
 /* ---- colour split / merge (ImageProcess.cpp:38-61, :114-139, :191-215, :276-308) ---- */
 
 int orc_rgb2yuv(const void* rgb, long rgb_stride, int w, int h, int channels, int type,

@@ -13,6 +13,7 @@
  *     against torch conv2d/conv_transpose2d (library routine); the constant
  *     image closed form; brute force on tiny images; translation equivariance.
  *   - orc_acnet_pass_f64: against torch float64 conv2d (library routine).
+ *   - orc_acnet8_pass_f64: against torch float64 conv2d / prelu / pixel_shuffle.
  *   - orc_from_float / orc_to_float: exhaustive closed-form checks.
  *   - orc_rgb2yuv / orc_yuv2rgb: closed-form per-pixel formulas and the
  *     reference's own round-trip bound (tests/core/src/ImageProcessTest.cpp:52-79).
@@ -72,6 +73,10 @@ int orc_fsrcnnx_pass(const void* src, long src_stride, int w, int h, int type, v
 
 /* The same network in double precision.  src is the toFloat'ed input
  * (w*h floats, dense); dst receives the saturated 2w x 2h output in double. */
+/* ACNet<8> in double precision, for the error of the 16-bit forms: src toFloat'ed, dst 2h x 2w
+ * saturated to [0,1]. */
+int orc_acnet8_pass_f64(const float* src, int w, int h, double* dst, const float* kernels, const float* biases,
+                        const float* alphas, int blocks, int threads);
 int orc_acnet_pass_f64(const float* src, int w, int h, double* dst,
                        const float* kernels, const float* biases, int threads);
 

@@ -138,17 +138,24 @@ static void processor_factory(const bool gpu)
         else CHECK(!p->ok() && std::strstr(p->error(), "no HIP device") != nullptr);
     }
     // precision suffixes, parsed whole: "bf16" contains "f16" and must still mean BF16.
-    // The FP32-only families reject 16-bit types before touching a device and name the
-    // precision they were asked for, so the parse is visible without a GPU.
+    // The FP32-only families (ArtCNN, FSRCNNX) reject 16-bit types before touching a device
+    // and name the precision they were asked for, so the parse is visible without a GPU.
     const struct { const char* type; const char* want; } suffixes[] = {
         { "hip-bf16", "BF16" }, { "hip:bf16", "BF16" }, { "HIP-BF16", "BF16" },
         { "hip-fp16", "FP16" }, { "hip:fp16", "FP16" }, { "hip-f16", "FP16" } };
     for (const auto& c : suffixes)
     {
-        auto q = ac::core::Processor::create(c.type, 0, "acnet-b8");
+        auto q = ac::core::Processor::create(c.type, 0, "artcnn-c4f16");
         CHECK(!q->ok() && std::strstr(q->error(), "FP32 only") != nullptr);
         const char* tail = std::strstr(q->error(), "asked for ");
         CHECK(tail != nullptr && std::strcmp(tail + 10, c.want) == 0);
+        // ACNet<8> and ACNetLegacy run every precision
+        for (const char* model : { "acnet-b8", "acnet-legacy-hdn0" })
+        {
+            auto r = ac::core::Processor::create(c.type, 0, model);
+            if (gpu) CHECK(r->ok());
+            else CHECK(!r->ok() && std::strstr(r->error(), "no HIP device") != nullptr);
+        }
     }
     for (const char* fp32 : { "hip", "hip-", "hip-fp32", "hip-bf16x" })
     {

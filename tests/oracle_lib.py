@@ -46,6 +46,7 @@ def orc():
         o.orc_resize2x.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, ctypes.c_int]
         o.orc_resize.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        o.orc_acnet8_pass_f64.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, P, ctypes.c_int, ctypes.c_int]
         o.orc_acnet8_pass.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, P, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int]
         o.orc_artcnn_pass.argtypes = [P, L, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, L, P, P, ctypes.c_int,
@@ -144,6 +145,17 @@ def oracle_pass8(src, kernels, biases, alphas, blocks, order=ORDER_FMA, threads=
     k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases, alphas))
     assert orc().orc_acnet8_pass(src.ctypes.data, src.strides[0], w, h, TYPE_OF[src.dtype], dst.ctypes.data,
                                  dst.strides[0], k.ctypes.data, b.ctypes.data, a.ctypes.data, blocks, order, threads) == 0
+    return dst
+
+
+def oracle_pass8_f64(src_float, kernels, biases, alphas, blocks, threads=0):
+    """ACNet<8> in float64: src_float is the toFloat'ed input (float32) -> saturated 2x output."""
+    s = np.ascontiguousarray(src_float, dtype=np.float32)
+    h, w = s.shape
+    dst = np.zeros((2 * h, 2 * w), np.float64)
+    k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (kernels, biases, alphas))
+    assert orc().orc_acnet8_pass_f64(s.ctypes.data, w, h, dst.ctypes.data, k.ctypes.data, b.ctypes.data, a.ctypes.data,
+                                     blocks, threads) == 0
     return dst
 
 

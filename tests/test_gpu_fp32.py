@@ -191,3 +191,56 @@ def test_invalid_arguments_launch_nothing(nets, cuda):
     finally:
         hiprt.hipFree(buf)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("variant", ["gan", "hdn0", "hdn3"])
+@pytest.mark.parametrize("w,h", [(1, 1), (5, 3), (63, 17), (130, 71)])
+def test_layer_split_form(nets, weights, variant, w, h):
+    """achip_pass_layers (layer 0, then layers 1..8 two per launch through f32 maps in a
+    workspace): f32 within 1e-5 and u8 within the u8 bar of the oracle."""
+    import ctypes
+    from anime4kcpp_amd import _native
+    L = _native.lib()
+    net = nets[variant]
+    k, b = weights[variant]
+    dev = torch.device("cuda", net.device)
+    ws = torch.empty(L.achip_pass8_workspace_bytes(w, h) + 256, dtype=torch.uint8, device=dev)
+    wp = (ws.data_ptr() + 255) // 256 * 256
+    for dt in (np.float32, np.uint8):
+        src = synth.flat_shaded(60 + w, w, h, dtype=dt)
+        s = to_dev(src, dev)
+        d = torch.empty((2 * h, 2 * w), dtype=s.dtype, device=dev)
+        es = s.element_size()
+        rc = L.achip_pass_layers(net.handle, ctypes.c_void_p(s.data_ptr()), w * es, hip.elem_type(s.dtype), w, h,
+                                 ctypes.c_void_p(d.data_ptr()), 2 * w * es, hip.elem_type(d.dtype), ctypes.c_void_p(wp),
+                                 L.achip_pass8_workspace_bytes(w, h), None)
+        assert rc == 0, _native.last_error()
+        torch.cuda.synchronize()
+        want = oracle_pass(src, k, b)
+        if dt == np.float32:
+            check_f32(to_host(d), want)
+        else:
+            check_int(to_host(d), want)
+
+
+def test_layer_split_1080p_u8(nets, weights):
+    import ctypes
+    from anime4kcpp_amd import _native
+    L = _native.lib()
+    net = nets["hdn1"]
+    k, b = weights["hdn1"]
+    src = synth.flat_shaded(61, 1920, 1080, dtype=np.uint8)
+    s = torch.from_numpy(src).cuda()
+    d = torch.empty((2160, 3840), dtype=torch.uint8, device="cuda")
+    ws = torch.empty(L.achip_pass8_workspace_bytes(1920, 1080), dtype=torch.uint8, device="cuda")
+    assert L.achip_pass_layers(net.handle, ctypes.c_void_p(s.data_ptr()), 1920, 0x001, 1920, 1080, ctypes.c_void_p(d.data_ptr()),
+                               3840, 0x001, ctypes.c_void_p(ws.data_ptr()), ws.numel(), None) == 0
+    torch.cuda.synchronize()
+    check_int(d.cpu().numpy(), oracle_pass(src, k, b))
+    # the workspace may not overlap src or dst, and an ACNet<8> net is refused
+    assert L.achip_pass_layers(net.handle, ctypes.c_void_p(s.data_ptr()), 1920, 0x001, 1920, 1080, ctypes.c_void_p(d.data_ptr()),
+                               3840, 0x001, ctypes.c_void_p(d.data_ptr()), ws.numel(), None) == -22
+    n8 = hip.Net8("b4", 0)
+    assert L.achip_pass_layers(n8.handle, ctypes.c_void_p(s.data_ptr()), 1920, 0x001, 1920, 1080, ctypes.c_void_p(d.data_ptr()),
+                               3840, 0x001, ctypes.c_void_p(ws.data_ptr()), ws.numel(), None) == -22
+    n8.close()

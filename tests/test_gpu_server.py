@@ -177,9 +177,9 @@ def test_invalid_requests(net):
 @pytest.mark.parametrize("model,cls,variant", [("acnet-b4-hdn", hip.Net8, "b4_hdn"), ("artcnn-c4f16-ds", hip.NetArt, "f16_ds"),
                                                ("fsrcnnx", hip.NetFsr, "f8")])
 def test_layered_nets_serve(cuda, model, cls, variant):
-    """ACNet<8>, ArtCNN and FSRCNNX nets in the serving loop (FP32): the mixed load of one-offs
-    and frames gives every request the same result as alone, and RGB one-offs equal
-    Processor::process with the same model."""
+    """ACNet<8>, ArtCNN and FSRCNNX nets in the serving loop (FP32, and FP16 for ACNet<8>): the
+    mixed load of one-offs and frames gives every request the same result as alone, and RGB
+    one-offs equal Processor::process with the same model."""
     net = cls(variant, 0)
     reqs = make_requests()
     with hip.Server(net, "fp32") as alone_srv:
@@ -198,6 +198,23 @@ def test_layered_nets_serve(cuda, model, cls, variant):
         if k == hip.REQ_RGB:
             rc, want = proc.process(p[0], f)
             assert rc == 0 and np.array_equal(mixed[i][0], want), f"request {i}"
-    with pytest.raises(Exception):
-        hip.Server(net, "fp16")                       # these families run in FP32 only
+    if cls is hip.Net8:
+        # ACNet<8> also serves in FP16: mixed equals alone, one-offs equal the "hip-fp16" processor
+        with hip.Server(net, "fp16") as alone_srv:
+            alone16 = run(alone_srv, reqs, together=False)
+        with hip.Server(net, "fp16", step_tiles=16) as mix_srv:
+            mixed16 = run(mix_srv, reqs, together=True)
+            assert mix_srv.stats()["failed"] == 0
+        for i, (a, m) in enumerate(zip(alone16, mixed16)):
+            for pa, pm in zip(a, m):
+                assert np.array_equal(pa, pm), f"fp16 request {i} differs between alone and mixed"
+        p16 = P.Processor("hip-fp16", 0, model)
+        assert p16.ok(), p16.error()
+        for i, (k, p, f, s) in enumerate(reqs):
+            if k == hip.REQ_RGB:
+                rc, want = p16.process(p[0], f)
+                assert rc == 0 and np.array_equal(mixed16[i][0], want), f"fp16 request {i}"
+    else:
+        with pytest.raises(Exception):
+            hip.Server(net, "fp16")                   # ArtCNN and FSRCNNX run in FP32 only
     net.close()

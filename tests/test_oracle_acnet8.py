@@ -68,6 +68,19 @@ def test_oracle_against_torch_float64(variant):
     assert np.abs(got.astype(np.float64) - want).max() < 5e-6
 
 
+@pytest.mark.parametrize("variant", ["b4", "b8_hdn", "b18_box_hdn"])
+def test_oracle_f64_against_torch_float64(variant):
+    """The double-precision restatement (the reference the 16-bit forms are measured against)
+    equals torch's float64 routines to rounding, and differs from a weight-order mistake."""
+    k, b, a, blocks = hip.load_weights8(variant)
+    x = synth.flat_shaded(9, 23, 17)
+    want = torch_pass8(x, k, b, a, blocks)
+    assert np.abs(OL.oracle_pass8_f64(x, k, b, a, blocks) - want).max() < 1e-12
+    bad = k.copy()
+    bad[72:72 + 576] = bad[72:72 + 576].reshape(8, 9, 8).transpose(0, 2, 1).reshape(-1)   # channel/tap swapped
+    assert np.abs(OL.oracle_pass8_f64(x, bad, b, a, blocks) - want).max() > 1e-4
+
+
 def test_planted_shuffle_mistake_fails():
     """A pixel shuffle with the sub-pixel axes swapped must not pass the torch check."""
     k, b, a, blocks = hip.load_weights8("b4_hdn")
