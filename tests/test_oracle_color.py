@@ -229,3 +229,97 @@ def test_interpolating_modes_pass_samples_through_at_3x(mode):
     x = np.random.default_rng(13).random((7, 9), dtype=np.float32)
     out = OL.resize(x, 27, 21, mode)
     assert np.abs(out[1::3, 1::3] - x).max() < 1e-6
+
+
+# ---- an independent pin: the published BT.601 definition -----------------------------------
+# The reference's constants (ImageProcess.cpp:38-61, :191-215) are the BT.601 ones rounded to
+# three decimals: luma weights Kr = 0.299, Kg = 0.587, Kb = 0.114 (ITU-R BT.601), chroma
+# scales 0.5 / (1 - Kb) and 0.5 / (1 - Kr), and the inverse matrix of that transform.  Deriving
+# every coefficient from the three published luma weights (not from the oracle's own table)
+# and comparing on f32 images, where no rounding to u8 can hide a misread digit, pins each of
+# the nine coefficients to within the reference's three-decimal rounding.
+KR, KG, KB = 0.299, 0.587, 0.114
+
+
+def bt601_forward(r, g, b):
+    y = KR * r + KG * g + KB * b
+    return y, 0.5 / (1 - KB) * (b - y) + 0.5, 0.5 / (1 - KR) * (r - y) + 0.5
+
+
+def bt601_inverse(y, u, v):
+    u, v = u - 0.5, v - 0.5
+    r = y + 2 * (1 - KR) * v
+    b = y + 2 * (1 - KB) * u
+    g = y - 2 * (1 - KB) * KB / KG * u - 2 * (1 - KR) * KR / KG * v
+    return r, g, b
+
+
+def orc_split_f32(rgb):
+    h, w, c = rgb.shape
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    y = np.zeros((h, w), np.float32)
+    uv = np.zeros((h, w, c - 1), np.float32)
+    assert OL.orc().orc_rgb2yuv(rgb.ctypes.data, rgb.strides[0], w, h, c, OL.F32, y.ctypes.data, y.strides[0],
+                                uv.ctypes.data, uv.strides[0]) == 0
+    return y, uv
+
+
+def orc_merge_f32(y, uv):
+    h, w = y.shape
+    c = uv.shape[2] + 1
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    uv = np.ascontiguousarray(uv, dtype=np.float32)
+    out = np.zeros((h, w, c), np.float32)
+    assert OL.orc().orc_yuv2rgb(y.ctypes.data, y.strides[0], uv.ctypes.data, uv.strides[0], w, h, c, OL.F32,
+                                out.ctypes.data, out.strides[0]) == 0
+    return out
+
+
+def unit_colours():
+    """The eight corners of the RGB cube plus a seeded spread inside it (f32)."""
+    corners = np.array([[r, g, b] for r in (0, 1) for g in (0, 1) for b in (0, 1)], np.float64)
+    rng = np.random.default_rng(601)
+    inner = rng.uniform(0.05, 0.95, (64, 3))
+    return np.concatenate([corners, inner]).reshape(8, 9, 3).astype(np.float32)
+
+
+def test_split_matches_bt601_definition():
+    rgb = unit_colours()
+    y, uv = orc_split_f32(rgb)
+    ey, eu, ev = bt601_forward(*(rgb[..., i].astype(np.float64) for i in range(3)))
+    sat = lambda a: np.clip(a, 0.0, 1.0)   # noqa: E731  (fromFloat saturates f32 too)
+    # three-decimal constants: |coefficient error| <= 5e-4 per unit of input
+    assert np.abs(y - sat(ey)).max() < 1.5e-3
+    assert np.abs(uv[..., 0] - sat(eu)).max() < 1.5e-3
+    assert np.abs(uv[..., 1] - sat(ev)).max() < 1.5e-3
+    # primaries single out each luma weight (any misread digit moves Y by >= 1e-3)
+    assert abs(float(y[0, 4]) - KR) < 1e-6 and abs(float(y[0, 2]) - KG) < 1e-6 and abs(float(y[0, 1]) - KB) < 1e-6
+
+
+def test_merge_matches_bt601_inverse():
+    rng = np.random.default_rng(602)
+    yuv = rng.uniform(0.2, 0.8, (6, 11, 3)).astype(np.float32)
+    out = orc_merge_f32(yuv[..., 0], yuv[..., 1:])
+    er, eg, eb = bt601_inverse(*(yuv[..., i].astype(np.float64) for i in range(3)))
+    for got, want in ((out[..., 0], er), (out[..., 1], eg), (out[..., 2], eb)):
+        want = np.clip(want, 0.0, 1.0)
+        assert np.abs(got - want).max() < 2e-3
+    # a chroma-only input isolates each inverse coefficient: v alone moves R by 1.403 v and
+    # G by -0.714 v, u alone moves B by 1.773 u and G by -0.344 u
+    y0 = np.full((1, 2), 0.5, np.float32)
+    uv = np.array([[[0.5, 0.6], [0.6, 0.5]]], np.float32)
+    o = orc_merge_f32(y0, uv)
+    assert abs(float(o[0, 0, 0]) - 0.5 - 0.1 * 2 * (1 - KR)) < 2e-4       # R from v
+    assert abs(float(o[0, 0, 1]) - 0.5 + 0.1 * 2 * (1 - KR) * KR / KG) < 2e-4
+    assert abs(float(o[0, 1, 2]) - 0.5 - 0.1 * 2 * (1 - KB)) < 2e-4       # B from u
+    assert abs(float(o[0, 1, 1]) - 0.5 + 0.1 * 2 * (1 - KB) * KB / KG) < 2e-4
+
+
+def test_planted_coefficient_misread_fails():
+    """A coefficient off in its third decimal (0.578 for 0.587) must fail the BT.601 pin
+    even though the oracle's own restatement would carry the same misread."""
+    rgb = unit_colours()
+    r, g, b = (rgb[..., i].astype(np.float64) for i in range(3))
+    bad_y = 0.299 * r + 0.578 * g + 0.114 * b
+    ey, _, _ = bt601_forward(r, g, b)
+    assert np.abs(np.clip(bad_y, 0, 1) - np.clip(ey, 0, 1)).max() >= 1.5e-3
