@@ -54,55 +54,121 @@ double ms_between(Clock::time_point a, Clock::time_point b)
 
 size_t round_block(size_t v) { return (std::max<size_t>(v, 1) + kBlock - 1) / kBlock * kBlock; }
 
-// Caching device allocator: blocks rounded to 2 MiB, reused by exact rounded size.  Blocks
-// are handed back only after the work that used them has completed.
+// Device allocator over a few large chunks: hipMalloc takes 0.05-0.3 ms a call and a
+// request needs up to eight buffers, so one hipMalloc per buffer would hold the scheduler
+// thread (and with it the device) for milliseconds whenever many requests arrive.  Chunks grow
+// geometrically (the first kFirstChunk, then as large as everything held so far, at most the
+// cap); blocks are 2 MiB-granular ranges of a chunk, taken best-fit, and freed ranges
+// coalesce with their free neighbours inside the same chunk.  Blocks are handed back only
+// after the work that used them has completed.  A chunk is returned to the device when it is
+// wholly free and an allocation fails, or when the pool is destroyed.
 class DevicePool {
 public:
-    ~DevicePool() { trim(0); }
+    static constexpr size_t kFirstChunk = size_t(256) << 20;
+    ~DevicePool() { trim(); }
+    void set_chunk_cap(size_t cap) { cap_ = std::max(round_block(cap), kFirstChunk); }
     hipError_t get(size_t bytes, void** out)
     {
         const size_t b = round_block(bytes);
-        auto it = cache_.find(b);
-        if (it != cache_.end()) {
-            *out = it->second;
-            cached_ -= b;
-            cache_.erase(it);
-            live_[*out] = b;
-            return hipSuccess;
+        auto it = by_size_.lower_bound(b);
+        if (it == by_size_.end()) {
+            const hipError_t e = grow(b);
+            if (e != hipSuccess) return e;
+            it = by_size_.lower_bound(b);
         }
-        hipError_t e = hipMalloc(out, b);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            trim(0);
-            e = hipMalloc(out, b);
-        }
-        if (e == hipSuccess) live_[*out] = b;
-        return e;
+        char* p = it->second;
+        const size_t have = it->first;
+        erase_free(p, have);
+        if (have > b) insert_free(p + b, have - b);
+        live_[p] = b;
+        *out = p;
+        return hipSuccess;
     }
-    void put(void* p)
+    void put(void* vp)
     {
-        if (!p) return;
-        auto it = live_.find(p);
+        if (!vp) return;
+        auto it = live_.find(vp);
         if (it == live_.end()) return;
-        cache_.emplace(it->second, p);
-        cached_ += it->second;
+        char* p = static_cast<char*>(vp);
+        size_t n = it->second;
         live_.erase(it);
+        auto next = free_.find(p + n);
+        if (next != free_.end() && !chunks_.count(p + n)) {
+            const size_t m = next->second;
+            erase_free(p + n, m);
+            n += m;
+        }
+        auto prev = free_.lower_bound(p);
+        if (prev != free_.begin() && !chunks_.count(p)) {
+            --prev;
+            if (prev->first + prev->second == p) {
+                char* q = prev->first;
+                const size_t m = prev->second;
+                erase_free(q, m);
+                p = q;
+                n += m;
+            }
+        }
+        insert_free(p, n);
     }
-    void trim(size_t keep)
+    // Wholly free chunks back to the device.
+    void trim()
     {
-        while (cached_ > keep && !cache_.empty()) {
-            auto it = std::prev(cache_.end());   // largest first
-            (void)hipFree(it->second);
-            cached_ -= it->first;
-            cache_.erase(it);
+        for (auto c = chunks_.begin(); c != chunks_.end();) {
+            auto f = free_.find(c->first);
+            if (f != free_.end() && f->second == c->second) {
+                erase_free(c->first, c->second);
+                (void)hipFree(c->first);
+                held_ -= c->second;
+                c = chunks_.erase(c);
+            } else {
+                ++c;
+            }
         }
     }
-    size_t cached() const { return cached_; }
 
 private:
-    std::unordered_map<void*, size_t> live_;
-    std::multimap<size_t, void*> cache_;
-    size_t cached_ = 0;
+    hipError_t grow(size_t b)
+    {
+        size_t n = std::max(b, std::min(cap_, std::max(kFirstChunk, held_)));
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            trim();
+            n = b;
+            e = hipMalloc(&p, n);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                return e;
+            }
+        }
+        chunks_[static_cast<char*>(p)] = n;
+        held_ += n;
+        insert_free(static_cast<char*>(p), n);
+        return hipSuccess;
+    }
+    void insert_free(char* p, size_t n)
+    {
+        free_[p] = n;
+        by_size_.emplace(n, p);
+    }
+    void erase_free(char* p, size_t n)
+    {
+        free_.erase(p);
+        for (auto r = by_size_.equal_range(n); r.first != r.second; ++r.first)
+            if (r.first->second == p) {
+                by_size_.erase(r.first);
+                break;
+            }
+    }
+
+    std::map<char*, size_t> chunks_;              // chunk base -> bytes
+    std::map<char*, size_t> free_;                // free range start -> bytes (within one chunk)
+    std::multimap<size_t, char*> by_size_;        // the same ranges by size, for best fit
+    std::unordered_map<void*, size_t> live_;      // handed-out block -> bytes
+    size_t held_ = 0;                             // bytes in chunks
+    size_t cap_ = size_t(4) << 30;
 };
 
 struct Plane {
@@ -750,12 +816,16 @@ void ACHipServer::run()
             const int e = start_frame(q);
             if (e) complete(q, e, "frame launch failed");
         }
+        // steps are built between one-off set-ups, so the passes of the first requests run
+        // while the uploads of later ones are still being issued
+        bool launched = false;
         for (const std::shared_ptr<Req>& q : oneoffs) {
             const int e = start_oneoff(q);
             if (e) complete(q, e, "request set-up failed");
+            poll();
+            while (build_step()) launched = true;
         }
         poll();
-        bool launched = false;
         while (build_step()) launched = true;
         if (!launched && busy_work()) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
@@ -777,6 +847,7 @@ int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipSer
     auto s = std::make_unique<ACHipServer>();
     s->net = net;
     s->cfg = c;
+    s->pool.set_chunk_cap(c.hbm_budget / 4);
     s->layered = layered;
     s->config = 0;
     if (achip_config_info(c.precision, s->config, &s->tw, &s->th, nullptr, nullptr, nullptr) != ACHIP_OK) return ACHIP_EINVAL;
@@ -827,7 +898,7 @@ void achip_server_destroy(ACHipServer* s)
     }
     for (auto* st : { s->batch, s->upload, s->download })
         if (st) (void)hipStreamDestroy(st);
-    s->pool.trim(0);
+    s->pool.trim();
     (void)hipSetDevice(dev);
     delete s;
 }

@@ -14,6 +14,8 @@
 // x = (dst x + 0.5) / scale - 0.5); when downscaling the filter is stretched by 1/scale;
 // weights normalised.  Horizontal taps are summed first for each source row, then the
 // rows vertically, in FP32.
+#include <type_traits>
+
 #include "acnet_device.hpp"
 #include "acnet_kernels.hpp"
 
@@ -544,6 +546,233 @@ __global__ __launch_bounds__(256) void resize_up_kernel(const void* __restrict__
     }
 }
 
+// ---- exact 2x / 4x upscale of u8 planes, four adjacent outputs per thread ----------------
+// The same sums as resize_up_kernel (horizontal taps of each source row first, then the
+// vertical taps, each product and sum rounded separately, taps in increasing order), for the
+// support-1 and support-2 filters (NT = 2 or 4 taps per output, every phase) on u8 data:
+// - a thread owns four adjacent output columns and walks kU8Rows output rows of its wave's
+//   band, keeping only the horizontal sums of the source rows still ahead in registers (the
+//   loops are unrolled, so which rows are live is known at compile time);
+// - a sum starts from its first product instead of 0 + the first product: the two differ
+//   only in the sign of an all-zero sum, which fromFloat maps to the same code;
+// - the source region is staged in LDS as float once (36 x 132 pixels at K = 2);
+// - MERGE: the resampled chroma codes index 256-entry tables of the products the inverse
+//   colour conversion forms from them (c / 255 - 0.5 times each coefficient, computed with
+//   the same operations), and Y's toFloat comes from a table the same way;
+// - outputs leave as whole dwords (a wave writes 768 contiguous bytes of RGB per row) when
+//   the planes' bases and pitches allow it, byte by byte otherwise.
+// Phase p of K (output x with x mod K = p) has its first tap at x div K - S when
+// (p + 1/2) / K - 1/2 < 0, else at x div K - S + 1 (S = NT / 2: floor(centre - S) + 1 with a
+// non-integer centre in (-1/2, 1/2)), so the staged region needs NT source pixels beyond the
+// tile's own on each axis in total.
+constexpr int kU8Cols = 256;   // output columns per workgroup (64 lanes x 4)
+
+template <int C, int K, int NT, int ROWS>
+struct U8Up {
+    static constexpr int S = NT / 2;
+    static constexpr int R = 4 * ROWS;             // output rows per workgroup (4 waves)
+    static constexpr int SW = kU8Cols / K + NT;    // staged source columns
+    static constexpr int SH = R / K + NT;          // staged source rows
+    static constexpr int HR = ROWS / K + NT;       // source rows one wave's band reads
+    // first staged row (relative to the wave's) of output row r's taps
+    static constexpr int q0(int r) { return r / K + (2 * (r % K) >= K ? 1 : 0); }
+    // staged column (relative to 4 * lane * (1 / K)) of column i's first tap
+    static constexpr int c0(int i) { return i / K + (2 * (i % K) >= K ? 1 : 0); }
+    static constexpr int HC = 4 / K + NT;          // staged columns one thread reads per row
+};
+
+// fromFloat<uint8_t> for finite v as a 32-bit code: the median of (v, 0, 1) is the
+// reference's saturation for every finite v except that -0 may come out as +0, and both
+// give code 0.  (The sums below are of finite u8 inputs, so no NaN reaches it.)
+__device__ inline unsigned u8_code(float v)
+{
+    return static_cast<unsigned>(__fadd_rn(__fmul_rn(__builtin_amdgcn_fmed3f(v, 0.0f, 1.0f), 255.0f), 0.5f));
+}
+
+template <int C, bool MERGE>
+struct U8Lut {
+    float y[256];
+    float2 u[256], v[256];   // (0.344 u, 1.773 u), (1.403 v, 0.714 v) with u = c / 255 - 0.5
+};
+template <int C>
+struct U8Lut<C, false> {};
+
+template <int C, int K, int NT, int ROWS, bool MERGE>
+__global__ __launch_bounds__(256) void resize_up_u8_kernel(const uint8_t* __restrict__ src, long ss, int sw, int sh,
+                                                          uint8_t* __restrict__ dst, long ds, int dw, int dh, int mode,
+                                                          const uint8_t* __restrict__ yp, long ys, int vec)
+{
+    using G = U8Up<C, K, NT, ROWS>;
+    constexpr int CO = MERGE ? C + 1 : C;   // output bytes per pixel
+    __shared__ float in[G::SH][G::SW][C];
+    __shared__ float wt[2][K][NT];          // [axis][phase][tap]
+    __shared__ U8Lut<C, MERGE> lut;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x0 = blockIdx.x * kU8Cols, y0 = blockIdx.y * G::R;
+    const int ix0 = x0 / K - G::S, iy0 = y0 / K - G::S;
+    if (tid < 2 * K) {
+        const int axis = tid / K, p = tid % K;
+        float w[NT];
+        int first = 0;
+        const int n = resize_taps(mode, p, axis ? sh : sw, axis ? dh : dw, w, NT, first);
+#pragma unroll
+        for (int j = 0; j < NT; j++) wt[axis][p][j] = j < n ? w[j] : 0.0f;
+    }
+    if constexpr (MERGE) {
+        const float f = to_float<uint8_t>(static_cast<uint8_t>(tid)), d = __fsub_rn(f, 0.5f);
+        lut.y[tid] = f;
+        lut.u[tid] = make_float2(__fmul_rn(0.344f, d), __fmul_rn(1.773f, d));
+        lut.v[tid] = make_float2(__fmul_rn(1.403f, d), __fmul_rn(0.714f, d));
+    }
+    for (int r = wave; r < G::SH; r += 4) {
+        const uint8_t* row = src + static_cast<long>(clampi(iy0 + r, 0, sh - 1)) * ss;
+        for (int c = lane; c < G::SW; c += 64) {
+            const uint8_t* p = row + static_cast<long>(clampi(ix0 + c, 0, sw - 1)) * C;
+#pragma unroll
+            for (int ch = 0; ch < C; ch++) in[r][c][ch] = to_float<uint8_t>(p[ch]);
+        }
+    }
+    __syncthreads();
+
+    float wx[K][NT], wy[K][NT];
+#pragma unroll
+    for (int p = 0; p < K; p++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) {
+            wx[p][j] = wt[0][p][j];
+            wy[p][j] = wt[1][p][j];
+        }
+    const int xs = lane * (4 / K);      // staged column of this thread's first tap, less c0
+    const int qb = wave * (ROWS / K);   // staged row of this wave's first tap, less q0
+    const int xo = x0 + 4 * lane;       // first output column
+    const bool full = xo + 4 <= dw;
+    const bool fast = full && vec;
+
+    // horizontal taps of staged row q for this thread's four columns
+    auto hrow = [&](int q, float (&h)[4][C]) {
+        float s[G::HC][C];
+#pragma unroll
+        for (int k = 0; k < G::HC; k++)
+#pragma unroll
+            for (int ch = 0; ch < C; ch++) s[k][ch] = in[qb + q][xs + k][ch];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int p = i % K, b = G::c0(i);
+#pragma unroll
+            for (int ch = 0; ch < C; ch++) {
+                float a = __fmul_rn(wx[p][0], s[b][ch]);
+#pragma unroll
+                for (int j = 1; j < NT; j++) a = __fadd_rn(a, __fmul_rn(wx[p][j], s[b + j][ch]));
+                h[i][ch] = a;
+            }
+        }
+    };
+
+    float h[G::HR][4][C];
+#pragma unroll
+    for (int q = 0; q < NT; q++) hrow(q, h[q]);
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+        if (r > 0 && G::q0(r) > G::q0(r - 1)) hrow(G::q0(r) + NT - 1, h[G::q0(r) + NT - 1]);
+        const int oy = y0 + wave * ROWS + r;
+        if (oy >= dh) break;
+        const int p = r % K;
+        unsigned code[4][C];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int ch = 0; ch < C; ch++) {
+                float a = __fmul_rn(wy[p][0], h[G::q0(r)][i][ch]);
+#pragma unroll
+                for (int j = 1; j < NT; j++) a = __fadd_rn(a, __fmul_rn(wy[p][j], h[G::q0(r) + j][i][ch]));
+                code[i][ch] = u8_code(a);
+            }
+        uint8_t* o = dst + static_cast<long>(oy) * ds + static_cast<long>(xo) * CO;
+        unsigned px[4][CO];
+        if constexpr (MERGE) {
+            const uint8_t* yr = yp + static_cast<long>(oy) * ys + xo;
+            unsigned yc[4];
+            if (fast) {
+                const unsigned w = *reinterpret_cast<const unsigned*>(yr);
+#pragma unroll
+                for (int i = 0; i < 4; i++) yc[i] = (w >> (8 * i)) & 0xffu;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++) yc[i] = xo + i < dw ? yr[i] : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float yy = lut.y[yc[i]];
+                const float2 lu = lut.u[code[i][0]], lv = lut.v[code[i][1]];
+                float r_ = __fadd_rn(yy, lv.x);
+                float g_ = __fsub_rn(__fsub_rn(yy, lu.x), lv.y);
+                float b_ = __fadd_rn(yy, lu.y);
+                if constexpr (C == 3) {
+                    const float a = lut.y[code[i][2]];
+                    unpremultiply(a, r_, g_, b_);
+                    px[i][3] = u8_code(a);
+                }
+                px[i][0] = u8_code(r_);
+                px[i][1] = u8_code(g_);
+                px[i][2] = u8_code(b_);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int ch = 0; ch < C; ch++) px[i][ch] = code[i][ch];
+        }
+        if (fast) {
+            // 4 * CO bytes as CO dwords, byte k of the run in byte k % 4 of dword k / 4
+            unsigned wds[CO];
+#pragma unroll
+            for (int d = 0; d < CO; d++) {
+                unsigned v = px[(4 * d) / CO][(4 * d) % CO];
+#pragma unroll
+                for (int k = 1; k < 4; k++) {
+                    const int e = 4 * d + k;
+                    v |= px[e / CO][e % CO] << (8 * k);
+                }
+                wds[d] = v;
+            }
+            unsigned* od = reinterpret_cast<unsigned*>(o);
+#pragma unroll
+            for (int d = 0; d < CO; d++) od[d] = wds[d];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (xo + i < dw)
+#pragma unroll
+                    for (int ch = 0; ch < CO; ch++) o[i * CO + ch] = static_cast<uint8_t>(px[i][ch]);
+        }
+    }
+}
+
+// the u8 kernel above handles this (type, channel count, filter) combination
+template <typename T, int C, bool MERGE>
+constexpr bool u8_up_kind() { return std::is_same_v<T, uint8_t> && (MERGE ? (C == 2 || C == 3) : (C == 1 || C == 2)); }
+
+// 8-row bands per wave, or 4-row bands when that would leave the grid under two
+// workgroups per CU (a 1080p frame's chroma planes: 272 workgroups of 8-row bands)
+template <int C, int K, int NT, bool MERGE>
+static void launch_up_u8(const void* src, long ss, int sw, int sh, void* dst, long ds, int dw, int dh, int mode, const void* y,
+                         long ys, int vec, hipStream_t stream)
+{
+    const int gx = (dw + kU8Cols - 1) / kU8Cols;
+    const auto* s8 = static_cast<const uint8_t*>(src);
+    const auto* y8 = static_cast<const uint8_t*>(y);
+    auto* d8 = static_cast<uint8_t*>(dst);
+    if (static_cast<long>(gx) * ((dh + 31) / 32) >= 512) {
+        const dim3 grid(gx, (dh + 31) / 32);
+        hipLaunchKernelGGL((resize_up_u8_kernel<C, K, NT, 8, MERGE>), grid, dim3(256), 0, stream, s8, ss, sw, sh, d8, ds, dw,
+                           dh, mode, y8, ys, vec);
+    } else {
+        const dim3 grid(gx, (dh + 15) / 16);
+        hipLaunchKernelGGL((resize_up_u8_kernel<C, K, NT, 4, MERGE>), grid, dim3(256), 0, stream, s8, ss, sw, sh, d8, ds, dw,
+                           dh, mode, y8, ys, vec);
+    }
+}
+
 // K when dst is exactly 2 or 4 times src on both axes (and small enough for exact centres)
 static int upscale_factor(int sw, int sh, int dw, int dh)
 {
@@ -553,10 +782,26 @@ static int upscale_factor(int sw, int sh, int dw, int dh)
     return 0;
 }
 
+static bool aligned4(const void* p, long pitch) { return (reinterpret_cast<uintptr_t>(p) % 4) == 0 && pitch % 4 == 0; }
+
 template <typename T, int C, bool MERGE>
 static hipError_t resize_up_t(int k, int mode, const void* src, long ss, int sw, int sh, void* dst, long ds, int dw, int dh,
                               const void* y, long ys, hipStream_t stream)
 {
+    if constexpr (u8_up_kind<T, C, MERGE>()) {
+        const float sup = resize_support(mode);
+        if (sup == 1.0f || sup == 2.0f) {
+            const int vec = aligned4(dst, ds) && (!MERGE || aligned4(y, ys)) ? 1 : 0;
+            if (sup == 1.0f) {
+                if (k == 2) launch_up_u8<C, 2, 2, MERGE>(src, ss, sw, sh, dst, ds, dw, dh, mode, y, ys, vec, stream);
+                else launch_up_u8<C, 4, 2, MERGE>(src, ss, sw, sh, dst, ds, dw, dh, mode, y, ys, vec, stream);
+            } else {
+                if (k == 2) launch_up_u8<C, 2, 4, MERGE>(src, ss, sw, sh, dst, ds, dw, dh, mode, y, ys, vec, stream);
+                else launch_up_u8<C, 4, 4, MERGE>(src, ss, sw, sh, dst, ds, dw, dh, mode, y, ys, vec, stream);
+            }
+            return hipGetLastError();
+        }
+    }
     const dim3 grid((dw + kUpW - 1) / kUpW, (dh + kUpH - 1) / kUpH), block(256);
     if (k == 2)
         hipLaunchKernelGGL((resize_up_kernel<T, C, 2, MERGE>), grid, block, 0, stream, src, ss, sw, sh, dst, ds, dw, dh, mode, y, ys);
@@ -628,7 +873,6 @@ static ColourPlanes split_planes(int channels, const void* rgb, long rs, void* y
     return pl;
 }
 
-static bool aligned4(const void* p, long pitch) { return (reinterpret_cast<uintptr_t>(p) % 4) == 0 && pitch % 4 == 0; }
 
 template <int KIND, bool SPLIT>
 static hipError_t colour_u8x4(const void* pk, long ps, void* y, long ys, void* c, long cs, int W, int H, hipStream_t stream)

@@ -312,19 +312,25 @@ def serving(net, prec, loads, seconds, seed=5):
         return (hip.REQ_RGB, [pool[int(rng.integers(len(pool)))]], 2 if rng.random() < 0.5 else 4, -1), True
 
     out = {}
-    # capacity: the same mix submitted all at once
-    with hip.Server(net, prec, hbm_budget=64 << 30) as srv:
-        reqs = [pick() for _ in range(120)]
-        ts = [(srv.submit(*r), keep) for r, keep in reqs]
+    srv = hip.Server(net, prec, hbm_budget=64 << 30)
+
+    def burst(n):
+        ts = [(srv.submit(*r), keep) for r, keep in (pick() for _ in range(n))]
         for t, keep in ts:
             srv.wait(t, 120000)
             if keep:
                 srv.release(t)
-        st = srv.stats()
-        out["saturated"] = {"requests": len(reqs), "output_mpix_per_s": round(st["output_mpix"] / st["busy_s"], 1),
-                            "requests_per_s": round(len(reqs) / st["busy_s"], 1), "steps": st["steps"]}
+
+    # one untimed burst first: a long-running server has its device pool, frame graphs and
+    # kernels loaded, and the process its copy engines set up, before the loads below
+    burst(120)
+    srv.stats(reset=True)
+    # capacity: the same mix submitted all at once
+    burst(120)
+    st = srv.stats(reset=True)
+    out["saturated"] = {"requests": 120, "output_mpix_per_s": round(st["output_mpix"] / st["busy_s"], 1),
+                        "requests_per_s": round(120 / st["busy_s"], 1), "steps": st["steps"]}
     for load in loads:
-        srv = hip.Server(net, prec, hbm_budget=64 << 30)
         tickets = []
         lock = threading.Lock()
         done = threading.Event()
@@ -366,8 +372,7 @@ def serving(net, prec, loads, seconds, seed=5):
                 tickets.append((t, keep))
         done.set()
         th.join()
-        st = srv.stats()
-        srv.close()
+        st = srv.stats(reset=True)
         out[str(load)] = {
             "offered_req_per_s": load, "frames": n_frame, "oneoffs": n_one,
             "frame_p50_ms": round(st["frame_p50_ms"], 3), "frame_p99_ms": round(st["frame_p99_ms"], 3),
@@ -375,6 +380,8 @@ def serving(net, prec, loads, seconds, seed=5):
             "output_mpix_per_s": round(st["output_mpix"] / st["busy_s"], 1) if st["busy_s"] > 0 else None,
             "steps": st["steps"], "graph_replays": st["graph_replays"], "failed": st["failed"],
         }
+    srv.close()
+    out["method"] = "one server per precision; an untimed burst of 120 requests of the mix, then each measurement"
     return out
 
 

@@ -77,6 +77,30 @@ def test_staged_upscale_multi_tile(cuda, k):
             compare_resize(got, OL.resize(src, 150 * k, 37 * k, mode), mode == 12)
 
 
+@pytest.mark.parametrize("k", [2, 4])
+def test_u8_upscale_four_pixel_kernel(cuda, k):
+    """The u8 exact-upscale kernel (four outputs per thread, 256 x 64 output tiles) for the
+    support-1 and support-2 filters: chroma resize + merge into RGB and RGBA, and the plain
+    resize of one- and two-channel planes, over several tiles with ragged right and bottom
+    edges, on dword-aligned pitches (output width 300 / 600) and unaligned ones (134 / 268:
+    Y, RGB and the last thread's four columns take the byte path), bit-exact."""
+    import torch
+    from anime4kcpp_amd import hip
+    for w, h in ((150, 37), (67, 21)):
+        W, H = k * w, k * h
+        for mode in (OL.RESIZE_CATMULL_ROM, OL.RESIZE_BILINEAR, 2):   # 2: Mitchell-Netravali
+            for c in (3, 4):
+                ys = rand_image((H, W), np.uint8, seed=W + c)
+                small = rand_image((h, w, c - 1), np.uint8, seed=w + c + mode)
+                got = hip.resize_yuv2rgb(torch.from_numpy(ys).cuda(), torch.from_numpy(small).cuda(), mode).cpu().numpy()
+                up = OL.resize(small, W, H, mode)
+                assert same(got, oracle_merge(np.concatenate([ys[..., None], up], axis=2))), (w, k, mode, c)
+            for c in (1, 2):
+                plane = rand_image((h, w) if c == 1 else (h, w, c), np.uint8, seed=3 * w + c + mode)
+                got = hip.resize(torch.from_numpy(plane).cuda(), (W, H), mode).cpu().numpy()
+                assert np.array_equal(got, OL.resize(plane, W, H, mode)), (w, k, mode, c)
+
+
 def test_resize_large_downscale_and_limits(cuda):
     """Down by 1/7 (29 taps per axis for Catmull-Rom) matches the oracle; a downscale past
     the tap limit is refused with an error, not truncated."""
