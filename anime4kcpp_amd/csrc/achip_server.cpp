@@ -36,6 +36,7 @@
 
 #include "achip.h"
 #include "achip_internal.hpp"
+#include "chunk_pool.hpp"
 
 namespace {
 
@@ -54,122 +55,17 @@ double ms_between(Clock::time_point a, Clock::time_point b)
 
 size_t round_block(size_t v) { return (std::max<size_t>(v, 1) + kBlock - 1) / kBlock * kBlock; }
 
-// Device allocator over a few large chunks: hipMalloc takes 0.05-0.3 ms a call and a
-// request needs up to eight buffers, so one hipMalloc per buffer would hold the scheduler
-// thread (and with it the device) for milliseconds whenever many requests arrive.  Chunks grow
-// geometrically (the first kFirstChunk, then as large as everything held so far, at most the
-// cap); blocks are 2 MiB-granular ranges of a chunk, taken best-fit, and freed ranges
-// coalesce with their free neighbours inside the same chunk.  Blocks are handed back only
-// after the work that used them has completed.  A chunk is returned to the device when it is
-// wholly free and an allocation fails, or when the pool is destroyed.
-class DevicePool {
-public:
-    static constexpr size_t kFirstChunk = size_t(256) << 20;
-    ~DevicePool() { trim(); }
-    void set_chunk_cap(size_t cap) { cap_ = std::max(round_block(cap), kFirstChunk); }
-    hipError_t get(size_t bytes, void** out)
+// hipMalloc / hipFree behind the serving loop's chunk pool
+struct HipChunks {
+    bool alloc(void** p, size_t bytes)
     {
-        const size_t b = round_block(bytes);
-        auto it = by_size_.lower_bound(b);
-        if (it == by_size_.end()) {
-            const hipError_t e = grow(b);
-            if (e != hipSuccess) return e;
-            it = by_size_.lower_bound(b);
-        }
-        char* p = it->second;
-        const size_t have = it->first;
-        erase_free(p, have);
-        if (have > b) insert_free(p + b, have - b);
-        live_[p] = b;
-        *out = p;
-        return hipSuccess;
+        if (hipMalloc(p, bytes) == hipSuccess) return true;
+        (void)hipGetLastError();
+        return false;
     }
-    void put(void* vp)
-    {
-        if (!vp) return;
-        auto it = live_.find(vp);
-        if (it == live_.end()) return;
-        char* p = static_cast<char*>(vp);
-        size_t n = it->second;
-        live_.erase(it);
-        auto next = free_.find(p + n);
-        if (next != free_.end() && !chunks_.count(p + n)) {
-            const size_t m = next->second;
-            erase_free(p + n, m);
-            n += m;
-        }
-        auto prev = free_.lower_bound(p);
-        if (prev != free_.begin() && !chunks_.count(p)) {
-            --prev;
-            if (prev->first + prev->second == p) {
-                char* q = prev->first;
-                const size_t m = prev->second;
-                erase_free(q, m);
-                p = q;
-                n += m;
-            }
-        }
-        insert_free(p, n);
-    }
-    // Wholly free chunks back to the device.
-    void trim()
-    {
-        for (auto c = chunks_.begin(); c != chunks_.end();) {
-            auto f = free_.find(c->first);
-            if (f != free_.end() && f->second == c->second) {
-                erase_free(c->first, c->second);
-                (void)hipFree(c->first);
-                held_ -= c->second;
-                c = chunks_.erase(c);
-            } else {
-                ++c;
-            }
-        }
-    }
-
-private:
-    hipError_t grow(size_t b)
-    {
-        size_t n = std::max(b, std::min(cap_, std::max(kFirstChunk, held_)));
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, n);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            trim();
-            n = b;
-            e = hipMalloc(&p, n);
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
-                return e;
-            }
-        }
-        chunks_[static_cast<char*>(p)] = n;
-        held_ += n;
-        insert_free(static_cast<char*>(p), n);
-        return hipSuccess;
-    }
-    void insert_free(char* p, size_t n)
-    {
-        free_[p] = n;
-        by_size_.emplace(n, p);
-    }
-    void erase_free(char* p, size_t n)
-    {
-        free_.erase(p);
-        for (auto r = by_size_.equal_range(n); r.first != r.second; ++r.first)
-            if (r.first->second == p) {
-                by_size_.erase(r.first);
-                break;
-            }
-    }
-
-    std::map<char*, size_t> chunks_;              // chunk base -> bytes
-    std::map<char*, size_t> free_;                // free range start -> bytes (within one chunk)
-    std::multimap<size_t, char*> by_size_;        // the same ranges by size, for best fit
-    std::unordered_map<void*, size_t> live_;      // handed-out block -> bytes
-    size_t held_ = 0;                             // bytes in chunks
-    size_t cap_ = size_t(4) << 30;
+    void release(void* p) { (void)hipFree(p); }
 };
+using DevicePool = achip::ChunkPool<HipChunks>;
 
 struct Plane {
     void* ptr = nullptr;
@@ -346,7 +242,7 @@ int ACHipServer::alloc(Plane& p, int w, int h, int ch, int es)
     p.ch = ch;
     p.es = es;
     p.stride = p.row();
-    return pool.get(p.bytes(), &p.ptr) == hipSuccess ? ACHIP_OK : ACHIP_ENOMEM;
+    return pool.get(p.bytes(), &p.ptr) ? ACHIP_OK : ACHIP_ENOMEM;
 }
 
 int ACHipServer::alloc_ws(Plane& p, int w, int h)
@@ -354,7 +250,7 @@ int ACHipServer::alloc_ws(Plane& p, int w, int h)
     const size_t bytes = achip_net_workspace_bytes(net, w, h);
     p.w = p.h = 0;
     p.stride = 0;
-    return pool.get(bytes, &p.ptr) == hipSuccess ? ACHIP_OK : ACHIP_ENOMEM;
+    return pool.get(bytes, &p.ptr) ? ACHIP_OK : ACHIP_ENOMEM;
 }
 
 // One 2x pass of the server's net from plane a into plane b: the fused ACNetLegacy kernel
