@@ -131,6 +131,11 @@ struct Slot {                          // one video stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::deque<std::shared_ptr<Req>> inflight;
+    // admitted frames not started yet: a stream's next frame starts once the one before has
+    // completed, because a host-to-device copy queued behind kernel work can hold the
+    // calling thread until that work drains (measured: ~10 ms per frame while pass steps
+    // fill the device), and the scheduler thread must not wait
+    std::deque<std::shared_ptr<Req>> waiting;
 };
 
 struct Step {
@@ -195,6 +200,7 @@ struct ACHipServer {
     bool build_layered_step(Step& step);
     hipError_t launch_finish(const Req& q);
     void poll();
+    void start_frames();
     void complete(const std::shared_ptr<Req>& q, int err, const std::string& msg);
     void free_req(Req& q);
     int alloc(Plane& p, int w, int h, int ch, int es = 1);
@@ -683,8 +689,21 @@ bool ACHipServer::busy_work() const
     for (const Step& s : steps)
         if (s.busy) return true;
     for (const Slot& s : slots)
-        if (!s.inflight.empty()) return true;
+        if (!s.inflight.empty() || !s.waiting.empty()) return true;
     return false;
+}
+
+// the next frame of every stream whose previous frame has completed
+void ACHipServer::start_frames()
+{
+    for (Slot& s : slots) {
+        while (s.inflight.empty() && !s.waiting.empty()) {
+            const std::shared_ptr<Req> q = s.waiting.front();
+            s.waiting.pop_front();
+            const int e = start_frame(q);
+            if (e) complete(q, e, "frame launch failed");
+        }
+    }
 }
 
 void ACHipServer::run()
@@ -708,10 +727,8 @@ void ACHipServer::run()
         }
         for (const std::shared_ptr<Req>& q : freed) free_req(*q);
         for (const std::shared_ptr<Req>& q : refused) complete(q, ACHIP_ENOMEM, "not admitted before shutdown");
-        for (const std::shared_ptr<Req>& q : frames) {
-            const int e = start_frame(q);
-            if (e) complete(q, e, "frame launch failed");
-        }
+        for (const std::shared_ptr<Req>& q : frames) slots[q->r.stream].waiting.push_back(q);
+        start_frames();
         // steps are built between one-off set-ups, so the passes of the first requests run
         // while the uploads of later ones are still being issued
         bool launched = false;
@@ -719,9 +736,11 @@ void ACHipServer::run()
             const int e = start_oneoff(q);
             if (e) complete(q, e, "request set-up failed");
             poll();
+            start_frames();
             while (build_step()) launched = true;
         }
         poll();
+        start_frames();
         while (build_step()) launched = true;
         if (!launched && busy_work()) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }

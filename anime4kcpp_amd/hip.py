@@ -88,6 +88,25 @@ def _torch_stream(stream, dev):
     return torch.cuda.ExternalStream(int(stream), device=dev)
 
 
+_TORCH_FIRST = False
+
+
+def _torch_first():
+    """Initialise torch's HIP context before the library's first device call.  The objects of
+    this module hand device tensors between torch and the library; when the library touched
+    the device first, torch's own lazy initialisation reported no GPU on these machines."""
+    global _TORCH_FIRST
+    if _TORCH_FIRST:
+        return
+    _TORCH_FIRST = True
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:   # no torch, or no device: the library reports its own errors
+        pass
+
+
 def device_count():
     return _native.lib().achip_device_count()
 
@@ -102,6 +121,7 @@ class Net:
         b = np.ascontiguousarray(biases, dtype=np.float32)
         if k.size != 4712 or b.size != 72:
             raise ValueError("ACNetLegacy needs 4712 kernel and 72 bias floats")
+        _torch_first()
         h = ctypes.c_void_p()
         check(_native.lib().achip_net_create(device, k.ctypes.data, b.ctypes.data, ctypes.byref(h)), "achip_net_create")
         self._h = h
@@ -209,6 +229,7 @@ class Net8:
         shipped variant's."""
         k, b, a, blocks = load_weights8(variant) if weights is None else weights
         k, b, a = (np.ascontiguousarray(x, dtype=np.float32) for x in (k, b, a))
+        _torch_first()
         h = ctypes.c_void_p()
         check(_native.lib().achip_net8_create(device, blocks, k.ctypes.data, b.ctypes.data, a.ctypes.data,
                                               ctypes.byref(h)), "achip_net8_create")
@@ -279,6 +300,7 @@ class NetArt(Net8):
 
     def __init__(self, variant="f16", device=0):
         k, b, f, blocks = load_weights_artcnn(variant)
+        _torch_first()
         h = ctypes.c_void_p()
         check(_native.lib().achip_net_artcnn_create(device, f, blocks, k.ctypes.data, b.ctypes.data, ctypes.byref(h)),
               "achip_net_artcnn_create")
@@ -318,6 +340,7 @@ class NetFsr(NetArt):
 
     def __init__(self, variant="f8", device=0):
         k, b, a, f, blocks = load_weights_fsrcnnx(variant)
+        _torch_first()
         h = ctypes.c_void_p()
         check(_native.lib().achip_net_fsrcnnx_create(device, f, blocks, k.ctypes.data, b.ctypes.data, a.ctypes.data,
                                                      ctypes.byref(h)), "achip_net_fsrcnnx_create")
