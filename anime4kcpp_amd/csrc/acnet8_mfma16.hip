@@ -2,9 +2,12 @@
 // as an FP16 or BF16 implicit GEMM on gfx950 MFMA (v_mfma_f32_16x16x32_{f16,bf16}), FP32
 // accumulation, and 8-channel 16-bit feature maps between launches (16 bytes per pixel: half
 // the HBM traffic of the FP32 path's maps):
-//   L0     conv3x3 1 -> 8 + PReLU in FP32 (the FP32 path's arithmetic), rounded to 16 bits
+//   L0     conv3x3 1 -> 8 + PReLU in FP32 (the FP32 path's arithmetic), rounded to 16 bits;
+//          with an even block count computed per tile in LDS together with block 1
+//          (conv16_head), else written as a map (acnet8_l0_16)
 //   L1..LB conv3x3 8 -> 8, bias and PReLU in FP32, rounded to 16 bits; two layers per launch
-//          with the map between them in LDS (conv16_pair), one for an odd count (conv16_layer)
+//          with the map between them in LDS (conv16_pair); the last block runs with the final
+//          layer in one conv16_pair, or alone before it (conv16_layer) when the pairs use all
 //   final  conv3x3 8 -> 4, pixel shuffle 2x, + the source pixel, all in FP32 from the FP32
 //          accumulators, then fromFloat to the output type
 // The MFMA work item is the ACNetLegacy 16-bit kernel's (acnet_mfma.hip): 16 consecutive pixels
@@ -27,6 +30,21 @@ namespace achip {
 
 long acnet8_map16_pitch(int w) { return (static_cast<long>(w) * 16 + 255) / 256 * 256; }
 
+// L0 of one pixel from its 9 clamped taps (row-major), in FP32, rounded to the 16-bit map
+template <typename T16>
+__device__ inline uint4 l0_pixel16(const float (&r)[9], const float* __restrict__ wts)
+{
+    float o[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int p = 0; p < 9; p++) s = __fadd_rn(s, __fmul_rn(r[p], wts[kA8OffW0 + p * 8 + n]));
+        o[n] = prelu(__fadd_rn(s, wts[kA8OffB0 + n]), wts[kA8OffAlpha0 + n]);
+    }
+    return make_uint4(pack16<T16>(o[0], o[1]), pack16<T16>(o[2], o[3]), pack16<T16>(o[4], o[5]), pack16<T16>(o[6], o[7]));
+}
+
 // ---- L0: one pixel per thread, 64 x 4 pixels per workgroup ----
 template <typename T16, typename TIN>
 __global__ __launch_bounds__(256) void acnet8_l0_16(const void* __restrict__ src, long ss, int W, int H,
@@ -42,16 +60,8 @@ __global__ __launch_bounds__(256) void acnet8_l0_16(const void* __restrict__ src
 #pragma unroll
         for (int dx = 0; dx < 3; dx++) r[dy * 3 + dx] = to_float<TIN>(row[clampi(x + dx - 1, 0, W - 1)]);
     }
-    float o[8];
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-        float s = 0.0f;
-#pragma unroll
-        for (int p = 0; p < 9; p++) s = __fadd_rn(s, __fmul_rn(r[p], wts[kA8OffW0 + p * 8 + n]));
-        o[n] = prelu(__fadd_rn(s, wts[kA8OffB0 + n]), wts[kA8OffAlpha0 + n]);
-    }
     uint4* px = reinterpret_cast<uint4*>(reinterpret_cast<char*>(feat) + static_cast<long>(y) * fp) + x;
-    *px = make_uint4(pack16<T16>(o[0], o[1]), pack16<T16>(o[2], o[3]), pack16<T16>(o[4], o[5]), pack16<T16>(o[6], o[7]));
+    *px = l0_pixel16<T16>(r, wts);
 }
 
 // Arguments of one 16-bit layer: A operands (uint4 per lane and patch column), FP32 bias and
@@ -187,6 +197,38 @@ __global__ __launch_bounds__(256) void conv16_layer(const uint4* __restrict__ fi
 // pixel inside, which is the replicate padding of the next layer.
 constexpr int kP16TW = 62, kP16TH = 32, kP16PW = 80, kP16PH = kP16TH + 4, kP16NT = 512;
 
+// The second layer of a two-layer launch, once layer A fills plane pixels (1..64, 1..34):
+// the replicate ring, then layer B (operands in op) over the 62 x 32 tile to fout or the
+// output image.
+template <typename T16, bool FINAL, typename TIN, typename TOUT>
+__device__ __forceinline__ void pair_second(uint4* __restrict__ plane, int W, int H, int x0, int y0, const Ops16<T16>& op,
+                                            uint4* __restrict__ fout, long fp, const Layer16& la)
+{
+    constexpr int NW = kP16NT / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
+    const int lane_rd = g * kP16PW + j;
+    // replicate ring: A's pixels outside the image take the nearest pixel inside
+    if (x0 - 1 < 0 || y0 - 1 < 0 || x0 + kP16TW >= W || y0 + kP16TH >= H) {
+        for (int e = tid; e < (kP16TH + 2) * 64; e += kP16NT) {
+            const int r = 1 + e / 64, c = 1 + (e & 63);
+            const int gy = y0 - 2 + r, gx = x0 - 2 + c;
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) continue;
+            plane[r * kP16PW + c] = plane[(clampi(gy, 0, H - 1) - (y0 - 2)) * kP16PW + clampi(gx, 0, W - 1) - (x0 - 2)];
+        }
+        __syncthreads();
+    }
+    // layer B over the 62 x 32 tile: patch rows 2 pair + 1 + g, columns 16 seg + j + t + 1
+    // (columns past A's region feed only lanes past the tile)
+    const int gx1 = min(x0 + kP16TW, W), gy1 = min(y0 + kP16TH, H);
+    for (int item = wave; item < 4 * (kP16TH / 2); item += NW) {
+        const int pair = item >> 2, seg = item & 3;
+        const f32x4 acc = op.item(plane + (2 * pair + 1) * kP16PW + 16 * seg + 1 + lane_rd);
+        const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
+        store16<T16, FINAL, TIN, TOUT>(op, acc, x, y, x < gx1 && y < gy1, half, fout, fp, la);
+    }
+}
+
 template <typename T16, bool FINAL, typename TIN, typename TOUT>
 __global__ __launch_bounds__(kP16NT) void conv16_pair(const uint4* __restrict__ fin, long fp, int W, int H,
                                                       uint4* __restrict__ fout, const Layer16 la0, const Layer16 la)
@@ -231,25 +273,50 @@ __global__ __launch_bounds__(kP16NT) void conv16_pair(const uint4* __restrict__ 
         }
     }
     __syncthreads();
-    // replicate ring: A's pixels outside the image take the nearest pixel inside
-    if (x0 - 1 < 0 || y0 - 1 < 0 || x0 + kP16TW >= W || y0 + kP16TH >= H) {
-        for (int e = tid; e < (kP16TH + 2) * 64; e += kP16NT) {
-            const int r = 1 + e / 64, c = 1 + (e & 63);
-            const int gy = y0 - 2 + r, gx = x0 - 2 + c;
-            if (gx >= 0 && gx < W && gy >= 0 && gy < H) continue;
-            plane[r * kP16PW + c] = plane[(clampi(gy, 0, H - 1) - (y0 - 2)) * kP16PW + clampi(gx, 0, W - 1) - (x0 - 2)];
-        }
-        __syncthreads();
+    pair_second<T16, FINAL, TIN, TOUT>(plane, W, H, x0, y0, op, fout, fp, la);
+}
+
+
+// ---- L0 and the first block in one launch (62 x 32 outputs, 512 threads) ----
+// The source tile (66 x 36 pixels from (x0 - 2, y0 - 2), clamped to the image, as f32) gives
+// L0 at plane pixels (1..64, 1..34) with l0_pixel16's arithmetic, then the first block runs as
+// conv16_pair's second layer: the map acnet8_l0_16 writes and conv16_pair reads never goes
+// through HBM.
+constexpr int kH16SW = kP16TW + 4, kH16SH = kP16TH + 4;
+
+template <typename T16, typename TIN>
+__global__ __launch_bounds__(kP16NT) void conv16_head(const void* __restrict__ src, long ss, int W, int H,
+                                                      uint4* __restrict__ fout, long fp, const float* __restrict__ wts,
+                                                      const Layer16 la)
+{
+    __shared__ uint4 plane[kP16PW * kP16PH];
+    __shared__ float img[kH16SH * kH16SW];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int x0 = tx * kP16TW, y0 = ty * kP16TH;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int half = (lane >> 4) & 1;
+    // img pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r), as plane pixel (c, r)
+    for (int e = tid; e < kH16SH * kH16SW; e += kP16NT) {
+        const int r = e / kH16SW, c = e - r * kH16SW;
+        const int sy = clampi(y0 - 2 + r, 0, H - 1), sx = clampi(x0 - 2 + c, 0, W - 1);
+        img[e] = to_float<TIN>(row_ptr<TIN>(src, ss, sy)[sx]);
     }
-    // layer B over the 62 x 32 tile: patch rows 2 pair + 1 + g, columns 16 seg + j + t + 1
-    // (columns past A's region feed only lanes past the tile)
-    const int gx1 = min(x0 + kP16TW, W), gy1 = min(y0 + kP16TH, H);
-    for (int item = wave; item < 4 * (kP16TH / 2); item += NW) {
-        const int pair = item >> 2, seg = item & 3;
-        const f32x4 acc = op.item(plane + (2 * pair + 1) * kP16PW + 16 * seg + 1 + lane_rd);
-        const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
-        store16<T16, FINAL, TIN, TOUT>(op, acc, x, y, x < gx1 && y < gy1, half, fout, fp, la);
+    Ops16<T16> op;
+    op.load(la, lane, half, false);
+    __syncthreads();
+    for (int e = tid; e < (kP16TH + 2) * (kP16TW + 2); e += kP16NT) {
+        const int r = 1 + e / (kP16TW + 2), c = 1 + e % (kP16TW + 2);
+        float q[9];
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) q[dy * 3 + dx] = img[(r - 1 + dy) * kH16SW + c - 1 + dx];
+        plane[r * kP16PW + c] = l0_pixel16<T16>(q, wts);
     }
+    __syncthreads();
+    pair_second<T16, false, float, float>(plane, W, H, x0, y0, op, fout, fp, la);
 }
 
 // ---- launch ----
@@ -258,8 +325,6 @@ template <typename T16, typename TIN, typename TOUT>
 static hipError_t acnet8_pass16_t(int blocks, const void* src, long ss, int W, int H, void* dst, long ds,
                                   const float* wts, const uint4* wa, uint4* f0, uint4* f1, long fp, hipStream_t stream)
 {
-    hipLaunchKernelGGL((acnet8_l0_16<T16, TIN>), dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, stream, src, ss, W, H,
-                       f0, fp, wts);
     const dim3 grid((W + kL16TW - 1) / kL16TW, (H + kL16TH - 1) / kL16TH);
     const dim3 pgrid((W + kP16TW - 1) / kP16TW, (H + kP16TH - 1) / kP16TH);
     constexpr long kLayerVecs = 3 * 64;   // uint4 A operands per layer
@@ -280,7 +345,16 @@ static hipError_t acnet8_pass16_t(int blocks, const void* src, long ss, int W, i
     uint4* in = f0;
     uint4* out = f1;
     int l = 1;
-    // blocks two per launch; with an odd count the last block pairs with the final layer
+    if (blocks % 2 == 0) {
+        // an even count: L0 with block 1, then blocks two per launch, the last with the final
+        // layer
+        hipLaunchKernelGGL((conv16_head<T16, TIN>), pgrid, dim3(kP16NT), 0, stream, src, ss, W, H, in, fp, wts, block_args(1));
+        l = 2;
+    } else {
+        hipLaunchKernelGGL((acnet8_l0_16<T16, TIN>), dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, stream, src, ss, W, H,
+                           in, fp, wts);
+    }
+    // blocks two per launch; the last one left over pairs with the final layer
     for (; l + 1 <= blocks; l += 2) {
         hipLaunchKernelGGL((conv16_pair<T16, false, float, float>), pgrid, dim3(kP16NT), 0, stream, in, fp, W, H, out,
                            block_args(l), block_args(l + 1));

@@ -443,8 +443,8 @@ def acnet8(frames_np, weights=None, steps=25):
     blocks, in FP32 (FP32 MFMA, f32 maps) and in FP16 / BF16 (16-bit MFMA, 16-bit maps); blocks
     two per launch.  FLOP per input pixel: 2 x (72 + 576 blocks + 288) MAC.  HBM bytes per
     input pixel: the u8 source read twice (L0, the final residual), 4 output bytes, and an
-    8-channel map (32 bytes FP32, 16 bytes 16-bit) written by L0, read and written once per
-    launch of blocks, and read by the final layer.  Error of each form against the FP64
+    8-channel map (32 bytes FP32, 16 bytes 16-bit) written by the first launch, read and
+    written once per map-to-map launch, and read by the last.  Error of each form against the FP64
     restatement on a 1080p f32 frame (B8 HDN)."""
     import numpy as np
     import torch
@@ -475,10 +475,14 @@ def acnet8(frames_np, weights=None, steps=25):
             flop = 2 * (72 + 576 * net.blocks + 288) * w * h
             peak = PEAK_FP32_TFLOPS if prec == "fp32" else PEAK_MFMA16_TFLOPS
             mb = 32 if prec == "fp32" else 16
+            # map-to-map launches: FP32 one per two blocks (and one for an odd block); 16-bit
+            # one per two blocks after L0 + block 1 (even counts) or L0 (odd), the last block
+            # running with the final layer
+            maps = (net.blocks + 1) // 2 if prec == "fp32" else (net.blocks - 1) // 2
             entry = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
                      "achieved_tflops": round(flop / sec / 1e12, 2), "peak_tflops": peak,
                      "frac": round(flop / sec / 1e12 / peak, 4),
-                     "hbm_bytes_per_frame": (2 + 4 + mb + 2 * mb * ((net.blocks + 1) // 2) + mb) * w * h}
+                     "hbm_bytes_per_frame": (2 + 4 + mb + 2 * mb * maps + mb) * w * h}
             if variant == "b8_hdn":
                 k, b, a, blocks = hip.load_weights8(variant)
                 xf = synth.flat_shaded(7, w, h)

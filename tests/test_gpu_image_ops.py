@@ -79,11 +79,12 @@ def test_staged_upscale_multi_tile(cuda, k):
 
 @pytest.mark.parametrize("k", [2, 4])
 def test_u8_upscale_four_pixel_kernel(cuda, k):
-    """The u8 exact-upscale kernel (four outputs per thread, 256 x 64 output tiles) for the
-    support-1 and support-2 filters: chroma resize + merge into RGB and RGBA, and the plain
-    resize of one- and two-channel planes, over several tiles with ragged right and bottom
-    edges, on dword-aligned pitches (output width 300 / 600) and unaligned ones (134 / 268:
-    Y, RGB and the last thread's four columns take the byte path), bit-exact."""
+    """The u8 exact-upscale kernel (four outputs per thread; 256 x 16 output tiles of 4-row
+    bands for small grids, 256 x 32 of 8-row bands from 512 workgroups) for the support-1 and
+    support-2 filters: chroma resize + merge into RGB and RGBA, and the plain resize of one-
+    and two-channel planes, over several tiles with ragged right and bottom edges, on
+    dword-aligned pitches (output width 300 / 600) and unaligned ones (134 / 268: Y, RGB and
+    the last thread's four columns take the byte path), bit-exact."""
     import torch
     from anime4kcpp_amd import hip
     for w, h in ((150, 37), (67, 21)):
@@ -99,6 +100,15 @@ def test_u8_upscale_four_pixel_kernel(cuda, k):
                 plane = rand_image((h, w) if c == 1 else (h, w, c), np.uint8, seed=3 * w + c + mode)
                 got = hip.resize(torch.from_numpy(plane).cuda(), (W, H), mode).cpu().numpy()
                 assert np.array_equal(got, OL.resize(plane, W, H, mode)), (w, k, mode, c)
+    # a grid of 512 workgroups or more takes the 8-row bands (1024 x 4096 output)
+    w, h = 1024 // k, 4096 // k
+    ys = rand_image((4096, 1024), np.uint8, seed=7 + k)
+    small = rand_image((h, w, 2), np.uint8, seed=9 + k)
+    got = hip.resize_yuv2rgb(torch.from_numpy(ys).cuda(), torch.from_numpy(small).cuda(), OL.RESIZE_CATMULL_ROM).cpu().numpy()
+    up = OL.resize(small, 1024, 4096, OL.RESIZE_CATMULL_ROM)
+    assert same(got, oracle_merge(np.concatenate([ys[..., None], up], axis=2)))
+    got = hip.resize(torch.from_numpy(small).cuda(), (1024, 4096), OL.RESIZE_BILINEAR).cpu().numpy()
+    assert np.array_equal(got, OL.resize(small, 1024, 4096, OL.RESIZE_BILINEAR))
 
 
 def test_resize_large_downscale_and_limits(cuda):
