@@ -1,9 +1,10 @@
 // ACNet<8> 2x pass (core/src/processor/cpu/CPUProcessor.cpp:1417-1436) on gfx950, layer by
 // layer, FP32 throughout:
-//   L0     conv3x3 1 -> 8 + PReLU            (Backend.hpp:58-75)        VALU, one pixel per thread
+//   L0     conv3x3 1 -> 8 + PReLU            (Backend.hpp:58-75)        VALU; with an even block
+//          count per tile in LDS together with block 1 (conv8_head), else one pixel per thread
 //   L1..LB conv3x3 8 -> 8 + PReLU            (Backend.hpp:95-98)        conv8_pair: two per launch
 //   final  conv3x3 8 -> 4, pixel shuffle 2x, + the source pixel  (Backend.hpp:116-133,
-//          Common.hpp:290-341)                                          conv8_layer<kShuffleSrc>
+//          Common.hpp:290-341)                  with the last block: conv8_pair<kShuffleSrc>
 // Features between launches are 8-channel f32 maps in HBM (32 bytes per pixel, rows padded
 // to 256 bytes); the blocks run two per launch (conv8_pair), so each map round trip covers
 // two layers at the cost of a one-pixel recompute ring; the depth (4, 8 or 18 blocks) is a
@@ -19,9 +20,23 @@ namespace achip {
 
 #pragma clang fp contract(off)
 
+// L0 of one pixel from its 9 clamped taps (row-major).  RELU: ACNetLegacy's layer 0 (ReLU; its
+// blob has the same w[p][n] and bias layout at the same offsets), else ACNet<8>'s (PReLU with
+// the blob's slopes).
+template <bool RELU>
+__device__ inline void l0_pixel8(const float (&r)[9], const float* __restrict__ wts, float (&o)[8])
+{
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        float s = 0.0f;
+#pragma unroll
+        for (int p = 0; p < 9; p++) s = __fadd_rn(s, __fmul_rn(r[p], wts[kA8OffW0 + p * 8 + n]));
+        const float v = __fadd_rn(s, wts[kA8OffB0 + n]);
+        o[n] = RELU ? fmaxf(v, 0.0f) : prelu(v, wts[kA8OffAlpha0 + n]);
+    }
+}
+
 // ---- L0: one pixel per thread, 64 x 4 pixels per workgroup ----
-// RELU: ACNetLegacy's layer 0 (ReLU; its blob has the same w[p][n] and bias layout at the
-// same offsets), else ACNet<8>'s (PReLU with the blob's slopes)
 template <typename TIN, bool RELU = false>
 __global__ __launch_bounds__(256) void acnet8_l0(const void* __restrict__ src, long ss, int W, int H,
                                                  float4* __restrict__ feat, long fp, const float* __restrict__ wts)
@@ -37,17 +52,57 @@ __global__ __launch_bounds__(256) void acnet8_l0(const void* __restrict__ src, l
         for (int dx = 0; dx < 3; dx++) r[dy * 3 + dx] = to_float<TIN>(row[clampi(x + dx - 1, 0, W - 1)]);
     }
     float o[8];
-#pragma unroll
-    for (int n = 0; n < 8; n++) {
-        float s = 0.0f;
-#pragma unroll
-        for (int p = 0; p < 9; p++) s = __fadd_rn(s, __fmul_rn(r[p], wts[kA8OffW0 + p * 8 + n]));
-        const float v = __fadd_rn(s, wts[kA8OffB0 + n]);
-        o[n] = RELU ? fmaxf(v, 0.0f) : prelu(v, wts[kA8OffAlpha0 + n]);
-    }
+    l0_pixel8<RELU>(r, wts, o);
     float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(feat) + static_cast<long>(y) * fp) + 2 * x;
     px[0] = make_float4(o[0], o[1], o[2], o[3]);
     px[1] = make_float4(o[4], o[5], o[6], o[7]);
+}
+
+// ---- L0 and the first block in one launch (62 x 16 outputs, 256 threads) ----
+// The source tile (66 x 20 pixels from (x0 - 2, y0 - 2), clamped, as f32) gives L0 at plane
+// pixels (1..64, 1..18) with l0_pixel8's arithmetic; the first block then runs as conv8_pair's
+// second layer, so the map between them never goes through HBM.
+constexpr int kH8SW = kP8TW + 4, kH8SH = kP8TH + 4;
+
+template <typename TIN>
+__global__ __launch_bounds__(256) void conv8_head(const void* __restrict__ src, long ss, int W, int H,
+                                                  float4* __restrict__ fout, long fp, const float* __restrict__ wts,
+                                                  const LayerArgs la)
+{
+    __shared__ float4 pa[kP8PW * kP8PH];
+    __shared__ float4 pb[kP8PW * kP8PH];
+    __shared__ float img[kH8SH * kH8SW];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int x0 = tx * kP8TW, y0 = ty * kP8TH;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int half = (lane >> 4) & 1;
+    // img pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r), as plane pixel (c, r)
+    for (int e = tid; e < kH8SH * kH8SW; e += 256) {
+        const int r = e / kH8SW, c = e - r * kH8SW;
+        img[e] = to_float<TIN>(row_ptr<TIN>(src, ss, clampi(y0 - 2 + r, 0, H - 1))[clampi(x0 - 2 + c, 0, W - 1)]);
+    }
+    float a[24];
+#pragma unroll
+    for (int s = 0; s < 24; s++) a[s] = la.wa[s * 64 + lane];
+    Epi8<kPrelu, float, float> epi;
+    epi.load(la, lane, half);
+    __syncthreads();
+    for (int e = tid; e < (kP8TH + 2) * 64; e += 256) {
+        const int r = 1 + e / 64, c = 1 + (e & 63);
+        float q[9];
+#pragma unroll
+        for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) q[dy * 3 + dx] = img[(r - 1 + dy) * kH8SW + c - 1 + dx];
+        float o[8];
+        l0_pixel8<false>(q, wts, o);
+        pa[r * kP8PW + c] = make_float4(o[0], o[1], o[2], o[3]);
+        pb[r * kP8PW + c] = make_float4(o[4], o[5], o[6], o[7]);
+    }
+    __syncthreads();
+    pair8_second<kPrelu, float, float>(pa, pb, W, H, x0, y0, a, epi, fp, fout, la);
 }
 
 // ---- launch ----
@@ -56,8 +111,7 @@ template <typename TIN, typename TOUT>
 static hipError_t acnet8_pass_t(int blocks, const void* src, long ss, int W, int H, void* dst, long ds,
                                 const float* wts, const float* wa, float4* f0, float4* f1, long fp, hipStream_t stream)
 {
-    hipLaunchKernelGGL((acnet8_l0<TIN>), dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, stream, src, ss, W, H, f0, fp, wts);
-    const dim3 grid = conv8_grid(W, H), pgrid = conv8_pair_grid(W, H), block(256);
+    const dim3 pgrid = conv8_pair_grid(W, H), block(256);
     auto block_args = [&](int l) {
         LayerArgs la{};
         la.wa = wa + static_cast<long>(l - 1) * kMfma32LayerFloats;
@@ -68,13 +122,17 @@ static hipError_t acnet8_pass_t(int blocks, const void* src, long ss, int W, int
     float4* in = f0;
     float4* out = f1;
     int l = 1;
-    for (; l + 1 <= blocks; l += 2) {   // blocks l and l + 1 in one launch
+    if (blocks % 2 == 0) {   // an even count: L0 with block 1 (conv8_head)
+        hipLaunchKernelGGL((conv8_head<TIN>), pgrid, block, 0, stream, src, ss, W, H, in, fp, wts, block_args(1));
+        l = 2;
+    } else {
+        hipLaunchKernelGGL((acnet8_l0<TIN>), dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, stream, src, ss, W, H, in, fp,
+                           wts);
+    }
+    // an odd number of blocks is left: two per launch, the last one with the final layer
+    for (; l < blocks; l += 2) {
         hipLaunchKernelGGL((conv8_pair<kPrelu, float, float>), pgrid, block, 0, stream, in, fp, W, H, out, block_args(l),
                            block_args(l + 1));
-        std::swap(in, out);
-    }
-    if (l == blocks) {                  // an odd block count leaves one
-        hipLaunchKernelGGL((conv8_layer<kPrelu, float, float>), grid, block, 0, stream, in, fp, W, H, out, block_args(l));
         std::swap(in, out);
     }
     LayerArgs la{};
@@ -84,7 +142,7 @@ static hipError_t acnet8_pass_t(int blocks, const void* src, long ss, int W, int
     la.ss = ss;
     la.dst = dst;
     la.ds = ds;
-    hipLaunchKernelGGL((conv8_layer<kShuffleSrc, TIN, TOUT>), grid, block, 0, stream, in, fp, W, H, out, la);
+    hipLaunchKernelGGL((conv8_pair<kShuffleSrc, TIN, TOUT>), pgrid, block, 0, stream, in, fp, W, H, out, block_args(blocks), la);
     return hipGetLastError();
 }
 

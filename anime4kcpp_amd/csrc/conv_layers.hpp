@@ -250,6 +250,41 @@ __global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ 
 // every layer of the reference applies (Common.hpp:120-143).
 constexpr int kP8TW = 62, kP8TH = 16, kP8PW = 80, kP8PH = kP8TH + 4;
 
+// The second layer of a two-layer launch, once layer A fills plane pixels (1..64, 1..18)
+// of pa/pb: the replicate ring, then layer B (A operands a, epilogue epi) over the 62 x 16 tile.
+template <int MODE, typename TIN, typename TOUT>
+__device__ __forceinline__ void pair8_second(float4* __restrict__ pa, float4* __restrict__ pb, int W, int H, int x0, int y0,
+                                             const float (&a)[24], const Epi8<MODE, TIN, TOUT>& epi, long fp,
+                                             float4* __restrict__ fout, const LayerArgs& la)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
+    const int lane_rd = g * kP8PW + j;
+    // replicate ring: A's pixels outside the image take the nearest pixel inside
+    if (x0 - 1 < 0 || y0 - 1 < 0 || x0 + kP8TW >= W || y0 + kP8TH >= H) {
+        for (int e = tid; e < 18 * 64; e += 256) {
+            const int r = 1 + e / 64, c = 1 + (e & 63);
+            const int gy = y0 - 2 + r, gx = x0 - 2 + c;
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) continue;
+            const int s = (clampi(gy, 0, H - 1) - (y0 - 2)) * kP8PW + clampi(gx, 0, W - 1) - (x0 - 2);
+            pa[r * kP8PW + c] = pa[s];
+            pb[r * kP8PW + c] = pb[s];
+        }
+        __syncthreads();
+    }
+    // layer B over the 62 x 16 tile: 32 items, 8 per wave; patch rows 2 pair + 1 + g,
+    // columns 16 seg + j + t + 1 (columns past A's region feed only lanes past the tile)
+    const int gx1 = min(x0 + kP8TW, W), gy1 = min(y0 + kP8TH, H);
+#pragma unroll 2
+    for (int item = wave; item < 32; item += 4) {
+        const int pair = item >> 2, seg = item & 3;
+        const f32x4m acc = item24(load_item(pa, pb, (2 * pair + 1) * kP8PW + 16 * seg + 1 + lane_rd), a, epi.bias);
+        const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
+        epi.apply(acc, x, y, x < gx1 && y < gy1, half, fp, fout, la);
+    }
+}
+
+
 template <int MODE, typename TIN, typename TOUT, int MODE_A = kPrelu>
 __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
                                                   float4* __restrict__ fout, const LayerArgs la0, const LayerArgs la)
@@ -305,28 +340,7 @@ __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin
         }
     }
     __syncthreads();
-    // replicate ring: A's pixels outside the image take the nearest pixel inside
-    if (x0 - 1 < 0 || y0 - 1 < 0 || x0 + kP8TW >= W || y0 + kP8TH >= H) {
-        for (int e = tid; e < 18 * 64; e += 256) {
-            const int r = 1 + e / 64, c = 1 + (e & 63);
-            const int gy = y0 - 2 + r, gx = x0 - 2 + c;
-            if (gx >= 0 && gx < W && gy >= 0 && gy < H) continue;
-            const int s = (clampi(gy, 0, H - 1) - (y0 - 2)) * kP8PW + clampi(gx, 0, W - 1) - (x0 - 2);
-            pa[r * kP8PW + c] = pa[s];
-            pb[r * kP8PW + c] = pb[s];
-        }
-        __syncthreads();
-    }
-    // layer B over the 62 x 16 tile: 32 items, 8 per wave; patch rows 2 pair + 1 + g,
-    // columns 16 seg + j + t + 1 (columns past A's region feed only lanes past the tile)
-    const int gx1 = min(x0 + kP8TW, W), gy1 = min(y0 + kP8TH, H);
-#pragma unroll 2
-    for (int item = wave; item < 32; item += 4) {
-        const int pair = item >> 2, seg = item & 3;
-        const f32x4m acc = item24(load_item(pa, pb, (2 * pair + 1) * kP8PW + 16 * seg + 1 + lane_rd), a, epi.bias);
-        const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
-        epi.apply(acc, x, y, x < gx1 && y < gy1, half, fp, fout, la);
-    }
+    pair8_second<MODE, TIN, TOUT>(pa, pb, W, H, x0, y0, a, epi, fp, fout, la);
 }
 
 inline dim3 conv8_pair_grid(int W, int H) { return dim3((W + kP8TW - 1) / kP8TW, (H + kP8TH - 1) / kP8TH); }

@@ -475,10 +475,9 @@ def acnet8(frames_np, weights=None, steps=25):
             flop = 2 * (72 + 576 * net.blocks + 288) * w * h
             peak = PEAK_FP32_TFLOPS if prec == "fp32" else PEAK_MFMA16_TFLOPS
             mb = 32 if prec == "fp32" else 16
-            # map-to-map launches: FP32 one per two blocks (and one for an odd block); 16-bit
-            # one per two blocks after L0 + block 1 (even counts) or L0 (odd), the last block
-            # running with the final layer
-            maps = (net.blocks + 1) // 2 if prec == "fp32" else (net.blocks - 1) // 2
+            # map-to-map launches: one per two blocks after L0 + block 1 (even counts) or L0
+            # (odd), the last block running with the final layer
+            maps = (net.blocks - 1) // 2
             entry = {"ms_per_frame": round(sec * 1e3, 4), "output_mpix_per_s": round(4 * w * h / sec / 1e6, 1),
                      "achieved_tflops": round(flop / sec / 1e12, 2), "peak_tflops": peak,
                      "frac": round(flop / sec / 1e12 / peak, 4),
