@@ -8,7 +8,8 @@
 //   - the whole of process(src, dst, factor) runs on the device (processAll): colour
 //     split, the chained 2x passes (factor 4, 8, ...), the resize of Y by the
 //     non-power-of-two remainder and of UV by the full factor, and the merge.  Only the
-//     input and the final output cross PCIe.  Intermediate images keep the input's
+//     input and the final output cross PCIe, pageable caller memory through two
+//     page-locked chunks of the thread's own.  Intermediate images keep the input's
 //     element type, as the reference's pass loop does (Processor.cpp:219-236).
 #include <hip/hip_runtime.h>
 
@@ -24,6 +25,7 @@
 #include "achip.h"
 #include "acnet_kernels.hpp"
 #include "core_internal.hpp"
+#include "host_staging.hpp"
 
 namespace ac::core::hip
 {
@@ -66,6 +68,7 @@ namespace ac::core::hip
             hipStream_t stream = nullptr;
             DeviceBuffer buf[8];
             DeviceBuffer ws;          // ACNet<8> / ArtCNN feature maps
+            detail::HostStaging staging;   // page-locked chunks for pageable caller images
             std::string error;
         };
         ThreadState& local();
@@ -74,6 +77,10 @@ namespace ac::core::hip
         bool run(const Image& src, Image& dst, int passes);
         bool pipeline(const Image& src, Image& dst, double factor);
         bool stream_ready(ThreadState& st);
+        // caller image rows <-> device rows on this thread's stream (through st.staging when
+        // the host memory is pageable); toHost returns once the rows are in host memory
+        bool toDevice(ThreadState& st, void* d, size_t dpitch, const void* h, size_t hpitch, size_t row, int rows);
+        bool toHost(ThreadState& st, void* h, size_t hpitch, const void* d, size_t dpitch, size_t row, int rows);
         // one 2x pass of the net on device buffers (ACNetLegacy: achip_pass; ACNet<8> and
         // ArtCNN: achip_pass_net with this thread's workspace)
         bool pass(ThreadState& st, const void* src, long sp, int stype, int w, int h, void* dst, long dp, int dtype);
@@ -203,6 +210,7 @@ ac::core::hip::HIPProcessorACNetLegacy::~HIPProcessorACNetLegacy() noexcept
             if (st.stream) (void)hipStreamSynchronize(st.stream);
             for (auto& b : st.buf) if (b.ptr) (void)hipFree(b.ptr);
             if (st.ws.ptr) (void)hipFree(st.ws.ptr);
+            st.staging.release();
             if (st.stream) (void)hipStreamDestroy(st.stream);
         }
     }
@@ -282,9 +290,7 @@ bool ac::core::hip::HIPProcessorACNetLegacy::run(const Image& src, Image& dst, c
     // buffers: [0] input, then ping-pong [1]/[2]; rows padded to 256 bytes
     size_t pitch = alignUp(static_cast<size_t>(w) * es, 256);
     if (!ensure(st.buf[0], pitch * h)) return false;
-    hipError_t e = hipMemcpy2DAsync(st.buf[0].ptr, pitch, src.ptr(), src.stride(), static_cast<size_t>(w) * es, h,
-                                    hipMemcpyHostToDevice, st.stream);
-    if (e != hipSuccess) return fail(std::string("hipMemcpy2DAsync(H2D): ") + hipGetErrorString(e));
+    if (!toDevice(st, st.buf[0].ptr, pitch, src.ptr(), src.stride(), static_cast<size_t>(w) * es, h)) return false;
     void* cur = st.buf[0].ptr;
     size_t curPitch = pitch;
     for (int i = 0; i < passes; i++)
@@ -300,12 +306,7 @@ bool ac::core::hip::HIPProcessorACNetLegacy::run(const Image& src, Image& dst, c
         w *= 2;
         h *= 2;
     }
-    e = hipMemcpy2DAsync(dst.ptr(), dst.stride(), cur, curPitch, static_cast<size_t>(w) * des, h,
-                         hipMemcpyDeviceToHost, st.stream);
-    if (e != hipSuccess) return fail(std::string("hipMemcpy2DAsync(D2H): ") + hipGetErrorString(e));
-    e = hipStreamSynchronize(st.stream);
-    if (e != hipSuccess) return fail(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-    return true;
+    return toHost(st, dst.ptr(), dst.stride(), cur, curPitch, static_cast<size_t>(w) * des, h);
 }
 
 void ac::core::hip::HIPProcessorACNetLegacy::process(const Image& src, Image& dst)
@@ -330,6 +331,18 @@ bool ac::core::hip::HIPProcessorACNetLegacy::stream_ready(ThreadState& st)
     hipError_t e = hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking);
     if (e != hipSuccess) { st.stream = nullptr; return fail(std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
     return true;
+}
+
+bool ac::core::hip::HIPProcessorACNetLegacy::toDevice(ThreadState& st, void* d, const size_t dpitch, const void* h,
+                                                      const size_t hpitch, const size_t row, const int rows)
+{
+    return st.staging.toDevice(d, dpitch, h, hpitch, row, rows, st.stream) ? true : fail(st.staging.error());
+}
+
+bool ac::core::hip::HIPProcessorACNetLegacy::toHost(ThreadState& st, void* h, const size_t hpitch, const void* d,
+                                                    const size_t dpitch, const size_t row, const int rows)
+{
+    return st.staging.toHost(h, hpitch, d, dpitch, row, rows, st.stream) ? true : fail(st.staging.error());
 }
 
 // Processor.cpp:199-276 on the device.  Buffers: [0] input, [1] Y, [2] UV, [3]/[4] pass
@@ -370,8 +383,7 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pipeline(const Image& src, Image& d
     // upload
     const long inPitch = pitch_of(w * C, es);
     if (!ensure(st.buf[0], inPitch * h)) return false;
-    if (!hipok(hipMemcpy2DAsync(st.buf[0].ptr, inPitch, src.ptr(), src.stride(), w * C * es, h, hipMemcpyHostToDevice, s), "hipMemcpy2DAsync(H2D)"))
-        return false;
+    if (!toDevice(st, st.buf[0].ptr, inPitch, src.ptr(), src.stride(), w * C * es, h)) return false;
     // colour split
     void* y = st.buf[0].ptr;
     long yPitch = inPitch;
@@ -424,9 +436,7 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pipeline(const Image& src, Image& d
     // download
     if (dst.empty()) dst.create(static_cast<int>(fw), static_cast<int>(fh), C, yT);
     if (dst.empty()) return fail("cannot allocate the destination image");
-    if (!hipok(hipMemcpy2DAsync(dst.ptr(), dst.stride(), out, outPitch, fw * C * dst.elementSize(), fh, hipMemcpyDeviceToHost, s),
-               "hipMemcpy2DAsync(D2H)")) return false;
-    return hipok(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return toHost(st, dst.ptr(), dst.stride(), out, outPitch, fw * C * dst.elementSize(), fh);
 }
 
 std::shared_ptr<ac::core::Processor> ac::core::detail::createHIP(const int device, const model::ACNetLegacy& model, const int precision)

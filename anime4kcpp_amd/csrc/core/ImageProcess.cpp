@@ -1,7 +1,8 @@
 // The ac::core image helpers (core/include/AC/Core/Image.hpp:14-319; behaviour of
 // core/src/ImageProcess.cpp and core/src/ImageResize.cpp) with every pixel operation on the
 // HIP device: the helper uploads its host images, runs one of the kernels of
-// acnet_color.hip / acnet_image.hip and downloads the result.  crop() only builds a view.
+// acnet_color.hip / acnet_image.hip and downloads the result (pageable host memory through
+// two page-locked chunks of the engine).  crop() only builds a view.
 //
 // Each call runs on the calling thread's current HIP device, on an engine (a stream and
 // grow-only device buffers) leased from a process-wide pool for the duration of the call,
@@ -20,6 +21,7 @@
 #include "AC/Core/Image.hpp"
 #include "acnet_kernels.hpp"
 #include "core_internal.hpp"
+#include "host_staging.hpp"
 
 namespace ac::core::detail::ops
 {
@@ -32,6 +34,7 @@ namespace ac::core::detail::ops
         hipStream_t stream = nullptr;
         void* buf[kSlots] = {};
         size_t bytes[kSlots] = {};
+        HostStaging staging;   // page-locked chunks for pageable images
     };
 
     // Engines live until the process ends (no teardown against the HIP runtime's own exit).
@@ -130,15 +133,13 @@ namespace ac::core::detail::ops
         {
             Dev d = alloc(img.width(), img.height(), img.channels(), img.type(), slot);
             if (failed) return d;
-            check(hipMemcpy2DAsync(d.p, d.pitch, img.ptr(), img.stride(), d.line(), d.h, hipMemcpyHostToDevice, e->stream),
-                  "hipMemcpy2DAsync(H2D)");
+            if (!e->staging.toDevice(d.p, d.pitch, img.ptr(), img.stride(), d.line(), d.h, e->stream)) fail(e->staging.error());
             return d;
         }
         bool download(const Dev& d, Image& img)
         {
             if (failed) return false;
-            return check(hipMemcpy2DAsync(img.ptr(), img.stride(), d.p, d.pitch, d.line(), d.h, hipMemcpyDeviceToHost, e->stream),
-                         "hipMemcpy2DAsync(D2H)");
+            return e->staging.toHost(img.ptr(), img.stride(), d.p, d.pitch, d.line(), d.h, e->stream) ? true : fail(e->staging.error());
         }
         bool finish()
         {

@@ -67,6 +67,23 @@ def test_resize_two_and_four_channels(cuda, c):
             assert np.array_equal(P.resize(src, (dw, dh), mode=mode), OL.resize(src, dw, dh, mode))
 
 
+def test_resize_host_images_beyond_one_staging_chunk(cuda):
+    """Host images of several 4 MiB staging chunks each way: a pageable source whose rows are
+    narrower than its stride (a column slice), its page-locked copy (torch's pinned memory)
+    and a pageable destination inside a wider array; all give the oracle's pixels."""
+    import torch
+    wide = rand_image((1100, 1700, 3), np.uint8, seed=5)
+    src = wide[:, :1500]                                     # 4.95 MB of rows 5.1 KB apart
+    want = OL.resize(np.ascontiguousarray(src), 3000, 2200, OL.RESIZE_BILINEAR)
+    assert np.array_equal(P.resize(src, (3000, 2200), mode=OL.RESIZE_BILINEAR), want)
+    pinned = torch.from_numpy(np.ascontiguousarray(src)).pin_memory().numpy()
+    assert np.array_equal(P.resize(pinned, (3000, 2200), mode=OL.RESIZE_BILINEAR), want)
+    big = np.zeros((2200, 3100, 3), np.uint8)
+    si, di = P.Image(src, mapped=True), P.Image(big[:, :3000], mapped=True)
+    _native.check_ac(_native.lib().ac_resize(si._p, di._p, 2.0, 2.0, OL.RESIZE_BILINEAR), "ac_resize")
+    assert np.array_equal(big[:, :3000], want) and not big[:, 3000:].any()
+
+
 @pytest.mark.parametrize("k", [2, 4])
 def test_staged_upscale_multi_tile(cuda, k):
     """Exact K-times upscales over many 64 x 32 output tiles and ragged edges, every type."""

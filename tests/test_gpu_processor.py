@@ -138,3 +138,35 @@ def test_type_string_selects_precision(cuda, type_str, precision):
     net.close()
     assert not np.array_equal(want["fp16"], want["bf16"])
     assert np.array_equal(out, want[precision])
+
+
+def test_staged_and_page_locked_host_images(proc, weights):
+    """Caller images larger than one 4 MiB staging chunk: pageable ones travel through the
+    page-locked chunks both ways (2 chunks up and 5 down for gray, 2 up and 7 down for RGB,
+    and a destination whose rows are narrower than its stride), page-locked ones (torch's
+    pinned memory, mapped) are copied directly; every route gives the same pixels, and the
+    gray one is checked against the oracle."""
+    import torch
+    gray = synth.flat_shaded(41, 2600, 1700, dtype=np.uint8)          # 4.4 MB in, 17.7 MB out
+    rc, out = proc.process(gray, 2.0)
+    assert rc == 0, proc.error()
+    check_int(out, oracle_pass(gray, *weights["hdn0"]))
+    pinned_src = torch.from_numpy(gray).pin_memory().numpy()
+    pinned_dst = torch.zeros((3400, 5200), dtype=torch.uint8).pin_memory().numpy()
+    rc, _ = proc.process(P.Image(pinned_src, mapped=True), 2.0, dst=pinned_dst)
+    assert rc == 0, proc.error()
+    assert np.array_equal(pinned_dst, out)
+
+    g = synth.flat_shaded(42, 1600, 1400, dtype=np.uint8)
+    rgb = np.ascontiguousarray(np.stack([g, np.roll(g, 3, 0), np.roll(g, 5, 1)], 2))   # 6.7 MB in, 27 MB out
+    rc, ref = proc.process(rgb, 2.0)
+    assert rc == 0, proc.error()
+    big = np.zeros((2800, 3300, 3), np.uint8)
+    rc, _ = proc.process(rgb, 2.0, dst=big[:, :3200])
+    assert rc == 0, proc.error()
+    assert np.array_equal(big[:, :3200], ref) and not big[:, 3200:].any()
+    pinned_rgb = torch.from_numpy(rgb).pin_memory().numpy()
+    pinned_out = torch.zeros((2800, 3200, 3), dtype=torch.uint8).pin_memory().numpy()
+    rc, _ = proc.process(P.Image(pinned_rgb, mapped=True), 2.0, dst=pinned_out)
+    assert rc == 0, proc.error()
+    assert np.array_equal(pinned_out, ref)
