@@ -104,6 +104,9 @@ SIGNATURES = {
     "achip_pass_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p]),
     "achip_pass_layers": (c_int, [c_void_p, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
                                   ctypes.c_size_t, c_void_p]),
+    "achip_pass_ws_bytes": (ctypes.c_size_t, [c_void_p, c_int, c_int, c_int, c_int, c_int]),
+    "achip_pass_ws": (c_int, [c_void_p, c_int, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
+                              ctypes.c_size_t, c_void_p]),
     "achip_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "achip_pass_batch_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p]),
     # include/AC/Core/Image.h

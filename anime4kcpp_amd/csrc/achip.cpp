@@ -225,6 +225,42 @@ std::tuple<int, int, int, int, int, int> tune_key(int device, int precision, int
     return { device, precision, st, dt, ceil_log2i(w), ceil_log2i(h) };
 }
 
+// the autotuner's form for achip_pass_ws, same key: true = the layer-split form
+std::map<std::tuple<int, int, int, int, int, int>, bool> g_split;
+
+int compute_units(int device)
+{
+    static std::mutex mtx;
+    static std::map<int, int> cache;
+    std::lock_guard<std::mutex> lk(mtx);
+    auto it = cache.find(device);
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        n = 256;
+    }
+    cache[device] = n;
+    return n;
+}
+
+// The form achip_pass_ws runs: the layer-split one only for FP32 ACNetLegacy passes, and there
+// the autotuner's verdict for the size class, else the rule measured on MI355X (u8 1080p:
+// fused 328 us, split 299 us; 4K: 1307 / 1042 us; 256 x 256: 82 / 52 us; 960 x 540: 84 /
+// 109 us): the fused kernel wins only where its grid of 64 x 32 tiles fits in one round of
+// the device's compute units and the image is larger than 256 x 256.
+bool split_preferred(const ACHipNet* net, int precision, int st, int dt, int w, int h)
+{
+    if (precision != ACHIP_FP32 || net->blocks || net->lfamily) return false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mtx);
+        auto it = g_split.find(tune_key(net->device, precision, st, dt, w, h));
+        if (it != g_split.end()) return it->second;
+    }
+    const long tiles = static_cast<long>((w + 63) / 64) * ((h + 31) / 32);
+    return !(tiles <= compute_units(net->device) && static_cast<long>(w) * h > 256L * 256);
+}
+
 } // namespace
 
 namespace achip {
@@ -750,6 +786,33 @@ int achip_pass_layers(ACHipNet* net, const void* src, long src_stride, int src_t
 
 int achip_net_device(const ACHipNet* net) { return net ? net->device : ACHIP_EINVAL; }
 
+size_t achip_pass_ws_bytes(ACHipNet* net, int precision, int src_type, int dst_type, int w, int h)
+{
+    if (!net || w <= 0 || h <= 0 || !elem_size(src_type) || !elem_size(dst_type)) return 0;
+    return split_preferred(net, precision, src_type, dst_type, w, h) ? achip_pass8_workspace_bytes(w, h) : 0;
+}
+
+int achip_pass_ws(ACHipNet* net, int precision, const void* src, long src_stride, int src_type, int w, int h, void* dst,
+                  long dst_stride, int dst_type, void* workspace, size_t workspace_bytes, void* stream)
+{
+    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    if (net->blocks || net->lfamily) return achip_pass(net, precision, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, stream);
+    const char* env = std::getenv("ACHIP_AUTOTUNE");
+    if (precision == ACHIP_FP32 && env && env[0] == '1' && w > 0 && h > 0) {   // tune an untuned class on first use
+        bool found;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mtx);
+            found = g_split.count(tune_key(net->device, precision, src_type, dst_type, w, h)) != 0;
+        }
+        int best = 0;
+        if (!found) (void)achip_autotune(net, precision, src_type, dst_type, w, h, &best, nullptr);
+    }
+    const size_t need = achip_pass_ws_bytes(net, precision, src_type, dst_type, w, h);
+    if (need && workspace && workspace_bytes >= need)
+        return achip_pass_layers(net, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, workspace, workspace_bytes, stream);
+    return achip_pass(net, precision, src, src_stride, src_type, w, h, dst, dst_stride, dst_type, stream);
+}
+
 int achip_config_count(int precision)
 {
     switch (precision) {
@@ -949,11 +1012,39 @@ int achip_autotune(ACHipNet* net, int precision, int src_type, int dst_type, int
         if (ms) ms[c] = t;
         if (t < best_ms) { best_ms = t; best_cfg = c; }
     }
+    // FP32 ACNetLegacy: the layer-split form too (same bits), for achip_pass_ws
+    int split = -1;
+    if (rc == ACHIP_OK && precision == ACHIP_FP32 && !net->blocks && !net->lfamily) {
+        const size_t wsb = achip_pass8_workspace_bytes(w, h);
+        void* ws = nullptr;
+        e = hipMalloc(&ws, wsb);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();   // no room for the maps: the fused kernel it is
+            split = 0;
+        } else {
+            rc = achip_pass_layers(net, s, sp, src_type, w, h, d, dp, dst_type, ws, wsb, st);
+            const int reps = 3;
+            (void)hipEventRecord(e0, st);
+            for (int r = 0; r < reps && rc == ACHIP_OK; r++)
+                rc = achip_pass_layers(net, s, sp, src_type, w, h, d, dp, dst_type, ws, wsb, st);
+            (void)hipEventRecord(e1, st);
+            e = hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (e == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+            (void)hipFree(ws);
+            if (e != hipSuccess) {
+                cleanup();
+                return hip_fail(e, "autotune run");
+            }
+            split = t / reps < best_ms ? 1 : 0;
+        }
+    }
     cleanup();
     if (rc) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tune_mtx);
         g_tune[tune_key(net->device, precision, src_type, dst_type, w, h)] = best_cfg;
+        if (split >= 0) g_split[tune_key(net->device, precision, src_type, dst_type, w, h)] = split == 1;
     }
     *best = best_cfg;
     return ACHIP_OK;

@@ -20,10 +20,8 @@ namespace achip {
 
 #pragma clang fp contract(off)
 
-// L0 of one pixel from its 9 clamped taps (row-major).  RELU: ACNetLegacy's layer 0 (ReLU; its
-// blob has the same w[p][n] and bias layout at the same offsets), else ACNet<8>'s (PReLU with
-// the blob's slopes).
-template <bool RELU>
+// L0 of one pixel from its 9 clamped taps (row-major): conv3x3 1 -> 8 + PReLU with the blob's
+// slopes, each product and sum rounded in the reference's order
 __device__ inline void l0_pixel8(const float (&r)[9], const float* __restrict__ wts, float (&o)[8])
 {
 #pragma unroll
@@ -31,13 +29,12 @@ __device__ inline void l0_pixel8(const float (&r)[9], const float* __restrict__ 
         float s = 0.0f;
 #pragma unroll
         for (int p = 0; p < 9; p++) s = __fadd_rn(s, __fmul_rn(r[p], wts[kA8OffW0 + p * 8 + n]));
-        const float v = __fadd_rn(s, wts[kA8OffB0 + n]);
-        o[n] = RELU ? fmaxf(v, 0.0f) : prelu(v, wts[kA8OffAlpha0 + n]);
+        o[n] = prelu(__fadd_rn(s, wts[kA8OffB0 + n]), wts[kA8OffAlpha0 + n]);
     }
 }
 
 // ---- L0: one pixel per thread, 64 x 4 pixels per workgroup ----
-template <typename TIN, bool RELU = false>
+template <typename TIN>
 __global__ __launch_bounds__(256) void acnet8_l0(const void* __restrict__ src, long ss, int W, int H,
                                                  float4* __restrict__ feat, long fp, const float* __restrict__ wts)
 {
@@ -52,7 +49,7 @@ __global__ __launch_bounds__(256) void acnet8_l0(const void* __restrict__ src, l
         for (int dx = 0; dx < 3; dx++) r[dy * 3 + dx] = to_float<TIN>(row[clampi(x + dx - 1, 0, W - 1)]);
     }
     float o[8];
-    l0_pixel8<RELU>(r, wts, o);
+    l0_pixel8(r, wts, o);
     float4* px = reinterpret_cast<float4*>(reinterpret_cast<char*>(feat) + static_cast<long>(y) * fp) + 2 * x;
     px[0] = make_float4(o[0], o[1], o[2], o[3]);
     px[1] = make_float4(o[4], o[5], o[6], o[7]);
@@ -97,7 +94,7 @@ __global__ __launch_bounds__(256) void conv8_head(const void* __restrict__ src, 
 #pragma unroll
             for (int dx = 0; dx < 3; dx++) q[dy * 3 + dx] = img[(r - 1 + dy) * kH8SW + c - 1 + dx];
         float o[8];
-        l0_pixel8<false>(q, wts, o);
+        l0_pixel8(q, wts, o);
         pa[r * kP8PW + c] = make_float4(o[0], o[1], o[2], o[3]);
         pb[r * kP8PW + c] = make_float4(o[4], o[5], o[6], o[7]);
     }
@@ -149,16 +146,86 @@ static hipError_t acnet8_pass_t(int blocks, const void* src, long ss, int W, int
 long acnet8_feature_pitch(int w) { return (static_cast<long>(w) * 32 + 255) / 256 * 256; }
 
 // ---- ACNetLegacy, layer by layer (the layer-split form of the fused FP32 pass) ----
-// L0 (conv3x3 1 -> 8 + ReLU, VALU) into an 8-channel f32 map, then layers 1..8 two per launch
-// on conv8_pair (ReLU; the last pair ends in the deconv 2x2 8 -> 1 straight to the output):
-// five launches, each map written and read once through HBM, a one-pixel recompute ring per
-// pair instead of the fused kernel's eight-pixel halo.
+// Layer 0 and layers 1-2 in one launch (legacy_head), then layers 3..8 two per launch on
+// conv8_pair (ReLU; the last pair ends in the deconv 2x2 8 -> 1 straight to the output):
+// four launches, each 8-channel f32 map written and read once through HBM, a one-pixel
+// recompute ring per layer instead of the fused kernel's eight-pixel halo.  Every layer
+// runs the fused kernel's own instructions on the same operands in the same order (layer 0:
+// three MFMAs per item from the bias; layers 1-8: item24; the deconv epilogue), so this
+// form gives the fused pass's bits.
+//
+// legacy_head, 62 x 16 outputs per 256-thread workgroup: the source tile, 82 x 22 pixels from
+// (x0 - 3, y0 - 3), clamped, in the source type, gives layer 0 at plane pixels (0..79, 0..19)
+// (five 16-pixel segments x 10 row pairs; plane pixel (c, r) is image pixel
+// (x0 - 2 + c, y0 - 2 + r)); plane pixels outside the image then take the nearest pixel
+// inside, as every layer's clamped taps see them (Common.hpp:120-143), and layers 1 and 2 run
+// as conv8_pair's two layers.
+constexpr int kLHW = 16 * 5 + 2, kLHH = kP8PH + 2;
+
+template <typename TIN>
+__global__ __launch_bounds__(256) void legacy_head(const void* __restrict__ src, long ss, int W, int H,
+                                                   float4* __restrict__ fout, long fp, const float* __restrict__ wts,
+                                                   const float* __restrict__ wa32, const LayerArgs la0, const LayerArgs la)
+{
+    __shared__ float4 pa[kP8PW * kP8PH];
+    __shared__ float4 pb[kP8PW * kP8PH];
+    __shared__ TIN img[kLHH * kLHW];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int x0 = tx * kP8TW, y0 = ty * kP8TH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
+    // img pixel (c, r) is image pixel (x0 - 3 + c, y0 - 3 + r)
+    for (int e = tid; e < kLHH * kLHW; e += 256) {
+        const int r = e / kLHW, c = e - r * kLHW;
+        img[e] = row_ptr<TIN>(src, ss, clampi(y0 - 3 + r, 0, H - 1))[clampi(x0 - 3 + c, 0, W - 1)];
+    }
+    const float a0 = wa32[lane], a1 = wa32[64 + lane], a2 = wa32[128 + lane];
+    const float* bl = wts + kOffB0 + 4 * half;
+    const f32x4m bias0 = { bl[0], bl[1], bl[2], bl[3] };
+    float a[24];
+#pragma unroll
+    for (int s = 0; s < 24; s++) a[s] = la0.wa[s * 64 + lane];
+    Epi8<kRelu, float, float> epa;
+    epa.load(la0, lane, half);
+    __syncthreads();
+    // layer 0: item (pair, seg) writes plane pixels (16 seg + j, 2 pair + ro); its B operands
+    // are img row 2 pair + g (image row y0 - 3 + 2 pair + g), columns 16 seg + j + t
+    {
+        float4* const pw = half ? pb : pa;
+        for (int item = wave; item < 50; item += 4) {
+            const int pair = item / 5, seg = item - 5 * pair;
+            const TIN* p = img + (2 * pair + g) * kLHW + 16 * seg + j;
+            f32x4m acc = mma32(a0, to_float<TIN>(p[0]), bias0);
+            acc = mma32(a1, to_float<TIN>(p[1]), acc);
+            acc = mma32(a2, to_float<TIN>(p[2]), acc);
+            pw[(2 * pair + ro) * kP8PW + 16 * seg + j] =
+                make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+        }
+    }
+    __syncthreads();
+    // layer 1 reads plane pixels (0..65, 0..19): those outside the image take the nearest inside
+    if (x0 - 2 < 0 || y0 - 2 < 0 || x0 + 63 >= W || y0 + 17 >= H) {
+        for (int e = tid; e < kP8PH * 66; e += 256) {
+            const int r = e / 66, c = e - r * 66;
+            const int gy = y0 - 2 + r, gx = x0 - 2 + c;
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) continue;
+            const int s = (clampi(gy, 0, H - 1) - (y0 - 2)) * kP8PW + clampi(gx, 0, W - 1) - (x0 - 2);
+            pa[r * kP8PW + c] = pa[s];
+            pb[r * kP8PW + c] = pb[s];
+        }
+        __syncthreads();
+    }
+    Epi8<kRelu, float, float> epi;
+    pair8_first<kRelu>(pa, pb, a, epa, epi, la);
+    pair8_second<kRelu, float, float>(pa, pb, W, H, x0, y0, a, epi, fp, fout, la);
+}
+
 template <typename TIN, typename TOUT>
 static hipError_t legacy_split_t(const void* src, long ss, int W, int H, void* dst, long ds, const float* wts,
                                  const float* wa32, float4* f0, float4* f1, long fp, hipStream_t stream)
 {
-    static_assert(kA8OffW0 == kOffW0 && kA8OffB0 == kOffB0, "layer 0 reads both blobs at the same offsets");
-    hipLaunchKernelGGL((acnet8_l0<TIN, true>), dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, stream, src, ss, W, H, f0, fp, wts);
     const dim3 pgrid = conv8_pair_grid(W, H), block(256);
     auto layer = [&](int l) {
         LayerArgs la{};
@@ -166,9 +233,10 @@ static hipError_t legacy_split_t(const void* src, long ss, int W, int H, void* d
         la.bias = wts + offB(l);
         return la;
     };
+    hipLaunchKernelGGL((legacy_head<TIN>), pgrid, block, 0, stream, src, ss, W, H, f0, fp, wts, wa32, layer(1), layer(2));
     float4* in = f0;
     float4* out = f1;
-    for (int l = 1; l <= 5; l += 2) {
+    for (int l = 3; l <= 5; l += 2) {
         hipLaunchKernelGGL((conv8_pair<kRelu, float, float, kRelu>), pgrid, block, 0, stream, in, fp, W, H, out, layer(l),
                            layer(l + 1));
         std::swap(in, out);

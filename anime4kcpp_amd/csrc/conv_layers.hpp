@@ -285,34 +285,19 @@ __device__ __forceinline__ void pair8_second(float4* __restrict__ pa, float4* __
 }
 
 
-template <int MODE, typename TIN, typename TOUT, int MODE_A = kPrelu>
-__global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
-                                                  float4* __restrict__ fout, const LayerArgs la0, const LayerArgs la)
+// Layer A of a two-layer launch: its input at plane pixels (0..65, 0..19) of pa/pb, its
+// output written back to (1..64, 1..18) -- 36 items, 9 per wave; item (pair, seg) reads
+// patch rows 2 pair + g, columns 16 seg + j + t.  Takes layer A's A operands in `a` and its
+// epilogue in epa; returns with layer B's (la) in `a` and epi, loaded while the other waves
+// finish layer A.
+template <int MODE_A, int MODE, typename TIN, typename TOUT>
+__device__ __forceinline__ void pair8_first(float4* __restrict__ pa, float4* __restrict__ pb, float (&a)[24],
+                                            const Epi8<MODE_A, float, float>& epa, Epi8<MODE, TIN, TOUT>& epi,
+                                            const LayerArgs& la)
 {
     static_assert(MODE_A == kPrelu || MODE_A == kRelu, "layer A: conv3x3 + PReLU or ReLU");
-    __shared__ float4 pa[kP8PW * kP8PH];
-    __shared__ float4 pb[kP8PW * kP8PH];
-    const int tiles_x = gridDim.x;
-    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
-    const int ty = t / tiles_x, tx = t - ty * tiles_x;
-    const int x0 = tx * kP8TW, y0 = ty * kP8TH;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
-
-    // plane pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r); stage 66 x 20, clamped
-    {
-        float4* const pl[2] = { pa, pb };
-        stage_map<256, kP8TW + 4, kP8PH, 2, kP8PW>(fin, fp, W, H, x0 - 2, y0 - 2, pl);
-    }
-    float a[24];
-#pragma unroll
-    for (int s = 0; s < 24; s++) a[s] = la0.wa[s * 64 + lane];
-    Epi8<MODE_A, float, float> epa;
-    epa.load(la0, lane, half);
-    __syncthreads();
-
-    // layer A over plane pixels (1..64, 1..18): 36 items, 9 per wave; item (pair, seg) reads
-    // patch rows 2 pair + g, columns 16 seg + j + t
     const int lane_rd = g * kP8PW + j;
     float4 res[9];
 #pragma unroll
@@ -325,10 +310,8 @@ __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin
         else
             res[ii] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
     }
-    // layer B's operands while the other waves finish
 #pragma unroll
     for (int s = 0; s < 24; s++) a[s] = la.wa[s * 64 + lane];
-    Epi8<MODE, TIN, TOUT> epi;
     epi.load(la, lane, half);
     __syncthreads();
     {
@@ -340,6 +323,33 @@ __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin
         }
     }
     __syncthreads();
+}
+
+template <int MODE, typename TIN, typename TOUT, int MODE_A = kPrelu>
+__global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
+                                                  float4* __restrict__ fout, const LayerArgs la0, const LayerArgs la)
+{
+    __shared__ float4 pa[kP8PW * kP8PH];
+    __shared__ float4 pb[kP8PW * kP8PH];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    const int x0 = tx * kP8TW, y0 = ty * kP8TH;
+    const int lane = threadIdx.x & 63, half = (lane >> 4) & 1;
+
+    // plane pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r); stage 66 x 20, clamped
+    {
+        float4* const pl[2] = { pa, pb };
+        stage_map<256, kP8TW + 4, kP8PH, 2, kP8PW>(fin, fp, W, H, x0 - 2, y0 - 2, pl);
+    }
+    float a[24];
+#pragma unroll
+    for (int s = 0; s < 24; s++) a[s] = la0.wa[s * 64 + lane];
+    Epi8<MODE_A, float, float> epa;
+    epa.load(la0, lane, half);
+    __syncthreads();
+    Epi8<MODE, TIN, TOUT> epi;
+    pair8_first<MODE_A>(pa, pb, a, epa, epi, la);
     pair8_second<MODE, TIN, TOUT>(pa, pb, W, H, x0, y0, a, epi, fp, fout, la);
 }
 

@@ -67,7 +67,7 @@ namespace ac::core::hip
         {
             hipStream_t stream = nullptr;
             DeviceBuffer buf[8];
-            DeviceBuffer ws;          // ACNet<8> / ArtCNN feature maps
+            DeviceBuffer ws;          // feature maps: ACNet<8>, ArtCNN, FSRCNNX, ACNetLegacy's layer-split form
             detail::HostStaging staging;   // page-locked chunks for pageable caller images
             std::string error;
         };
@@ -81,8 +81,9 @@ namespace ac::core::hip
         // the host memory is pageable); toHost returns once the rows are in host memory
         bool toDevice(ThreadState& st, void* d, size_t dpitch, const void* h, size_t hpitch, size_t row, int rows);
         bool toHost(ThreadState& st, void* h, size_t hpitch, const void* d, size_t dpitch, size_t row, int rows);
-        // one 2x pass of the net on device buffers (ACNetLegacy: achip_pass; ACNet<8> and
-        // ArtCNN: achip_pass_net with this thread's workspace)
+        // one 2x pass of the net on device buffers (ACNetLegacy: achip_pass_ws, lending the
+        // workspace to the layer-split form; ACNet<8>, ArtCNN and FSRCNNX: achip_pass8_precision
+        // or achip_pass_net with this thread's workspace)
         bool pass(ThreadState& st, const void* src, long sp, int stype, int w, int h, void* dst, long dp, int dtype);
         void finishInit();
 
@@ -195,8 +196,20 @@ bool ac::core::hip::HIPProcessorACNetLegacy::pass(ThreadState& st, const void* s
         rc = acnet8 ? achip_pass8_precision(net, precision, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream)
                     : achip_pass_net(net, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream);
     }
-    else rc = achip_pass(net, precision, src, sp, stype, w, h, dst, dp, dtype, st.stream);
-    return rc == ACHIP_OK ? true : fail(std::string(layered ? "achip_pass_net: " : "achip_pass: ") + achip_last_error());
+    else
+    {
+        // the faster form for the size; the layer-split one borrows this thread's map buffer
+        // (same bits as the fused kernel, which runs when that buffer cannot be had)
+        const size_t need = achip_pass_ws_bytes(net, precision, stype, dtype, w, h);
+        if (need && st.ws.bytes < need)
+        {
+            if (st.ws.ptr) { (void)hipFree(st.ws.ptr); st.ws = DeviceBuffer{}; }
+            if (hipMalloc(&st.ws.ptr, need) == hipSuccess) st.ws.bytes = need;
+            else { st.ws.ptr = nullptr; (void)hipGetLastError(); }
+        }
+        rc = achip_pass_ws(net, precision, src, sp, stype, w, h, dst, dp, dtype, st.ws.ptr, st.ws.bytes, st.stream);
+    }
+    return rc == ACHIP_OK ? true : fail(std::string(layered ? "achip_pass_net: " : "achip_pass_ws: ") + achip_last_error());
 }
 
 ac::core::hip::HIPProcessorACNetLegacy::~HIPProcessorACNetLegacy() noexcept

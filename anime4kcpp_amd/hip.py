@@ -143,9 +143,18 @@ class Net:
     def handle(self):
         return self._h
 
-    def upscale(self, src, dst=None, precision="fp32", config=None, stream=None, out_dtype=None):
+    def workspace_bytes(self, w, h, precision="fp32", src_dtype=None, dst_dtype=None):
+        """Device bytes achip_pass_ws wants for a w x h pass (0: the fused kernel runs)."""
+        import torch
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        st = elem_type(src_dtype or torch.uint8)
+        return _native.lib().achip_pass_ws_bytes(self._h, prec, st, elem_type(dst_dtype) if dst_dtype else st, w, h)
+
+    def upscale(self, src, dst=None, precision="fp32", config=None, stream=None, out_dtype=None, workspace=None):
         """One 2x pass: src (H x W, 1 channel) -> dst (2H x 2W); launches on `stream`
-        (a torch.cuda.Stream or raw hipStream_t int; default: torch's current stream)."""
+        (a torch.cuda.Stream or raw hipStream_t int; default: torch's current stream).
+        With a device tensor `workspace` (and no config) the pass runs through achip_pass_ws:
+        the faster form for the size when the workspace holds workspace_bytes()."""
         import torch
         if src.dim() != 2 or not src.is_cuda:
             raise ValueError("src must be a 2-D device tensor")
@@ -165,7 +174,10 @@ class Net:
                 ctypes.c_void_p(dst.data_ptr()), dst.stride(0) * dst.element_size(), elem_type(dst.dtype),
                 ctypes.c_void_p(stream))
         L = _native.lib()
-        if config is None:
+        if workspace is not None and config is None:
+            check(L.achip_pass_ws(self._h, prec, *args[:-1], ctypes.c_void_p(workspace.data_ptr()),
+                                  workspace.numel() * workspace.element_size(), args[-1]), "achip_pass_ws")
+        elif config is None:
             check(L.achip_pass(self._h, prec, *args), "achip_pass")
         else:
             check(L.achip_pass_config(self._h, prec, config, *args), "achip_pass_config")

@@ -3,12 +3,14 @@
 One step = one 2x pass over a batch of synthetic 1080p u8 luma frames already resident in
 HBM (two frames per weight set: GAN, HDN0, HDN1, HDN2, HDN3), written to resident 4K
 outputs.  Prints one JSON line (rank 0):
-  value      whole-job output Mpix/s of the fused FP32 kernel over the timed steps (the
-             FP16-/BF16-MFMA forms are timed on the same frames and reported beside it,
-             with their error against the FP64 restatement)
-  roofline   achieved FP32 TFLOP/s of the fused kernel (9424 algorithmic FLOP per input
-             pixel x pixels per launch / mean launch time from HIP events on the launch
-             stream) against the MI355X FP32 peak
+  value      whole-job output Mpix/s of the FP32 pass the processor runs (achip_pass_ws with
+             a workspace: at 1080p the layer-split form, four launches, the fused kernel's
+             bits) over the timed steps; the fused kernel alone and the FP16-/BF16-MFMA forms
+             are timed on the same frames and reported beside it, the 16-bit forms with their
+             error against the FP64 restatement
+  roofline   achieved FP32 TFLOP/s of that pass (9424 algorithmic FLOP per input pixel x
+             pixels per pass / mean pass time from HIP events on the launch stream) against
+             the MI355X FP32 peak
   cpu_baseline  the reference CPU 2x pass (its own per-ISA kernels, oracle/_ref, or the
              oracle's restatement when that is absent) on a bounded sample of the same
              frames, timed on the host cores
@@ -31,10 +33,15 @@ PEAK_MFMA16_TFLOPS = 2500.0       # MI355X published dense FP16/BF16 MFMA peak
 PEAK_HBM_GBS = 8000.0             # MI355X published HBM3E bandwidth
 PEAK_PCIE_GBS = 64.0              # PCIe 5.0 x16, one direction
 FRAMES_PER_VARIANT = 2
-# HBM bytes per 1080p u8 launch of the default configuration of each precision, measured with
-# rocprofv3 --pmc FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000) against a 256 MiB copy
-# (profiles/acnet_fused_1080p_u8.md); counters cannot be read from inside this process.
-MEASURED_TRAFFIC = {"fp32": 11.24e6, "fp16": 11.08e6, "bf16": 11.08e6}
+# HBM bytes per 1080p u8 pass of the default configuration of each precision (fp32_split: the
+# layer-split form's four launches together), measured with rocprofv3 --pmc FETCH_SIZE (x2.000)
+# and WRITE_SIZE (x1.000) against a 256 MiB copy (profiles/acnet_fused_1080p_u8.md); counters
+# cannot be read from inside this process.
+MEASURED_TRAFFIC = {"fp32": 11.24e6, "fp16": 11.08e6, "bf16": 11.08e6, "fp32_split": 415.2e6}
+# bytes per input pixel the layer-split form moves by design: the u8 source, a 32-byte f32 map
+# written by the first launch, read and written by the two middle ones and read by the last,
+# and the four u8 output pixels
+SPLIT_FORM_BYTES_PER_PX = 1 + 32 * 6 + 4
 
 
 def cpu_model():
@@ -102,13 +109,15 @@ def cpu_baseline(frames_u8, weights, min_seconds=10.0):
     return out
 
 
-def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, stream):
-    """Time `steps` steps of one precision; returns (wall seconds, mean launch seconds)."""
+def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, stream, workspace=None):
+    """Time `steps` steps of one precision; returns (wall seconds, mean seconds per pass on
+    the stream).  With a workspace the passes run through achip_pass_ws (the processor's
+    pass: the faster form for the size), else through achip_pass (the fused kernel)."""
     import torch
 
     def step():
         for i, v in order:
-            nets[v].upscale(srcs[i], dsts[i], precision=prec, config=config, stream=stream)
+            nets[v].upscale(srcs[i], dsts[i], precision=prec, config=config, stream=stream, workspace=workspace)
 
     for _ in range(warmup):
         step()
@@ -126,8 +135,8 @@ def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, stream)
 
 
 def layer_split_pass(net, src, dst, ws, stream):
-    """One ACNetLegacy FP32 pass in the layer-split form (achip_pass_layers: layer 0, then
-    layers 1..8 two per launch through f32 maps in `ws`)."""
+    """One ACNetLegacy FP32 pass in the layer-split form (achip_pass_layers: layers 0-2, then
+    layers 3..8 two per launch through f32 maps in `ws`)."""
     import ctypes
     from anime4kcpp_amd import _native, hip
     h, w = src.shape
@@ -137,33 +146,6 @@ def layer_split_pass(net, src, dst, ws, stream):
                                          ctypes.c_void_p(ws.data_ptr()), ws.numel(), ctypes.c_void_p(stream))
     if rc != 0:
         raise RuntimeError(f"achip_pass_layers: {_native.last_error()}")
-
-
-def time_layer_split(nets, srcs, dsts, order, steps, warmup, stream):
-    """The same frames in the layer-split FP32 form; returns (wall seconds, mean seconds per
-    frame on the stream)."""
-    import torch
-    from anime4kcpp_amd import _native
-    h, w = srcs[0].shape
-    ws = torch.empty(_native.lib().achip_pass8_workspace_bytes(w, h), dtype=torch.uint8, device=srcs[0].device)
-    st = stream.cuda_stream
-
-    def step():
-        for i, v in order:
-            layer_split_pass(nets[v], srcs[i], dsts[i], ws, st)
-
-    for _ in range(warmup):
-        step()
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    return time.perf_counter() - t0, e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
 
 
 def error_vs_fp64(nets, weights, prec):
@@ -219,7 +201,7 @@ def batched_uneven(nets, prec, seed=2024, n=64):
                 + ("/mfma" if hip.configs(prec)[c]["p"] == 0 else "/valu") + "/halo-recompute" for c in sweep}
     sweep_ms = {names[c]: round(t * 1e3, 3) for c, t in sweep.items()}
     if prec == "fp32":
-        # the layer-split form (one image at a time: 5 launches, f32 maps through HBM)
+        # the layer-split form (one image at a time: 4 launches, f32 maps through HBM)
         lws = torch.empty(_native.lib().achip_pass8_workspace_bytes(2048, 2048), dtype=torch.uint8, device=dev)
         st = torch.cuda.current_stream(dev).cuda_stream
         sweep_ms["62x16/256thr/mfma/layer-split (per image)"] = round(
@@ -595,15 +577,33 @@ def main():
     out_px_per_step = len(srcs) * 4 * W_IN * H_IN
 
     precs = [p for p in args.precisions.split(",") if p]
+    # the processor's pass (achip_pass_ws): each thread lends it a workspace, and at 1080p FP32
+    # it runs the layer-split form (4 launches, the fused kernel's bits); an explicit --config
+    # runs that fused configuration
+    ws_bytes = {p: (nets["hdn0"].workspace_bytes(W_IN, H_IN, precision=p) if args.config is None else 0) for p in precs}
+    workspaces = {p: torch.empty(n, dtype=torch.uint8, device=dev) if n else None for p, n in ws_bytes.items()}
     timings = {}
     for prec in precs:
-        timings[prec] = time_precision(nets, srcs, dsts, order, prec, args.steps, args.warmup, args.config, stream)
+        timings[prec] = time_precision(nets, srcs, dsts, order, prec, args.steps, args.warmup, args.config, stream,
+                                       workspaces[prec])
 
     main_prec = precs[0]
-    wall, mean_launch_s = timings[main_prec]
+    wall, mean_pass_s = timings[main_prec]
     value = out_px_per_step * args.steps / 1e6 / wall
-    achieved = FLOP_PER_INPUT_PX * W_IN * H_IN / mean_launch_s / 1e12
+    achieved = FLOP_PER_INPUT_PX * W_IN * H_IN / mean_pass_s / 1e12
     kcfg = hip.configs(main_prec)[args.config or 0]
+    split = ws_bytes[main_prec] > 0
+    if split:
+        path = "achip_pass_ws: layer-split form, 4 launches (legacy_head, conv8_pair x2, conv8_pair + deconv), 8-channel f32 maps in HBM"
+        engine = "FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulate"
+        traffic_key = main_prec + "_split"
+    else:
+        path = {"fp32": "fused FP32 kernel", "fp16": "FP16-MFMA fused kernel", "bf16": "BF16-MFMA fused kernel"}[main_prec]
+        engine = (("FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulate" if kcfg["p"] == 0 else
+                   "FP32 VALU (packed FP32 disabled; non-packed FMA caps at half the FP32 peak)")
+                  if main_prec == "fp32" else f"{main_prec.upper()} MFMA, FP32 accumulate")
+        traffic_key = main_prec
+    peak = PEAK_FP32_TFLOPS if main_prec == "fp32" else PEAK_MFMA16_TFLOPS
 
     result = {
         "metric": "output Mpix/s, 1080p->4K 2x (FP32 vs FP16-MFMA)",
@@ -619,39 +619,40 @@ def main():
         "dtype": {"fp32": "f32", "fp16": "f16", "bf16": "bf16"}[main_prec],
         "data": "synthetic",
         "config": {
-            "workload": "ACNetLegacy 2x pass, 1920x1080 u8 luma -> 3840x2160 u8, one fused HIP kernel per frame",
-            "path": {"fp32": "fused FP32 kernel", "fp16": "FP16-MFMA fused kernel",
-                     "bf16": "BF16-MFMA fused kernel"}[main_prec],
+            "workload": "ACNetLegacy 2x pass, 1920x1080 u8 luma -> 3840x2160 u8, the processor's pass per frame",
+            "path": path,
             "frames_per_step": len(srcs),
             "weights": "GAN, HDN0, HDN1, HDN2, HDN3 (2 frames each)",
             "content": "seeded flat-shaded polygons + Bezier strokes + noise (sigma 0.01), resident in HBM",
-            "kernel_config": kcfg,
+            "kernel_config": None if split else kcfg,
+            "workspace_bytes": ws_bytes[main_prec],
         },
         "roofline": {
             "bound": "mfma",
-            "engine": (("FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulate" if kcfg["p"] == 0 else
-                        "FP32 VALU (packed FP32 disabled; non-packed FMA caps at half the FP32 peak)")
-                       if main_prec == "fp32" else f"{main_prec.upper()} MFMA, FP32 accumulate"),
+            "engine": engine,
+            "unit_of_work": "one 2x pass of a 1080p frame" + (" (its 4 launches)" if split else " (one launch)"),
             "achieved": round(achieved, 2),
-            "peak": PEAK_FP32_TFLOPS if main_prec == "fp32" else PEAK_MFMA16_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": round(achieved / (PEAK_FP32_TFLOPS if main_prec == "fp32" else PEAK_MFMA16_TFLOPS), 4),
-            "traffic": MEASURED_TRAFFIC.get(main_prec) if not args.config else None,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE per launch, profiles/acnet_fused_1080p_u8.md",
+            "frac": round(achieved / peak, 4),
+            "traffic": MEASURED_TRAFFIC.get(traffic_key) if args.config is None else None,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE per pass, profiles/acnet_fused_1080p_u8.md",
             "algorithmic_bytes": 5 * W_IN * H_IN,
-            "mean_launch_us": round(mean_launch_s * 1e6, 2),
-            "flop_per_launch": FLOP_PER_INPUT_PX * W_IN * H_IN,
+            "form_bytes": SPLIT_FORM_BYTES_PER_PX * W_IN * H_IN if split else 5 * W_IN * H_IN,
+            "mean_pass_us": round(mean_pass_s * 1e6, 2),
+            "flop_per_pass": FLOP_PER_INPUT_PX * W_IN * H_IN,
         },
     }
-    if "fp32" in precs:
-        # the layer-split alternative of the same FP32 pass (five launches, f32 maps through HBM)
-        wl, ml = time_layer_split(nets, srcs, dsts, order, args.steps, args.warmup, stream)
-        achl = FLOP_PER_INPUT_PX * W_IN * H_IN / ml / 1e12
-        result["fp32_layer_split"] = {
-            "value": round(out_px_per_step * args.steps / 1e6 / wl, 1), "unit": "Mpix/s",
-            "ms_per_step": round(wl * 1e3 / args.steps, 4), "mean_frame_us": round(ml * 1e6, 2),
-            "achieved_tflops": round(achl, 2), "frac": round(achl / PEAK_FP32_TFLOPS, 4),
-            "form": "layer 0, then layers 1..8 two per launch (conv8_pair), 8-channel f32 maps in HBM: 5 launches"}
+    if split:
+        # the fused kernel on the same frames (achip_pass, one launch per frame)
+        wf, mf = time_precision(nets, srcs, dsts, order, main_prec, args.steps, args.warmup, None, stream)
+        achf = FLOP_PER_INPUT_PX * W_IN * H_IN / mf / 1e12
+        result[main_prec + "_fused"] = {
+            "value": round(out_px_per_step * args.steps / 1e6 / wf, 1), "unit": "Mpix/s",
+            "ms_per_step": round(wf * 1e3 / args.steps, 4), "mean_launch_us": round(mf * 1e6, 2),
+            "achieved_tflops": round(achf, 2), "frac": round(achf / peak, 4),
+            "traffic": MEASURED_TRAFFIC.get(main_prec), "kernel_config": kcfg,
+            "form": "one fused kernel per frame, 64x32 tiles with an 8-pixel recomputed halo"}
     for prec in precs[1:]:
         w2, ml2 = timings[prec]
         v2 = out_px_per_step * args.steps / 1e6 / w2

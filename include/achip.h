@@ -96,14 +96,30 @@ ACHIP_API int achip_pass_batch(ACHipNet* net, int precision, int config, int src
                                const ACHipImage* images, int n, void* workspace, size_t workspace_bytes,
                                void* stream);
 
-/* ACNetLegacy's FP32 2x pass layer by layer instead of fused: layer 0, then layers 1..8 two
- * per launch through 8-channel f32 maps in `workspace` (achip_pass8_workspace_bytes(w, h)
- * bytes, 256-byte aligned, device memory of the net's device, not overlapping src or dst):
- * five launches, each map through HBM once, a one-pixel recompute ring per launch instead of
- * the fused kernel's eight-pixel halo; results within the FP32 tolerance of the fused pass.
- * The reference's CPU pass runs the layers this way (CPUProcessor.cpp:1371-1390). */
+/* ACNetLegacy's FP32 2x pass layer by layer instead of fused: layers 0-2 in one launch, then
+ * layers 3..8 two per launch through 8-channel f32 maps in `workspace`
+ * (achip_pass8_workspace_bytes(w, h) bytes, 256-byte aligned, device memory of the net's
+ * device, not overlapping src or dst): four launches, each map through HBM once, a one-pixel
+ * recompute ring per layer instead of the fused kernel's eight-pixel halo.  Every layer runs
+ * the fused FP32 kernel's instructions on the same operands, so the results are its bits.
+ * The reference's CPU pass runs the layers one by one (CPUProcessor.cpp:1371-1390). */
 ACHIP_API int achip_pass_layers(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h, void* dst,
                                 long dst_stride, int dst_type, void* workspace, size_t workspace_bytes, void* stream);
+
+/* The 2x pass in the faster of its forms for the image's size, given a workspace.  Replaces
+ * achip_pass where the caller can lend device memory (the processor's per-thread buffers do;
+ * the reference's processors run one form, CUDAProcessor.cpp:383-422).  For FP32 ACNetLegacy
+ * passes the forms are the fused kernel and the layer-split form of achip_pass_layers, which
+ * give the same bits; the choice is the autotuner's for the size class (achip_autotune times
+ * both), else a rule measured on MI355X: the fused kernel where its 64 x 32 tiles fit in one
+ * round of the device's compute units on an image of more than 256 x 256 pixels, the
+ * layer-split form elsewhere (1080p u8: 299 against 328 us).  achip_pass_ws_bytes gives the
+ * workspace bytes the chosen form needs, 0 for the fused kernel and every other net or
+ * precision; with a smaller workspace (or none) the pass runs the fused kernel, as achip_pass
+ * does.  Workspace rules as for achip_pass_layers. */
+ACHIP_API size_t achip_pass_ws_bytes(ACHipNet* net, int precision, int src_type, int dst_type, int w, int h);
+ACHIP_API int achip_pass_ws(ACHipNet* net, int precision, const void* src, long src_stride, int src_type, int w, int h,
+                            void* dst, long dst_stride, int dst_type, void* workspace, size_t workspace_bytes, void* stream);
 
 /* achip_pass_batch without a workspace argument and without any allocation, host or device.
  * No counterpart in the reference: it is for callers that cannot allocate (code under HIP
@@ -130,7 +146,9 @@ ACHIP_API int achip_config_info(int precision, int i, int* tw, int* th, int* nt,
  * of `precision` on w x h images (src_type -> dst_type) on the net's device, on temporary
  * buffers, a few launches each, and remember the fastest for that size class (log2 of
  * width and height).  Fills *best and, if ms is not NULL, the mean launch time of each
- * configuration (achip_config_count(precision) floats). */
+ * configuration (achip_config_count(precision) floats).  For an FP32 ACNetLegacy net it also
+ * times the layer-split form and remembers for achip_pass_ws whether it beat the fastest
+ * configuration. */
 ACHIP_API int achip_autotune(ACHipNet* net, int precision, int src_type, int dst_type, int w, int h,
                              int* best, float* ms);
 

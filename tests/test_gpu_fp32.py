@@ -194,10 +194,12 @@ def test_invalid_arguments_launch_nothing(nets, cuda):
 
 
 @pytest.mark.parametrize("variant", ["gan", "hdn0", "hdn3"])
-@pytest.mark.parametrize("w,h", [(1, 1), (5, 3), (63, 17), (130, 71)])
+@pytest.mark.parametrize("w,h", [(1, 1), (5, 3), (63, 17), (130, 71), (62, 16), (64, 18), (125, 33)])
 def test_layer_split_form(nets, weights, variant, w, h):
-    """achip_pass_layers (layer 0, then layers 1..8 two per launch through f32 maps in a
-    workspace): f32 within 1e-5 and u8 within the u8 bar of the oracle."""
+    """achip_pass_layers (layers 0-2 in one launch, then layers 3..8 two per launch through
+    f32 maps in a workspace): the fused MFMA kernel's bits in both of its configurations (the
+    same instructions on the same operands), and so f32 within 1e-5 and u8 within the u8 bar
+    of the oracle."""
     import ctypes
     from anime4kcpp_amd import _native
     L = _native.lib()
@@ -216,11 +218,14 @@ def test_layer_split_form(nets, weights, variant, w, h):
                                  L.achip_pass8_workspace_bytes(w, h), None)
         assert rc == 0, _native.last_error()
         torch.cuda.synchronize()
+        got = to_host(d)
+        for config in (0, 1):
+            assert np.array_equal(got, gpu_pass(net, src, dt, config=config)), (dt, config)
         want = oracle_pass(src, k, b)
         if dt == np.float32:
-            check_f32(to_host(d), want)
+            check_f32(got, want)
         else:
-            check_int(to_host(d), want)
+            check_int(got, want)
 
 
 def test_layer_split_1080p_u8(nets, weights):
@@ -236,7 +241,9 @@ def test_layer_split_1080p_u8(nets, weights):
     assert L.achip_pass_layers(net.handle, ctypes.c_void_p(s.data_ptr()), 1920, 0x001, 1920, 1080, ctypes.c_void_p(d.data_ptr()),
                                3840, 0x001, ctypes.c_void_p(ws.data_ptr()), ws.numel(), None) == 0
     torch.cuda.synchronize()
-    check_int(d.cpu().numpy(), oracle_pass(src, k, b))
+    got = d.cpu().numpy()
+    assert np.array_equal(got, gpu_pass(net, src, np.uint8, config=0))
+    check_int(got, oracle_pass(src, k, b))
     # the workspace may not overlap src or dst, and an ACNet<8> net is refused
     assert L.achip_pass_layers(net.handle, ctypes.c_void_p(s.data_ptr()), 1920, 0x001, 1920, 1080, ctypes.c_void_p(d.data_ptr()),
                                3840, 0x001, ctypes.c_void_p(d.data_ptr()), ws.numel(), None) == -22
@@ -244,3 +251,25 @@ def test_layer_split_1080p_u8(nets, weights):
     assert L.achip_pass_layers(n8.handle, ctypes.c_void_p(s.data_ptr()), 1920, 0x001, 1920, 1080, ctypes.c_void_p(d.data_ptr()),
                                3840, 0x001, ctypes.c_void_p(ws.data_ptr()), ws.numel(), None) == -22
     n8.close()
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (130, 71), (256, 256), (960, 540), (1280, 720)])
+def test_pass_ws_picks_a_form_with_the_fused_bits(nets, weights, w, h):
+    """achip_pass_ws runs the layer-split form where the size rule prefers it (a workspace of
+    achip_pass_ws_bytes) and the fused kernel elsewhere, or when the workspace is short: the
+    output is the fused kernel's, bit for bit, and within the u8 bar of the oracle."""
+    net = nets["hdn2"]
+    src = synth.flat_shaded(300 + w, w, h, dtype=np.uint8)
+    s = torch.from_numpy(src).cuda()
+    need = net.workspace_bytes(w, h)
+    tiles = ((w + 63) // 64) * ((h + 31) // 32)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    assert (need > 0) == (not (tiles <= cus and w * h > 256 * 256)), (need, tiles, cus)
+    fused = net.upscale(s, config=0)
+    for ws_bytes in (need, need // 2, 0):
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device="cuda")
+        got = net.upscale(s, workspace=ws if ws_bytes else None)
+        torch.cuda.synchronize()
+        assert torch.equal(got, fused), (w, h, ws_bytes)
+    assert net.workspace_bytes(w, h, precision="fp16") == 0
+    check_int(fused.cpu().numpy(), oracle_pass(src, *weights["hdn2"]))
