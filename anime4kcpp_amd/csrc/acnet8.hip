@@ -55,14 +55,14 @@ __global__ __launch_bounds__(256) void acnet8_l0(const void* __restrict__ src, l
     px[1] = make_float4(o[4], o[5], o[6], o[7]);
 }
 
-// ---- L0 and the first block in one launch (62 x 16 outputs, 256 threads) ----
+// ---- L0 and the first block in one launch (62 x 16 outputs, kP8NT threads) ----
 // The source tile (66 x 20 pixels from (x0 - 2, y0 - 2), clamped, as f32) gives L0 at plane
 // pixels (1..64, 1..18) with l0_pixel8's arithmetic; the first block then runs as conv8_pair's
 // second layer, so the map between them never goes through HBM.
 constexpr int kH8SW = kP8TW + 4, kH8SH = kP8TH + 4;
 
 template <typename TIN>
-__global__ __launch_bounds__(256) void conv8_head(const void* __restrict__ src, long ss, int W, int H,
+__global__ __launch_bounds__(kP8NT) void conv8_head(const void* __restrict__ src, long ss, int W, int H,
                                                   float4* __restrict__ fout, long fp, const float* __restrict__ wts,
                                                   const LayerArgs la)
 {
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void conv8_head(const void* __restrict__ src, 
     const int tid = threadIdx.x, lane = tid & 63;
     const int half = (lane >> 4) & 1;
     // img pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r), as plane pixel (c, r)
-    for (int e = tid; e < kH8SH * kH8SW; e += 256) {
+    for (int e = tid; e < kH8SH * kH8SW; e += kP8NT) {
         const int r = e / kH8SW, c = e - r * kH8SW;
         img[e] = to_float<TIN>(row_ptr<TIN>(src, ss, clampi(y0 - 2 + r, 0, H - 1))[clampi(x0 - 2 + c, 0, W - 1)]);
     }
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void conv8_head(const void* __restrict__ src, 
     Epi8<kPrelu, float, float> epi;
     epi.load(la, lane, half);
     __syncthreads();
-    for (int e = tid; e < (kP8TH + 2) * 64; e += 256) {
+    for (int e = tid; e < (kP8TH + 2) * 64; e += kP8NT) {
         const int r = 1 + e / 64, c = 1 + (e & 63);
         float q[9];
 #pragma unroll
@@ -108,7 +108,7 @@ template <typename TIN, typename TOUT>
 static hipError_t acnet8_pass_t(int blocks, const void* src, long ss, int W, int H, void* dst, long ds,
                                 const float* wts, const float* wa, float4* f0, float4* f1, long fp, hipStream_t stream)
 {
-    const dim3 pgrid = conv8_pair_grid(W, H), block(256);
+    const dim3 pgrid = conv8_pair_grid(W, H), block(kP8NT);
     auto block_args = [&](int l) {
         LayerArgs la{};
         la.wa = wa + static_cast<long>(l - 1) * kMfma32LayerFloats;
@@ -154,7 +154,7 @@ long acnet8_feature_pitch(int w) { return (static_cast<long>(w) * 32 + 255) / 25
 // three MFMAs per item from the bias; layers 1-8: item24; the deconv epilogue), so this
 // form gives the fused pass's bits.
 //
-// legacy_head, 62 x 16 outputs per 256-thread workgroup: the source tile, 82 x 22 pixels from
+// legacy_head, 62 x 16 outputs per workgroup of kP8NT threads: the source tile, 82 x 22 pixels from
 // (x0 - 3, y0 - 3), clamped, in the source type, gives layer 0 at plane pixels (0..79, 0..19)
 // (five 16-pixel segments x 10 row pairs; plane pixel (c, r) is image pixel
 // (x0 - 2 + c, y0 - 2 + r)); plane pixels outside the image then take the nearest pixel
@@ -163,7 +163,7 @@ long acnet8_feature_pitch(int w) { return (static_cast<long>(w) * 32 + 255) / 25
 constexpr int kLHW = 16 * 5 + 2, kLHH = kP8PH + 2;
 
 template <typename TIN>
-__global__ __launch_bounds__(256) void legacy_head(const void* __restrict__ src, long ss, int W, int H,
+__global__ __launch_bounds__(kP8NT) void legacy_head(const void* __restrict__ src, long ss, int W, int H,
                                                    float4* __restrict__ fout, long fp, const float* __restrict__ wts,
                                                    const float* __restrict__ wa32, const LayerArgs la0, const LayerArgs la)
 {
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void legacy_head(const void* __restrict__ src,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
     // img pixel (c, r) is image pixel (x0 - 3 + c, y0 - 3 + r)
-    for (int e = tid; e < kLHH * kLHW; e += 256) {
+    for (int e = tid; e < kLHH * kLHW; e += kP8NT) {
         const int r = e / kLHW, c = e - r * kLHW;
         img[e] = row_ptr<TIN>(src, ss, clampi(y0 - 3 + r, 0, H - 1))[clampi(x0 - 3 + c, 0, W - 1)];
     }
@@ -194,8 +194,10 @@ __global__ __launch_bounds__(256) void legacy_head(const void* __restrict__ src,
     // are img row 2 pair + g (image row y0 - 3 + 2 pair + g), columns 16 seg + j + t
     {
         float4* const pw = half ? pb : pa;
-        for (int item = wave; item < 50; item += 4) {
-            const int pair = item / 5, seg = item - 5 * pair;
+#pragma unroll
+        for (int i0 = 0; i0 < (50 + kP8NW - 1) / kP8NW; i0++) {
+            const int item = wave + kP8NW * i0, pair = item / 5, seg = item - 5 * pair;
+            if (item >= 50) break;   // uniform across the wave
             const TIN* p = img + (2 * pair + g) * kLHW + 16 * seg + j;
             f32x4m acc = mma32(a0, to_float<TIN>(p[0]), bias0);
             acc = mma32(a1, to_float<TIN>(p[1]), acc);
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(256) void legacy_head(const void* __restrict__ src,
     __syncthreads();
     // layer 1 reads plane pixels (0..65, 0..19): those outside the image take the nearest inside
     if (x0 - 2 < 0 || y0 - 2 < 0 || x0 + 63 >= W || y0 + 17 >= H) {
-        for (int e = tid; e < kP8PH * 66; e += 256) {
+        for (int e = tid; e < kP8PH * 66; e += kP8NT) {
             const int r = e / 66, c = e - r * 66;
             const int gy = y0 - 2 + r, gx = x0 - 2 + c;
             if (gx >= 0 && gx < W && gy >= 0 && gy < H) continue;
@@ -226,7 +228,7 @@ template <typename TIN, typename TOUT>
 static hipError_t legacy_split_t(const void* src, long ss, int W, int H, void* dst, long ds, const float* wts,
                                  const float* wa32, float4* f0, float4* f1, long fp, hipStream_t stream)
 {
-    const dim3 pgrid = conv8_pair_grid(W, H), block(256);
+    const dim3 pgrid = conv8_pair_grid(W, H), block(kP8NT);
     auto layer = [&](int l) {
         LayerArgs la{};
         la.wa = wa32 + kMfma32L0Floats + static_cast<long>(l - 1) * kMfma32LayerFloats;

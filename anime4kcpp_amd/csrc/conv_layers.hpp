@@ -249,6 +249,8 @@ __global__ __launch_bounds__(C::NT) void conv8_layer(const float4* __restrict__ 
 // outside the image with the nearest pixel inside, so layer B sees the replicate padding
 // every layer of the reference applies (Common.hpp:120-143).
 constexpr int kP8TW = 62, kP8TH = 16, kP8PW = 80, kP8PH = kP8TH + 4;
+// threads per two-layer workgroup, its waves, and layer A's items per wave
+constexpr int kP8NT = 256, kP8NW = kP8NT / 64, kP8AI = (36 + kP8NW - 1) / kP8NW;
 
 // The second layer of a two-layer launch, once layer A fills plane pixels (1..64, 1..18)
 // of pa/pb: the replicate ring, then layer B (A operands a, epilogue epi) over the 62 x 16 tile.
@@ -262,7 +264,7 @@ __device__ __forceinline__ void pair8_second(float4* __restrict__ pa, float4* __
     const int lane_rd = g * kP8PW + j;
     // replicate ring: A's pixels outside the image take the nearest pixel inside
     if (x0 - 1 < 0 || y0 - 1 < 0 || x0 + kP8TW >= W || y0 + kP8TH >= H) {
-        for (int e = tid; e < 18 * 64; e += 256) {
+        for (int e = tid; e < 18 * 64; e += kP8NT) {
             const int r = 1 + e / 64, c = 1 + (e & 63);
             const int gy = y0 - 2 + r, gx = x0 - 2 + c;
             if (gx >= 0 && gx < W && gy >= 0 && gy < H) continue;
@@ -272,12 +274,13 @@ __device__ __forceinline__ void pair8_second(float4* __restrict__ pa, float4* __
         }
         __syncthreads();
     }
-    // layer B over the 62 x 16 tile: 32 items, 8 per wave; patch rows 2 pair + 1 + g,
+    // layer B over the 62 x 16 tile: 32 items dealt round-robin to the waves; patch rows 2 pair + 1 + g,
     // columns 16 seg + j + t + 1 (columns past A's region feed only lanes past the tile)
     const int gx1 = min(x0 + kP8TW, W), gy1 = min(y0 + kP8TH, H);
-#pragma unroll 2
-    for (int item = wave; item < 32; item += 4) {
-        const int pair = item >> 2, seg = item & 3;
+    static_assert(32 % kP8NW == 0, "layer B's items split evenly over the waves");
+#pragma unroll
+    for (int ib = 0; ib < 32 / kP8NW; ib++) {   // unrolled: the next item's LDS reads issue early
+        const int item = wave + kP8NW * ib, pair = item >> 2, seg = item & 3;
         const f32x4m acc = item24(load_item(pa, pb, (2 * pair + 1) * kP8PW + 16 * seg + 1 + lane_rd), a, epi.bias);
         const int x = x0 + 16 * seg + j, y = y0 + 2 * pair + ro;
         epi.apply(acc, x, y, x < gx1 && y < gy1, half, fp, fout, la);
@@ -286,7 +289,7 @@ __device__ __forceinline__ void pair8_second(float4* __restrict__ pa, float4* __
 
 
 // Layer A of a two-layer launch: its input at plane pixels (0..65, 0..19) of pa/pb, its
-// output written back to (1..64, 1..18) -- 36 items, 9 per wave; item (pair, seg) reads
+// output written back to (1..64, 1..18) -- 36 items, round-robin over the waves; item (pair, seg) reads
 // patch rows 2 pair + g, columns 16 seg + j + t.  Takes layer A's A operands in `a` and its
 // epilogue in epa; returns with layer B's (la) in `a` and epi, loaded while the other waves
 // finish layer A.
@@ -299,16 +302,18 @@ __device__ __forceinline__ void pair8_first(float4* __restrict__ pa, float4* __r
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
     const int lane_rd = g * kP8PW + j;
-    float4 res[9];
+    float4 res[kP8AI];
 #pragma unroll
-    for (int ii = 0; ii < 9; ii++) {
-        const int item = wave + 4 * ii, pair = item >> 2, seg = item & 3;
-        const f32x4m acc = item24(load_item(pa, pb, 2 * pair * kP8PW + 16 * seg + lane_rd), a, epa.bias);
-        if constexpr (MODE_A == kPrelu)
-            res[ii] = make_float4(prelu(acc[0], epa.alpha[0]), prelu(acc[1], epa.alpha[1]), prelu(acc[2], epa.alpha[2]),
-                                  prelu(acc[3], epa.alpha[3]));
-        else
-            res[ii] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+    for (int ii = 0; ii < kP8AI; ii++) {
+        const int item = wave + kP8NW * ii, pair = item >> 2, seg = item & 3;
+        if (36 % kP8NW == 0 || item < 36) {   // uniform across the wave
+            const f32x4m acc = item24(load_item(pa, pb, 2 * pair * kP8PW + 16 * seg + lane_rd), a, epa.bias);
+            if constexpr (MODE_A == kPrelu)
+                res[ii] = make_float4(prelu(acc[0], epa.alpha[0]), prelu(acc[1], epa.alpha[1]), prelu(acc[2], epa.alpha[2]),
+                                      prelu(acc[3], epa.alpha[3]));
+            else
+                res[ii] = make_float4(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f), fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+        }
     }
 #pragma unroll
     for (int s = 0; s < 24; s++) a[s] = la.wa[s * 64 + lane];
@@ -317,16 +322,16 @@ __device__ __forceinline__ void pair8_first(float4* __restrict__ pa, float4* __r
     {
         float4* const pw = half ? pb : pa;
 #pragma unroll
-        for (int ii = 0; ii < 9; ii++) {
-            const int item = wave + 4 * ii, pair = item >> 2, seg = item & 3;
-            pw[(1 + 2 * pair + ro) * kP8PW + 1 + 16 * seg + j] = res[ii];
+        for (int ii = 0; ii < kP8AI; ii++) {
+            const int item = wave + kP8NW * ii, pair = item >> 2, seg = item & 3;
+            if (36 % kP8NW == 0 || item < 36) pw[(1 + 2 * pair + ro) * kP8PW + 1 + 16 * seg + j] = res[ii];
         }
     }
     __syncthreads();
 }
 
 template <int MODE, typename TIN, typename TOUT, int MODE_A = kPrelu>
-__global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
+__global__ __launch_bounds__(kP8NT) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
                                                   float4* __restrict__ fout, const LayerArgs la0, const LayerArgs la)
 {
     __shared__ float4 pa[kP8PW * kP8PH];
@@ -340,7 +345,7 @@ __global__ __launch_bounds__(256) void conv8_pair(const float4* __restrict__ fin
     // plane pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r); stage 66 x 20, clamped
     {
         float4* const pl[2] = { pa, pb };
-        stage_map<256, kP8TW + 4, kP8PH, 2, kP8PW>(fin, fp, W, H, x0 - 2, y0 - 2, pl);
+        stage_map<kP8NT, kP8TW + 4, kP8PH, 2, kP8PW>(fin, fp, W, H, x0 - 2, y0 - 2, pl);
     }
     float a[24];
 #pragma unroll

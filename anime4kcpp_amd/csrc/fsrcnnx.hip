@@ -108,10 +108,10 @@ static hipError_t fsrcnnx_pass_t(int blocks, const void* src, long ss, int W, in
     if constexpr (F == 8) {   // two layers per launch where the first of them is a plain PReLU block
         for (; l + 1 <= blocks; l += 2) {
             if (l + 1 < blocks)
-                hipLaunchKernelGGL((conv8_pair<kPrelu, float, float>), conv8_pair_grid(W, H), dim3(256), 0, stream, in, fp, W,
+                hipLaunchKernelGGL((conv8_pair<kPrelu, float, float>), conv8_pair_grid(W, H), dim3(kP8NT), 0, stream, in, fp, W,
                                    H, out, body_args(l), body_args(l + 1));
             else
-                hipLaunchKernelGGL((conv8_pair<kPreluConv1x1, float, float>), conv8_pair_grid(W, H), dim3(256), 0, stream, in,
+                hipLaunchKernelGGL((conv8_pair<kPreluConv1x1, float, float>), conv8_pair_grid(W, H), dim3(kP8NT), 0, stream, in,
                                    fp, W, H, out, body_args(l), body_args(l + 1));
             advance();
         }
