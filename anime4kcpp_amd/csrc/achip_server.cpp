@@ -183,6 +183,7 @@ struct ACHipServer {
     int step_next = 0;
     void* ws_dev[kRing] = {};
     void* ws_host[kRing] = {};
+    void* ws_host_dev[kRing] = {};   // ws_host as the device sees it
     size_t ws_bytes = 0;
 
     // statistics (under mtx)
@@ -229,8 +230,37 @@ size_t request_bytes(const ACHipRequest& r, const ACHipFrameFormat& fmt, int typ
     return b;
 }
 
-int copy_plane(void* dst, long dstride, const void* src, long sstride, long row_bytes, int rows, hipStream_t st)
+// p itself when [p, p + bytes) is page-locked host memory that the device sees at the same
+// addresses (both ends checked), else nullptr (pageable or device memory, or another mapping).
+void* host_mapping(const void* p, size_t bytes)
 {
+    const char* ends[2] = { static_cast<const char*>(p), static_cast<const char*>(p) + bytes - 1 };
+    for (const char* q : ends) {
+        hipPointerAttribute_t a{};
+        if (hipPointerGetAttributes(&a, q) != hipSuccess) {
+            (void)hipGetLastError();   // pageable memory the runtime has never seen
+            return nullptr;
+        }
+        if (a.type != hipMemoryTypeHost || a.devicePointer != q) return nullptr;
+    }
+    return const_cast<void*>(p);
+}
+
+// With `by_kernel`, page-locked host planes move by a kernel through their device mapping, on
+// the stream like any other launch.  Frames use it: the runtime's copy calls onto the frame
+// streams blocked the scheduler thread for 6-7 ms at a time, now and then, with the device
+// idle meanwhile (one call in several hundred; none of the one-off uploads did).  One-offs
+// keep the copy engines, which move their large planes without holding CUs.  Pageable and
+// device planes always take the runtime's copies.
+int copy_plane(void* dst, long dstride, const void* src, long sstride, long row_bytes, int rows, hipStream_t st,
+               bool by_kernel)
+{
+    if (rows <= 0 || row_bytes <= 0) return ACHIP_OK;
+    const void* ms = by_kernel ? host_mapping(src, static_cast<size_t>(rows - 1) * sstride + row_bytes) : nullptr;
+    void* md = by_kernel && !ms ? host_mapping(dst, static_cast<size_t>(rows - 1) * dstride + row_bytes) : nullptr;
+    if (ms || md)
+        return achip::launch_copy_rows(ms ? ms : src, sstride, md ? md : dst, dstride, row_bytes, rows, st) == hipSuccess
+                   ? ACHIP_OK : ACHIP_EIO;
     hipError_t e;
     if (dstride == row_bytes && sstride == row_bytes)   // dense planes: one linear copy
         e = hipMemcpyAsync(dst, src, static_cast<size_t>(row_bytes) * rows, hipMemcpyDefault, st);
@@ -420,9 +450,9 @@ int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
             return e;
         }
     }
-    int e = copy_plane(s.y0.ptr, s.y0.stride, r.src[0], r.src_stride[0], s.y0.row(), s.h, s.stream);
+    int e = copy_plane(s.y0.ptr, s.y0.stride, r.src[0], r.src_stride[0], s.y0.row(), s.h, s.stream, true);
     for (int i = 0; i < c.n && !e; i++)
-        e = copy_plane(s.c0[i].ptr, s.c0[i].stride, r.src[1 + i], r.src_stride[1 + i], s.c0[i].row(), s.c0[i].h, s.stream);
+        e = copy_plane(s.c0[i].ptr, s.c0[i].stride, r.src[1 + i], r.src_stride[1 + i], s.c0[i].row(), s.c0[i].h, s.stream, true);
     if (e) return e;
     if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return ACHIP_EIO;
     {
@@ -430,9 +460,9 @@ int ACHipServer::start_frame(const std::shared_ptr<Req>& q)
         replays++;
     }
     if (r.dst[0]) {
-        e = copy_plane(r.dst[0], r.dst_stride[0], s.y1.ptr, s.y1.stride, s.y1.row(), s.y1.h, s.stream);
+        e = copy_plane(r.dst[0], r.dst_stride[0], s.y1.ptr, s.y1.stride, s.y1.row(), s.y1.h, s.stream, true);
         for (int i = 0; i < c.n && !e; i++)
-            e = copy_plane(r.dst[1 + i], r.dst_stride[1 + i], s.c1[i].ptr, s.c1[i].stride, s.c1[i].row(), s.c1[i].h, s.stream);
+            e = copy_plane(r.dst[1 + i], r.dst_stride[1 + i], s.c1[i].ptr, s.c1[i].stride, s.c1[i].row(), s.c1[i].h, s.stream, true);
         if (e) return e;
     }
     if (hipEventCreateWithFlags(&q->ev_done, hipEventDisableTiming) != hipSuccess) return ACHIP_EIO;
@@ -461,10 +491,10 @@ int ACHipServer::start_oneoff(const std::shared_ptr<Req>& q)
     }
     if (!e && layered) e = alloc_ws(q->ws, W << (P - 1), H << (P - 1));   // the largest pass
     if (e) return e;
-    if (r.kind == ACHIP_REQ_RGB) e = copy_plane(q->in.ptr, q->in.stride, r.src[0], r.src_stride[0], q->in.row(), H, upload);
-    else e = copy_plane(q->luma[0].ptr, q->luma[0].stride, r.src[0], r.src_stride[0], q->luma[0].row(), H, upload);
+    if (r.kind == ACHIP_REQ_RGB) e = copy_plane(q->in.ptr, q->in.stride, r.src[0], r.src_stride[0], q->in.row(), H, upload, false);
+    else e = copy_plane(q->luma[0].ptr, q->luma[0].stride, r.src[0], r.src_stride[0], q->luma[0].row(), H, upload, false);
     for (int i = 0; i < c.n && !e; i++)
-        e = copy_plane(q->c0[i].ptr, q->c0[i].stride, r.src[1 + i], r.src_stride[1 + i], q->c0[i].row(), c.h, upload);
+        e = copy_plane(q->c0[i].ptr, q->c0[i].stride, r.src[1 + i], r.src_stride[1 + i], q->c0[i].row(), c.h, upload, false);
     if (e) return e;
     if (hipEventCreateWithFlags(&q->ev_up, hipEventDisableTiming) != hipSuccess) return ACHIP_EIO;
     if (hipEventRecord(q->ev_up, upload) != hipSuccess) return ACHIP_EIO;
@@ -527,7 +557,8 @@ bool ACHipServer::build_step()
     const size_t bytes = 256 + static_cast<size_t>(n + n16) * sizeof(achip::BatchDesc);
     char* ws = static_cast<char*>(ws_dev[step_next]);
     const auto* ddesc = reinterpret_cast<const achip::BatchDesc*>(ws + 256);
-    hipError_t e = hipMemcpyAsync(ws, hdr, bytes, hipMemcpyHostToDevice, batch);
+    hipError_t e = achip::launch_copy_rows(ws_host_dev[step_next], static_cast<long>(bytes), ws, static_cast<long>(bytes),
+                                           static_cast<long>(bytes), 1, batch);
     if (e == hipSuccess && n)
         e = achip::launch_batch(net, cfg.precision, config, kU8, kU8, ddesc, n, static_cast<int>(total), reinterpret_cast<int*>(ws),
                                 cfg.max_workgroups, batch);
@@ -648,11 +679,11 @@ void ACHipServer::poll()
             }
             int e = ACHIP_OK;
             const Plane& y = q->luma[q->passes];
-            if (r.kind == ACHIP_REQ_RGB) e = copy_plane(r.dst[0], r.dst_stride[0], q->out.ptr, q->out.stride, q->out.row(), q->out.h, download);
-            else e = copy_plane(r.dst[0], r.dst_stride[0], y.ptr, y.stride, y.row(), y.h, download);
+            if (r.kind == ACHIP_REQ_RGB) e = copy_plane(r.dst[0], r.dst_stride[0], q->out.ptr, q->out.stride, q->out.row(), q->out.h, download, false);
+            else e = copy_plane(r.dst[0], r.dst_stride[0], y.ptr, y.stride, y.row(), y.h, download, false);
             const int nc = r.kind == ACHIP_REQ_RGB ? 0 : chroma_of(q->fmt, r.width, r.height).n;
             for (int i = 0; i < nc && !e; i++)
-                e = copy_plane(r.dst[1 + i], r.dst_stride[1 + i], q->c1[i].ptr, q->c1[i].stride, q->c1[i].row(), q->c1[i].h, download);
+                e = copy_plane(r.dst[1 + i], r.dst_stride[1 + i], q->c1[i].ptr, q->c1[i].stride, q->c1[i].row(), q->c1[i].h, download, false);
             if (!e && hipEventCreateWithFlags(&q->ev_done, hipEventDisableTiming) != hipSuccess) e = ACHIP_EIO;
             if (!e) {
                 q->own_done_event = true;
@@ -775,6 +806,7 @@ int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipSer
     s->ws_bytes = 256 + static_cast<size_t>(kMaxStepItems) * sizeof(achip::BatchDesc);
     for (int i = 0; i < kRing && ok; i++) {
         ok = hipMalloc(&s->ws_dev[i], s->ws_bytes) == hipSuccess && hipHostMalloc(&s->ws_host[i], s->ws_bytes, 0) == hipSuccess &&
+             hipHostGetDevicePointer(&s->ws_host_dev[i], s->ws_host[i], 0) == hipSuccess &&
              hipEventCreateWithFlags(&s->steps[i].ev, hipEventDisableTiming) == hipSuccess;
     }
     (void)hipSetDevice(dev);

@@ -2,6 +2,8 @@
 // (core/src/ImageProcess.cpp:310-797): type conversion (astype, copy), integer shifts
 // (shl, shr), pixelShuffle and per-pixel byte runs (extract, insert).  All are one pass
 // over HBM; a thread handles one element (or one pixel's run), rows in blockIdx.y.
+#include <algorithm>
+
 #include "acnet_device.hpp"
 #include "acnet_kernels.hpp"
 
@@ -83,6 +85,37 @@ __global__ __launch_bounds__(256) void copy_runs_kernel(const unsigned char* __r
     }
 }
 
+// rows x units elements of T (the widest unit that every address, stride and row length is a
+// multiple of); either side may be page-locked host memory seen through its device mapping.
+// A small grid walks the elements with four loads in flight per thread: over PCIe the copy
+// is latency-bound, so few workgroups keep the link busy and leave the CUs to the passes.
+template <typename T>
+__global__ __launch_bounds__(256) void copy_rows_kernel(const unsigned char* __restrict__ src, long ss,
+                                                       unsigned char* __restrict__ dst, long ds, unsigned units,
+                                                       unsigned total)
+{
+    const unsigned stride = gridDim.x * 256u;
+    for (unsigned i0 = blockIdx.x * 256u + threadIdx.x; i0 < total; i0 += 4u * stride) {
+        T v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const unsigned i = i0 + k * stride;
+            if (i < total) {
+                const unsigned y = i / units, u = i - y * units;
+                v[k] = reinterpret_cast<const T*>(src + static_cast<long>(y) * ss)[u];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const unsigned i = i0 + k * stride;
+            if (i < total) {
+                const unsigned y = i / units, u = i - y * units;
+                reinterpret_cast<T*>(dst + static_cast<long>(y) * ds)[u] = v[k];
+            }
+        }
+    }
+}
+
 // ---- launchers ----------------------------------------------------------------------------
 
 // rows beyond the grid's y extent are strided over inside the kernels
@@ -160,6 +193,39 @@ hipError_t launch_pixel_shuffle(int stype, int dtype, const void* src, long ss, 
     case F32: return shuffle_to<float>(dtype, src, ss, sc, up, w, h, dc, dst, ds, stream);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_copy_rows(const void* src, long ss, void* dst, long ds, long row_bytes, int rows, hipStream_t stream)
+{
+    if (row_bytes <= 0 || rows <= 0) return hipSuccess;
+    const auto* s8 = static_cast<const unsigned char*>(src);
+    auto* d8 = static_cast<unsigned char*>(dst);
+    if (ss == row_bytes && ds == row_bytes) {   // dense planes: one row, so wide units fit more often
+        row_bytes *= rows;
+        rows = 1;
+        ss = ds = row_bytes;
+    }
+    if (static_cast<unsigned long>(row_bytes) * rows >= (1ul << 31)) return hipErrorInvalidValue;   // keeps the unsigned index walk from wrapping
+    auto go = [&](auto kernel, long unit, const unsigned char* s0, unsigned char* d0, long bytes) {
+        const unsigned units = static_cast<unsigned>(bytes / unit), total = units * static_cast<unsigned>(rows);
+        if (!total) return;
+        const unsigned blocks = std::min<unsigned>(64u, (total + 1023u) / 1024u);
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, stream, s0, ss, d0, ds, units, total);
+    };
+    const unsigned long bases = reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst);
+    const unsigned long steps = static_cast<unsigned long>(ss) | static_cast<unsigned long>(ds);
+    if (rows == 1 && bases % 16 == 0) {   // one row: 16-byte units, then the last bytes
+        const long head = row_bytes / 16 * 16;
+        go(copy_rows_kernel<uint4>, 16, s8, d8, head);
+        if (head < row_bytes) go(copy_rows_kernel<unsigned char>, 1, s8 + head, d8 + head, row_bytes - head);
+    } else if ((bases | steps | static_cast<unsigned long>(row_bytes)) % 16 == 0) {
+        go(copy_rows_kernel<uint4>, 16, s8, d8, row_bytes);
+    } else if ((bases | steps | static_cast<unsigned long>(row_bytes)) % 4 == 0) {
+        go(copy_rows_kernel<unsigned int>, 4, s8, d8, row_bytes);
+    } else {
+        go(copy_rows_kernel<unsigned char>, 1, s8, d8, row_bytes);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_copy_runs(const void* src, long ss, int spx, int soff, void* dst, long ds, int dpx, int doff, int len,

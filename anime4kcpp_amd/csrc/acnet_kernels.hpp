@@ -197,6 +197,9 @@ hipError_t launch_shift(int type, int left, int bits, const void* src, long ss, 
 // dst (sw * up) x (sh * up) with dc channels
 hipError_t launch_pixel_shuffle(int stype, int dtype, const void* src, long ss, int sw, int sh, int sc, int up,
                                 void* dst, long ds, hipStream_t stream);
+// rows x row_bytes bytes from src (rows ss bytes apart) to dst (ds apart) by a kernel; either
+// pointer may be the device mapping of page-locked host memory
+hipError_t launch_copy_rows(const void* src, long ss, void* dst, long ds, long row_bytes, int rows, hipStream_t stream);
 // for every pixel of a w x h grid, copy len bytes from src + y*ss + x*spx + soff to
 // dst + y*ds + x*dpx + doff (channel extract / insert)
 hipError_t launch_copy_runs(const void* src, long ss, int spx, int soff, void* dst, long ds, int dpx, int doff, int len,

@@ -218,3 +218,30 @@ def test_layered_nets_serve(cuda, model, cls, variant):
         with pytest.raises(Exception):
             hip.Server(net, "fp16")                   # ArtCNN and FSRCNNX run in FP32 only
     net.close()
+
+
+def test_page_locked_planes_equal_pageable(net):
+    """Page-locked input and output planes (torch's pinned memory) give the same outputs as
+    pageable ones, for every request of the mix: the frames' planes then move by the library's
+    row copy kernel through their device mapping (16-byte units for these dense planes), the
+    one-offs' by the copy engines, pageable ones by the runtime's staged copies.  One more RGB
+    input is a column slice of a wider pinned buffer (rows 750 bytes apart, 600 used)."""
+    def pinned(a):
+        return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().numpy()
+
+    # plus two frames of a stream whose planes are not whole 16-byte units (a byte tail to copy)
+    reqs = make_requests() + [(hip.REQ_YUV420, yuv(300 + i, 202, 114), 2, 2) for i in range(2)]
+    first_rgb = next(i for i, r in enumerate(reqs) if r[0] == hip.REQ_RGB and r[1][0].shape == (150, 200, 3))
+    wide = torch.zeros((150, 250, 3), dtype=torch.uint8).pin_memory().numpy()
+    wide[:, :200] = reqs[first_rgb][1][0]
+    preqs = [(k, [pinned(p) for p in planes], f, s) for k, planes, f, s in reqs] + [(hip.REQ_RGB, [wide[:, :200]], 2, -1)]
+    with hip.Server(net, "fp32") as srv:
+        want = run(srv, reqs, together=True)
+        outs = [[pinned(o) for o in out_planes(k, p, f)] for k, p, f, _ in preqs]
+        tickets = [srv.submit(k, p, f, s, outs[i]) for i, (k, p, f, s) in enumerate(preqs)]
+        for t in tickets:
+            srv.wait(t, 60000)
+    for i, (a, b) in enumerate(zip(want, outs)):
+        for pa, pb in zip(a, b):
+            assert np.array_equal(pa, pb), f"request {i}: page-locked planes differ from pageable ones"
+    assert np.array_equal(outs[-1][0], want[first_rgb][0])
