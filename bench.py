@@ -265,7 +265,7 @@ def precision_4k_x4(nets, weights, precs, crop=1024):
     return res
 
 
-def serving(net, prec, loads, seconds, seed=5):
+def serving(net, prec, loads, seconds, seed=5, bursts=5):
     """Poisson arrivals at each offered load (requests/s): 60% frames of 8 concurrent 1080p
     YUV420 u8 streams (Y through the network, UV bilinear 2x, hipGraph replays), 40% one-off
     RGB u8 images with sides uniform in [128, 4096] and factor 2 or 4.  Inputs are
@@ -307,11 +307,20 @@ def serving(net, prec, loads, seconds, seed=5):
     # kernels loaded, and the process its copy engines set up, before the loads below
     burst(120)
     srv.stats(reset=True)
-    # capacity: the same mix submitted all at once
-    burst(120)
-    st = srv.stats(reset=True)
-    out["saturated"] = {"requests": 120, "output_mpix_per_s": round(st["output_mpix"] / st["busy_s"], 1),
-                        "requests_per_s": round(120 / st["busy_s"], 1), "steps": st["steps"]}
+    # capacity: the same mix submitted all at once, `bursts` times (each burst draws its own
+    # 120 requests); the median burst is the figure, the spread beside it
+    rates, req_rates, steps = [], [], 0
+    for _ in range(max(1, bursts)):
+        burst(120)
+        st = srv.stats(reset=True)
+        rates.append(st["output_mpix"] / st["busy_s"])
+        req_rates.append(120 / st["busy_s"])
+        steps += st["steps"]
+    import statistics
+    out["saturated"] = {"requests": 120, "bursts": len(rates), "output_mpix_per_s": round(statistics.median(rates), 1),
+                        "min_mpix_per_s": round(min(rates), 1), "max_mpix_per_s": round(max(rates), 1),
+                        "per_burst_mpix_per_s": [round(r, 1) for r in rates],
+                        "requests_per_s": round(statistics.median(req_rates), 1), "steps": steps}
     for load in loads:
         tickets = []
         lock = threading.Lock()
@@ -363,7 +372,8 @@ def serving(net, prec, loads, seconds, seed=5):
             "steps": st["steps"], "graph_replays": st["graph_replays"], "failed": st["failed"],
         }
     srv.close()
-    out["method"] = "one server per precision; an untimed burst of 120 requests of the mix, then each measurement"
+    out["method"] = ("one server per precision; an untimed burst of 120 requests of the mix, then the saturated bursts "
+                     "(median), then each offered load")
     return out
 
 
@@ -547,6 +557,7 @@ def main():
     ap.add_argument("--serving-loads", default="fp16:300,1200;fp32:60,240",
                     help="offered loads (requests/s) per precision")
     ap.add_argument("--serving-seconds", type=float, default=3.0)
+    ap.add_argument("--serving-bursts", type=int, default=5, help="saturated bursts of 120 requests (median reported)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -688,12 +699,15 @@ def main():
         result["serving"] = {}
         for spec in args.serving_loads.split(";"):
             p, loads = spec.split(":")
-            result["serving"][p] = serving(nets["hdn0"], p, [float(x) for x in loads.split(",")], args.serving_seconds)
+            result["serving"][p] = serving(nets["hdn0"], p, [float(x) for x in loads.split(",")], args.serving_seconds,
+                                           bursts=args.serving_bursts)
         if not args.no_acnet8:   # the reference's default model (acnet-f8b8-hdn), FP32 and FP16
             from anime4kcpp_amd import hip as _hip
             n8 = _hip.Net8("b8_hdn", 0)
-            result["serving"]["acnet8_b8_hdn_fp32"] = serving(n8, "fp32", [60.0, 240.0], args.serving_seconds)
-            result["serving"]["acnet8_b8_hdn_fp16"] = serving(n8, "fp16", [300.0, 1200.0], args.serving_seconds)
+            result["serving"]["acnet8_b8_hdn_fp32"] = serving(n8, "fp32", [60.0, 240.0], args.serving_seconds,
+                                                              bursts=args.serving_bursts)
+            result["serving"]["acnet8_b8_hdn_fp16"] = serving(n8, "fp16", [300.0, 1200.0], args.serving_seconds,
+                                                              bursts=args.serving_bursts)
             n8.close()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(frames_np, weights)
