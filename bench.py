@@ -216,9 +216,10 @@ def batched_uneven(nets, prec, seed=2024, n=64):
 
 
 def precision_4k_x4(nets, weights, precs, crop=1024):
-    """3840x2160 f32 -> 15360x8640 f32 (two chained 2x passes) per precision: Mpix/s of the
-    two-pass chain, and each pass's error against the FP64 restatement on a crop (interior
-    only: the crop's own borders differ from the full image's)."""
+    """3840x2160 f32 -> 15360x8640 f32 (two chained 2x passes) per precision, each pass the
+    processor's (achip_pass_ws with a workspace: in FP32 the layer-split form at these sizes):
+    Mpix/s of the two-pass chain, and each pass's error against the FP64 restatement on a
+    crop (interior only: the crop's own borders differ from the full image's)."""
     import numpy as np
     import torch
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -234,14 +235,17 @@ def precision_4k_x4(nets, weights, precs, crop=1024):
     out_mpix = out.numel() / 1e6
     res = {}
     for prec in precs:
-        net.upscale(src, mid, precision=prec)
-        net.upscale(mid, out, precision=prec)
+        wsb = max(net.workspace_bytes(3840, 2160, precision=prec, src_dtype=torch.float32),
+                  net.workspace_bytes(7680, 4320, precision=prec, src_dtype=torch.float32))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev) if wsb else None
+        net.upscale(src, mid, precision=prec, workspace=ws)
+        net.upscale(mid, out, precision=prec, workspace=ws)
         torch.cuda.synchronize()
         reps = 3
         t0 = time.perf_counter()
         for _ in range(reps):
-            net.upscale(src, mid, precision=prec)
-            net.upscale(mid, out, precision=prec)
+            net.upscale(src, mid, precision=prec, workspace=ws)
+            net.upscale(mid, out, precision=prec, workspace=ws)
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / reps
         errs = {}
@@ -257,7 +261,9 @@ def precision_4k_x4(nets, weights, precs, crop=1024):
             mse = float((e ** 2).mean())
             errs[f"pass{pi + 1}"] = {"max_abs": float(e.max()), "mean_abs": float(e.mean()),
                                      "psnr_db": round(10 * np.log10(1.0 / mse), 2) if mse > 0 else None}
-        res[prec] = {"ms": round(t * 1e3, 3), "output_mpix_per_s": round(out_mpix / t, 1), "error_vs_fp64": errs}
+        res[prec] = {"ms": round(t * 1e3, 3), "output_mpix_per_s": round(out_mpix / t, 1), "error_vs_fp64": errs,
+                     "form": "layer-split" if wsb else "fused"}
+        del ws
     base = res[precs[0]]["output_mpix_per_s"]
     for prec in precs[1:]:
         res[prec]["vs_" + precs[0]] = round(res[prec]["output_mpix_per_s"] / base, 2)
