@@ -5,9 +5,11 @@ HBM (two frames per weight set: GAN, HDN0, HDN1, HDN2, HDN3), written to residen
 outputs.  Prints one JSON line (rank 0):
   value      whole-job output Mpix/s of the FP32 pass the processor runs (achip_pass_ws with
              a workspace: at 1080p the layer-split form, four launches, the fused kernel's
-             bits) over the timed steps; the fused kernel alone and the FP16-/BF16-MFMA forms
-             are timed on the same frames and reported beside it, the 16-bit forms with their
-             error against the FP64 restatement
+             bits) over the timed steps, the step's frames alternating over two streams as
+             two calling threads would drive the processor; the same on one stream, the fused
+             kernel alone and the FP16-/BF16-MFMA forms are timed on the same frames and
+             reported beside it, the 16-bit forms with their error against the FP64
+             restatement
   roofline   achieved FP32 TFLOP/s of that pass (9424 algorithmic FLOP per input pixel x
              pixels per pass / mean pass time from HIP events on the launch stream) against
              the MI355X FP32 peak
@@ -109,15 +111,22 @@ def cpu_baseline(frames_u8, weights, min_seconds=10.0):
     return out
 
 
-def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, stream, workspace=None):
-    """Time `steps` steps of one precision; returns (wall seconds, mean seconds per pass on
-    the stream).  With a workspace the passes run through achip_pass_ws (the processor's
-    pass: the faster form for the size), else through achip_pass (the fused kernel)."""
+def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, streams, workspaces=None):
+    """Time `steps` steps of one precision; returns (wall seconds, seconds per pass).  Frame n
+    of a step runs on streams[n % len(streams)], each stream with its own workspace, as the
+    processor runs each calling thread's passes on that thread's own stream.  The time per
+    pass is the step's span on the device (HIP events: after every stream's end, less the
+    first stream's start) over its frames.  With workspaces the passes run through
+    achip_pass_ws (the processor's pass: the faster form for the size), else through
+    achip_pass (the fused kernel)."""
     import torch
+    ns = len(streams)
 
     def step():
-        for i, v in order:
-            nets[v].upscale(srcs[i], dsts[i], precision=prec, config=config, stream=stream, workspace=workspace)
+        for n, (i, v) in enumerate(order):
+            k = n % ns
+            nets[v].upscale(srcs[i], dsts[i], precision=prec, config=config, stream=streams[k],
+                            workspace=workspaces[k] if workspaces else None)
 
     for _ in range(warmup):
         step()
@@ -125,10 +134,16 @@ def time_precision(nets, srcs, dsts, order, prec, steps, warmup, config, stream,
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    e0.record(stream)
+    e0.record(streams[0])
+    for st in streams[1:]:
+        st.wait_event(e0)
     for _ in range(steps):
         step()
-    e1.record(stream)
+    for st in streams[1:]:
+        done = torch.cuda.Event()
+        done.record(st)
+        streams[0].wait_event(done)
+    e1.record(streams[0])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall, e0.elapsed_time(e1) / 1e3 / (steps * len(srcs))
@@ -553,6 +568,7 @@ def main():
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 error report")
     ap.add_argument("--precisions", default="fp32,fp16,bf16")
     ap.add_argument("--config", type=int, default=None, help="kernel tile configuration")
+    ap.add_argument("--streams", type=int, default=2, help="streams the step's frames alternate over")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-image batched measurement")
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K f32 x4 precision measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
@@ -597,11 +613,16 @@ def main():
     # the processor's pass (achip_pass_ws): each thread lends it a workspace, and at 1080p FP32
     # it runs the layer-split form (4 launches, the fused kernel's bits); an explicit --config
     # runs that fused configuration
+    # The step's frames alternate over --streams streams (default 2), each with its own
+    # workspace: the way a caller with several threads drives the processor (one stream per
+    # thread), so one frame's launches fill the rounds another's leave idle.
+    streams = [stream] + [torch.cuda.Stream(dev) for _ in range(max(1, args.streams) - 1)]
     ws_bytes = {p: (nets["hdn0"].workspace_bytes(W_IN, H_IN, precision=p) if args.config is None else 0) for p in precs}
-    workspaces = {p: torch.empty(n, dtype=torch.uint8, device=dev) if n else None for p, n in ws_bytes.items()}
+    workspaces = {p: [torch.empty(n, dtype=torch.uint8, device=dev) for _ in streams] if n else None
+                  for p, n in ws_bytes.items()}
     timings = {}
     for prec in precs:
-        timings[prec] = time_precision(nets, srcs, dsts, order, prec, args.steps, args.warmup, args.config, stream,
+        timings[prec] = time_precision(nets, srcs, dsts, order, prec, args.steps, args.warmup, args.config, streams,
                                        workspaces[prec])
 
     main_prec = precs[0]
@@ -643,11 +664,14 @@ def main():
             "content": "seeded flat-shaded polygons + Bezier strokes + noise (sigma 0.01), resident in HBM",
             "kernel_config": None if split else kcfg,
             "workspace_bytes": ws_bytes[main_prec],
+            "streams": len(streams),
         },
         "roofline": {
             "bound": "mfma",
             "engine": engine,
-            "unit_of_work": "one 2x pass of a 1080p frame" + (" (its 4 launches)" if split else " (one launch)"),
+            "unit_of_work": "one 2x pass of a 1080p frame" + (" (its 4 launches)" if split else " (one launch)")
+                            + (f"; the step's frames alternate over {len(streams)} streams, so passes overlap and the time"
+                               " per pass is the step's device time over its frames" if len(streams) > 1 else ""),
             "achieved": round(achieved, 2),
             "peak": peak,
             "unit": "TFLOP/s",
@@ -660,9 +684,17 @@ def main():
             "flop_per_pass": FLOP_PER_INPUT_PX * W_IN * H_IN,
         },
     }
+    if len(streams) > 1:
+        # the same pass with every frame on one stream (one calling thread)
+        w1, m1 = time_precision(nets, srcs, dsts, order, main_prec, args.steps, args.warmup, args.config, [stream],
+                                workspaces[main_prec][:1] if workspaces[main_prec] else None)
+        a1 = FLOP_PER_INPUT_PX * W_IN * H_IN / m1 / 1e12
+        result["one_stream"] = {"value": round(out_px_per_step * args.steps / 1e6 / w1, 1), "unit": "Mpix/s",
+                                "ms_per_step": round(w1 * 1e3 / args.steps, 4), "mean_pass_us": round(m1 * 1e6, 2),
+                                "achieved_tflops": round(a1, 2), "frac": round(a1 / peak, 4)}
     if split:
-        # the fused kernel on the same frames (achip_pass, one launch per frame)
-        wf, mf = time_precision(nets, srcs, dsts, order, main_prec, args.steps, args.warmup, None, stream)
+        # the fused kernel on the same frames (achip_pass, one launch per frame), same streams
+        wf, mf = time_precision(nets, srcs, dsts, order, main_prec, args.steps, args.warmup, None, streams)
         achf = FLOP_PER_INPUT_PX * W_IN * H_IN / mf / 1e12
         result[main_prec + "_fused"] = {
             "value": round(out_px_per_step * args.steps / 1e6 / wf, 1), "unit": "Mpix/s",
