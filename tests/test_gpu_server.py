@@ -223,18 +223,34 @@ def test_layered_nets_serve(cuda, model, cls, variant):
 def test_page_locked_planes_equal_pageable(net):
     """Page-locked input and output planes (torch's pinned memory) give the same outputs as
     pageable ones, for every request of the mix: the frames' planes then move by the library's
-    row copy kernel through their device mapping (16-byte units for these dense planes), the
-    one-offs' by the copy engines, pageable ones by the runtime's staged copies.  One more RGB
-    input is a column slice of a wider pinned buffer (rows 750 bytes apart, 600 used)."""
+    row copy kernel through their device mapping (dense planes as one row of 16-byte units and
+    a byte tail; strided views in 16- or 4-byte units per row), the one-offs' by the copy
+    engines, pageable ones by the runtime's staged copies.  One more RGB input is a column
+    slice of a wider pinned buffer (rows 750 bytes apart, 600 used)."""
     def pinned(a):
         return torch.from_numpy(np.ascontiguousarray(a)).pin_memory().numpy()
 
     # plus two frames of a stream whose planes are not whole 16-byte units (a byte tail to copy)
     reqs = make_requests() + [(hip.REQ_YUV420, yuv(300 + i, 202, 114), 2, 2) for i in range(2)]
+    # and frames whose page-locked planes are column slices of wider buffers (rows 256 / 128
+    # bytes apart): 16-byte units for 192 / 96-pixel rows, 4-byte units for 200 / 100
+    strided = []
+    for stream, w in ((3, 192), (4, 200)):
+        for i in range(2):
+            planes = yuv(400 + 10 * stream + i, w, 64)
+            views = []
+            for pl in planes:
+                buf = torch.zeros((pl.shape[0], 256 if pl.shape[1] > 128 else 128), dtype=torch.uint8).pin_memory().numpy()
+                buf[:, :pl.shape[1]] = pl
+                views.append(buf[:, :pl.shape[1]])
+            reqs.append((hip.REQ_YUV420, planes, 2, stream))
+            strided.append((len(reqs) - 1, views))
     first_rgb = next(i for i, r in enumerate(reqs) if r[0] == hip.REQ_RGB and r[1][0].shape == (150, 200, 3))
     wide = torch.zeros((150, 250, 3), dtype=torch.uint8).pin_memory().numpy()
     wide[:, :200] = reqs[first_rgb][1][0]
     preqs = [(k, [pinned(p) for p in planes], f, s) for k, planes, f, s in reqs] + [(hip.REQ_RGB, [wide[:, :200]], 2, -1)]
+    for i, views in strided:
+        preqs[i] = (preqs[i][0], views, preqs[i][2], preqs[i][3])
     with hip.Server(net, "fp32") as srv:
         want = run(srv, reqs, together=True)
         outs = [[pinned(o) for o in out_planes(k, p, f)] for k, p, f, _ in preqs]
