@@ -104,6 +104,8 @@ SIGNATURES = {
     "achip_pass_batch": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p]),
     "achip_pass_layers": (c_int, [c_void_p, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
                                   ctypes.c_size_t, c_void_p]),
+    "achip_pass_batch_layers_bytes": (ctypes.c_size_t, [c_void_p, c_int]),
+    "achip_pass_batch_layers": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p]),
     "achip_pass_ws_bytes": (ctypes.c_size_t, [c_void_p, c_int, c_int, c_int, c_int, c_int]),
     "achip_pass_ws": (c_int, [c_void_p, c_int, c_void_p, c_long, c_int, c_int, c_int, c_void_p, c_long, c_int, c_void_p,
                               ctypes.c_size_t, c_void_p]),

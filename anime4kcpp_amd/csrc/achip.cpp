@@ -1074,6 +1074,82 @@ int achip_pass_batch(ACHipNet* net, int precision, int config, int src_type, int
                                static_cast<hipStream_t>(stream));
 }
 
+namespace {
+
+// workspace of achip_pass_batch_layers: the descriptors, then each image's two maps
+size_t split_batch_layout(const ACHipImage* images, int n, std::vector<size_t>* offsets)
+{
+    const size_t header = (static_cast<size_t>(n) * sizeof(achip::SplitDesc) + 255) / 256 * 256;
+    size_t off = header;
+    for (int i = 0; i < n; i++) {
+        if (offsets) offsets->push_back(off);
+        const ACHipImage& im = images[i];
+        if (im.w > 0 && im.h > 0) off += (achip_pass8_workspace_bytes(im.w, im.h) + 255) / 256 * 256;
+    }
+    return off;
+}
+
+} // namespace
+
+size_t achip_pass_batch_layers_bytes(const ACHipImage* images, int n)
+{
+    if (!images || n <= 0 || n > (1 << 24)) return 0;
+    return split_batch_layout(images, n, nullptr);
+}
+
+int achip_pass_batch_layers(ACHipNet* net, int src_type, int dst_type, const ACHipImage* images, int n, void* workspace,
+                            size_t workspace_bytes, void* stream)
+{
+    if (!net) return fail(ACHIP_EINVAL, "net is NULL");
+    if (net->blocks || net->lfamily) return fail(ACHIP_EINVAL, "achip_pass_batch_layers takes an ACNetLegacy net");
+    if (!images || n <= 0 || n > (1 << 24)) return fail(ACHIP_EINVAL, "batch of %d images", n);
+    std::vector<size_t> offsets;
+    const size_t need = split_batch_layout(images, n, &offsets);
+    if (!workspace || workspace_bytes < need)
+        return fail(ACHIP_EINVAL, "workspace of %zu bytes, the batch needs %zu", workspace_bytes, need);
+    if (reinterpret_cast<uintptr_t>(workspace) % 256) return fail(ACHIP_EINVAL, "workspace not 256-byte aligned");
+    int r = check_device_buffer(workspace, need, net->device, "workspace");
+    if (r) return r;
+    std::vector<achip::SplitDesc> desc(static_cast<size_t>(n));
+    std::vector<achip::BatchSpan> spans;
+    spans.reserve(2 * static_cast<size_t>(n) + 1);
+    char* ws = static_cast<char*>(workspace);
+    long total = 0;
+    for (int i = 0; i < n; i++) {
+        const ACHipImage& im = images[i];
+        r = validate(net, im.src, im.src_stride, src_type, im.w, im.h, im.dst, im.dst_stride, dst_type);
+        if (r) {
+            std::string msg = g_error;
+            return fail(r, "image %d: %s", i, msg.c_str());
+        }
+        const uintptr_t s0 = reinterpret_cast<uintptr_t>(im.src), d0 = reinterpret_cast<uintptr_t>(im.dst);
+        spans.push_back({ s0, s0 + static_cast<size_t>(im.h - 1) * im.src_stride + static_cast<size_t>(im.w) * (src_type & 0xff), false, i });
+        spans.push_back({ d0, d0 + static_cast<size_t>(2 * im.h - 1) * im.dst_stride + static_cast<size_t>(2 * im.w) * (dst_type & 0xff), true, i });
+        const long fp = achip::acnet8_feature_pitch(im.w);
+        auto* m0 = reinterpret_cast<float4*>(ws + offsets[i]);
+        auto* m1 = reinterpret_cast<float4*>(ws + offsets[i] + static_cast<size_t>(fp) * im.h);
+        const int tx = (im.w + 61) / 62, ty = (im.h + 15) / 16;
+        desc[i] = achip::SplitDesc{ im.src, im.dst, m0, m1, im.src_stride, im.dst_stride, fp, im.w, im.h, tx, static_cast<int>(total) };
+        total += static_cast<long>(tx) * ty;
+        if (total > (1L << 30)) return fail(ACHIP_EINVAL, "batch too large (%ld tiles)", total);
+    }
+    // the maps and descriptors are written by the launches: no image may overlap them, and no
+    // image's dst another image's src or dst
+    spans.push_back({ reinterpret_cast<uintptr_t>(workspace), reinterpret_cast<uintptr_t>(workspace) + need, true, -1 });
+    r = achip::check_disjoint_writes(spans.data(), static_cast<int>(spans.size()));
+    if (r) return r;
+    DeviceGuard g(net->device);
+    if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    // from pageable memory the copy is staged before the call returns, so `desc` may go
+    hipError_t e = hipMemcpyAsync(workspace, desc.data(), static_cast<size_t>(n) * sizeof(achip::SplitDesc),
+                                  hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = achip::launch_legacy_split_batch(src_type, dst_type, reinterpret_cast<const achip::SplitDesc*>(workspace), n,
+                                             static_cast<int>(total), net->wts, net->wa32, st);
+    return e == hipSuccess ? ACHIP_OK : hip_fail(e, "kernel launch");
+}
+
 const char* achip_last_error(void) { return g_error.c_str(); }
 
 } // extern "C"

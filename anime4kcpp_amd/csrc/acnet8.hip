@@ -163,17 +163,12 @@ long acnet8_feature_pitch(int w) { return (static_cast<long>(w) * 32 + 255) / 25
 constexpr int kLHW = 16 * 5 + 2, kLHH = kP8PH + 2;
 
 template <typename TIN>
-__global__ __launch_bounds__(kP8NT) void legacy_head(const void* __restrict__ src, long ss, int W, int H,
-                                                   float4* __restrict__ fout, long fp, const float* __restrict__ wts,
-                                                   const float* __restrict__ wa32, const LayerArgs la0, const LayerArgs la)
+__device__ __forceinline__ void legacy_head_tile(const void* __restrict__ src, long ss, int W, int H,
+                                                 float4* __restrict__ fout, long fp, const float* __restrict__ wts,
+                                                 const float* __restrict__ wa32, const LayerArgs& la0, const LayerArgs& la,
+                                                 int x0, int y0, float4* __restrict__ pa, float4* __restrict__ pb,
+                                                 TIN* __restrict__ img)
 {
-    __shared__ float4 pa[kP8PW * kP8PH];
-    __shared__ float4 pb[kP8PW * kP8PH];
-    __shared__ TIN img[kLHH * kLHW];
-    const int tiles_x = gridDim.x;
-    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
-    const int ty = t / tiles_x, tx = t - ty * tiles_x;
-    const int x0 = tx * kP8TW, y0 = ty * kP8TH;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int g = lane >> 4, j = lane & 15, half = g & 1, ro = g >> 1;
     // img pixel (c, r) is image pixel (x0 - 3 + c, y0 - 3 + r)
@@ -224,6 +219,57 @@ __global__ __launch_bounds__(kP8NT) void legacy_head(const void* __restrict__ sr
     pair8_second<kRelu, float, float>(pa, pb, W, H, x0, y0, a, epi, fp, fout, la);
 }
 
+
+template <typename TIN>
+__global__ __launch_bounds__(kP8NT) void legacy_head(const void* __restrict__ src, long ss, int W, int H,
+                                                   float4* __restrict__ fout, long fp, const float* __restrict__ wts,
+                                                   const float* __restrict__ wa32, const LayerArgs la0, const LayerArgs la)
+{
+    __shared__ float4 pa[kP8PW * kP8PH];
+    __shared__ float4 pb[kP8PW * kP8PH];
+    __shared__ TIN img[kLHH * kLHW];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    legacy_head_tile<TIN>(src, ss, W, H, fout, fp, wts, wa32, la0, la, tx * kP8TW, ty * kP8TH, pa, pb, img);
+}
+
+// ---- the layer-split form over a batch of images: every launch takes the tiles of all of
+// them (numbered image by image, XCD-aware), each image with its own two maps ----
+template <typename TIN>
+__global__ __launch_bounds__(kP8NT) void legacy_head_batch(const SplitDesc* __restrict__ desc, int n, int total,
+                                                         const float* __restrict__ wts, const float* __restrict__ wa32,
+                                                         const LayerArgs la0, const LayerArgs la)
+{
+    __shared__ float4 pa[kP8PW * kP8PH];
+    __shared__ float4 pb[kP8PW * kP8PH];
+    __shared__ TIN img[kLHH * kLHW];
+    const int t = xcd_tile(blockIdx.x, total);
+    const SplitDesc d = find_tile(desc, n, t);
+    const int local = t - d.first_tile, ty = local / d.tiles_x, tx = local - ty * d.tiles_x;
+    legacy_head_tile<TIN>(d.src, d.sstride, d.w, d.h, d.m0, d.fp, wts, wa32, la0, la, tx * kP8TW, ty * kP8TH, pa, pb, img);
+}
+
+// from_m1: the launch reads map 1 and writes map 0 (else the other way); kReluDeconv writes
+// the image's output instead of a map
+template <int MODE, typename TOUT>
+__global__ __launch_bounds__(kP8NT) void pair8_batch(const SplitDesc* __restrict__ desc, int n, int total, int from_m1,
+                                                   const LayerArgs la0, const LayerArgs la)
+{
+    __shared__ float4 pa[kP8PW * kP8PH];
+    __shared__ float4 pb[kP8PW * kP8PH];
+    const int t = xcd_tile(blockIdx.x, total);
+    const SplitDesc d = find_tile(desc, n, t);
+    const int local = t - d.first_tile, ty = local / d.tiles_x, tx = local - ty * d.tiles_x;
+    LayerArgs lb = la;
+    if constexpr (MODE == kReluDeconv) {
+        lb.dst = d.dst;
+        lb.ds = d.dstride;
+    }
+    pair8_tile<MODE, float, TOUT, kRelu>(from_m1 ? d.m1 : d.m0, d.fp, d.w, d.h, from_m1 ? d.m0 : d.m1, la0, lb,
+                                         tx * kP8TW, ty * kP8TH, pa, pb);
+}
+
 template <typename TIN, typename TOUT>
 static hipError_t legacy_split_t(const void* src, long ss, int W, int H, void* dst, long ds, const float* wts,
                                  const float* wa32, float4* f0, float4* f1, long fp, hipStream_t stream)
@@ -258,6 +304,53 @@ hipError_t launch_legacy_split(int stype, int dtype, const void* src, long ss, i
     float4* f0 = static_cast<float4*>(workspace);
     float4* f1 = reinterpret_cast<float4*>(static_cast<char*>(workspace) + fp * H);
 #define CALL(TI, TO) legacy_split_t<TI, TO>(src, ss, W, H, dst, ds, wts, wa32, f0, f1, fp, stream)
+    switch (stype) {
+    case U8: switch (dtype) {
+        case U8: return CALL(uint8_t, uint8_t); case U16: return CALL(uint8_t, uint16_t);
+        case F16: return CALL(uint8_t, __half); case F32: return CALL(uint8_t, float);
+        default: return hipErrorInvalidValue; }
+    case U16: switch (dtype) {
+        case U8: return CALL(uint16_t, uint8_t); case U16: return CALL(uint16_t, uint16_t);
+        case F16: return CALL(uint16_t, __half); case F32: return CALL(uint16_t, float);
+        default: return hipErrorInvalidValue; }
+    case F16: switch (dtype) {
+        case U8: return CALL(__half, uint8_t); case U16: return CALL(__half, uint16_t);
+        case F16: return CALL(__half, __half); case F32: return CALL(__half, float);
+        default: return hipErrorInvalidValue; }
+    case F32: switch (dtype) {
+        case U8: return CALL(float, uint8_t); case U16: return CALL(float, uint16_t);
+        case F16: return CALL(float, __half); case F32: return CALL(float, float);
+        default: return hipErrorInvalidValue; }
+    default: return hipErrorInvalidValue;
+    }
+#undef CALL
+}
+
+template <typename TIN, typename TOUT>
+static hipError_t legacy_split_batch_t(const SplitDesc* desc, int n, int total, const float* wts, const float* wa32,
+                                       hipStream_t stream)
+{
+    const dim3 grid(total), block(kP8NT);
+    auto layer = [&](int l) {
+        LayerArgs la{};
+        la.wa = wa32 + kMfma32L0Floats + static_cast<long>(l - 1) * kMfma32LayerFloats;
+        la.bias = wts + offB(l);
+        return la;
+    };
+    hipLaunchKernelGGL((legacy_head_batch<TIN>), grid, block, 0, stream, desc, n, total, wts, wa32, layer(1), layer(2));
+    hipLaunchKernelGGL((pair8_batch<kRelu, float>), grid, block, 0, stream, desc, n, total, 0, layer(3), layer(4));
+    hipLaunchKernelGGL((pair8_batch<kRelu, float>), grid, block, 0, stream, desc, n, total, 1, layer(5), layer(6));
+    LayerArgs last = layer(8);
+    last.k2 = wts + kOffK2;
+    hipLaunchKernelGGL((pair8_batch<kReluDeconv, TOUT>), grid, block, 0, stream, desc, n, total, 0, layer(7), last);
+    return hipGetLastError();
+}
+
+hipError_t launch_legacy_split_batch(int stype, int dtype, const SplitDesc* desc, int n, int total, const float* wts,
+                                     const float* wa32, hipStream_t stream)
+{
+    if (n <= 0 || total <= 0) return hipSuccess;
+#define CALL(TI, TO) legacy_split_batch_t<TI, TO>(desc, n, total, wts, wa32, stream)
     switch (stype) {
     case U8: switch (dtype) {
         case U8: return CALL(uint8_t, uint8_t); case U16: return CALL(uint8_t, uint16_t);

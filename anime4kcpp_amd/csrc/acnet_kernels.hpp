@@ -18,8 +18,21 @@ struct BatchDesc {
     int ty0;
 };
 
+// One image of a batched layer-split pass: its images, its two 8-channel f32 maps (rows fp
+// bytes apart) and its 62 x 16-pixel tiles, numbered image by image from first_tile.
+struct SplitDesc {
+    const void* src;
+    void* dst;
+    float4* m0;
+    float4* m1;
+    long sstride, dstride, fp;
+    int w, h;
+    int tiles_x, first_tile;
+};
+
 // The image that owns tile t (binary search over first_tile; desc sorted by it).
-__device__ inline BatchDesc find_tile(const BatchDesc* __restrict__ desc, int n, int t)
+template <typename D>
+__device__ inline D find_tile(const D* __restrict__ desc, int n, int t)
 {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
@@ -90,6 +103,10 @@ hipError_t launch_acnet8(int blocks, int stype, int dtype, const void* src, long
 // acnet8_feature_pitch(W) * H bytes).
 hipError_t launch_legacy_split(int stype, int dtype, const void* src, long ss, int W, int H, void* dst, long ds,
                                const float* wts, const float* wa32, void* workspace, hipStream_t stream);
+// The same for n images (one element type pair) in four launches over the tiles of all of
+// them, `total` in all; desc is device memory.
+hipError_t launch_legacy_split_batch(int stype, int dtype, const SplitDesc* desc, int n, int total, const float* wts,
+                                     const float* wa32, hipStream_t stream);
 
 // ACNet<8> 2x pass with the conv3x3 layers on 16-bit MFMA (acnet8_mfma16.hip): bf16 = 0 FP16,
 // 1 BF16 operands; wa16 that type's per-lane A operands of the blocks and the final layer

@@ -330,18 +330,14 @@ __device__ __forceinline__ void pair8_first(float4* __restrict__ pa, float4* __r
     __syncthreads();
 }
 
-template <int MODE, typename TIN, typename TOUT, int MODE_A = kPrelu>
-__global__ __launch_bounds__(kP8NT) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
-                                                  float4* __restrict__ fout, const LayerArgs la0, const LayerArgs la)
+// Both layers of a two-layer launch on the tile whose output starts at (x0, y0); pa/pb are the
+// workgroup's LDS planes.
+template <int MODE, typename TIN, typename TOUT, int MODE_A>
+__device__ __forceinline__ void pair8_tile(const float4* __restrict__ fin, long fp, int W, int H, float4* __restrict__ fout,
+                                           const LayerArgs& la0, const LayerArgs& la, int x0, int y0,
+                                           float4* __restrict__ pa, float4* __restrict__ pb)
 {
-    __shared__ float4 pa[kP8PW * kP8PH];
-    __shared__ float4 pb[kP8PW * kP8PH];
-    const int tiles_x = gridDim.x;
-    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
-    const int ty = t / tiles_x, tx = t - ty * tiles_x;
-    const int x0 = tx * kP8TW, y0 = ty * kP8TH;
     const int lane = threadIdx.x & 63, half = (lane >> 4) & 1;
-
     // plane pixel (c, r) is image pixel (x0 - 2 + c, y0 - 2 + r); stage 66 x 20, clamped
     {
         float4* const pl[2] = { pa, pb };
@@ -356,6 +352,18 @@ __global__ __launch_bounds__(kP8NT) void conv8_pair(const float4* __restrict__ f
     Epi8<MODE, TIN, TOUT> epi;
     pair8_first<MODE_A>(pa, pb, a, epa, epi, la);
     pair8_second<MODE, TIN, TOUT>(pa, pb, W, H, x0, y0, a, epi, fp, fout, la);
+}
+
+template <int MODE, typename TIN, typename TOUT, int MODE_A = kPrelu>
+__global__ __launch_bounds__(kP8NT) void conv8_pair(const float4* __restrict__ fin, long fp, int W, int H,
+                                                  float4* __restrict__ fout, const LayerArgs la0, const LayerArgs la)
+{
+    __shared__ float4 pa[kP8PW * kP8PH];
+    __shared__ float4 pb[kP8PW * kP8PH];
+    const int tiles_x = gridDim.x;
+    const int t = xcd_tile(blockIdx.y * tiles_x + blockIdx.x, tiles_x * gridDim.y);
+    const int ty = t / tiles_x, tx = t - ty * tiles_x;
+    pair8_tile<MODE, TIN, TOUT, MODE_A>(fin, fp, W, H, fout, la0, la, tx * kP8TW, ty * kP8TH, pa, pb);
 }
 
 inline dim3 conv8_pair_grid(int W, int H) { return dim3((W + kP8TW - 1) / kP8TW, (H + kP8TH - 1) / kP8TH); }

@@ -217,6 +217,47 @@ class Net:
         self._keep = (workspace, arr)    # alive until the next call on this net
         return dsts
 
+    def upscale_batch_layers(self, srcs, dsts=None, stream=None, workspace=None, out_dtype=None):
+        """FP32 2x passes of many images (same element types) in the layer-split form, four
+        launches over the tiles of all of them (achip_pass_batch_layers); outputs are the
+        fused kernel's bits.  workspace: a device tensor of batch_layers_bytes(srcs) bytes, or
+        None to allocate one.  Returns the list of outputs."""
+        import torch
+        if not srcs:
+            return []
+        dev = srcs[0].device
+        if dsts is None:
+            dsts = [torch.empty((2 * s.shape[0], 2 * s.shape[1]), dtype=out_dtype or s.dtype, device=dev) for s in srcs]
+        st = elem_type(srcs[0].dtype)
+        dt = elem_type(dsts[0].dtype)
+        arr = (_native.ACHipImage * len(srcs))()
+        for i, (s, d) in enumerate(zip(srcs, dsts)):
+            if elem_type(s.dtype) != st or elem_type(d.dtype) != dt:
+                raise TypeError("a batch has one source and one destination element type")
+            if s.stride(1) != 1 or d.stride(1) != 1 or tuple(d.shape) != (2 * s.shape[0], 2 * s.shape[1]):
+                raise ValueError(f"image {i}: bad shape or layout")
+            arr[i] = _native.ACHipImage(s.data_ptr(), s.stride(0) * s.element_size(), d.data_ptr(),
+                                        d.stride(0) * d.element_size(), s.shape[1], s.shape[0])
+        L = _native.lib()
+        need = L.achip_pass_batch_layers_bytes(arr, len(srcs))
+        if workspace is None:
+            workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+        if stream is None:
+            stream = torch.cuda.current_stream(dev).cuda_stream
+        elif hasattr(stream, "cuda_stream"):
+            stream = stream.cuda_stream
+        check(L.achip_pass_batch_layers(self._h, st, dt, arr, len(srcs), ctypes.c_void_p(workspace.data_ptr()),
+                                        workspace.numel(), ctypes.c_void_p(stream)), "achip_pass_batch_layers")
+        self._keep = (workspace, arr)
+        return dsts
+
+    def batch_layers_bytes(self, srcs):
+        """Workspace bytes of upscale_batch_layers for these images."""
+        arr = (_native.ACHipImage * len(srcs))()
+        for i, s in enumerate(srcs):
+            arr[i] = _native.ACHipImage(0, 0, 0, 0, s.shape[1], s.shape[0])
+        return _native.lib().achip_pass_batch_layers_bytes(arr, len(srcs))
+
 
     def autotune(self, w, h, precision="fp32", src_dtype=None, dst_dtype=None):
         """Time every kernel configuration of `precision` on w x h images (achip_autotune) and

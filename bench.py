@@ -216,18 +216,43 @@ def batched_uneven(nets, prec, seed=2024, n=64):
                 + ("/mfma" if hip.configs(prec)[c]["p"] == 0 else "/valu") + "/halo-recompute" for c in sweep}
     sweep_ms = {names[c]: round(t * 1e3, 3) for c, t in sweep.items()}
     if prec == "fp32":
-        # the layer-split form (one image at a time: 4 launches, f32 maps through HBM)
-        lws = torch.empty(_native.lib().achip_pass8_workspace_bytes(2048, 2048), dtype=torch.uint8, device=dev)
-        st = torch.cuda.current_stream(dev).cuda_stream
+        # the layer-split form (one image at a time: 4 launches, f32 maps through HBM), on one
+        # stream and with the images alternating over two (a workspace each)
+        nbytes = _native.lib().achip_pass8_workspace_bytes(2048, 2048)
+        lws = [torch.empty(nbytes, dtype=torch.uint8, device=dev) for _ in range(2)]
+        sts = [torch.cuda.current_stream(dev), torch.cuda.Stream(dev)]
+        st = sts[0].cuda_stream
         sweep_ms["62x16/256thr/mfma/layer-split (per image)"] = round(
-            timed(lambda: [layer_split_pass(net, s, d, lws, st) for s, d in zip(srcs, dsts)]) * 1e3, 3)
-    return {
+            timed(lambda: [layer_split_pass(net, s, d, lws[0], st) for s, d in zip(srcs, dsts)]) * 1e3, 3)
+
+        def two_streams():
+            ev = torch.cuda.Event()
+            ev.record(sts[0])
+            sts[1].wait_event(ev)
+            for i, (s, d) in enumerate(zip(srcs, dsts)):
+                layer_split_pass(net, s, d, lws[i % 2], sts[i % 2].cuda_stream)
+            ev2 = torch.cuda.Event()
+            ev2.record(sts[1])
+            sts[0].wait_event(ev2)
+        sweep_ms["62x16/256thr/mfma/layer-split (per image, two streams)"] = round(timed(two_streams) * 1e3, 3)
+        bws = torch.empty(net.batch_layers_bytes(srcs), dtype=torch.uint8, device=dev)
+        sweep_ms["62x16/256thr/mfma/layer-split batched (4 launches over all images)"] = round(
+            timed(lambda: net.upscale_batch_layers(srcs, dsts, workspace=bws)) * 1e3, 3)
+        del bws
+    out = {
         "images": n, "input_mpix": round(sum(s.numel() for s in srcs) / 1e6, 2), "output_mpix": round(out_mpix, 2),
         "one_launch": {"images_per_s": round(n / tb, 1), "mpix_per_s": round(out_mpix / tb, 1), "ms": round(tb * 1e3, 3)},
         "separate_launches": {"images_per_s": round(n / sep, 1), "mpix_per_s": round(out_mpix / sep, 1), "ms": round(sep * 1e3, 3)},
         "autotune_sweep_ms": sweep_ms,
         "chosen_config": best,
     }
+    key = "62x16/256thr/mfma/layer-split batched (4 launches over all images)"
+    if key in sweep_ms:
+        tl = sweep_ms[key] / 1e3
+        out["layer_split_batched"] = {"images_per_s": round(n / tl, 1), "mpix_per_s": round(out_mpix / tl, 1),
+                                      "ms": round(tl * 1e3, 3), "vs_one_launch": round(tb / tl, 2)}
+    out["fastest"] = min(sweep_ms, key=sweep_ms.get)
+    return out
 
 
 def precision_4k_x4(nets, weights, precs, crop=1024):

@@ -106,6 +106,17 @@ ACHIP_API int achip_pass_batch(ACHipNet* net, int precision, int config, int src
 ACHIP_API int achip_pass_layers(ACHipNet* net, const void* src, long src_stride, int src_type, int w, int h, void* dst,
                                 long dst_stride, int dst_type, void* workspace, size_t workspace_bytes, void* stream);
 
+/* The layer-split form over a batch of n images of uneven sizes (all src_type -> dst_type),
+ * in four launches that each take the tiles of every image (no counterpart in the
+ * reference, whose callers run one pass per image: tools/benchmark/src/Benchmark.cpp:58-63).
+ * `workspace`: device memory of achip_pass_batch_layers_bytes(images, n) bytes, 256-byte
+ * aligned, the descriptors and every image's two 8-channel f32 maps, not overlapping any
+ * image; no image's dst may overlap another image's src or dst.  Outputs are the fused
+ * kernel's bits, as achip_pass_layers' are. */
+ACHIP_API size_t achip_pass_batch_layers_bytes(const ACHipImage* images, int n);
+ACHIP_API int achip_pass_batch_layers(ACHipNet* net, int src_type, int dst_type, const ACHipImage* images, int n,
+                                      void* workspace, size_t workspace_bytes, void* stream);
+
 /* The 2x pass in the faster of its forms for the image's size, given a workspace.  Replaces
  * achip_pass where the caller can lend device memory (the processor's per-thread buffers do;
  * the reference's processors run one form, CUDAProcessor.cpp:383-422).  For FP32 ACNetLegacy

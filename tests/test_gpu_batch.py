@@ -124,3 +124,47 @@ def test_autotune_choice_is_used(net, cuda, prec):
     expl = net.upscale(src, precision=prec, config=best)
     torch.cuda.synchronize()
     assert torch.equal(auto, expl)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.float32])
+def test_batch_layers_equals_fused(net, weights, cuda, dtype):
+    """achip_pass_batch_layers (the layer-split form, four launches over the 62 x 16 tiles of
+    every image): each image's output is the fused kernel's alone, bit for bit, for tiny,
+    tile-edge and uneven sizes, and within the oracle's tolerance."""
+    dims = [(1, 1), (5, 3), (62, 16), (63, 17), (130, 71)] + [(int(w), int(h)) for w, h in sizes(21, 9, 16, 400)]
+    srcs = [synth.flat_shaded(80 + i, w, h, dtype=dtype) for i, (w, h) in enumerate(dims)]
+    dsrc = [torch.from_numpy(s).to(cuda) for s in srcs]
+    out = net.upscale_batch_layers(dsrc)
+    torch.cuda.synchronize()
+    k, b = weights["hdn2"]
+    for i, s in enumerate(dsrc):
+        alone = net.upscale(s, config=0)
+        torch.cuda.synchronize()
+        assert torch.equal(alone, out[i]), f"image {i} {dims[i]} differs"
+        want = oracle_pass(srcs[i], k, b)
+        (check_int if dtype == np.uint8 else check_f32)(out[i].cpu().numpy(), want)
+
+
+def test_batch_layers_rejects_bad_arguments(net, cuda):
+    """A short workspace, a dst overlapping another image's src, and a non-ACNetLegacy net are
+    refused before any launch (ACHIP_EINVAL)."""
+    L = _native.lib()
+    srcs = [torch.zeros((40, 50), dtype=torch.uint8, device=cuda) for _ in range(2)]
+    dsts = [torch.zeros((80, 100), dtype=torch.uint8, device=cuda) for _ in range(2)]
+    arr = (_native.ACHipImage * 2)()
+    for i in range(2):
+        arr[i] = _native.ACHipImage(srcs[i].data_ptr(), 50, dsts[i].data_ptr(), 100, 50, 40)
+    need = L.achip_pass_batch_layers_bytes(arr, 2)
+    ws = torch.empty(need, dtype=torch.uint8, device=cuda)
+    assert L.achip_pass_batch_layers(net.handle, 1, 1, arr, 2, ctypes.c_void_p(ws.data_ptr()), need - 256, None) == -22
+    big = torch.zeros((80, 100), dtype=torch.uint8, device=cuda)
+    arr[1] = _native.ACHipImage(big.data_ptr(), 100, big.data_ptr() + 0, 100, 50, 40)   # its src is its own dst
+    assert L.achip_pass_batch_layers(net.handle, 1, 1, arr, 2, ctypes.c_void_p(ws.data_ptr()), need, None) == -22
+    arr[1] = _native.ACHipImage(srcs[1].data_ptr(), 50, srcs[0].data_ptr(), 100, 25, 20)   # dst over image 0's src
+    assert L.achip_pass_batch_layers(net.handle, 1, 1, arr, 2, ctypes.c_void_p(ws.data_ptr()), need, None) == -22
+    n8 = hip.Net8("b4", 0)
+    arr[1] = _native.ACHipImage(srcs[1].data_ptr(), 50, dsts[1].data_ptr(), 100, 50, 40)
+    assert L.achip_pass_batch_layers(n8.handle, 1, 1, arr, 2, ctypes.c_void_p(ws.data_ptr()), need, None) == -22
+    n8.close()
+    assert L.achip_pass_batch_layers(net.handle, 1, 1, arr, 2, ctypes.c_void_p(ws.data_ptr()), need, None) == 0
+    torch.cuda.synchronize()

@@ -44,6 +44,8 @@ def test_host_side_errors():
     assert L.achip_config_info(0, 999, None, None, None, None, None) == -22
     assert L.achip_pass_ws_bytes(None, 0, 1, 1, 1920, 1080) == 0
     assert L.achip_pass_ws(None, 0, None, 0, 1, 1, 1, None, 0, 1, None, 0, None) == -22
+    assert L.achip_pass_batch_layers_bytes(None, 0) == 0
+    assert L.achip_pass_batch_layers(None, 1, 1, None, 0, None, 0, None) == -22
     if L.achip_device_count() == 0:
         import numpy as np
         k = np.zeros(4712, np.float32)
