@@ -181,7 +181,7 @@ def error_vs_fp64(nets, weights, prec):
             "psnr_db": round(10 * np.log10(1.0 / mse), 2) if mse > 0 else None}
 
 
-def batched_uneven(nets, prec, seed=2024, n=64):
+def batched_uneven(nets, prec, seed=2024, n=64, streams=None):
     """64 u8 images, sides uniform in [128, 2048] (seeded): one persistent launch with a tile
     work-queue per configuration (the autotuner's sweep) against 64 separate launches."""
     import numpy as np
@@ -220,7 +220,9 @@ def batched_uneven(nets, prec, seed=2024, n=64):
         # stream and with the images alternating over two (a workspace each)
         nbytes = _native.lib().achip_pass8_workspace_bytes(2048, 2048)
         lws = [torch.empty(nbytes, dtype=torch.uint8, device=dev) for _ in range(2)]
-        sts = [torch.cuda.current_stream(dev), torch.cuda.Stream(dev)]
+        # the main section's two streams: a new stream here would change which hardware queues
+        # the serving section's streams share (4 per process by default)
+        sts = list(streams[:2]) if streams and len(streams) >= 2 else [torch.cuda.current_stream(dev), torch.cuda.Stream(dev)]
         st = sts[0].cuda_stream
         sweep_ms["62x16/256thr/mfma/layer-split (per image)"] = round(
             timed(lambda: [layer_split_pass(net, s, d, lws[0], st) for s, d in zip(srcs, dsts)]) * 1e3, 3)
@@ -747,7 +749,7 @@ def main():
             else:
                 result[f"{p}_mfma"]["error_vs_fp64"] = e
     if not args.no_batch:
-        result["batched_uneven"] = {p: batched_uneven(nets, p) for p in precs[:2]}
+        result["batched_uneven"] = {p: batched_uneven(nets, p, streams=streams) for p in precs[:2]}
     if not args.no_4k:
         result["precision_4k_x4"] = precision_4k_x4(nets, weights, precs)
     if not args.no_stages:
