@@ -567,10 +567,10 @@ __global__ __launch_bounds__(256) void resize_up_kernel(const void* __restrict__
 // tile's own on each axis in total.
 constexpr int kU8Cols = 256;   // output columns per workgroup (64 lanes x 4)
 
-template <int C, int K, int NT, int ROWS>
+template <int C, int K, int NT, int ROWS, int NWV>
 struct U8Up {
     static constexpr int S = NT / 2;
-    static constexpr int R = 4 * ROWS;             // output rows per workgroup (4 waves)
+    static constexpr int R = NWV * ROWS;           // output rows per workgroup (NWV waves)
     static constexpr int SW = kU8Cols / K + NT;    // staged source columns
     static constexpr int SH = R / K + NT;          // staged source rows
     static constexpr int HR = ROWS / K + NT;       // source rows one wave's band reads
@@ -583,10 +583,11 @@ struct U8Up {
 
 // fromFloat<uint8_t> for finite v as a 32-bit code: the median of (v, 0, 1) is the
 // reference's saturation for every finite v except that -0 may come out as +0, and both
-// give code 0.  (The sums below are of finite u8 inputs, so no NaN reaches it.)
+// give code 0.  (The sums below are of finite u8 inputs, so no NaN reaches it.)  The
+// multiply and add are one FMA: see u8_scale_code.
 __device__ inline unsigned u8_code(float v)
 {
-    return static_cast<unsigned>(__fadd_rn(__fmul_rn(__builtin_amdgcn_fmed3f(v, 0.0f, 1.0f), 255.0f), 0.5f));
+    return u8_scale_code(__builtin_amdgcn_fmed3f(v, 0.0f, 1.0f));
 }
 
 template <int C, bool MERGE>
@@ -597,12 +598,12 @@ struct U8Lut {
 template <int C>
 struct U8Lut<C, false> {};
 
-template <int C, int K, int NT, int ROWS, bool MERGE>
-__global__ __launch_bounds__(256) void resize_up_u8_kernel(const uint8_t* __restrict__ src, long ss, int sw, int sh,
+template <int C, int K, int NT, int ROWS, int NWV, bool MERGE>
+__global__ __launch_bounds__(64 * NWV) void resize_up_u8_kernel(const uint8_t* __restrict__ src, long ss, int sw, int sh,
                                                           uint8_t* __restrict__ dst, long ds, int dw, int dh, int mode,
                                                           const uint8_t* __restrict__ yp, long ys, int vec)
 {
-    using G = U8Up<C, K, NT, ROWS>;
+    using G = U8Up<C, K, NT, ROWS, NWV>;
     constexpr int CO = MERGE ? C + 1 : C;   // output bytes per pixel
     __shared__ float in[G::SH][G::SW][C];
     __shared__ float wt[2][K][NT];          // [axis][phase][tap]
@@ -618,13 +619,14 @@ __global__ __launch_bounds__(256) void resize_up_u8_kernel(const uint8_t* __rest
 #pragma unroll
         for (int j = 0; j < NT; j++) wt[axis][p][j] = j < n ? w[j] : 0.0f;
     }
-    if constexpr (MERGE) {
-        const float f = to_float<uint8_t>(static_cast<uint8_t>(tid)), d = __fsub_rn(f, 0.5f);
-        lut.y[tid] = f;
-        lut.u[tid] = make_float2(__fmul_rn(0.344f, d), __fmul_rn(1.773f, d));
-        lut.v[tid] = make_float2(__fmul_rn(1.403f, d), __fmul_rn(0.714f, d));
-    }
-    for (int r = wave; r < G::SH; r += 4) {
+    if constexpr (MERGE)
+        for (int i = tid; i < 256; i += 64 * NWV) {
+            const float f = to_float<uint8_t>(static_cast<uint8_t>(i)), d = __fsub_rn(f, 0.5f);
+            lut.y[i] = f;
+            lut.u[i] = make_float2(__fmul_rn(0.344f, d), __fmul_rn(1.773f, d));
+            lut.v[i] = make_float2(__fmul_rn(1.403f, d), __fmul_rn(0.714f, d));
+        }
+    for (int r = wave; r < G::SH; r += NWV) {
         const uint8_t* row = src + static_cast<long>(clampi(iy0 + r, 0, sh - 1)) * ss;
         for (int c = lane; c < G::SW; c += 64) {
             const uint8_t* p = row + static_cast<long>(clampi(ix0 + c, 0, sw - 1)) * C;
@@ -764,11 +766,11 @@ static void launch_up_u8(const void* src, long ss, int sw, int sh, void* dst, lo
     auto* d8 = static_cast<uint8_t*>(dst);
     if (static_cast<long>(gx) * ((dh + 31) / 32) >= 512) {
         const dim3 grid(gx, (dh + 31) / 32);
-        hipLaunchKernelGGL((resize_up_u8_kernel<C, K, NT, 8, MERGE>), grid, dim3(256), 0, stream, s8, ss, sw, sh, d8, ds, dw,
+        hipLaunchKernelGGL((resize_up_u8_kernel<C, K, NT, 8, 4, MERGE>), grid, dim3(256), 0, stream, s8, ss, sw, sh, d8, ds, dw,
                            dh, mode, y8, ys, vec);
     } else {
         const dim3 grid(gx, (dh + 15) / 16);
-        hipLaunchKernelGGL((resize_up_u8_kernel<C, K, NT, 4, MERGE>), grid, dim3(256), 0, stream, s8, ss, sw, sh, d8, ds, dw,
+        hipLaunchKernelGGL((resize_up_u8_kernel<C, K, NT, 4, 4, MERGE>), grid, dim3(256), 0, stream, s8, ss, sw, sh, d8, ds, dw,
                            dh, mode, y8, ys, vec);
     }
 }

@@ -66,9 +66,14 @@ template <> __device__ inline float to_float<float>(float v) { return v; }
 // with the multiply and the add rounded separately and the conversion truncating.
 template <typename T> __device__ inline T from_float(float v);
 __device__ inline float saturate01(float v) { return v < 0.0f ? 0.0f : (v < 1.0f ? v : 1.0f); }
+// (unsigned)(s * 255 + 0.5f) for s in [0, 1], the multiply and the add rounded separately,
+// computed as one FMA: the truncated results are equal for every float s in [0, 1] (all
+// 1,065,353,217 of them compared on the CPU; the windows where the two roundings could
+// part are checked in tests/test_device_math.py).
+__device__ inline unsigned u8_scale_code(float s) { return static_cast<unsigned>(__fmaf_rn(s, 255.0f, 0.5f)); }
 template <> __device__ inline uint8_t from_float<uint8_t>(float v)
 {
-    return static_cast<uint8_t>(static_cast<unsigned>(__fadd_rn(__fmul_rn(saturate01(v), 255.0f), 0.5f)));
+    return static_cast<uint8_t>(u8_scale_code(saturate01(v)));
 }
 template <> __device__ inline uint16_t from_float<uint16_t>(float v)
 {
