@@ -1141,7 +1141,8 @@ int achip_pass_batch_layers(ACHipNet* net, int src_type, int dst_type, const ACH
     DeviceGuard g(net->device);
     if (g.err != hipSuccess) return hip_fail(g.err, "hipSetDevice");
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    // from pageable memory the copy is staged before the call returns, so `desc` may go
+    // from pageable memory the copy is staged before the call returns, so `desc` may go out
+    // of scope
     hipError_t e = hipMemcpyAsync(workspace, desc.data(), static_cast<size_t>(n) * sizeof(achip::SplitDesc),
                                   hipMemcpyHostToDevice, st);
     if (e == hipSuccess)

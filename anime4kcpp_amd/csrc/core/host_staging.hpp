@@ -24,7 +24,8 @@ namespace ac::core::detail
         static constexpr size_t kChunk = size_t(4) << 20;
 
         // Rows of `row` bytes from host `h` (pitch hpitch) to device `d` (pitch dpitch),
-        // queued on `s`.  Returns once the host rows may be changed again.
+        // queued on `s`.  Pageable rows may be changed again once this returns; page-locked
+        // rows are read by the DMA itself, so they must stay unchanged until `s` has passed it.
         bool toDevice(void* d, size_t dpitch, const void* h, size_t hpitch, size_t row, int rows, hipStream_t s)
         {
             const int per = perChunk(row, rows);
