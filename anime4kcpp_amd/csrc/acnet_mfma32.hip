@@ -128,7 +128,12 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
                                             TIN* __restrict__ img)
 {
     using C = Mfma32Tile<TW, TH, NT, sizeof(TIN)>;
-    const int tid = threadIdx.x;
+    // The thread index passes through an empty asm statement, so nothing derived from it
+    // (LDS offsets, operand loads) is hoisted out of a persistent kernel's tile loop: held
+    // across the loop, those values pushed the 1024-thread batch kernels past their 128
+    // VGPRs into scratch spills (18 and 12 VGPRs; none with this).
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int bx = x0 - 8, by = y0 - 8;
@@ -331,11 +336,15 @@ __global__ __launch_bounds__(NT) void acnet_mfma32_batch(const BatchDesc* __rest
     __shared__ float4 pb[C::PW * C::PH];
     __shared__ TIN img[C::IW * C::IH];
     __shared__ int s_tile;
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
-        __syncthreads();
-        const int t = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: keeps the tile state scalar
-        if (t >= total) break;
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+    __syncthreads();
+    int t = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: keeps the tile state scalar
+    while (t < total) {
+        // The next tile is drawn as this one starts and read once it is done, so the atomic's
+        // round trip overlaps the tile instead of leaving the CU idle between tiles (one
+        // workgroup per CU).  Each workgroup still draws exactly one index past its last tile.
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1);
         const BatchDesc d = find_tile(desc, n, t);
         const int local = t - d.first_tile;
         const int ly = local / d.tiles_x;
@@ -343,7 +352,10 @@ __global__ __launch_bounds__(NT) void acnet_mfma32_batch(const BatchDesc* __rest
         const int ty = d.ty0 + ly;
         mfma32_tile<TIN, TOUT, TW, TH, NT>(d.src, d.sstride, d.w, d.h, d.dst, d.dstride, wts, wa,
                                            tx * TW, ty * TH, pa, pb, img);
+        __syncthreads();                    // the tile's LDS and s_tile are free again
+        if (threadIdx.x == 0) s_tile = next;
         __syncthreads();
+        t = __builtin_amdgcn_readfirstlane(s_tile);
     }
 }
 

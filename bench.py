@@ -206,9 +206,13 @@ def batched_uneven(nets, prec, seed=2024, n=64, streams=None):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / reps
 
+    # three rounds over the configurations, the best round of each: a configuration measured
+    # first in a single round read 5-15% slow
     sweep = {}
-    for c in range(len(hip.configs(prec))):
-        sweep[c] = timed(lambda: net.upscale_batch(srcs, dsts, precision=prec, config=c, workspace=ws_t))
+    for _ in range(3):
+        for c in range(len(hip.configs(prec))):
+            sweep[c] = min(sweep.get(c, float("inf")),
+                           timed(lambda: net.upscale_batch(srcs, dsts, precision=prec, config=c, workspace=ws_t)))
     best = min(sweep, key=sweep.get)
     sep = timed(lambda: [net.upscale(s, d, precision=prec, config=best) for s, d in zip(srcs, dsts)])
     tb = sweep[best]
