@@ -149,7 +149,9 @@ ACHIP_API int achip_pass_batch_noalloc(ACHipNet* net, int precision, int config,
 /* Kernel configurations of a precision: tile width/height (input pixels), threads per
  * workgroup, pixels per thread run of the FP32 VALU kernel (0 for the MFMA kernels), LDS
  * bytes for f32 input.  FP32: the FP32-MFMA kernel first (configuration 0 is the default),
- * then the FP32 VALU kernel.  Returns the count, or fills config i. */
+ * then the FP32 VALU kernel.  FP16/BF16: the 64x32 tile at 512 threads (configuration 0, the
+ * default for u8 input) and the 64x64 tile at 1024 threads (configuration 1, the default for
+ * wider inputs).  Returns the count, or fills config i. */
 ACHIP_API int achip_config_count(int precision);
 ACHIP_API int achip_config_info(int precision, int i, int* tw, int* th, int* nt, int* p, int* lds_bytes);
 
