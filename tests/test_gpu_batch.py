@@ -49,6 +49,20 @@ def test_batch_equals_alone(net, weights, cuda, prec, dtype):
                 check_f32(got, want)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+@pytest.mark.parametrize("config", [0, 1])
+def test_batch_one_tile(net, cuda, prec, config):
+    """A batch of one image of one tile: every workgroup of the persistent grid but one draws
+    an index past the end at once, and the one that runs the tile draws its next index while
+    running it; the output equals the image alone."""
+    for w, h in [(1, 1), (7, 5)]:
+        s = torch.from_numpy(synth.flat_shaded(90 + w, w, h, dtype=np.uint8)).to(cuda)
+        out = net.upscale_batch([s], precision=prec, config=config)
+        alone = net.upscale(s, precision=prec, config=config)
+        torch.cuda.synchronize()
+        assert torch.equal(out[0], alone), f"{prec} config {config}: {w}x{h} differs"
+
+
 @pytest.mark.parametrize("config", [0, 1])
 def test_batch_64_uneven_u8(net, weights, cuda, config):
     """64 u8 images with sides uniform in [128, 2048] (seeded), one launch; spot-checked
