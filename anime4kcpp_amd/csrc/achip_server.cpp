@@ -788,7 +788,7 @@ int achip_server_create(ACHipNet* net, const ACHipServerConfig* config, ACHipSer
     const bool layered = net->blocks || net->lfamily;
     if (net->lfamily && c.precision != ACHIP_FP32) return ACHIP_EINVAL;   // ArtCNN and FSRCNNX run in FP32 only
     if (c.hbm_budget == 0) c.hbm_budget = static_cast<size_t>(16) << 30;
-    if (c.step_tiles <= 0) c.step_tiles = c.precision == ACHIP_FP32 ? 2048 : 12288;   // 8.4 / 25.2 Mpix
+    if (c.step_tiles <= 0) c.step_tiles = c.precision == ACHIP_FP32 ? 2048 : 12288;   // 4.2 / 25.2 Mpix
     if (c.max_workgroups < 0) return ACHIP_EINVAL;
     auto s = std::make_unique<ACHipServer>();
     s->net = net;

@@ -106,8 +106,9 @@ __device__ __forceinline__ void mfma_tile(const void* __restrict__ src, long sst
     constexpr int P = C::kPh;
     // The thread index passes through an empty asm statement, so nothing derived from it
     // (LDS offsets, operand loads) is hoisted out of a persistent kernel's tile loop: held
-    // across the loop, those values pushed the 1024-thread batch kernels past their 128
-    // VGPRs into scratch spills (18 and 12 VGPRs; none with this).
+    // across the loop, those values pushed the 1024-thread batch kernel past its 128 VGPRs
+    // into scratch spills (18 VGPRs) and the 512-thread one to 160 VGPRs, one workgroup per
+    // CU (126 VGPRs for both and no spills with this).
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int lane = tid & 63;
@@ -307,8 +308,9 @@ __global__ __launch_bounds__(NT) void acnet_mfma_batch(const BatchDesc* __restri
     int t = __builtin_amdgcn_readfirstlane(s_tile);   // uniform: keeps the tile state scalar
     while (t < total) {
         // The next tile is drawn as this one starts and read once it is done, so the atomic's
-        // round trip overlaps the tile instead of leaving the CU idle between tiles (one
-        // workgroup per CU).  Each workgroup still draws exactly one index past its last tile.
+        // round trip overlaps the tile instead of leaving the CU idle between tiles (one or
+        // two workgroups per CU).  Each workgroup still draws exactly one index past its last
+        // tile.
         int next = 0;
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
         const BatchDesc d = find_tile(desc, n, t);

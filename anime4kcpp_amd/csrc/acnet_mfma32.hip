@@ -130,8 +130,8 @@ __device__ __forceinline__ void mfma32_tile(const void* __restrict__ src, long s
     using C = Mfma32Tile<TW, TH, NT, sizeof(TIN)>;
     // The thread index passes through an empty asm statement, so nothing derived from it
     // (LDS offsets, operand loads) is hoisted out of a persistent kernel's tile loop: held
-    // across the loop, those values pushed the 1024-thread batch kernels past their 128
-    // VGPRs into scratch spills (18 and 12 VGPRs; none with this).
+    // across the loop, those values pushed the 1024-thread batch kernel past its 128 VGPRs
+    // into scratch spills (12 VGPRs; 114 VGPRs and no spills with this).
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int lane = tid & 63;
@@ -342,7 +342,8 @@ __global__ __launch_bounds__(NT) void acnet_mfma32_batch(const BatchDesc* __rest
     while (t < total) {
         // The next tile is drawn as this one starts and read once it is done, so the atomic's
         // round trip overlaps the tile instead of leaving the CU idle between tiles (one
-        // workgroup per CU).  Each workgroup still draws exactly one index past its last tile.
+        // 1024-thread workgroup per CU).  Each workgroup still draws exactly one index past
+        // its last tile.
         int next = 0;
         if (threadIdx.x == 0) next = atomicAdd(counter, 1);
         const BatchDesc d = find_tile(desc, n, t);
