@@ -997,26 +997,32 @@ int achip_autotune(ACHipNet* net, int precision, int src_type, int dst_type, int
         cleanup();
         return hip_fail(e, "autotune setup");
     }
+    // Two rounds over the configurations, each one's faster round: in a single round the
+    // configuration timed first read 5-15% slow.
+    std::vector<float> cfg_ms(static_cast<size_t>(n), 1e30f);
+    for (int round = 0; round < 2 && rc == ACHIP_OK; round++)
+        for (int c = 0; c < n && rc == ACHIP_OK; c++) {
+            rc = achip_pass_config(net, precision, c, s, sp, src_type, w, h, d, dp, dst_type, st);   // warm-up
+            if (rc) break;
+            const int reps = 3;
+            (void)hipEventRecord(e0, st);
+            for (int r = 0; r < reps && rc == ACHIP_OK; r++)
+                rc = achip_pass_config(net, precision, c, s, sp, src_type, w, h, d, dp, dst_type, st);
+            (void)hipEventRecord(e1, st);
+            e = hipEventSynchronize(e1);
+            if (e != hipSuccess) {
+                cleanup();
+                return hip_fail(e, "autotune run");
+            }
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, e0, e1);
+            cfg_ms[static_cast<size_t>(c)] = std::min(cfg_ms[static_cast<size_t>(c)], t / reps);
+        }
     float best_ms = 1e30f;
     int best_cfg = 0;
     for (int c = 0; c < n && rc == ACHIP_OK; c++) {
-        rc = achip_pass_config(net, precision, c, s, sp, src_type, w, h, d, dp, dst_type, st);   // warm-up
-        if (rc) break;
-        const int reps = 3;
-        (void)hipEventRecord(e0, st);
-        for (int r = 0; r < reps && rc == ACHIP_OK; r++)
-            rc = achip_pass_config(net, precision, c, s, sp, src_type, w, h, d, dp, dst_type, st);
-        (void)hipEventRecord(e1, st);
-        e = hipEventSynchronize(e1);
-        if (e != hipSuccess) {
-            cleanup();
-            return hip_fail(e, "autotune run");
-        }
-        float t = 0.0f;
-        (void)hipEventElapsedTime(&t, e0, e1);
-        t /= reps;
-        if (ms) ms[c] = t;
-        if (t < best_ms) { best_ms = t; best_cfg = c; }
+        if (ms) ms[c] = cfg_ms[static_cast<size_t>(c)];
+        if (cfg_ms[static_cast<size_t>(c)] < best_ms) { best_ms = cfg_ms[static_cast<size_t>(c)]; best_cfg = c; }
     }
     // FP32 ACNetLegacy: the layer-split form too (same bits), for achip_pass_ws
     int split = -1;
